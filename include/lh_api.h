@@ -1,0 +1,144 @@
+/*
+ * lh_api.h -- C ABI of the MI355X liquid-step library (liblh.so).
+ *
+ * The reference exposes no FFI: its boundary is the C++ class surface of
+ * Scene<3> (Scene.h:14-48), LiquidModel<3> (LiquidModel.h:11-61) and
+ * CGSolver<3> (PressureSolver.h:81-145).  Every entry point below replaces one
+ * method of that surface (cited per function).  The C++ facade in
+ * include/lh/liquid.hpp and the Python binding in liquidhairinteration_amd/
+ * are thin wrappers over these functions.
+ *
+ * Conventions
+ *  - every call returns an int status (LH_OK == 0); no exception crosses the ABI;
+ *    lh_last_error(ctx) returns the message of the last failure on ctx;
+ *  - the context owns all device memory; host pointers are borrowed for the
+ *    duration of the call (copy in / copy out);
+ *  - calls on one context are serialised on the context's HIP stream and return
+ *    after completion; one context per host thread;
+ *  - grid arrays use the reference Array3 layout (Array.h:78-83): x-major,
+ *    z fastest, index = (x*Ny + y)*Nz + z, with the shape of each field given by
+ *    lh_field_shape();
+ *  - particle arrays: pos[3*i+a], vel[3*i+a], C[9*i + 3*col + row] (the Eigen
+ *    column-major affine matrix of Particle<3>, ParticleSystem.h:5-22).
+ */
+#ifndef LH_API_H
+#define LH_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define LH_OK 0
+#define LH_ERR_CONFIG 1        /* bad / missing JSON key (SceneParameter.h:22-43 throws) */
+#define LH_ERR_OOM 2           /* device allocation failed */
+#define LH_ERR_NOT_CONVERGED 3 /* PCG hit maxIterations (PressureSolver.cpp:929-931 asserts) */
+#define LH_ERR_NAN 4           /* NaN/Inf alpha or beta (PressureSolver.cpp:884-887, 908-911) */
+#define LH_ERR_DEVICE 5        /* HIP runtime error */
+#define LH_ERR_ARG 6           /* invalid argument (sizes, ids, null pointers) */
+#define LH_ERR_DOMAIN 7        /* a particle left the grid box (binning needs in-box cells) */
+
+/* ---- field ids for lh_get_field / lh_set_field ------------------------ */
+enum lh_field {
+  LH_FIELD_VX = 0,          /* (N+1,N,N)   Grid<3>::vx       Grid.h:152 */
+  LH_FIELD_VY = 1,          /* (N,N+1,N)   Grid<3>::vy */
+  LH_FIELD_VZ = 2,          /* (N,N,N+1)   Grid<3>::vz */
+  LH_FIELD_TEMPVX = 3,      /* extrapolation ping-pong buffers, Grid.h:157-159 */
+  LH_FIELD_TEMPVY = 4,
+  LH_FIELD_TEMPVZ = 5,
+  LH_FIELD_LIQUID_WX = 6,   /* solid face openness, Grid.h:165-167 */
+  LH_FIELD_LIQUID_WY = 7,
+  LH_FIELD_LIQUID_WZ = 8,
+  LH_FIELD_PRESSURE = 9,    /* (N,N,N) */
+  LH_FIELD_LIQUID_PHI = 10, /* (N,N,N) */
+  LH_FIELD_SOLID_PHI = 11,  /* (N+1,N+1,N+1) nodal */
+  LH_FIELD_IS_VALID = 12,   /* (N+1,N+1,N+1) uint8, Grid.h:160 */
+  LH_FIELD_RHS = 13,        /* (N,N,N) CGSolver<3>::m_rhs of the last solve */
+  LH_FIELD_COUNT = 14
+};
+
+/* ---- solver / model options (mirror the JSON keys) -------------------- */
+enum lh_precond { LH_PRECOND_NONE = 0, LH_PRECOND_JACOBI = 1, LH_PRECOND_MULTIGRID = 2 };
+enum lh_tol_mode { LH_TOL_ABSOLUTE = 0, LH_TOL_RELATIVE = 1 };
+/* reference: Q4/Q5 smoother semantics of PressureSolver.cpp:821-837 with z[0] kept
+ * between applications; symmetric: z <- z + w(r-Az)/D, z[0] zeroed per application,
+ * post-smoothing in reverse colour order (a fixed SPD preconditioner). */
+enum lh_smoother { LH_SMOOTHER_REFERENCE = 0, LH_SMOOTHER_SYMMETRIC = 1 };
+
+typedef struct lh_solve_stats {
+  int iterations;          /* PCG iterations run */
+  int converged;           /* 1 if the stop rule was met */
+  int outer_iterations;    /* FP64 refinement sweeps (0 in plain FP32 mode) */
+  int warm_start_used;     /* 1 if a learned initial guess was kept */
+  double initial_residual; /* sqrt(sum r0^2) as the solver measures it */
+  double final_residual;   /* last residual the stop rule saw */
+  double rhs_norm;         /* ||b||_2 */
+  double true_residual;    /* FP64 ||b - A x||_2 / ||b||_2 over liquid cells (refined mode) */
+  double solve_ms;         /* device time of the solve (hip events) */
+} lh_solve_stats;
+
+typedef struct lh_step_stats {
+  lh_solve_stats solve;
+  double divergence_mean;  /* LiquidModel<3>::ComputeDivergence, LiquidModel.cpp:1065-1083 */
+  double divergence_linf;  /* max |div| over cells with liquidPhi <= 0 */
+  int fluid_cells;         /* count of cells with liquidPhi <= 0 */
+  float max_velocity;      /* max_p max_a |v_pa| (LiquidModel.h:49-55) */
+  float cfl_dt;            /* clamp(dx / max_velocity, 0.001, 0.1) */
+  double sim_time;         /* simulated time after the step */
+  double step_ms;          /* device time of the whole step */
+} lh_step_stats;
+
+typedef struct lh_ctx lh_ctx;
+
+/* ---- lifecycle: Scene<3>(configFile) + LiquidModel<3> ctor, Scene.cpp:13-20 */
+int lh_create(const char* json_path, lh_ctx** out);
+int lh_create_from_string(const char* json_text, lh_ctx** out);
+void lh_destroy(lh_ctx* ctx);
+const char* lh_last_error(const lh_ctx* ctx);
+/* grid geometry: Grid<3> ctor, Grid.cpp:86-98 */
+int lh_grid_info(const lh_ctx* ctx, int dims[3], float min_coord[3], float dx[3]);
+int lh_field_shape(const lh_ctx* ctx, int field, int shape[3]);
+
+/* ---- LiquidModel<3> driver surface -------------------------------------- */
+int lh_init(lh_ctx* ctx);                            /* LiquidModel.cpp:506-512 */
+int lh_pre_step(lh_ctx* ctx, float dt);              /* LiquidModel.cpp:514-517 */
+int lh_step(lh_ctx* ctx, float dt);                  /* LiquidModel.cpp:519-528 */
+int lh_post_step(lh_ctx* ctx, float dt, lh_step_stats* stats); /* LiquidModel.cpp:530-537 */
+int lh_get_cfl(lh_ctx* ctx, float* dt);              /* LiquidModel.h:49-55 */
+/* main.cpp:47-53 loop without rendering: nsteps x {PreStep, Step, PostStep, dt=GetCFL}.
+ * *dt is the first step's dt on entry and the next dt on return. */
+int lh_run_steps(lh_ctx* ctx, int nsteps, float* dt, lh_step_stats* last);
+
+/* ---- stage entry points (public stage methods of LiquidModel.h:25-47) ---- */
+int lh_compute_solid_phi(lh_ctx* ctx);               /* LiquidModel.cpp:562-611 */
+int lh_clear_grid(lh_ctx* ctx);                      /* Grid.cpp:157-169 */
+int lh_compute_liquid_phi(lh_ctx* ctx);              /* LiquidModel.cpp:539-560 */
+int lh_p2g(lh_ctx* ctx);                             /* LiquidModel.cpp:613-721 */
+int lh_apply_gravity(lh_ctx* ctx, float dt);         /* LiquidModel.cpp:898-910 */
+int lh_solve_pressure(lh_ctx* ctx, float dt, lh_solve_stats* stats); /* CGSolver<3>::Solve, PressureSolver.cpp:839-934 */
+int lh_velocity_update(lh_ctx* ctx, float dt);       /* LiquidModel.cpp:834-889 */
+int lh_extrapolate(lh_ctx* ctx);                     /* LiquidModel.cpp:937-1063 */
+int lh_g2p(lh_ctx* ctx);                             /* LiquidModel.cpp:723-823 */
+int lh_advect(lh_ctx* ctx, float dt);                /* LiquidModel.cpp:912-935 */
+int lh_divergence(lh_ctx* ctx, double* mean, double* linf, int* count); /* LiquidModel.cpp:1065-1083 */
+
+/* ---- data ---------------------------------------------------------------- */
+int lh_num_particles(const lh_ctx* ctx, int64_t* n);
+/* ParticleSystem<3>::Resize + SetPosition/SetVelocity/SetAffineMatrix
+ * (ParticleSystem.cpp:10-21, ParticleSystem.h:44-49); vel / C may be NULL (zero). */
+int lh_set_particles(lh_ctx* ctx, const float* pos, const float* vel, const float* C, int64_t n);
+int lh_get_particles(lh_ctx* ctx, float* pos, float* vel, float* C, int64_t n);
+/* count = number of elements of the field (uint8 for LH_FIELD_IS_VALID, float otherwise) */
+int lh_get_field(lh_ctx* ctx, int field, void* host, size_t count);
+int lh_set_field(lh_ctx* ctx, int field, const void* host, size_t count);
+/* simulated time and the tank-acceleration clock */
+int lh_set_time(lh_ctx* ctx, double t);
+int lh_get_time(const lh_ctx* ctx, double* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LH_API_H */

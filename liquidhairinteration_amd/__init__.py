@@ -1,0 +1,366 @@
+"""MI355X liquid step: Python binding of liblh.so (the C ABI in include/lh_api.h).
+
+The classes mirror the reference's C++ surface so that code written against it
+reads the same:
+
+    scene = Scene("liquidhairinteration_amd/configs/pure_liquid_3d.json")   # Scene<3>(configFile)
+    model = scene.GetLiquidModel()                                          # LiquidModel<3>&
+    model.Init()
+    dt = 0.01
+    while running:                                                          # main.cpp:47-53
+        model.PreStep(dt); model.Step(dt); model.PostStep(dt)
+        dt = model.GetCFL()
+
+Errors raise LHError (a RuntimeError, as the reference's std::runtime_error /
+json::out_of_range).  All compute runs in the HIP library; there is no CPU
+fallback: importing works without a GPU, but creating a Scene needs liblh.so
+and a visible MI355X.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["Scene", "LiquidModel", "CGSolver", "LHError", "library_path", "load_library", "FIELDS",
+           "SolveStats", "StepStats", "config_path"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_NAME = "liblh.so"
+
+LH_OK = 0
+ERROR_NAMES = {1: "config", 2: "out of memory", 3: "not converged", 4: "NaN detected", 5: "device", 6: "argument",
+               7: "domain"}
+
+FIELDS = {"vx": 0, "vy": 1, "vz": 2, "tempVx": 3, "tempVy": 4, "tempVz": 5, "liquidWx": 6, "liquidWy": 7,
+          "liquidWz": 8, "pressure": 9, "liquidPhi": 10, "solidPhi": 11, "isValid": 12, "rhs": 13}
+
+
+class LHError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s error (%d): %s" % (ERROR_NAMES.get(code, "unknown"), code, msg))
+        self.code = code
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("outer_iterations", C.c_int),
+                ("warm_start_used", C.c_int), ("initial_residual", C.c_double), ("final_residual", C.c_double),
+                ("rhs_norm", C.c_double), ("true_residual", C.c_double), ("solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class StepStats(C.Structure):
+    _fields_ = [("solve", SolveStats), ("divergence_mean", C.c_double), ("divergence_linf", C.c_double),
+                ("fluid_cells", C.c_int), ("max_velocity", C.c_float), ("cfl_dt", C.c_float),
+                ("sim_time", C.c_double), ("step_ms", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "solve"}
+        d["solve"] = self.solve.as_dict()
+        return d
+
+
+def library_path():
+    return os.path.join(HERE, _LIB_NAME)
+
+
+def config_path(name):
+    """Path of a config shipped in liquidhairinteration_amd/configs."""
+    return os.path.join(HERE, "configs", name if name.endswith(".json") else name + ".json")
+
+
+_lib = None
+
+
+def load_library():
+    """Load liblh.so (built by __graft_entry__.build() / `make -C liquidhairinteration_amd/csrc`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise LHError(5, "HIP library %s is not built; run __graft_entry__.build()" % path)
+    L = C.CDLL(path)
+    vp, fp, dp, ip = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_int)
+    sigs = {
+        "lh_create": ([C.c_char_p, C.POINTER(vp)], C.c_int),
+        "lh_create_from_string": ([C.c_char_p, C.POINTER(vp)], C.c_int),
+        "lh_destroy": ([vp], None),
+        "lh_last_error": ([vp], C.c_char_p),
+        "lh_grid_info": ([vp, ip, fp, fp], C.c_int),
+        "lh_field_shape": ([vp, C.c_int, ip], C.c_int),
+        "lh_init": ([vp], C.c_int),
+        "lh_pre_step": ([vp, C.c_float], C.c_int),
+        "lh_step": ([vp, C.c_float], C.c_int),
+        "lh_post_step": ([vp, C.c_float, C.POINTER(StepStats)], C.c_int),
+        "lh_get_cfl": ([vp, fp], C.c_int),
+        "lh_run_steps": ([vp, C.c_int, fp, C.POINTER(StepStats)], C.c_int),
+        "lh_compute_solid_phi": ([vp], C.c_int),
+        "lh_clear_grid": ([vp], C.c_int),
+        "lh_compute_liquid_phi": ([vp], C.c_int),
+        "lh_p2g": ([vp], C.c_int),
+        "lh_apply_gravity": ([vp, C.c_float], C.c_int),
+        "lh_solve_pressure": ([vp, C.c_float, C.POINTER(SolveStats)], C.c_int),
+        "lh_velocity_update": ([vp, C.c_float], C.c_int),
+        "lh_extrapolate": ([vp], C.c_int),
+        "lh_g2p": ([vp], C.c_int),
+        "lh_advect": ([vp, C.c_float], C.c_int),
+        "lh_divergence": ([vp, dp, dp, ip], C.c_int),
+        "lh_num_particles": ([vp, C.POINTER(C.c_int64)], C.c_int),
+        "lh_set_particles": ([vp, fp, fp, fp, C.c_int64], C.c_int),
+        "lh_get_particles": ([vp, fp, fp, fp, C.c_int64], C.c_int),
+        "lh_get_field": ([vp, C.c_int, vp, C.c_size_t], C.c_int),
+        "lh_set_field": ([vp, C.c_int, vp, C.c_size_t], C.c_int),
+        "lh_set_time": ([vp, C.c_double], C.c_int),
+        "lh_get_time": ([vp, dp], C.c_int),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _fptr(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class _Handle:
+    def __init__(self, config):
+        self.L = load_library()
+        h = C.c_void_p()
+        if isinstance(config, dict):
+            import json
+            rc = self.L.lh_create_from_string(json.dumps(config).encode(), C.byref(h))
+        else:
+            rc = self.L.lh_create(os.fspath(config).encode(), C.byref(h))
+        self.h = h
+        if rc != LH_OK:
+            msg = self.L.lh_last_error(h).decode() if h.value else "create failed"
+            if h.value:
+                self.L.lh_destroy(h)
+            self.h = C.c_void_p()
+            raise LHError(rc, msg)
+
+    def check(self, rc):
+        if rc != LH_OK:
+            raise LHError(rc, self.L.lh_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if self.h and self.h.value:
+            self.L.lh_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ParticleSystem:
+    """Host view of ParticleSystem<3> (ParticleSystem.h:24-61): copies in and out of HBM."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def NumParticles(self):
+        n = C.c_int64()
+        self._h.check(self._h.L.lh_num_particles(self._h.h, C.byref(n)))
+        return int(n.value)
+
+    def get(self):
+        n = self.NumParticles()
+        pos = np.empty((n, 3), np.float32)
+        vel = np.empty((n, 3), np.float32)
+        Cm = np.empty((n, 9), np.float32)
+        self._h.check(self._h.L.lh_get_particles(self._h.h, _fptr(pos), _fptr(vel), _fptr(Cm), n))
+        return pos, vel, Cm
+
+    def set(self, pos, vel=None, Cm=None):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        vel = None if vel is None else np.ascontiguousarray(vel, np.float32).reshape(n, 3)
+        Cm = None if Cm is None else np.ascontiguousarray(Cm, np.float32).reshape(n, 9)
+        self._h.check(self._h.L.lh_set_particles(self._h.h, _fptr(pos), _fptr(vel), _fptr(Cm), n))
+
+    def GetPositions(self):
+        return self.get()[0]
+
+    def GetVelocities(self):
+        return self.get()[1]
+
+    def GetAffineMatrices(self):
+        """(n, 3, 3) with [i, row, col] (the Eigen column-major matrix of each particle)."""
+        Cm = self.get()[2]
+        return Cm.reshape(-1, 3, 3).transpose(0, 2, 1)
+
+
+class CGSolver:
+    """CGSolver<3> (PressureSolver.h:81-145) bound to a scene's grid."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self.last_stats = None
+
+    def Solve(self, liquidDensity=None, dt=0.01):
+        """Solve(rho, dt): builds RHS/LHS and runs MG-PCG; writes grid.pressure.
+
+        The density comes from the scene config (the reference passes
+        m_scene.GetLiquidDensity(), LiquidModel.cpp:830); liquidDensity is accepted
+        for signature parity and must match it when given.
+        """
+        st = SolveStats()
+        rc = self._h.L.lh_solve_pressure(self._h.h, float(dt), C.byref(st))
+        self.last_stats = st.as_dict()
+        if rc not in (LH_OK, 3):
+            self._h.check(rc)
+        return self.last_stats
+
+
+class LiquidModel:
+    """LiquidModel<3> (LiquidModel.h:11-82) on the GPU."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self._ps = ParticleSystem(handle)
+        self._solver = CGSolver(handle)
+        self.last_step = None
+
+    # LiquidModel.cpp:506-537
+    def Init(self):
+        self._h.check(self._h.L.lh_init(self._h.h))
+
+    def PreStep(self, dt):
+        self._h.check(self._h.L.lh_pre_step(self._h.h, float(dt)))
+
+    def Step(self, dt):
+        rc = self._h.L.lh_step(self._h.h, float(dt))
+        if rc not in (LH_OK, 3):
+            self._h.check(rc)
+
+    def PostStep(self, dt):
+        st = StepStats()
+        self._h.check(self._h.L.lh_post_step(self._h.h, float(dt), C.byref(st)))
+        self.last_step = st.as_dict()
+        return self.last_step
+
+    def GetCFL(self):
+        v = C.c_float()
+        self._h.check(self._h.L.lh_get_cfl(self._h.h, C.byref(v)))
+        return float(v.value)
+
+    def RunSteps(self, nsteps, dt):
+        """nsteps x {PreStep, Step, PostStep, dt = GetCFL()} in one native call; returns (next dt, stats)."""
+        d = C.c_float(float(dt))
+        st = StepStats()
+        self._h.check(self._h.L.lh_run_steps(self._h.h, int(nsteps), C.byref(d), C.byref(st)))
+        return float(d.value), st.as_dict()
+
+    # public stage methods, LiquidModel.h:25-47
+    def ComputeLiquidPhi(self):
+        self._h.check(self._h.L.lh_compute_liquid_phi(self._h.h))
+
+    def ComputeSolidPhi(self):
+        self._h.check(self._h.L.lh_compute_solid_phi(self._h.h))
+
+    def ClearGrid(self):
+        self._h.check(self._h.L.lh_clear_grid(self._h.h))
+
+    def Particle2Grid(self):
+        self._h.check(self._h.L.lh_p2g(self._h.h))
+
+    def Grid2Particle(self):
+        self._h.check(self._h.L.lh_g2p(self._h.h))
+
+    def ApplyGravity(self, dt):
+        self._h.check(self._h.L.lh_apply_gravity(self._h.h, float(dt)))
+
+    def SolvePressure(self, dt):
+        st = self._solver.Solve(None, dt)
+        self._h.check(self._h.L.lh_velocity_update(self._h.h, float(dt)))
+        return st
+
+    def Projection(self, dt):
+        return self.SolvePressure(dt)
+
+    def VelocityUpdate(self, dt):
+        self._h.check(self._h.L.lh_velocity_update(self._h.h, float(dt)))
+
+    def Extrapolate(self):
+        self._h.check(self._h.L.lh_extrapolate(self._h.h))
+
+    def AdvectParticle(self, dt):
+        self._h.check(self._h.L.lh_advect(self._h.h, float(dt)))
+
+    def ComputeDivergence(self):
+        """Mean weighted divergence over liquid cells (LiquidModel.cpp:1065-1083); also returns L-inf and count."""
+        m, l, c = C.c_double(), C.c_double(), C.c_int()
+        self._h.check(self._h.L.lh_divergence(self._h.h, C.byref(m), C.byref(l), C.byref(c)))
+        return m.value, l.value, c.value
+
+    def GetParticleSystem(self):
+        return self._ps
+
+    def GetSolver(self):
+        return self._solver
+
+    # grid access (Grid<3> public members)
+    def field_shape(self, name):
+        f = FIELDS[name] if isinstance(name, str) else int(name)
+        s = (C.c_int * 3)()
+        self._h.check(self._h.L.lh_field_shape(self._h.h, f, s))
+        return tuple(s)
+
+    def get_field(self, name):
+        f = FIELDS[name] if isinstance(name, str) else int(name)
+        shp = self.field_shape(f)
+        a = np.empty(shp, np.uint8 if f == FIELDS["isValid"] else np.float32)
+        self._h.check(self._h.L.lh_get_field(self._h.h, f, a.ctypes.data, a.size))
+        return a
+
+    def set_field(self, name, arr):
+        f = FIELDS[name] if isinstance(name, str) else int(name)
+        shp = self.field_shape(f)
+        a = np.ascontiguousarray(arr, np.uint8 if f == FIELDS["isValid"] else np.float32).reshape(shp)
+        self._h.check(self._h.L.lh_set_field(self._h.h, f, a.ctypes.data, a.size))
+
+    @property
+    def time(self):
+        t = C.c_double()
+        self._h.check(self._h.L.lh_get_time(self._h.h, C.byref(t)))
+        return t.value
+
+    @time.setter
+    def time(self, t):
+        self._h.check(self._h.L.lh_set_time(self._h.h, float(t)))
+
+
+class Scene:
+    """Scene<3>(configFile) (Scene.h:14-48): parses the JSON config and owns the liquid model."""
+
+    def __init__(self, config):
+        self._h = _Handle(config)
+        dims = (C.c_int * 3)()
+        mn = (C.c_float * 3)()
+        dx = (C.c_float * 3)()
+        self._h.check(self._h.L.lh_grid_info(self._h.h, dims, mn, dx))
+        self.grid_dimension = tuple(dims)
+        self.min_coord = np.array(list(mn), np.float32)
+        self.dx = np.array(list(dx), np.float32)
+        self._model = LiquidModel(self._h)
+
+    def GetLiquidModel(self):
+        return self._model
+
+    def GetGridDimension(self):
+        return self.grid_dimension
+
+    def GetMinCoord(self):
+        return self.min_coord
+
+    def close(self):
+        self._h.close()

@@ -1,0 +1,501 @@
+// k_particles.hip -- particle side of the liquid step on gfx950.
+//
+//  * binning: particles are counting-sorted into half-cell bins (2N per axis,
+//    x-major, z fastest) and physically permuted into bin order every step;
+//    inside a bin they are ordered by particle id, so the order is a pure
+//    function of positions and ids (deterministic).  Replaces the spatial hash
+//    of NeighborSearch.h:59-131.
+//  * ComputeLiquidPhi (LiquidModel.cpp:539-560) as a per-cell gather: a cell
+//    takes the min over particles whose cell lies within +-ceil(r/dx) of it.
+//    min is order-free, so the result equals the reference's bit for bit.
+//  * Particle2Grid (LiquidModel.cpp:613-721) as a per-face gather with no float
+//    atomics.  A face of component a is reached by particles with |gridPos -
+//    faceCentre| < 1.5 on every axis, i.e. by exactly 6 half-cells per axis; the
+//    6 half-cells along z are contiguous in bin order, so a face reads 36
+//    contiguous particle runs.  Normalisation, gravity (LiquidModel.cpp:898-910)
+//    and the optional tank acceleration are fused into the epilogue.
+//  * Grid2Particle + AdvectParticle + GetCFL's max|v| (LiquidModel.cpp:723-935,
+//    LiquidModel.h:49-55) fused into one per-particle kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "lh_ctx.h"
+#include "lh_math.h"
+
+namespace lh {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kBlock * kScanItems;
+
+inline unsigned grid_for(size_t n) {
+  size_t b = div_up(n, kBlock);
+  if (b > 65536 * 8) b = 65536 * 8;
+  return (unsigned)(b ? b : 1);
+}
+
+// ------------------------------------------------------------------ scan
+// Exclusive scan of int counts, one 1024-element tile per block.
+__global__ void k_scan_tile(const int* __restrict__ in, int* __restrict__ out, size_t n, int* __restrict__ sums) {
+  __shared__ int s[kBlock];
+  size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int t = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    size_t i = base + k;
+    v[k] = (i < n) ? in[i] : 0;
+    t += v[k];
+  }
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    int a = ((int)threadIdx.x >= off) ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += a;
+    __syncthreads();
+  }
+  int run = s[threadIdx.x] - t;  // exclusive prefix of this thread
+  for (int k = 0; k < kScanItems; ++k) {
+    size_t i = base + k;
+    if (i < n) out[i] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == kBlock - 1) sums[blockIdx.x] = s[kBlock - 1];
+}
+__global__ void k_scan_add(int* __restrict__ out, size_t n, const int* __restrict__ offs) {
+  size_t i = (size_t)blockIdx.x * kScanTile + threadIdx.x;
+  int o = offs[blockIdx.x];
+  for (int k = 0; k < kScanItems; ++k, i += kBlock)
+    if (i < n) out[i] += o;
+}
+__global__ void k_set_total(const int* __restrict__ out, const int* __restrict__ in, size_t n, int* __restrict__ tot) {
+  *tot = (n ? out[n - 1] + in[n - 1] : 0);
+}
+
+// exclusive scan of in[0..n) into out[0..n], out[n] = total. tmp must hold the
+// block sums of every level.
+void scan_excl(Ctx& c, const int* in, int* out, size_t n, int* tmp) {
+  size_t nb = div_up(n, kScanTile);
+  int* sums = tmp;
+  int* sums_scan = tmp + nb;
+  hipLaunchKernelGGL(k_scan_tile, dim3((unsigned)nb), dim3(kBlock), 0, c.s, in, out, n, sums);
+  if (nb > 1) {
+    scan_excl(c, sums, sums_scan, nb, tmp + 2 * nb + 1);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(kBlock), 0, c.s, out, n, sums_scan);
+  }
+  hipLaunchKernelGGL(k_set_total, dim3(1), dim3(1), 0, c.s, out, in, n, out + n);
+}
+
+// ------------------------------------------------------------------ binning
+struct BinP {
+  int H[3];
+  float mn[3], dx[3];
+};
+
+__global__ void k_bin_key(const float* __restrict__ pb, int64_t cap, int64_t P, BinP b, int* __restrict__ key,
+                          int* __restrict__ count, int* __restrict__ err) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
+    int h[3];
+    bool bad = false;
+    for (int a = 0; a < 3; ++a) {
+      float g = (pb[(size_t)a * cap + i] - b.mn[a]) / b.dx[a];
+      float g2 = 2.0f * g;
+      if (!(g2 >= 0.0f) || !(g2 < (float)b.H[a])) {
+        bad = true;
+        g2 = (g2 >= (float)b.H[a]) ? (float)(b.H[a] - 1) : 0.0f;
+      }
+      h[a] = (int)g2;
+    }
+    if (bad) atomicOr(err, 1);
+    int k = (h[0] * b.H[1] + h[1]) * b.H[2] + h[2];
+    key[i] = k;
+    atomicAdd(&count[k], 1);
+  }
+}
+
+__global__ void k_bin_scatter(const int* __restrict__ key, int64_t P, const int* __restrict__ start,
+                              int* __restrict__ count, int* __restrict__ slot_owner) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
+    int k = key[i];
+    int s = start[k] + atomicSub(&count[k], 1) - 1;  // leaves count at zero for the next step
+    slot_owner[s] = (int)i;
+  }
+}
+
+// place particle i at start[key] + (number of particles in its bin with a smaller id)
+__global__ void k_bin_permute(const float* __restrict__ src, float* __restrict__ dst, int64_t cap, int64_t P,
+                              const int* __restrict__ key, const int* __restrict__ start,
+                              const int* __restrict__ slot_owner) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
+    int k = key[i];
+    int s0 = start[k], s1 = start[k + 1];
+    int myid = __float_as_int(src[(size_t)PA_ID * cap + i]);
+    int rank = 0;
+    for (int s = s0; s < s1; ++s) {
+      int j = slot_owner[s];
+      int idj = __float_as_int(src[(size_t)PA_ID * cap + j]);
+      rank += (idj < myid) ? 1 : 0;
+    }
+    size_t d = (size_t)(s0 + rank);
+#pragma unroll
+    for (int a = 0; a < kParticleAttrs; ++a) dst[(size_t)a * cap + d] = src[(size_t)a * cap + i];
+  }
+}
+
+// ------------------------------------------------------------------ liquid phi
+__global__ void k_liquid_phi(float* __restrict__ phi, const float* __restrict__ pb, int64_t cap,
+                             const int* __restrict__ start, GridP g, int H1, int H2, int dW0, int dW1, int dW2,
+                             float radius, float init) {
+  const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+  size_t total = (size_t)N0 * N1 * N2;
+  const float* px = pb + (size_t)PA_X * cap;
+  const float* py = px + cap;
+  const float* pz = py + cap;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    int zi = (int)(i % N2), yi = (int)((i / N2) % N1), xi = (int)(i / ((size_t)N1 * N2));
+    float cpx = ((float)xi + 0.5f) * g.dx[0] + g.mn[0];
+    float cpy = ((float)yi + 0.5f) * g.dx[1] + g.mn[1];
+    float cpz = ((float)zi + 0.5f) * g.dx[2] + g.mn[2];
+    int hx0 = max(0, 2 * (xi - dW0)), hx1 = min(2 * N0 - 1, 2 * (xi + dW0) + 1);
+    int hy0 = max(0, 2 * (yi - dW1)), hy1 = min(2 * N1 - 1, 2 * (yi + dW1) + 1);
+    int hz0 = max(0, 2 * (zi - dW2)), hz1 = min(2 * N2 - 1, 2 * (zi + dW2) + 1);
+    float best = init;
+    for (int hx = hx0; hx <= hx1; ++hx)
+      for (int hy = hy0; hy <= hy1; ++hy) {
+        size_t k0 = ((size_t)hx * H1 + hy) * H2;
+        int s0 = start[k0 + hz0], s1 = start[k0 + hz1 + 1];
+        for (int s = s0; s < s1; ++s) {
+          float ph = norm3(cpx - px[s], cpy - py[s], cpz - pz[s]) - radius;
+          best = (best < ph) ? best : ph;
+        }
+      }
+    phi[i] = best;
+  }
+}
+
+// ------------------------------------------------------------------ P2G
+struct P2GP {
+  GridP g;
+  int a;             // component
+  int H[3];
+  float gdt;         // gravity * dt, added to interior vy faces
+  int tank_axis;
+  float adt;         // tank acceleration * dt on interior faces of tank_axis
+};
+
+__global__ void k_p2g(float* __restrict__ vout, const float* __restrict__ pb, int64_t cap,
+                      const int* __restrict__ start, P2GP q) {
+  const int a = q.a;
+  const int d0 = q.g.N[0] + (a == 0), d1 = q.g.N[1] + (a == 1), d2 = q.g.N[2] + (a == 2);
+  size_t total = (size_t)d0 * d1 * d2;
+  const float* px = pb + (size_t)PA_X * cap;
+  const float* py = px + cap;
+  const float* pz = py + cap;
+  const float* pv = pb + (size_t)(PA_VX + a) * cap;
+  const float* c0 = pb + (size_t)(PA_C + 3 * a) * cap;  // C.col(a)
+  const float* c1 = c0 + cap;
+  const float* c2 = c1 + cap;
+  const float mn0 = q.g.mn[0], mn1 = q.g.mn[1], mn2 = q.g.mn[2];
+  const float dx0 = q.g.dx[0], dx1 = q.g.dx[1], dx2 = q.g.dx[2];
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    int c[3];
+    c[2] = (int)(i % d2);
+    c[1] = (int)((i / d2) % d1);
+    c[0] = (int)(i / ((size_t)d1 * d2));
+    int lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      int l = (k == a) ? 2 * c[k] - 3 : 2 * c[k] - 2;
+      lo[k] = max(l, 0);
+      hi[k] = min(l + 5, q.H[k] - 1);
+    }
+    const float ccx = (a == 0) ? (float)c[0] : (float)c[0] + 0.5f;
+    const float ccy = (a == 1) ? (float)c[1] : (float)c[1] + 0.5f;
+    const float ccz = (a == 2) ? (float)c[2] : (float)c[2] + 0.5f;
+    const float fx = ccx * dx0 + mn0, fy = ccy * dx1 + mn1, fz = ccz * dx2 + mn2;
+    float ws = 0.f, vs = 0.f;
+    for (int hx = lo[0]; hx <= hi[0]; ++hx)
+      for (int hy = lo[1]; hy <= hi[1]; ++hy) {
+        size_t k0 = ((size_t)hx * q.H[1] + hy) * q.H[2];
+        int s0 = start[k0 + lo[2]], s1 = start[k0 + hi[2] + 1];
+        for (int s = s0; s < s1; ++s) {
+          float ppx = px[s], ppy = py[s], ppz = pz[s];
+          float gx = (ppx - mn0) / dx0, gy = (ppy - mn1) / dx1, gz = (ppz - mn2) / dx2;
+          float w = quad_w(gx - ccx) * quad_w(gy - ccy) * quad_w(gz - ccz);
+          if (w == 0.f) continue;
+          float d0v = ppx - fx, d1v = ppy - fy, d2v = ppz - fz;
+          float dot = sum3(c0[s] * d0v, c1[s] * d1v, c2[s] * d2v);
+          ws += w;
+          vs += w * (pv[s] + dot);
+        }
+      }
+    float v = !is_small(ws) ? vs / ws : 0.0f;
+    // interior ranges of ApplyGravity (a == 1) and of the tank acceleration
+    if (a == 1 || a == q.tank_axis) {
+      bool in = true;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        int h = (k == a) ? q.g.N[k] - 1 : q.g.N[k] - 2;
+        if (c[k] < 1 || c[k] > h) in = false;
+      }
+      if (in) {
+        if (a == 1) v += q.gdt;
+        if (a == q.tank_axis) v += q.adt;
+      }
+    }
+    vout[i] = v;
+  }
+}
+
+// ------------------------------------------------------------------ G2P + advect
+struct GP {
+  GridP g;
+  float dt;
+  int dW[3];
+};
+
+__device__ __forceinline__ float solid_phi_in(const float* __restrict__ sp, const GridP& g, float x, float y,
+                                              float z) {
+  float lx = (x - g.mn[0]) / g.dx[0];
+  float ly = (y - g.mn[1]) / g.dx[1];
+  float lz = (z - g.mn[2]) / g.dx[2];
+  int xi, yi, zi;
+  float fx, fy, fz;
+  barycentric(lx, 0, g.N[0] + 1, xi, fx);
+  barycentric(ly, 0, g.N[1] + 1, yi, fy);
+  barycentric(lz, 0, g.N[2] + 1, zi, fz);
+  const int n1 = g.N[1] + 1, n2 = g.N[2] + 1;
+  auto S = [&](int a, int b, int c) { return sp[((size_t)a * n1 + b) * n2 + c]; };
+  return trilerp(S(xi, yi, zi), S(xi + 1, yi, zi), S(xi, yi + 1, zi), S(xi + 1, yi + 1, zi), S(xi, yi, zi + 1),
+                 S(xi + 1, yi, zi + 1), S(xi, yi + 1, zi + 1), S(xi + 1, yi + 1, zi + 1), fx, fy, fz);
+}
+__device__ __forceinline__ bool in_grid(const GridP& g, const float mx[3], float x, float y, float z) {
+  return g.mn[0] <= x && g.mn[1] <= y && g.mn[2] <= z && x <= mx[0] && y <= mx[1] && z <= mx[2];
+}
+// central-difference normalised gradient at a point whose +-eps samples are all
+// inside the grid (Grid.cpp:183-191)
+__device__ void grad_from_samples(float left, float right, float bottom, float top, float front, float back,
+                                  float out[3]) {
+  const float eps = 1e-4f;
+  const float two_eps = (float)(2.0 * (double)eps);
+  float a = (right - left) / two_eps, b = (top - bottom) / two_eps, c = (front - back) / two_eps;
+  float n2 = a * a + (b * b + c * c);
+  if (n2 > 0.f) {
+    float n = sqrtf(n2);
+    a = a / n;
+    b = b / n;
+    c = c / n;
+  }
+  out[0] = a;
+  out[1] = b;
+  out[2] = c;
+}
+__device__ void grad_inner(const float* sp, const GridP& g, float x, float y, float z, float out[3]) {
+  const float eps = 1e-4f;
+  grad_from_samples(solid_phi_in(sp, g, x - eps, y, z), solid_phi_in(sp, g, x + eps, y, z),
+                    solid_phi_in(sp, g, x, y - eps, z), solid_phi_in(sp, g, x, y + eps, z),
+                    solid_phi_in(sp, g, x, y, z + eps), solid_phi_in(sp, g, x, y, z - eps), out);
+}
+// Grid<3>::GetSolidPhi (Grid.cpp:128-155) with the out-of-grid extrapolation
+__device__ float solid_phi_any(const float* sp, const GridP& g, const float mx[3], float x, float y, float z) {
+  if (in_grid(g, mx, x, y, z)) return solid_phi_in(sp, g, x, y, z);
+  float vx = clampf(x, g.mn[0] + g.dx[0], mx[0] - g.dx[0]);
+  float vy = clampf(y, g.mn[1] + g.dx[1], mx[1] - g.dx[1]);
+  float vz = clampf(z, g.mn[2] + g.dx[2], mx[2] - g.dx[2]);
+  float gr[3];
+  grad_inner(sp, g, vx, vy, vz, gr);
+  return solid_phi_in(sp, g, vx, vy, vz) + gr[0] * (x - vx) + gr[1] * (y - vy) + gr[2] * (z - vz);
+}
+// Grid<3>::GetSolidGradient (Grid.cpp:175-192)
+__device__ void solid_grad_any(const float* sp, const GridP& g, const float mx[3], float x, float y, float z,
+                               float out[3]) {
+  if (!in_grid(g, mx, x, y, z)) {
+    float vx = clampf(x, g.mn[0] + g.dx[0], mx[0] - g.dx[0]);
+    float vy = clampf(y, g.mn[1] + g.dx[1], mx[1] - g.dx[1]);
+    float vz = clampf(z, g.mn[2] + g.dx[2], mx[2] - g.dx[2]);
+    grad_inner(sp, g, vx, vy, vz, out);
+    return;
+  }
+  const float eps = 1e-4f;
+  grad_from_samples(solid_phi_any(sp, g, mx, x - eps, y, z), solid_phi_any(sp, g, mx, x + eps, y, z),
+                    solid_phi_any(sp, g, mx, x, y - eps, z), solid_phi_any(sp, g, mx, x, y + eps, z),
+                    solid_phi_any(sp, g, mx, x, y, z + eps), solid_phi_any(sp, g, mx, x, y, z - eps), out);
+}
+
+__global__ void k_g2p_advect(float* __restrict__ pb, int64_t cap, int64_t P, const float* __restrict__ vx,
+                             const float* __restrict__ vy, const float* __restrict__ vz,
+                             const float* __restrict__ sp, GP q, float mx0, float mx1, float mx2, int do_g2p,
+                             int do_advect, unsigned* __restrict__ maxbits) {
+  const float mx[3] = {mx0, mx1, mx2};
+  float local_max = 0.f;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
+    float pos[3] = {pb[(size_t)PA_X * cap + i], pb[(size_t)(PA_X + 1) * cap + i], pb[(size_t)(PA_X + 2) * cap + i]};
+    float vel[3];
+    if (do_g2p) {
+      float g[3];
+      int ci[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        g[k] = (pos[k] - q.g.mn[k]) / q.g.dx[k];
+        ci[k] = (int)g[k];
+      }
+      float Cm[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Cm[k] = 0.f;
+      const float* V[3] = {vx, vy, vz};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        int dims[3] = {q.g.N[0], q.g.N[1], q.g.N[2]};
+        dims[a] += 1;
+        // per axis: up to 3 taps with nonzero weight, in ascending index order
+        int idx[3][3];
+        float w[3][3], gw[3][3];
+        int nt[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          nt[k] = 0;
+          for (int t = -q.dW[k]; t <= q.dW[k]; ++t) {
+            int j = ci[k] + t;
+            if (j < 0 || j > dims[k] - 1) continue;
+            float cc = (k == a) ? (float)j : (float)j + 0.5f;
+            float d = g[k] - cc;
+            float wv = quad_w(d);
+            if (wv == 0.f || nt[k] >= 3) continue;
+            idx[k][nt[k]] = j;
+            w[k][nt[k]] = wv;
+            gw[k][nt[k]] = quad_gw(d);
+            nt[k]++;
+          }
+        }
+        float acc = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
+        const float* Va = V[a];
+        for (int ix = 0; ix < nt[0]; ++ix)
+          for (int iy = 0; iy < nt[1]; ++iy)
+            for (int iz = 0; iz < nt[2]; ++iz) {
+              float wX = w[0][ix], wY = w[1][iy], wZ = w[2][iz];
+              float gX = gw[0][ix], gY = gw[1][iy], gZ = gw[2][iz];
+              float weight = wX * wY * wZ;
+              float g0 = gX * wY * wZ, g1 = wX * gY * wZ, g2 = wX * wY * gZ;
+              float fv = Va[((size_t)idx[0][ix] * dims[1] + idx[1][iy]) * dims[2] + idx[2][iz]];
+              acc += fv * weight;
+              c0 += fv * g0;
+              c1 += fv * g1;
+              c2 += fv * g2;
+            }
+        vel[a] = acc;
+        Cm[3 * a + 0] = c0;
+        Cm[3 * a + 1] = c1;
+        Cm[3 * a + 2] = c2;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) pb[(size_t)(PA_VX + k) * cap + i] = vel[k];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) pb[(size_t)(PA_C + k) * cap + i] = Cm[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) vel[k] = pb[(size_t)(PA_VX + k) * cap + i];
+    }
+    float m3 = max3(fabsf(vel[0]), fabsf(vel[1]), fabsf(vel[2]));
+    local_max = (local_max > m3) ? local_max : m3;
+    if (do_advect) {
+      float p0 = pos[0] + vel[0] * q.dt, p1 = pos[1] + vel[1] * q.dt, p2 = pos[2] + vel[2] * q.dt;
+      float s = solid_phi_any(sp, q.g, mx, p0, p1, p2);
+      if (s < 0.0f) {
+        float gr[3];
+        solid_grad_any(sp, q.g, mx, p0, p1, p2, gr);
+        float sh = s + kEps;
+        p0 -= sh * gr[0];
+        p1 -= sh * gr[1];
+        p2 -= sh * gr[2];
+      }
+      pb[(size_t)PA_X * cap + i] = p0;
+      pb[(size_t)(PA_X + 1) * cap + i] = p1;
+      pb[(size_t)(PA_X + 2) * cap + i] = p2;
+    }
+  }
+  // block max, then one atomic per block (max of non-negative floats is order-free)
+  __shared__ float sm[kBlock];
+  sm[threadIdx.x] = local_max;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(maxbits, __float_as_uint(sm[0]));
+}
+
+}  // namespace
+
+int run_binning(Ctx& c) {
+  c.bins_valid = false;
+  if (c.P == 0) return LH_OK;
+  BinP b;
+  for (int a = 0; a < 3; ++a) {
+    b.H[a] = c.H[a];
+    b.mn[a] = c.g.mn[a];
+    b.dx[a] = c.g.dx[a];
+  }
+  hipMemsetAsync(c.dflag + 1, 0, sizeof(int), c.s);
+  unsigned gp = grid_for((size_t)c.P);
+  hipLaunchKernelGGL(k_bin_key, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, b, c.pkey, c.bin_count,
+                     c.dflag + 1);
+  scan_excl(c, c.bin_count, c.bin_start, c.nbins, c.scan_tmp);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, c.s, c.pkey, c.P, c.bin_start, c.bin_count, c.pslot);
+  hipLaunchKernelGGL(k_bin_permute, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.pbuf[1 - c.cur], c.cap, c.P,
+                     c.pkey, c.bin_start, c.pslot);
+  c.cur = 1 - c.cur;
+  c.bins_valid = true;
+  return LH_OK;
+}
+
+void launch_liquid_phi(Ctx& c) {
+  if (c.P == 0 || !c.bins_valid) return;
+  hipLaunchKernelGGL(k_liquid_phi, dim3(grid_for(c.ncell)), dim3(kBlock), 0, c.s, c.phi, c.pbuf[c.cur], c.cap,
+                     c.bin_start, c.g, c.H[1], c.H[2], c.dW[0], c.dW[1], c.dW[2], c.radius, 3 * c.g.dx[0]);
+}
+
+void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
+  if (c.P == 0 || !c.bins_valid) return;
+  P2GP q;
+  q.g = c.g;
+  for (int k = 0; k < 3; ++k) q.H[k] = c.H[k];
+  q.gdt = 0.f;
+  q.adt = 0.f;
+  q.tank_axis = -1;
+  if (fuse_body_forces) {
+    q.gdt = c.cfg.gravity * dt;
+    if (c.cfg.tank_axis >= 0 && c.cfg.tank_axis <= 2 && c.cfg.tank_amp != 0.f) {
+      double acc = (double)c.cfg.tank_amp * std::sin(2.0 * M_PI * (double)c.cfg.tank_freq * c.time);
+      q.adt = (float)acc * dt;
+      q.tank_axis = c.cfg.tank_axis;
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    q.a = a;
+    P2GP qa = q;
+    if (!fuse_body_forces || a != 1) qa.gdt = 0.f;
+    // gravity applies only to vy; keep the a==1 interior test for it
+    hipLaunchKernelGGL(k_p2g, dim3(grid_for(c.nface[a])), dim3(kBlock), 0, c.s, c.v[a], c.pbuf[c.cur], c.cap,
+                       c.bin_start, qa);
+  }
+}
+
+void launch_g2p_advect(Ctx& c, bool do_g2p, bool do_advect, float dt) {
+  hipMemsetAsync(c.dmaxv, 0, sizeof(unsigned), c.s);
+  if (c.P == 0) return;
+  GP q;
+  q.g = c.g;
+  q.dt = dt;
+  for (int k = 0; k < 3; ++k) q.dW[k] = c.dW[k];
+  unsigned gp = grid_for((size_t)c.P);
+  hipLaunchKernelGGL(k_g2p_advect, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, c.v[0], c.v[1], c.v[2],
+                     c.solid_phi, q, c.cfg.mx[0], c.cfg.mx[1], c.cfg.mx[2], do_g2p ? 1 : 0, do_advect ? 1 : 0,
+                     c.dmaxv);
+}
+
+}  // namespace lh

@@ -1,0 +1,576 @@
+// k_solver.hip -- CGSolver<3> (PressureSolver.cpp:422-934) on gfx950.
+//
+// Operator storage per multigrid level (17 B/cell instead of the reference's
+// 28 B of Adiag + Acoef[6]):
+//   adiag      Adiag exactly as BuildLHS accumulates it (PressureSolver.cpp:544-664)
+//   t[a]       term = W*dt/(rho*dx^2) of the +a face of the cell (0 if W <= 0 or
+//              the face leaves the level's box); the -a face of a cell is the
+//              +a face of its -a neighbour
+//   cls        0: phi > 0 (no row), 1: phi == 0 or NaN, 2: phi < 0
+// The reference's off-diagonal Acoef[k] of row c towards neighbour n equals
+// -term if phi_c <= 0 and phi_n < 0, else 0, so GetAx (PressureSolver.cpp:700-717)
+// is  Adiag*z_c + sum_k [cls_c >= 1 and cls_n == 2] * z_n * (-term_k)
+// in the reference's order (+x, -x, +y, -y, +z, -z).  Adding z_n * 0 in the
+// reference is an exact no-op, so skipped terms leave the result unchanged.
+//
+// Dot products (Array3::Reduce, Array.h:111-122, and the denom / residual sums
+// of Solve) accumulate in FP64 over fixed per-block partials and a fixed-order
+// final sum, so they are deterministic; the totals are rounded to float like
+// the reference's Scalar members m_oldRZ / m_newRZ / residual.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "lh_ctx.h"
+#include "lh_math.h"
+
+namespace lh {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(size_t n) {
+  size_t b = div_up(n, kBlock);
+  if (b > 65536 * 8) b = 65536 * 8;
+  return (unsigned)(b ? b : 1);
+}
+
+struct LvP {
+  int d[3];
+  const float* adiag;
+  const float* t0;
+  const float* t1;
+  const float* t2;
+  const uint8_t* cls;
+};
+
+LvP lvp(const Level& L) {
+  LvP p;
+  for (int k = 0; k < 3; ++k) p.d[k] = L.d[k];
+  p.adiag = L.adiag;
+  p.t0 = L.t[0];
+  p.t1 = L.t[1];
+  p.t2 = L.t[2];
+  p.cls = L.cls;
+  return p;
+}
+
+__device__ __forceinline__ float get_ax(const LvP& L, const float* __restrict__ z, int x, int y, int w, size_t c) {
+  float res = z[c] * L.adiag[c];
+  if (L.cls[c] == 0) return res;
+  const size_t sx = (size_t)L.d[1] * L.d[2], sy = (size_t)L.d[2];
+  if (x + 1 < L.d[0] && L.cls[c + sx] == 2) res += z[c + sx] * (-L.t0[c]);
+  if (x - 1 >= 0 && L.cls[c - sx] == 2) res += z[c - sx] * (-L.t0[c - sx]);
+  if (y + 1 < L.d[1] && L.cls[c + sy] == 2) res += z[c + sy] * (-L.t1[c]);
+  if (y - 1 >= 0 && L.cls[c - sy] == 2) res += z[c - sy] * (-L.t1[c - sy]);
+  if (w + 1 < L.d[2] && L.cls[c + 1] == 2) res += z[c + 1] * (-L.t2[c]);
+  if (w - 1 >= 0 && L.cls[c - 1] == 2) res += z[c - 1] * (-L.t2[c - 1]);
+  return res;
+}
+
+// ---------------------------------------------------------------- assembly
+struct FineP {
+  int N[3];
+  const float* phi;
+  const float* lw0;
+  const float* lw1;
+  const float* lw2;
+};
+
+// GetApproxLiquidPhi, PressureSolver.cpp:476-494
+__device__ __forceinline__ float approx_phi(const FineP& F, int x, int y, int z, int l) {
+  const size_t s1 = (size_t)F.N[2], s0 = (size_t)F.N[1] * F.N[2];
+  if (l == 0) return F.phi[x * s0 + y * s1 + z];
+  int f = 1 << l, h = (1 << (l - 1)) - 1;
+  int bx = f * x + h, by = f * y + h, bz = f * z + h;
+  const float* p = F.phi;
+  auto P = [&](int a, int b, int c) { return p[(size_t)a * s0 + (size_t)b * s1 + c]; };
+  return trilerp(P(bx, by, bz), P(bx + 1, by, bz), P(bx, by + 1, bz), P(bx + 1, by + 1, bz), P(bx, by, bz + 1),
+                 P(bx + 1, by, bz + 1), P(bx, by + 1, bz + 1), P(bx + 1, by + 1, bz + 1), 0.5f, 0.5f, 0.5f);
+}
+// GetApproxWx/Wy/Wz, PressureSolver.cpp:496-542
+__device__ __forceinline__ float approx_w(const FineP& F, int a, int x, int y, int z, int l) {
+  int d1 = F.N[1] + (a == 1), d2 = F.N[2] + (a == 2);
+  const float* w = (a == 0) ? F.lw0 : (a == 1) ? F.lw1 : F.lw2;
+  auto W = [&](int i, int j, int k) { return w[((size_t)i * d1 + j) * d2 + k]; };
+  if (l == 0) return W(x, y, z);
+  int f = 1 << l, h = (1 << (l - 1)) - 1;
+  if (a == 0) {
+    int bx = f * x, by = f * y + h, bz = f * z + h;
+    return bilerp(W(bx, by, bz), W(bx, by + 1, bz), W(bx, by, bz + 1), W(bx, by + 1, bz + 1), 0.5f, 0.5f);
+  } else if (a == 1) {
+    int bx = f * x + h, by = f * y, bz = f * z + h;
+    return bilerp(W(bx, by, bz), W(bx + 1, by, bz), W(bx, by, bz + 1), W(bx + 1, by, bz + 1), 0.5f, 0.5f);
+  }
+  int bx = f * x + h, by = f * y + h, bz = f * z;
+  return bilerp(W(bx, by, bz), W(bx + 1, by, bz), W(bx, by + 1, bz), W(bx + 1, by + 1, bz), 0.5f, 0.5f);
+}
+
+__global__ void k_build_lhs(float* __restrict__ adiag, float* __restrict__ t0, float* __restrict__ t1,
+                            float* __restrict__ t2, uint8_t* __restrict__ cls, FineP F, int l, int d0, int d1,
+                            int d2, float dt, float rho, float dxl0, float dxl1, float dxl2) {
+  size_t total = (size_t)d0 * d1 * d2;
+  const int dims[3] = {d0, d1, d2};
+  const float dxl[3] = {dxl0, dxl1, dxl2};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    int c[3];
+    c[2] = (int)(i % d2);
+    c[1] = (int)((i / d2) % d1);
+    c[0] = (int)(i / ((size_t)d1 * d2));
+    float cphi = approx_phi(F, c[0], c[1], c[2], l);
+    cls[i] = (cphi > 0.0f) ? 0 : ((cphi < 0.0f) ? 2 : 1);
+    // +a face terms (independent of this cell's class: the neighbour's row uses them)
+    float tp[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      tp[a] = 0.f;
+      if (c[a] + 1 < dims[a]) {
+        int f[3] = {c[0], c[1], c[2]};
+        f[a] += 1;
+        float wv = approx_w(F, a, f[0], f[1], f[2], l);
+        if (wv > 0.0f) tp[a] = wv * dt / (rho * dxl[a] * dxl[a]);
+      }
+    }
+    t0[i] = tp[0];
+    t1[i] = tp[1];
+    t2[i] = tp[2];
+    float diag = 0.0f;
+    if (!(cphi > 0.0f)) {
+      // right, left, top, bottom, front, back (PressureSolver.cpp:557-661)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const int a = k >> 1;
+        const int s = (k & 1) ? -1 : 1;
+        if (s > 0 ? !(c[a] + 1 < dims[a]) : !(c[a] - 1 >= 0)) continue;
+        int f[3] = {c[0], c[1], c[2]};
+        if (s > 0) f[a] += 1;
+        float wv = approx_w(F, a, f[0], f[1], f[2], l);
+        if (!(wv > 0.0f)) continue;
+        float term = wv * dt / (rho * dxl[a] * dxl[a]);
+        int nb[3] = {c[0], c[1], c[2]};
+        nb[a] += s;
+        float ph = approx_phi(F, nb[0], nb[1], nb[2], l);
+        if (ph < 0.0f) {
+          diag += term;
+        } else {
+          float frac = max_tenth(fraction_in_region(cphi, ph));
+          diag += term / frac;
+        }
+      }
+    }
+    adiag[i] = diag;
+  }
+}
+
+// BuildRHS, PressureSolver.cpp:682-698
+__global__ void k_build_rhs(float* __restrict__ b, const float* __restrict__ phi, const float* __restrict__ vx,
+                            const float* __restrict__ vy, const float* __restrict__ vz, const float* __restrict__ lwx,
+                            const float* __restrict__ lwy, const float* __restrict__ lwz, GridP g) {
+  const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+  size_t total = (size_t)N0 * N1 * N2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    float r = 0.0f;
+    if (!(phi[i] > 0.0f)) {
+      int zi = (int)(i % N2), yi = (int)((i / N2) % N1), xi = (int)(i / ((size_t)N1 * N2));
+      size_t fx0 = ((size_t)xi * N1 + yi) * N2 + zi, fx1 = fx0 + (size_t)N1 * N2;
+      size_t fy0 = ((size_t)xi * (N1 + 1) + yi) * N2 + zi, fy1 = fy0 + N2;
+      size_t fz0 = ((size_t)xi * N1 + yi) * (N2 + 1) + zi, fz1 = fz0 + 1;
+      r -= lwx[fx1] * vx[fx1] / g.dx[0];
+      r += lwx[fx0] * vx[fx0] / g.dx[0];
+      r -= lwy[fy1] * vy[fy1] / g.dx[1];
+      r += lwy[fy0] * vy[fy0] / g.dx[1];
+      r -= lwz[fz1] * vz[fz1] / g.dx[2];
+      r += lwz[fz0] * vz[fz0] / g.dx[2];
+    }
+    b[i] = r;
+  }
+}
+
+// ---------------------------------------------------------------- V-cycle
+// Smooth, PressureSolver.cpp:821-837.  mode 0: (1. - w) * z + w * term with the
+// reference's double/float promotion; mode 1: z + w * term.
+__global__ void k_smooth(float* __restrict__ z, const float* __restrict__ r, LvP L, int phase, float w, int mode) {
+  const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
+  size_t total = (size_t)d0 * d1 * d2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    int zi = (int)(i % d2), yi = (int)((i / d2) % d1), xi = (int)(i / ((size_t)d1 * d2));
+    if (((xi + yi + zi) & 1) != phase) continue;
+    float ad = L.adiag[i];
+    if (is_small(ad)) continue;
+    float term = (r[i] - get_ax(L, z, xi, yi, zi, i)) / ad;
+    if (mode == 0) {
+      float wt = w * term;
+      z[i] = (float)((1.0 - (double)w) * (double)z[i] + (double)wt);
+    } else {
+      z[i] = z[i] + w * term;
+    }
+  }
+}
+
+// Restrict, PressureSolver.cpp:796-819
+__global__ void k_restrict(float* __restrict__ rc, float* __restrict__ zc, const float* __restrict__ rf,
+                           const float* __restrict__ zf, LvP Lf, int c0, int c1, int c2) {
+  size_t total = (size_t)c0 * c1 * c2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    int zi = (int)(i % c2), yi = (int)((i / c2) % c1), xi = (int)(i / ((size_t)c1 * c2));
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      int x = 2 * xi + ((k >> 2) & 1), y = 2 * yi + ((k >> 1) & 1), w = 2 * zi + (k & 1);
+      size_t f = ((size_t)x * Lf.d[1] + y) * Lf.d[2] + w;
+      float d = rf[f] - get_ax(Lf, zf, x, y, w, f);
+      acc = (float)((double)acc + 0.125 * (double)d);
+    }
+    rc[i] = acc;
+    zc[i] = 0.0f;
+  }
+}
+
+// Prolong, PressureSolver.cpp:774-794
+__global__ void k_prolong(float* __restrict__ zf, const float* __restrict__ zc, int f1, int f2, int c0, int c1,
+                          int c2) {
+  size_t total = (size_t)c0 * c1 * c2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    int zi = (int)(i % c2), yi = (int)((i / c2) % c1), xi = (int)(i / ((size_t)c1 * c2));
+    float v = zc[i];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      int x = 2 * xi + ((k >> 2) & 1), y = 2 * yi + ((k >> 1) & 1), w = 2 * zi + (k & 1);
+      zf[((size_t)x * f1 + y) * f2 + w] += v;
+    }
+  }
+}
+
+// PrecondType::None / Jacobi, PressureSolver.cpp:721-743
+__global__ void k_precond_simple(float* __restrict__ z, const float* __restrict__ r, const float* __restrict__ adiag,
+                                 const uint8_t* __restrict__ cls, size_t n, int jacobi) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    if (jacobi) {
+      float dn = adiag[i];
+      if (is_small(dn)) continue;
+      z[i] = r[i] / dn;
+    } else {
+      if (cls[i] == 0) continue;
+      z[i] = r[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- PCG vectors
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partial) {
+  __shared__ double s[NV][kBlock];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int st = kBlock / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partial[(size_t)k * gridDim.x + blockIdx.x] = s[k][0];
+  }
+}
+
+// r = b on liquid cells (else 0); x = p = z0 = 0 (Init + PressureSolver.cpp:845-853)
+__global__ void k_cg_init(float* __restrict__ r, float* __restrict__ x, float* __restrict__ p, float* __restrict__ z,
+                          const float* __restrict__ b, const uint8_t* __restrict__ cls, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    r[i] = (cls[i] != 0) ? b[i] : 0.0f;
+    x[i] = 0.0f;
+    p[i] = 0.0f;
+    z[i] = 0.0f;
+  }
+}
+
+// p = z on liquid cells; partials of r.z and r.r over all cells (:856-867)
+__global__ void k_cg_first(float* __restrict__ p, const float* __restrict__ z, const float* __restrict__ r,
+                           const uint8_t* __restrict__ cls, size_t n, double* __restrict__ partial) {
+  double v[2] = {0.0, 0.0};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    float zi = z[i], ri = r[i];
+    if (cls[i] != 0) p[i] = zi;
+    v[0] += (double)ri * (double)zi;
+    v[1] += (double)ri * (double)ri;
+  }
+  block_partials<2>(v, partial);
+}
+
+// q = A p and partial p.q over interior liquid cells (:873-882)
+__global__ void k_cg_spmv(float* __restrict__ q, const float* __restrict__ p, LvP L, double* __restrict__ partial) {
+  const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
+  size_t total = (size_t)d0 * d1 * d2;
+  double v[1] = {0.0};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    int zi = (int)(i % d2), yi = (int)((i / d2) % d1), xi = (int)(i / ((size_t)d1 * d2));
+    if (xi < 1 || xi > d0 - 2 || yi < 1 || yi > d1 - 2 || zi < 1 || zi > d2 - 2) continue;
+    if (L.cls[i] == 0) continue;
+    float ap = get_ax(L, p, xi, yi, zi, i);
+    q[i] = ap;
+    v[0] += (double)p[i] * (double)ap;
+  }
+  block_partials<1>(v, partial);
+}
+
+// x += alpha p; r -= alpha q; partial r.r over interior liquid cells (:889-902)
+__global__ void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
+                               const float* __restrict__ q, const uint8_t* __restrict__ cls,
+                               const float* __restrict__ sc, int d0, int d1, int d2, double* __restrict__ partial) {
+  const float alpha = sc[2];
+  size_t total = (size_t)d0 * d1 * d2;
+  double v[1] = {0.0};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    int zi = (int)(i % d2), yi = (int)((i / d2) % d1), xi = (int)(i / ((size_t)d1 * d2));
+    if (xi < 1 || xi > d0 - 2 || yi < 1 || yi > d1 - 2 || zi < 1 || zi > d2 - 2) continue;
+    if (cls[i] == 0) continue;
+    x[i] += alpha * p[i];
+    float rn = r[i] - alpha * q[i];
+    r[i] = rn;
+    v[0] += (double)rn * (double)rn;
+  }
+  block_partials<1>(v, partial);
+}
+
+__global__ void k_cg_dot_rz(const float* __restrict__ r, const float* __restrict__ z, size_t n,
+                            double* __restrict__ partial) {
+  double v[1] = {0.0};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    v[0] += (double)r[i] * (double)z[i];
+  block_partials<1>(v, partial);
+}
+
+// p = z + beta p on liquid cells (:913-921)
+__global__ void k_cg_update_p(float* __restrict__ p, const float* __restrict__ z, const uint8_t* __restrict__ cls,
+                              const float* __restrict__ sc, size_t n) {
+  const float beta = sc[3];
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    if (cls[i] != 0) p[i] = z[i] + beta * p[i];
+}
+
+__device__ double sum_partials_block(const double* __restrict__ part, int n) {
+  __shared__ double s[kBlock];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) a += part[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int k = kBlock / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+    __syncthreads();
+  }
+  double t = s[0];
+  __syncthreads();
+  return t;
+}
+
+// scalar kernels (one block): sc = [oldRZ, newRZ, alpha, beta, residual, denom]
+// sc64 = [rz64, rr64, denom64, res64]
+__global__ void k_sc_first(const double* __restrict__ part, int nb, float* __restrict__ sc, double* __restrict__ sc64) {
+  double rz = sum_partials_block(part, nb);
+  double rr = sum_partials_block(part + nb, nb);
+  if (threadIdx.x == 0) {
+    sc[0] = (float)rz;
+    sc[4] = sqrtf((float)rr);
+    sc64[0] = rz;
+    sc64[1] = rr;
+  }
+}
+__global__ void k_sc_alpha(const double* __restrict__ part, int nb, float* __restrict__ sc, int* __restrict__ flag) {
+  double dn = sum_partials_block(part, nb);
+  if (threadIdx.x == 0) {
+    float denom = (float)dn;
+    float alpha = sc[0] / denom;
+    sc[5] = denom;
+    sc[2] = alpha;
+    if (isnan(alpha) || isinf(alpha)) atomicOr(flag, 1);
+  }
+}
+__global__ void k_sc_residual(const double* __restrict__ part, int nb, float* __restrict__ sc) {
+  double rr = sum_partials_block(part, nb);
+  if (threadIdx.x == 0) sc[4] = sqrtf((float)rr);
+}
+__global__ void k_sc_beta(const double* __restrict__ part, int nb, float* __restrict__ sc, int* __restrict__ flag) {
+  double rz = sum_partials_block(part, nb);
+  if (threadIdx.x == 0) {
+    float newRZ = (float)rz;
+    float beta = newRZ / sc[0];
+    sc[1] = newRZ;
+    sc[3] = beta;
+    sc[0] = newRZ;  // m_oldRZ = m_newRZ (:922)
+    if (isnan(beta) || isinf(beta)) atomicOr(flag, 2);
+  }
+}
+
+}  // namespace
+
+void launch_build_rhs(Ctx& c) {
+  hipLaunchKernelGGL(k_build_rhs, dim3(grid_for(c.ncell)), dim3(kBlock), 0, c.s, c.rhs, c.phi, c.v[0], c.v[1], c.v[2],
+                     c.lw[0], c.lw[1], c.lw[2], c.g);
+}
+
+void launch_build_lhs(Ctx& c, float dt) {
+  FineP F;
+  for (int k = 0; k < 3; ++k) F.N[k] = c.g.N[k];
+  F.phi = c.phi;
+  F.lw0 = c.lw[0];
+  F.lw1 = c.lw[1];
+  F.lw2 = c.lw[2];
+  for (size_t l = 0; l < c.lv.size(); ++l) {
+    Level& L = c.lv[l];
+    float f = (float)(1 << l);
+    hipLaunchKernelGGL(k_build_lhs, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.adiag, L.t[0], L.t[1], L.t[2], L.cls,
+                       F, (int)l, L.d[0], L.d[1], L.d[2], dt, c.cfg.density, f * c.g.dx[0], f * c.g.dx[1],
+                       f * c.g.dx[2]);
+  }
+}
+
+namespace {
+
+void smooth(Ctx& c, int l, int phase, int mode) {
+  Level& L = c.lv[l];
+  hipLaunchKernelGGL(k_smooth, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), phase, c.cfg.omega, mode);
+}
+
+// ApplyPreconditioner, PressureSolver.cpp:719-772 (symmetric mode: z[0] = 0 first,
+// post-smoothing and the second half of the coarsest sweeps in reverse colour order)
+void apply_precond(Ctx& c) {
+  Level& L0 = c.lv[0];
+  if (c.cfg.precond != LH_PRECOND_MULTIGRID) {
+    hipLaunchKernelGGL(k_precond_simple, dim3(grid_for(L0.n)), dim3(kBlock), 0, c.s, L0.z, L0.r, L0.adiag, L0.cls,
+                       L0.n, c.cfg.precond == LH_PRECOND_JACOBI ? 1 : 0);
+    return;
+  }
+  const int L = (int)c.lv.size();
+  const int sym = (c.cfg.smoother == LH_SMOOTHER_SYMMETRIC);
+  const int mode = sym ? 1 : 0;
+  if (sym) launch_fill_f32(c, L0.z, L0.n, 0.f);
+  for (int l = 1; l < L; ++l) {
+    for (int it = 0; it < c.cfg.num_pre; ++it) {
+      smooth(c, l - 1, 0, mode);
+      smooth(c, l - 1, 1, mode);
+    }
+    Level& Lc = c.lv[l];
+    Level& Lf = c.lv[l - 1];
+    hipLaunchKernelGGL(k_restrict, dim3(grid_for(Lc.n)), dim3(kBlock), 0, c.s, Lc.r, Lc.z, Lf.r, Lf.z, lvp(Lf),
+                       Lc.d[0], Lc.d[1], Lc.d[2]);
+  }
+  for (int it = 0; it < c.cfg.num_final; ++it) {
+    smooth(c, L - 1, 0, mode);
+    smooth(c, L - 1, 1, mode);
+  }
+  if (sym)
+    for (int it = 0; it < c.cfg.num_final; ++it) {
+      smooth(c, L - 1, 1, mode);
+      smooth(c, L - 1, 0, mode);
+    }
+  for (int l = L - 2; l >= 0; --l) {
+    Level& Lc = c.lv[l + 1];
+    Level& Lf = c.lv[l];
+    hipLaunchKernelGGL(k_prolong, dim3(grid_for(Lc.n)), dim3(kBlock), 0, c.s, Lf.z, Lc.z, Lf.d[1], Lf.d[2], Lc.d[0],
+                       Lc.d[1], Lc.d[2]);
+    for (int it = 0; it < c.cfg.num_post; ++it) {
+      if (sym) {
+        smooth(c, l, 1, mode);
+        smooth(c, l, 0, mode);
+      } else {
+        smooth(c, l, 0, mode);
+        smooth(c, l, 1, mode);
+      }
+    }
+  }
+}
+
+}  // namespace
+
+// CGSolver<3>::Solve, PressureSolver.cpp:839-934: Init, BuildRHS, BuildLHS, PCG
+// from x = 0, pressure = x.
+int run_solve(Ctx& c, float dt, lh_solve_stats* st) {
+  lh_solve_stats S;
+  std::memset(&S, 0, sizeof(S));
+  hipEventRecord(c.ev0, c.s);
+  // Init (:666-680): r, z of every level and x, p are zeroed
+  for (auto& L : c.lv) {
+    launch_fill_f32(c, L.r, L.n, 0.f);
+    launch_fill_f32(c, L.z, L.n, 0.f);
+  }
+  launch_build_rhs(c);
+  launch_build_lhs(c, dt);
+  Level& L0 = c.lv[0];
+  const size_t n = L0.n;
+  const int nb = c.npartial;
+  hipMemsetAsync(c.dflag + 2, 0, sizeof(int), c.s);
+  hipLaunchKernelGGL(k_cg_init, dim3(grid_for(n)), dim3(kBlock), 0, c.s, L0.r, c.x, c.p, L0.z, c.rhs, L0.cls, n);
+  apply_precond(c);
+  hipLaunchKernelGGL(k_cg_first, dim3(nb), dim3(kBlock), 0, c.s, c.p, L0.z, L0.r, L0.cls, n, c.partial);
+  hipLaunchKernelGGL(k_sc_first, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dsc64);
+  hipMemcpyAsync(c.h_pin, c.dsc, 8 * sizeof(float), hipMemcpyDeviceToHost, c.s);
+  if (!hip_ok(c, hipStreamSynchronize(c.s), "solve init")) return LH_ERR_DEVICE;
+  float hsc[8];
+  std::memcpy(hsc, c.h_pin, sizeof(hsc));
+  float residual = hsc[4];
+  const float res0 = residual;
+  const double tol = (double)c.cfg.tol;
+  auto done = [&](float res) {
+    if (c.cfg.tol_mode == LH_TOL_ABSOLUTE) return res < c.cfg.tol;
+    return (double)res <= tol * (double)res0;
+  };
+  LvP P0 = lvp(L0);
+  int it = 0;
+  bool conv = false, nan = false;
+  for (it = 0; it < c.cfg.max_iter; ++it) {
+    if (done(residual)) {
+      conv = true;
+      break;
+    }
+    hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.q, c.p, P0, c.partial);
+    hipLaunchKernelGGL(k_sc_alpha, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dflag + 2);
+    hipLaunchKernelGGL(k_cg_update_xr, dim3(nb), dim3(kBlock), 0, c.s, c.x, L0.r, c.p, c.q, L0.cls, c.dsc, L0.d[0],
+                       L0.d[1], L0.d[2], c.partial);
+    hipLaunchKernelGGL(k_sc_residual, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc);
+    apply_precond(c);
+    hipLaunchKernelGGL(k_cg_dot_rz, dim3(nb), dim3(kBlock), 0, c.s, L0.r, L0.z, n, c.partial);
+    hipLaunchKernelGGL(k_sc_beta, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dflag + 2);
+    hipLaunchKernelGGL(k_cg_update_p, dim3(grid_for(n)), dim3(kBlock), 0, c.s, c.p, L0.z, L0.cls, c.dsc, n);
+    hipMemcpyAsync(c.h_pin, c.dsc, 8 * sizeof(float), hipMemcpyDeviceToHost, c.s);
+    hipMemcpyAsync(c.h_flag, c.dflag + 2, sizeof(int), hipMemcpyDeviceToHost, c.s);
+    if (!hip_ok(c, hipStreamSynchronize(c.s), "solve iteration")) return LH_ERR_DEVICE;
+    std::memcpy(hsc, c.h_pin, sizeof(hsc));
+    residual = hsc[4];
+    if (*c.h_flag) {
+      nan = true;
+      ++it;
+      break;
+    }
+  }
+  if (!conv && !nan && it >= c.cfg.max_iter) conv = done(residual);
+  // m_grid.pressure.CopyFrom(m_x), :933
+  hipMemcpyAsync(c.pressure, c.x, n * sizeof(float), hipMemcpyDeviceToDevice, c.s);
+  hipEventRecord(c.ev1, c.s);
+  if (!hip_ok(c, hipEventSynchronize(c.ev1), "solve")) return LH_ERR_DEVICE;
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c.ev0, c.ev1);
+  S.iterations = it;
+  S.converged = conv ? 1 : 0;
+  S.initial_residual = res0;
+  S.final_residual = residual;
+  S.rhs_norm = res0;
+  S.solve_ms = ms;
+  if (st) *st = S;
+  if (nan) {
+    c.err = "PCG produced a NaN/Inf alpha or beta";
+    return LH_ERR_NAN;
+  }
+  if (!conv) {
+    c.err = "PCG did not reach the tolerance within maxIterations";
+    return LH_ERR_NOT_CONVERGED;
+  }
+  return LH_OK;
+}
+
+}  // namespace lh

@@ -1,0 +1,851 @@
+// lh_api.hip -- the C ABI of include/lh_api.h: scene construction from the
+// reference's JSON config (Scene.cpp:13-20, SceneParameter.h:9-152), device
+// allocation (Grid.cpp:86-98, PressureSolver.cpp:422-447), initial particles
+// (LiquidModel.cpp:506-512, ParticleSystem.cpp:44-64) and the step driver
+// (LiquidModel.cpp:514-537, main.cpp:47-53).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include "lh_ctx.h"
+#include "lh_json.h"
+#include "lh_math.h"
+
+using lh::Ctx;
+
+struct lh_ctx {
+  Ctx c;
+};
+
+namespace lh {
+namespace {
+
+// ------------------------------------------------------------------ config
+void parse_vec3(const Json& j, float out[3]) {
+  for (int a = 0; a < 3; ++a) out[a] = j.at((size_t)a).as_float();
+}
+
+void emit_sdf(const Json& n, std::vector<float>& prog) {
+  const std::string& t = n.at("type").as_string();
+  if (t == "Rect" || t == "Sphere") {
+    float c[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+    if (n.has("center")) parse_vec3(n.at("center"), c);
+    if (n.has("halfExtent")) parse_vec3(n.at("halfExtent"), h);
+    float rad = n.has("radius") ? n.at("radius").as_float() : 0.f;
+    float ins = (n.has("useInside") && n.at("useInside").as_bool()) ? 1.f : 0.f;
+    float node[12] = {t == "Rect" ? 0.f : 1.f, c[0], c[1], c[2], h[0], h[1], h[2], rad, ins, 0, 0, 0};
+    prog.insert(prog.end(), node, node + 12);
+    return;
+  }
+  float kind;
+  if (t == "Union") kind = 10;
+  else if (t == "Intersection") kind = 11;
+  else if (t == "Subtract" || t == "Substract") kind = 12;
+  else if (t == "None") kind = 13;
+  else throw std::runtime_error("Failed to recognize OperatorType");  // Boundary.h:52-54
+  if (kind == 13) {
+    emit_sdf(n.at("child"), prog);
+  } else {
+    emit_sdf(n.at("left"), prog);
+    emit_sdf(n.at("right"), prog);
+  }
+  float node[12] = {kind, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  prog.insert(prog.end(), node, node + 12);
+}
+
+Config parse_config(const std::string& text) {
+  Json j = Json::parse(text);
+  Config c;
+  // SolverParameter (SceneParameter.h:20-44)
+  const Json& s = j.at("SolverParameter");
+  c.max_iter = s.at("maxIterations").as_int();
+  c.tol = s.at("tolerance").as_float();
+  c.num_level = s.at("numLevel").as_int();
+  c.num_pre = s.at("numPreSmooth").as_int();
+  c.num_post = s.at("numPostSmooth").as_int();
+  c.num_final = s.at("numFinalSmooth").as_int();
+  c.omega = s.at("smoothingFactor").as_float();
+  const std::string& pt = s.at("preconditionerType").as_string();
+  if (pt == "None") c.precond = LH_PRECOND_NONE;
+  else if (pt == "Jacobi") c.precond = LH_PRECOND_JACOBI;
+  else if (pt == "Multigrid") c.precond = LH_PRECOND_MULTIGRID;
+  else throw std::runtime_error("Unknown preconditioner type");
+  std::string tm = s.get("toleranceMode", "absolute");
+  if (tm == "absolute") c.tol_mode = LH_TOL_ABSOLUTE;
+  else if (tm == "relative") c.tol_mode = LH_TOL_RELATIVE;
+  else throw std::runtime_error("Unknown toleranceMode " + tm);
+  std::string sm = s.get("smoother", "reference");
+  if (sm == "reference") c.smoother = LH_SMOOTHER_REFERENCE;
+  else if (sm == "symmetric") c.smoother = LH_SMOOTHER_SYMMETRIC;
+  else throw std::runtime_error("Unknown smoother " + sm);
+  // LiquidParameter (SceneParameter.h:55-63)
+  const Json& lq = j.at("LiquidParameter");
+  c.density = lq.at("density").as_float();
+  c.viscosity = lq.at("viscosity").as_float();
+  c.num_particles = lq.at("numParticles").as_int();
+  c.surface_tension = lq.at("surfaceTension").as_float();
+  c.correct_cycle = lq.at("correctCycle").as_int();
+  // SceneParameter (SceneParameter.h:139-151)
+  c.cfg_dt = j.at("dt").as_float();
+  c.gravity = j.at("gravity").as_float();
+  const Json& gd = j.at("gridDimension");
+  const Json& mn = j.at("minCoord");
+  const Json& mx = j.at("maxCoord");
+  for (int a = 0; a < 3; ++a) {
+    c.N[a] = gd.at((size_t)a).as_int();
+    c.mn[a] = mn.at((size_t)a).as_float();
+    c.mx[a] = mx.at((size_t)a).as_float();
+  }
+  // build extensions (all optional; defaults reproduce the reference)
+  if (j.has("Boundary")) emit_sdf(j.at("Boundary"), c.sdf);
+  else {
+    // Scene.cpp:18: OperatorBoundary(None) over RectBoundary(0, 0.9, useInside)
+    float rect[12] = {0, 0, 0, 0, 0.9f, 0.9f, 0.9f, 0, 1.f, 0, 0, 0};
+    float none[12] = {13, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    c.sdf.assign(rect, rect + 12);
+    c.sdf.insert(c.sdf.end(), none, none + 12);
+  }
+  if (c.sdf.size() / 12 > (size_t)kMaxSdfNodes) throw std::runtime_error("Boundary tree too large");
+  if (j.has("FluidBlocks")) {
+    const Json& bl = j.at("FluidBlocks");
+    for (size_t b = 0; b < bl.size(); ++b) {
+      const Json& B = bl.at(b);
+      float lo[3], hi[3], v[3] = {0, 0, 0};
+      parse_vec3(B.at("min"), lo);
+      parse_vec3(B.at("max"), hi);
+      if (B.has("velocity")) parse_vec3(B.at("velocity"), v);
+      float sig = B.has("velocityNoise") ? B.at("velocityNoise").as_float() : 0.f;
+      float rec[10] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], v[0], v[1], v[2], sig};
+      c.blocks.insert(c.blocks.end(), rec, rec + 10);
+    }
+  }
+  c.k_ppc = (int)j.get("particlesPerCellAxis", 2.0);
+  c.seed = (uint64_t)j.get("seed", 0.0);
+  if (j.has("TankAcceleration")) {
+    const Json& t = j.at("TankAcceleration");
+    c.tank_amp = (float)t.get("amplitude", 0.0);
+    c.tank_freq = (float)t.get("frequency", 1.0);
+    c.tank_axis = (int)t.get("axis", 0.0);
+  }
+  std::string prec = j.get("precision", "fp32");
+  if (prec == "refined") c.refined = 1;
+  else if (prec != "fp32") throw std::runtime_error("Unknown precision " + prec);
+  c.refined_tol = j.get("refinedTolerance", 1e-9);
+  c.first_dt = (float)j.get("firstDt", 0.01);
+  c.steps = (int)j.get("steps", 0.0);
+  for (int a = 0; a < 3; ++a)
+    if (c.N[a] < 3) throw std::runtime_error("gridDimension must be >= 3 on every axis");
+  return c;
+}
+
+// ------------------------------------------------------------------ memory
+template <class T>
+bool dalloc(Ctx& c, T*& p, size_t n) {
+  p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
+  if (e != hipSuccess) {
+    c.err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
+    p = nullptr;
+    return false;
+  }
+  return true;
+}
+
+int alloc_ctx(Ctx& c) {
+  GridP& g = c.g;
+  for (int a = 0; a < 3; ++a) {
+    g.N[a] = c.cfg.N[a];
+    g.mn[a] = c.cfg.mn[a];
+    float interval = c.cfg.mx[a] - c.cfg.mn[a];
+    g.dx[a] = (float)(interval / (float)c.cfg.N[a]);  // Grid.cpp:94-97
+  }
+  // ParticleSystem(numParticles, dx.x): m_defaultRadius = 2 * dx.x
+  c.radius = 2.0f * g.dx[0];
+  for (int a = 0; a < 3; ++a) {
+    c.dW[a] = (int)std::ceil(c.radius / g.dx[a]);
+    if (c.dW[a] < 2) {
+      c.err = "cells longer than dx.x on some axis are not supported (ceil(2dx.x/dx_a) < 2)";
+      return LH_ERR_CONFIG;
+    }
+  }
+  const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+  c.ncell = (size_t)N0 * N1 * N2;
+  c.nface[0] = (size_t)(N0 + 1) * N1 * N2;
+  c.nface[1] = (size_t)N0 * (N1 + 1) * N2;
+  c.nface[2] = (size_t)N0 * N1 * (N2 + 1);
+  c.nnode = (size_t)(N0 + 1) * (N1 + 1) * (N2 + 1);
+  size_t fmax = std::max(c.nface[0], std::max(c.nface[1], c.nface[2]));
+  bool ok = true;
+  for (int a = 0; a < 3; ++a) {
+    ok = ok && dalloc(c, c.v[a], c.nface[a]) && dalloc(c, c.tv[a], c.nface[a]) && dalloc(c, c.lw[a], c.nface[a]);
+  }
+  ok = ok && dalloc(c, c.pressure, c.ncell) && dalloc(c, c.phi, c.ncell) && dalloc(c, c.rhs, c.ncell) &&
+       dalloc(c, c.solid_phi, c.nnode) && dalloc(c, c.is_valid, c.nnode) && dalloc(c, c.xstate, fmax);
+  c.nsdf = (int)(c.cfg.sdf.size() / 12);
+  ok = ok && dalloc(c, c.d_sdf, c.cfg.sdf.size());
+  for (int a = 0; a < 3; ++a) c.H[a] = 2 * c.cfg.N[a];
+  c.nbins = (size_t)c.H[0] * c.H[1] * c.H[2];
+  if (c.nbins >= (size_t)0x7fffffff) {
+    c.err = "grid too large for 32-bit bin offsets";
+    return LH_ERR_CONFIG;
+  }
+  ok = ok && dalloc(c, c.bin_count, c.nbins) && dalloc(c, c.bin_start, c.nbins + 1) &&
+       dalloc(c, c.scan_tmp, 3 * div_up(c.nbins, 1024) + 64);
+  // multigrid levels
+  int L = std::max(1, c.cfg.num_level);
+  if (c.cfg.precond != LH_PRECOND_MULTIGRID) L = 1;
+  c.lv.resize(L);
+  for (int l = 0; l < L; ++l) {
+    Level& q = c.lv[l];
+    for (int a = 0; a < 3; ++a) q.d[a] = c.cfg.N[a] / (1 << l);
+    if (q.d[0] < 1 || q.d[1] < 1 || q.d[2] < 1) {
+      c.err = "numLevel too deep for gridDimension";
+      return LH_ERR_CONFIG;
+    }
+    q.n = (size_t)q.d[0] * q.d[1] * q.d[2];
+    ok = ok && dalloc(c, q.adiag, q.n) && dalloc(c, q.t[0], q.n) && dalloc(c, q.t[1], q.n) &&
+         dalloc(c, q.t[2], q.n) && dalloc(c, q.cls, q.n) && dalloc(c, q.r, q.n) && dalloc(c, q.z, q.n);
+  }
+  c.npartial = 1024;
+  ok = ok && dalloc(c, c.p, c.ncell) && dalloc(c, c.x, c.ncell) && dalloc(c, c.q, c.ncell) &&
+       dalloc(c, c.partial, 4 * (size_t)c.npartial) && dalloc(c, c.dsc, 16) && dalloc(c, c.dsc64, 16) &&
+       dalloc(c, c.dmaxv, 4) && dalloc(c, c.dflag, 16);
+  if (!ok) return LH_ERR_OOM;
+  if (hipHostMalloc((void**)&c.h_pin, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c.h_flag, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    c.err = "hipHostMalloc failed";
+    return LH_ERR_OOM;
+  }
+  hipMemcpyAsync(c.d_sdf, c.cfg.sdf.data(), c.cfg.sdf.size() * sizeof(float), hipMemcpyHostToDevice, c.s);
+  hipMemsetAsync(c.bin_count, 0, c.nbins * sizeof(int), c.s);
+  hipMemsetAsync(c.is_valid, 0, c.nnode, c.s);
+  hipMemsetAsync(c.dflag, 0, 16 * sizeof(int), c.s);
+  launch_fill_f32(c, c.solid_phi, c.nnode, 0.f);
+  for (int a = 0; a < 3; ++a) {
+    launch_fill_f32(c, c.lw[a], c.nface[a], 0.f);
+    launch_fill_f32(c, c.v[a], c.nface[a], 0.f);
+    launch_fill_f32(c, c.tv[a], c.nface[a], 0.f);
+  }
+  launch_fill_f32(c, c.pressure, c.ncell, 0.f);
+  launch_fill_f32(c, c.phi, c.ncell, 3 * g.dx[0]);
+  launch_fill_f32(c, c.rhs, c.ncell, 0.f);
+  for (auto& q : c.lv) {
+    launch_fill_f32(c, q.adiag, q.n, 0.f);
+    for (int a = 0; a < 3; ++a) launch_fill_f32(c, q.t[a], q.n, 0.f);
+    launch_fill_u8(c, q.cls, q.n, 0);
+    launch_fill_f32(c, q.r, q.n, 0.f);
+    launch_fill_f32(c, q.z, q.n, 0.f);
+  }
+  if (!hip_ok(c, hipStreamSynchronize(c.s), "init")) return LH_ERR_DEVICE;
+  return LH_OK;
+}
+
+void free_ctx(Ctx& c) {
+  auto F = [](void* p) {
+    if (p) hipFree(p);
+  };
+  for (int a = 0; a < 3; ++a) {
+    F(c.v[a]);
+    F(c.tv[a]);
+    F(c.lw[a]);
+  }
+  F(c.pressure); F(c.phi); F(c.rhs); F(c.solid_phi); F(c.is_valid); F(c.xstate); F(c.d_sdf);
+  F(c.bin_count); F(c.bin_start); F(c.scan_tmp); F(c.pkey); F(c.pslot);
+  F(c.pbuf[0]); F(c.pbuf[1]);
+  for (auto& q : c.lv) {
+    F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z);
+  }
+  F(c.p); F(c.x); F(c.q); F(c.partial); F(c.dsc); F(c.dsc64); F(c.dmaxv); F(c.dflag);
+  if (c.h_pin) hipHostFree(c.h_pin);
+  if (c.h_flag) hipHostFree(c.h_flag);
+  if (c.ev0) hipEventDestroy(c.ev0);
+  if (c.ev1) hipEventDestroy(c.ev1);
+  if (c.s) hipStreamDestroy(c.s);
+}
+
+int ensure_particle_capacity(Ctx& c, int64_t n) {
+  if (n <= c.cap && c.pbuf[0]) return LH_OK;
+  if (c.pbuf[0]) hipFree(c.pbuf[0]);
+  if (c.pbuf[1]) hipFree(c.pbuf[1]);
+  if (c.pkey) hipFree(c.pkey);
+  if (c.pslot) hipFree(c.pslot);
+  c.pbuf[0] = c.pbuf[1] = nullptr;
+  c.pkey = c.pslot = nullptr;
+  int64_t cap = std::max<int64_t>(n, 1);
+  bool ok = dalloc(c, c.pbuf[0], (size_t)kParticleAttrs * cap) && dalloc(c, c.pbuf[1], (size_t)kParticleAttrs * cap) &&
+            dalloc(c, c.pkey, cap) && dalloc(c, c.pslot, cap);
+  if (!ok) {
+    c.cap = 0;
+    return LH_ERR_OOM;
+  }
+  c.cap = cap;
+  c.cur = 0;
+  return LH_OK;
+}
+
+// upload particles in AoS (pos[3n], vel[3n], C[9n]) order; ids 0..n-1
+int upload_particles(Ctx& c, const float* pos, const float* vel, const float* C, int64_t n) {
+  int rc = ensure_particle_capacity(c, n);
+  if (rc) return rc;
+  c.P = n;
+  c.bins_valid = false;
+  if (n == 0) return LH_OK;
+  std::vector<float> soa((size_t)kParticleAttrs * n);
+  for (int64_t i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      soa[(size_t)(PA_X + a) * n + i] = pos[3 * i + a];
+      soa[(size_t)(PA_VX + a) * n + i] = vel ? vel[3 * i + a] : 0.f;
+    }
+    for (int k = 0; k < 9; ++k) soa[(size_t)(PA_C + k) * n + i] = C ? C[9 * i + k] : 0.f;
+    int id = (int)i;
+    std::memcpy(&soa[(size_t)PA_ID * n + i], &id, 4);
+  }
+  for (int k = 0; k < kParticleAttrs; ++k)
+    hipMemcpyAsync(c.pbuf[c.cur] + (size_t)k * c.cap, soa.data() + (size_t)k * n, n * sizeof(float),
+                   hipMemcpyHostToDevice, c.s);
+  if (!hip_ok(c, hipStreamSynchronize(c.s), "upload particles")) return LH_ERR_DEVICE;
+  return LH_OK;
+}
+
+// ParticleSystem<3>::SpawnParticles, ParticleSystem.cpp:44-64
+void spawn_lattice(Ctx& c, std::vector<float>& pos) {
+  const float* mn = c.cfg.mn;
+  const float* mx = c.cfg.mx;
+  float lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    float off = 0.3f * (mx[a] - mn[a]);  // Scalar(0.3) * (max - min), LiquidModel.cpp:509
+    lo[a] = mn[a] + off;
+    hi[a] = mx[a] - off;
+  }
+  float interval = (float)(0.5 * (double)c.radius);
+  int d[3];
+  for (int a = 0; a < 3; ++a) d[a] = (int)((hi[a] - lo[a]) / interval);
+  int64_t total = (int64_t)d[0] * d[1] * d[2];
+  int64_t n = std::min<int64_t>(c.cfg.num_particles, total);
+  pos.assign(3 * n, 0.f);
+  for (int xi = 0; xi < d[0]; ++xi)
+    for (int yi = 0; yi < d[1]; ++yi)
+      for (int zi = 0; zi < d[2]; ++zi) {
+        int64_t idx = (int64_t)xi * d[1] * d[2] + (int64_t)yi * d[2] + zi;
+        if (idx >= n) break;
+        pos[3 * idx] = (float)xi * interval + lo[0];
+        pos[3 * idx + 1] = (float)yi * interval + lo[1];
+        pos[3 * idx + 2] = (float)zi * interval + lo[2];
+      }
+}
+
+// Jittered block seeding (build extension): k^3 particles per cell whose centre
+// lies inside the block, one per sub-cell at min + ((k*i + s) + u) * (dx/k),
+// u from a SplitMix64 hash of (seed, block, cell, sub-cell, axis); velocity
+// v0 + sigma * N(0,1) (Box-Muller in FP64); particles inside solids dropped.
+inline uint64_t splitmix(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline uint64_t rng_u64(uint64_t seed, uint64_t block, uint64_t cell, uint64_t sub, uint64_t stream) {
+  uint64_t h = splitmix(seed);
+  h = splitmix(h ^ block);
+  h = splitmix(h ^ cell);
+  return splitmix(h ^ (sub * 16 + stream));
+}
+void seed_blocks(Ctx& c, std::vector<float>& pos, std::vector<float>& vel) {
+  const int k = std::max(1, c.cfg.k_ppc);
+  const GridP& g = c.g;
+  const float* prog = c.cfg.sdf.data();
+  const int nprog = (int)(c.cfg.sdf.size() / 12);
+  size_t nb = c.cfg.blocks.size() / 10;
+  for (size_t b = 0; b < nb; ++b) {
+    const float* B = &c.cfg.blocks[10 * b];
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = g.N[a];
+      hi[a] = -1;
+      for (int i = 0; i < g.N[a]; ++i) {
+        float cc = ((float)i + 0.5f) * g.dx[a] + g.mn[a];
+        if (cc >= B[a] && cc <= B[3 + a]) {
+          lo[a] = std::min(lo[a], i);
+          hi[a] = i;
+        }
+      }
+    }
+    for (int xi = lo[0]; xi <= hi[0]; ++xi)
+      for (int yi = lo[1]; yi <= hi[1]; ++yi)
+        for (int zi = lo[2]; zi <= hi[2]; ++zi) {
+          uint64_t cell = ((uint64_t)xi * g.N[1] + yi) * g.N[2] + zi;
+          for (int s = 0; s < k * k * k; ++s) {
+            int ss[3] = {s / (k * k), (s / k) % k, s % k};
+            int cc[3] = {xi, yi, zi};
+            float p[3];
+            for (int a = 0; a < 3; ++a) {
+              float u = (float)(rng_u64(c.cfg.seed, b, cell, s, a) >> 40) * (1.0f / 16777216.0f);
+              p[a] = g.mn[a] + ((float)(k * cc[a] + ss[a]) + u) * (g.dx[a] / (float)k);
+            }
+            if (!(sdf_eval(prog, nprog, p[0], p[1], p[2]) > 0.0f)) continue;
+            float v[3];
+            for (int a = 0; a < 3; ++a) {
+              v[a] = B[6 + a];
+              if (B[9] > 0.f) {
+                uint64_t h1 = rng_u64(c.cfg.seed, b, cell, s, 3 + 2 * a);
+                uint64_t h2 = rng_u64(c.cfg.seed, b, cell, s, 4 + 2 * a);
+                double u1 = ((double)(h1 >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+                double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);
+                double nrm = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+                v[a] = (float)((double)B[6 + a] + (double)B[9] * nrm);
+              }
+            }
+            for (int a = 0; a < 3; ++a) {
+              pos.push_back(p[a]);
+              vel.push_back(v[a]);
+            }
+          }
+        }
+  }
+}
+
+size_t field_count(const Ctx& c, int f, int shape[3]) {
+  const int* N = c.g.N;
+  int s[3] = {N[0], N[1], N[2]};
+  switch (f) {
+    case LH_FIELD_VX: case LH_FIELD_TEMPVX: case LH_FIELD_LIQUID_WX: s[0] += 1; break;
+    case LH_FIELD_VY: case LH_FIELD_TEMPVY: case LH_FIELD_LIQUID_WY: s[1] += 1; break;
+    case LH_FIELD_VZ: case LH_FIELD_TEMPVZ: case LH_FIELD_LIQUID_WZ: s[2] += 1; break;
+    case LH_FIELD_SOLID_PHI: case LH_FIELD_IS_VALID: s[0] += 1; s[1] += 1; s[2] += 1; break;
+    case LH_FIELD_PRESSURE: case LH_FIELD_LIQUID_PHI: case LH_FIELD_RHS: break;
+    default: return 0;
+  }
+  if (shape) {
+    shape[0] = s[0]; shape[1] = s[1]; shape[2] = s[2];
+  }
+  return (size_t)s[0] * s[1] * s[2];
+}
+
+void* field_dev(Ctx& c, int f) {
+  switch (f) {
+    case LH_FIELD_VX: return c.v[0];
+    case LH_FIELD_VY: return c.v[1];
+    case LH_FIELD_VZ: return c.v[2];
+    case LH_FIELD_TEMPVX: return c.tv[0];
+    case LH_FIELD_TEMPVY: return c.tv[1];
+    case LH_FIELD_TEMPVZ: return c.tv[2];
+    case LH_FIELD_LIQUID_WX: return c.lw[0];
+    case LH_FIELD_LIQUID_WY: return c.lw[1];
+    case LH_FIELD_LIQUID_WZ: return c.lw[2];
+    case LH_FIELD_PRESSURE: return c.pressure;
+    case LH_FIELD_LIQUID_PHI: return c.phi;
+    case LH_FIELD_SOLID_PHI: return c.solid_phi;
+    case LH_FIELD_IS_VALID: return c.is_valid;
+    case LH_FIELD_RHS: return c.rhs;
+    default: return nullptr;
+  }
+}
+
+int check_domain(Ctx& c) {
+  hipMemcpyAsync(c.h_flag + 1, c.dflag + 1, sizeof(int), hipMemcpyDeviceToHost, c.s);
+  if (!hip_ok(c, hipStreamSynchronize(c.s), "binning")) return LH_ERR_DEVICE;
+  if (c.h_flag[1]) {
+    c.err = "a particle left the grid box [minCoord, maxCoord)";
+    return LH_ERR_DOMAIN;
+  }
+  return LH_OK;
+}
+
+int read_maxv(Ctx& c) {
+  hipMemcpyAsync(c.h_flag + 4, c.dmaxv, sizeof(unsigned), hipMemcpyDeviceToHost, c.s);
+  if (!hip_ok(c, hipStreamSynchronize(c.s), "max velocity")) return LH_ERR_DEVICE;
+  std::memcpy(&c.last_maxv, c.h_flag + 4, sizeof(float));
+  return LH_OK;
+}
+
+// ---- step pieces
+int do_pre_step(Ctx& c) {
+  launch_clear(c);  // Grid<3>::Clear
+  int rc = run_binning(c);
+  if (rc) return rc;
+  launch_liquid_phi(c);  // ComputeLiquidPhi
+  return LH_OK;
+}
+
+int do_step(Ctx& c, float dt, lh_solve_stats* st) {
+  lh_solve_stats local;
+  std::memset(&local, 0, sizeof(local));
+  if (!c.bins_valid && c.P > 0) {
+    int rc = run_binning(c);
+    if (rc) return rc;
+  }
+  launch_p2g(c, true, dt);  // Particle2Grid + ApplyGravity (+ tank acceleration)
+  if (c.P == 0) launch_gravity(c, dt);
+  int solve_rc = LH_OK;
+  if (c.P > 0) {  // SolvePressure returns early without particles (LiquidModel.cpp:828)
+    solve_rc = run_solve(c, dt, &local);
+    if (solve_rc == LH_ERR_DEVICE || solve_rc == LH_ERR_NAN) {
+      if (st) *st = local;
+      return solve_rc;
+    }
+    launch_velocity_update(c, dt);
+  }
+  if (run_extrapolate(c) < 0) return LH_ERR_DEVICE;
+  launch_g2p_advect(c, true, true, dt);  // Grid2Particle + AdvectParticle (+ max|v|)
+  c.bins_valid = false;
+  c.time += (double)dt;
+  int rc = check_domain(c);
+  if (rc) return rc;
+  rc = read_maxv(c);
+  if (st) *st = local;
+  return rc;
+}
+
+float cfl_from(const Ctx& c) {
+  // GetCFL: clamp(dx / maxVel, 0.001, 0.1), LiquidModel.h:49-55
+  return clampf(c.g.dx[0] / c.last_maxv, 0.001f, 0.1f);
+}
+
+template <class F>
+int guarded(lh_ctx* ctx, F&& f) {
+  if (!ctx) return LH_ERR_ARG;
+  try {
+    return f(ctx->c);
+  } catch (const json_error& e) {
+    ctx->c.err = e.what();
+    return LH_ERR_CONFIG;
+  } catch (const std::exception& e) {
+    ctx->c.err = e.what();
+    return LH_ERR_CONFIG;
+  }
+}
+
+int create_common(const std::string& text, lh_ctx** out) {
+  if (!out) return LH_ERR_ARG;
+  *out = nullptr;
+  lh_ctx* ctx = new lh_ctx();
+  Ctx& c = ctx->c;
+  try {
+    c.cfg = parse_config(text);
+  } catch (const std::exception& e) {
+    c.err = e.what();
+    *out = ctx;  // returned so the caller can read lh_last_error, then lh_destroy
+    return LH_ERR_CONFIG;
+  }
+  if (!hip_ok(c, hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking), "stream") ||
+      !hip_ok(c, hipEventCreate(&c.ev0), "event") || !hip_ok(c, hipEventCreate(&c.ev1), "event")) {
+    *out = ctx;
+    return LH_ERR_DEVICE;
+  }
+  int rc = alloc_ctx(c);
+  *out = ctx;
+  return rc;
+}
+
+}  // namespace
+}  // namespace lh
+
+using namespace lh;
+
+extern "C" {
+
+int lh_create(const char* json_path, lh_ctx** out) {
+  if (!json_path) return LH_ERR_ARG;
+  std::ifstream f(json_path);
+  if (!f.is_open()) {
+    if (out) {
+      *out = new lh_ctx();
+      (*out)->c.err = std::string("cannot open config ") + json_path;
+    }
+    return LH_ERR_CONFIG;
+  }
+  std::stringstream ss;
+  ss << f.rdbuf();
+  return create_common(ss.str(), out);
+}
+
+int lh_create_from_string(const char* json_text, lh_ctx** out) {
+  if (!json_text) return LH_ERR_ARG;
+  return create_common(json_text, out);
+}
+
+void lh_destroy(lh_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->c.s) hipStreamSynchronize(ctx->c.s);
+  free_ctx(ctx->c);
+  delete ctx;
+}
+
+const char* lh_last_error(const lh_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int lh_grid_info(const lh_ctx* ctx, int dims[3], float min_coord[3], float dx[3]) {
+  if (!ctx) return LH_ERR_ARG;
+  for (int a = 0; a < 3; ++a) {
+    if (dims) dims[a] = ctx->c.g.N[a];
+    if (min_coord) min_coord[a] = ctx->c.g.mn[a];
+    if (dx) dx[a] = ctx->c.g.dx[a];
+  }
+  return LH_OK;
+}
+
+int lh_field_shape(const lh_ctx* ctx, int field, int shape[3]) {
+  if (!ctx) return LH_ERR_ARG;
+  return field_count(ctx->c, field, shape) ? LH_OK : LH_ERR_ARG;
+}
+
+int lh_compute_solid_phi(lh_ctx* ctx) {
+  return guarded(ctx, [](Ctx& c) {
+    launch_solid_phi(c);
+    c.solid_ready = true;
+    return hip_ok(c, hipStreamSynchronize(c.s), "solid phi") ? LH_OK : LH_ERR_DEVICE;
+  });
+}
+
+int lh_init(lh_ctx* ctx) {
+  return guarded(ctx, [](Ctx& c) {
+    // LiquidModel<3>::Init: ComputeSolidPhi, then spawn (LiquidModel.cpp:506-512)
+    launch_solid_phi(c);
+    c.solid_ready = true;
+    std::vector<float> pos, vel;
+    if (!c.cfg.blocks.empty()) seed_blocks(c, pos, vel);
+    else spawn_lattice(c, pos);
+    int64_t n = (int64_t)(pos.size() / 3);
+    c.time = 0.0;
+    return upload_particles(c, pos.data(), vel.empty() ? nullptr : vel.data(), nullptr, n);
+  });
+}
+
+int lh_clear_grid(lh_ctx* ctx) {
+  return guarded(ctx, [](Ctx& c) {
+    launch_clear(c);
+    return hip_ok(c, hipStreamSynchronize(c.s), "clear") ? LH_OK : LH_ERR_DEVICE;
+  });
+}
+
+int lh_compute_liquid_phi(lh_ctx* ctx) {
+  return guarded(ctx, [](Ctx& c) {
+    int rc = run_binning(c);
+    if (rc) return rc;
+    launch_liquid_phi(c);
+    return check_domain(c);
+  });
+}
+
+int lh_pre_step(lh_ctx* ctx, float dt) {
+  (void)dt;
+  return guarded(ctx, [](Ctx& c) {
+    int rc = do_pre_step(c);
+    if (rc) return rc;
+    return check_domain(c);
+  });
+}
+
+int lh_p2g(lh_ctx* ctx) {
+  return guarded(ctx, [](Ctx& c) {
+    if (!c.bins_valid) {
+      int rc = run_binning(c);
+      if (rc) return rc;
+    }
+    launch_p2g(c, false, 0.f);
+    return check_domain(c);
+  });
+}
+
+int lh_apply_gravity(lh_ctx* ctx, float dt) {
+  return guarded(ctx, [dt](Ctx& c) {
+    launch_gravity(c, dt);
+    return hip_ok(c, hipStreamSynchronize(c.s), "gravity") ? LH_OK : LH_ERR_DEVICE;
+  });
+}
+
+int lh_solve_pressure(lh_ctx* ctx, float dt, lh_solve_stats* stats) {
+  return guarded(ctx, [dt, stats](Ctx& c) {
+    if (c.P == 0) {
+      if (stats) std::memset(stats, 0, sizeof(*stats));
+      return LH_OK;
+    }
+    return run_solve(c, dt, stats);
+  });
+}
+
+int lh_velocity_update(lh_ctx* ctx, float dt) {
+  return guarded(ctx, [dt](Ctx& c) {
+    if (c.P > 0) launch_velocity_update(c, dt);
+    return hip_ok(c, hipStreamSynchronize(c.s), "velocity update") ? LH_OK : LH_ERR_DEVICE;
+  });
+}
+
+int lh_extrapolate(lh_ctx* ctx) {
+  return guarded(ctx, [](Ctx& c) {
+    if (run_extrapolate(c) < 0) return LH_ERR_DEVICE;
+    return hip_ok(c, hipStreamSynchronize(c.s), "extrapolate") ? LH_OK : LH_ERR_DEVICE;
+  });
+}
+
+int lh_g2p(lh_ctx* ctx) {
+  return guarded(ctx, [](Ctx& c) {
+    launch_g2p_advect(c, true, false, 0.f);
+    return read_maxv(c);
+  });
+}
+
+int lh_advect(lh_ctx* ctx, float dt) {
+  return guarded(ctx, [dt](Ctx& c) {
+    launch_g2p_advect(c, false, true, dt);
+    c.bins_valid = false;
+    return read_maxv(c);
+  });
+}
+
+int lh_divergence(lh_ctx* ctx, double* mean, double* linf, int* count) {
+  return guarded(ctx, [=](Ctx& c) {
+    launch_divergence(c, mean, linf, count);
+    return hip_ok(c, hipGetLastError(), "divergence") ? LH_OK : LH_ERR_DEVICE;
+  });
+}
+
+int lh_step(lh_ctx* ctx, float dt) {
+  return guarded(ctx, [dt](Ctx& c) { return do_step(c, dt, nullptr); });
+}
+
+int lh_post_step(lh_ctx* ctx, float dt, lh_step_stats* stats) {
+  (void)dt;
+  return guarded(ctx, [stats](Ctx& c) {
+    if (stats) {
+      launch_divergence(c, &stats->divergence_mean, &stats->divergence_linf, &stats->fluid_cells);
+      stats->max_velocity = c.last_maxv;
+      stats->cfl_dt = cfl_from(c);
+      stats->sim_time = c.time;
+    }
+    return LH_OK;
+  });
+}
+
+int lh_get_cfl(lh_ctx* ctx, float* dt) {
+  return guarded(ctx, [dt](Ctx& c) {
+    if (!dt) return LH_ERR_ARG;
+    *dt = cfl_from(c);
+    return LH_OK;
+  });
+}
+
+int lh_run_steps(lh_ctx* ctx, int nsteps, float* dt, lh_step_stats* last) {
+  return guarded(ctx, [=](Ctx& c) {
+    if (!dt) return LH_ERR_ARG;
+    for (int k = 0; k < nsteps; ++k) {
+      lh_step_stats st;
+      std::memset(&st, 0, sizeof(st));
+      hipEvent_t e0, e1;
+      hipEventCreate(&e0);
+      hipEventCreate(&e1);
+      hipEventRecord(e0, c.s);
+      int rc = do_pre_step(c);
+      if (rc == LH_OK) rc = do_step(c, *dt, &st.solve);
+      hipEventRecord(e1, c.s);
+      hipEventSynchronize(e1);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+      if (rc != LH_OK && rc != LH_ERR_NOT_CONVERGED) return rc;
+      st.step_ms = ms;
+      if (last && k == nsteps - 1) {
+        launch_divergence(c, &st.divergence_mean, &st.divergence_linf, &st.fluid_cells);
+      }
+      st.max_velocity = c.last_maxv;
+      st.cfl_dt = cfl_from(c);
+      st.sim_time = c.time;
+      *dt = st.cfl_dt;  // dt = liquidModel.GetCFL(), main.cpp:53
+      if (last) *last = st;
+    }
+    return LH_OK;
+  });
+}
+
+int lh_num_particles(const lh_ctx* ctx, int64_t* n) {
+  if (!ctx || !n) return LH_ERR_ARG;
+  *n = ctx->c.P;
+  return LH_OK;
+}
+
+int lh_set_particles(lh_ctx* ctx, const float* pos, const float* vel, const float* C, int64_t n) {
+  return guarded(ctx, [=](Ctx& c) {
+    if (n < 0 || (n > 0 && !pos)) return LH_ERR_ARG;
+    if (n >= (int64_t)0x7fffffff) {
+      c.err = "particle count exceeds 32-bit bin offsets";
+      return LH_ERR_ARG;
+    }
+    return upload_particles(c, pos, vel, C, n);
+  });
+}
+
+int lh_get_particles(lh_ctx* ctx, float* pos, float* vel, float* C, int64_t n) {
+  return guarded(ctx, [=](Ctx& c) {
+    if (n != c.P) {
+      c.err = "lh_get_particles: n does not match the particle count";
+      return LH_ERR_ARG;
+    }
+    if (n == 0) return LH_OK;
+    std::vector<float> soa((size_t)kParticleAttrs * n);
+    for (int k = 0; k < kParticleAttrs; ++k)
+      hipMemcpyAsync(soa.data() + (size_t)k * n, c.pbuf[c.cur] + (size_t)k * c.cap, n * sizeof(float),
+                     hipMemcpyDeviceToHost, c.s);
+    if (!hip_ok(c, hipStreamSynchronize(c.s), "download particles")) return LH_ERR_DEVICE;
+    for (int64_t i = 0; i < n; ++i) {
+      int id;
+      std::memcpy(&id, &soa[(size_t)PA_ID * n + i], 4);
+      if (id < 0 || id >= n) {
+        c.err = "corrupt particle id";
+        return LH_ERR_DEVICE;
+      }
+      for (int a = 0; a < 3; ++a) {
+        if (pos) pos[3 * (size_t)id + a] = soa[(size_t)(PA_X + a) * n + i];
+        if (vel) vel[3 * (size_t)id + a] = soa[(size_t)(PA_VX + a) * n + i];
+      }
+      if (C)
+        for (int k = 0; k < 9; ++k) C[9 * (size_t)id + k] = soa[(size_t)(PA_C + k) * n + i];
+    }
+    return LH_OK;
+  });
+}
+
+int lh_get_field(lh_ctx* ctx, int field, void* host, size_t count) {
+  return guarded(ctx, [=](Ctx& c) {
+    size_t n = field_count(c, field, nullptr);
+    if (!n || n != count || !host) {
+      c.err = "lh_get_field: bad field id or element count";
+      return LH_ERR_ARG;
+    }
+    size_t bytes = n * (field == LH_FIELD_IS_VALID ? 1 : 4);
+    hipMemcpyAsync(host, field_dev(c, field), bytes, hipMemcpyDeviceToHost, c.s);
+    return hip_ok(c, hipStreamSynchronize(c.s), "get field") ? LH_OK : LH_ERR_DEVICE;
+  });
+}
+
+int lh_set_field(lh_ctx* ctx, int field, const void* host, size_t count) {
+  return guarded(ctx, [=](Ctx& c) {
+    size_t n = field_count(c, field, nullptr);
+    if (!n || n != count || !host) {
+      c.err = "lh_set_field: bad field id or element count";
+      return LH_ERR_ARG;
+    }
+    size_t bytes = n * (field == LH_FIELD_IS_VALID ? 1 : 4);
+    hipMemcpyAsync(field_dev(c, field), host, bytes, hipMemcpyHostToDevice, c.s);
+    return hip_ok(c, hipStreamSynchronize(c.s), "set field") ? LH_OK : LH_ERR_DEVICE;
+  });
+}
+
+int lh_set_time(lh_ctx* ctx, double t) {
+  if (!ctx) return LH_ERR_ARG;
+  ctx->c.time = t;
+  return LH_OK;
+}
+
+int lh_get_time(const lh_ctx* ctx, double* t) {
+  if (!ctx || !t) return LH_ERR_ARG;
+  *t = ctx->c.time;
+  return LH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+// lh_ctx.h -- device-side state of one liquid scene and the launchers of the
+// HIP kernels (k_grid.hip, k_particles.hip, k_solver.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/lh_api.h"
+
+namespace lh {
+
+// Scene configuration: the reference's four parameter structs
+// (SceneParameter.h:9-152) plus the build's optional keys.
+struct Config {
+  int N[3] = {0, 0, 0};
+  float mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+  float gravity = -9.81f, density = 1000.f, cfg_dt = 0.01f, viscosity = 0.f, surface_tension = 0.f;
+  int correct_cycle = 8;
+  int num_particles = 0;
+  // SolverParameter
+  int max_iter = 100;
+  float tol = 1e-5f;
+  int tol_mode = LH_TOL_ABSOLUTE;
+  int precond = LH_PRECOND_MULTIGRID;
+  int num_level = 1, num_pre = 5, num_post = 5, num_final = 10;
+  float omega = 0.5f;
+  int smoother = LH_SMOOTHER_REFERENCE;
+  // mixed-precision refinement (FP32 inner MGPCG under an FP64 residual loop)
+  int refined = 0;
+  double refined_tol = 1e-9;
+  int max_outer = 50;
+  // boundary SDF program (postfix, 12 floats per node)
+  std::vector<float> sdf;
+  // jittered fluid blocks: min[3], max[3], vel[3], sigma
+  std::vector<float> blocks;
+  int k_ppc = 2;
+  uint64_t seed = 0;
+  // tank acceleration a(t) = amp * sin(2 pi freq t) on axis
+  float tank_amp = 0.f, tank_freq = 0.f;
+  int tank_axis = -1;
+  float first_dt = 0.01f;
+  int steps = 0;
+};
+
+struct GridP {
+  int N[3];
+  float mn[3], dx[3];
+};
+
+struct Level {
+  int d[3] = {0, 0, 0};
+  size_t n = 0;
+  float* adiag = nullptr;
+  float* t[3] = {nullptr, nullptr, nullptr};  // term of the +axis face of each cell
+  uint8_t* cls = nullptr;                     // 0: phi > 0, 1: phi == 0 (or NaN), 2: phi < 0
+  float* r = nullptr;
+  float* z = nullptr;
+};
+
+// attributes of one particle in the SoA particle buffer (kParticleAttrs x cap)
+enum { PA_X = 0, PA_VX = 3, PA_C = 6, PA_ID = 15, kParticleAttrs = 16 };
+
+struct Ctx {
+  Config cfg;
+  GridP g;
+  int dW[3] = {2, 2, 2};  // ceil(radius / dx_a), radius = 2 dx_x (ParticleSystem.cpp:6-8)
+  float radius = 0.f;
+  hipStream_t s = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // grid (Grid<3>, Grid.h:137-170)
+  float *v[3] = {}, *tv[3] = {}, *lw[3] = {};
+  float *pressure = nullptr, *phi = nullptr, *solid_phi = nullptr, *rhs = nullptr;
+  uint8_t* is_valid = nullptr;  // (N+1)^3
+  uint8_t* xstate = nullptr;    // extrapolation sweep state, sized for the largest face array
+  size_t nface[3] = {0, 0, 0}, ncell = 0, nnode = 0;
+  float* d_sdf = nullptr;
+  int nsdf = 0;
+  // particles, SoA, double buffered for the per-step bin-order permutation
+  int64_t P = 0, cap = 0;
+  float* pbuf[2] = {nullptr, nullptr};
+  int cur = 0;
+  // half-cell bins: 2N per axis
+  int H[3] = {0, 0, 0};
+  size_t nbins = 0;
+  int* bin_count = nullptr;  // kept at zero between steps (decremented by the scatter)
+  int* bin_start = nullptr;  // nbins + 1
+  int* pkey = nullptr;       // per particle bin key
+  int* pslot = nullptr;      // per particle provisional slot
+  int* scan_tmp = nullptr;   // scan block sums
+  bool bins_valid = false;
+  // solver (CGSolver<3>)
+  std::vector<Level> lv;
+  float *p = nullptr, *x = nullptr, *q = nullptr;
+  double* partial = nullptr;  // per-block FP64 partial sums
+  int npartial = 0;
+  float* dsc = nullptr;       // device scalars: [0] oldRZ [1] newRZ [2] alpha [3] beta [4] residual ...
+  double* dsc64 = nullptr;    // device FP64 scalars
+  unsigned* dmaxv = nullptr;  // max |v| bits
+  int* dflag = nullptr;       // misc device flags (extrapolation progress, domain errors)
+  double* h_pin = nullptr;    // pinned host scalars
+  int* h_flag = nullptr;
+  double time = 0.0;
+  float last_maxv = 0.f;
+  bool solid_ready = false;
+  std::string err;
+};
+
+// ---- launchers (k_grid.hip)
+void launch_fill_f32(Ctx& c, float* a, size_t n, float v);
+void launch_fill_u8(Ctx& c, uint8_t* a, size_t n, uint8_t v);
+void launch_solid_phi(Ctx& c);
+void launch_clear(Ctx& c);
+void launch_velocity_update(Ctx& c, float dt);
+// returns number of propagation rounds used; <0 on device error
+int run_extrapolate(Ctx& c);
+void launch_divergence(Ctx& c, double* h_mean, double* h_linf, int* h_count);
+void launch_gravity(Ctx& c, float dt);
+// ---- launchers (k_particles.hip)
+int run_binning(Ctx& c);  // LH_OK or LH_ERR_DOMAIN
+void launch_liquid_phi(Ctx& c);
+void launch_p2g(Ctx& c, bool fuse_body_forces, float dt);
+void launch_g2p_advect(Ctx& c, bool do_g2p, bool do_advect, float dt);
+// ---- launchers (k_solver.hip)
+int run_solve(Ctx& c, float dt, lh_solve_stats* st);  // CGSolver<3>::Solve
+void launch_build_rhs(Ctx& c);
+void launch_build_lhs(Ctx& c, float dt);
+
+// device error check helper
+bool hip_ok(Ctx& c, hipError_t e, const char* what);
+
+inline size_t div_up(size_t a, size_t b) { return (a + b - 1) / b; }
+inline float* pattr(Ctx& c, int buf, int k) { return c.pbuf[buf] + (size_t)k * (size_t)c.cap; }
+
+}  // namespace lh

@@ -1,0 +1,1334 @@
+// lh_oracle.cpp -- CPU oracle for the liquid step.  TEST INFRASTRUCTURE ONLY.
+//
+// A line-by-line FP32 restatement of the reference's 3D liquid path
+// (LiquidModel.cpp:478-1083, PressureSolver.cpp:422-934, Grid.cpp:86-192,
+// Array.h:67-126, Kernel.h:12-31, MathUtils.h:8-34, MathUtils.cpp:29-35,75-88,
+// Boundary.h:27-94, ParticleSystem.cpp:6-64, LiquidModel.h:49-55).  It is the
+// checker of the HIP library, never the thing measured or shipped: only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+//
+// Determinism choices (the reference's OpenMP code is racy or order-free):
+//  * CGSolver denom / residual sums get a proper reduction (the reference has
+//    no reduction clause, PressureSolver.cpp:874-898).
+//  * Reduce / denom / residual: dot_mode 0 sums in FP32 in index order
+//    (a serial run of Array.h:111-122); dot_mode 1 sums in FP64 and rounds the
+//    total to float, which is what the HIP library does.
+//  * P2G: p2g_order 0 adds particle contributions in particle-index order
+//    (a serial run of the atomics at LiquidModel.cpp:640-688); p2g_order 1 adds
+//    them per face in half-cell-bin order (bins x-major, z fastest; particles in
+//    a bin by ascending index), the order of the HIP gather.
+//  * Extrapolate runs sequentially in (x,y,z) order with in-place validity
+//    (LiquidModel.cpp:953-978), which fixes the std::vector<bool> race.
+//  * Eigen fixed-size reductions of length 3 evaluate as x0 + (x1 + x2)
+//    (Eigen Redux.h redux_novec_unroller splits Length=3 as 1 + 2); this is
+//    used for norm(), dot() and maxCoeff().
+//
+// Parity status per function is listed in the header comment of each section
+// ("pinned by:").  Built with -ffp-contract=off: MSVC /fp:precise forms no FMA.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+using i64 = long long;
+
+// ---------------------------------------------------------------- params
+}  // namespace
+
+extern "C" {
+struct orc_params {
+  int N[3];
+  float mn[3], mx[3];
+  float gravity, density;
+  int max_iter;
+  float tol;
+  int tol_mode;   // 0 absolute (reference), 1 relative to ||b||
+  int precond;    // 0 None, 1 Jacobi, 2 Multigrid
+  int num_level, num_pre, num_post, num_final;
+  float omega;
+  int dot_mode;   // 0 fp32 serial, 1 fp64
+  int smoother;   // 0 reference (Q4/Q5), 1 symmetric
+  int p2g_order;  // 0 particle order, 1 half-cell bin order
+  float tank_amp, tank_freq;
+  int tank_axis;  // -1: none
+  int nsdf;       // number of SDF program nodes (12 floats each)
+  const float* sdf;
+};
+struct orc_solve_stats {
+  int iterations, converged;
+  double initial_residual, final_residual, rhs_norm;
+};
+}
+
+namespace {
+
+// ------------------------------------------------------------ Array3
+// Array.h:67-126: x-major, z fastest.
+struct A3 {
+  int d[3] = {1, 1, 1};
+  std::vector<float> v;
+  void resize(int x, int y, int z) { d[0] = x; d[1] = y; d[2] = z; v.assign((size_t)x * y * z, 0.f); }
+  inline size_t id(int x, int y, int z) const { return ((size_t)x * d[1] + y) * d[2] + z; }
+  inline float& operator()(int x, int y, int z) { return v[id(x, y, z)]; }
+  inline float operator()(int x, int y, int z) const { return v[id(x, y, z)]; }
+  void fill(float a) { std::fill(v.begin(), v.end(), a); }
+};
+
+// ------------------------------------------------------------ L0 numerics
+// Kernel.h:12-17 / 26-31.  pinned by: closed forms (w(0)=.75, w(.5)=.5,
+// w(1)=.125, w(1.5)=0, partition of unity, first moment) in tests.
+inline float quad_w(float d) {
+  d = std::fabs(d);
+  if (d < 0.5f) return 0.75f - d * d;
+  else if (d < 1.5f) return 0.5f * (1.5f - d) * (1.5f - d);
+  return 0.0f;
+}
+inline float quad_gw(float d) {
+  d = std::fabs(d);
+  if (d < 0.5f) return -2.f * d;
+  else if (d < 1.5f) return d - 1.5f;
+  return 0.f;
+}
+// MathUtils.h:8-34
+const float kEps = 1e-8f;  // Definition.h:93
+inline bool is_small(float v) { return std::fabs(v) < kEps; }
+inline float clampf(float v, float lo, float hi) {  // min(maxVal, max(minVal, val)) with the windows.h macros
+  float t = (lo > v) ? lo : v;
+  return (hi < t) ? hi : t;
+}
+inline float lerp(float a, float b, float f) { return a + clampf(f, 0.0f, 1.0f) * (b - a); }
+inline float bilerp(float v00, float v10, float v01, float v11, float f1, float f2) {
+  float a = lerp(v00, v10, f1);
+  float b = lerp(v01, v11, f1);
+  return lerp(a, b, clampf(f2, 0.0f, 1.0f));
+}
+inline float trilerp(float v000, float v100, float v010, float v110, float v001, float v101, float v011,
+                     float v111, float f1, float f2, float f3) {
+  return lerp(bilerp(v000, v100, v010, v110, f1, f2), bilerp(v001, v101, v011, v111, f1, f2),
+              clampf(f3, 0.0f, 1.0f));
+}
+// MathUtils.cpp:29-35.  pinned by: closed-form table in tests.
+inline float fraction_in_region(float l, float r) {
+  float ans = 0.0f;
+  if (l < 0.0f && r < 0.0f) ans = 1.0f;
+  else if (l < 0.0f && 0.0f < r) ans = l / (l - r);
+  else if (l > 0.0f && 0.0f > r) ans = r / (r - l);
+  return ans;
+}
+// max(0.1, frac) with the windows.h macro: compared in double, result stored as float.
+inline float max_tenth(float frac) { return (0.1 > (double)frac) ? (float)0.1 : frac; }
+// MathUtils.cpp:75-88
+inline void barycentric(float x, int lo, int hi, int& idx, float& frac) {
+  int i = (int)std::floor(x);
+  float f = x - (float)i;
+  if (i < lo) { i = lo; f = 0.0f; }
+  else if (i > hi - 2) { i = hi - 2; f = 1.0f; }
+  idx = i; frac = f;
+}
+// Eigen length-3 reductions (see file header).
+inline float sum3(float a, float b, float c) { return a + (b + c); }
+inline float norm3(float a, float b, float c) { return std::sqrt(a * a + (b * b + c * c)); }
+inline float max3(float a, float b, float c) {
+  float bc = (b < c) ? c : b;
+  return (a < bc) ? bc : a;
+}
+
+// ------------------------------------------------------------ SDF program
+// Boundary.h:27-94 as a postfix program (12 floats per node):
+//   [kind, cx, cy, cz, hx, hy, hz, radius, useInside, 0, 0, 0]
+//   kind 0 Rect leaf, 1 Sphere leaf, 10 Union(min), 11 Intersection(max),
+//   12 Subtract max(a,-b), 13 None (identity on the single child).
+// pinned by: box-SDF closed forms in tests.
+float sdf_eval(const std::vector<float>& prog, float px, float py, float pz) {
+  float st[32];
+  int sp = 0;
+  size_t n = prog.size() / 12;
+  for (size_t k = 0; k < n; ++k) {
+    const float* q = &prog[12 * k];
+    int kind = (int)q[0];
+    if (kind == 0) {  // RectBoundary::Compute, Boundary.h:83-90
+      float ax = std::fabs(px - q[1]) - q[4];
+      float ay = std::fabs(py - q[2]) - q[5];
+      float az = std::fabs(pz - q[3]) - q[6];
+      float mx = ax > 0.f ? ax : 0.f, my = ay > 0.f ? ay : 0.f, mz = az > 0.f ? az : 0.f;
+      float outside = norm3(mx, my, mz);
+      float mc = max3(ax, ay, az);
+      float inside = (mc < 0.f) ? mc : 0.f;
+      st[sp++] = (q[8] != 0.f) ? -outside - inside : outside + inside;
+    } else if (kind == 1) {  // sphere: |p - c| - r (negated with useInside)
+      float d = norm3(px - q[1], py - q[2], pz - q[3]) - q[7];
+      st[sp++] = (q[8] != 0.f) ? -d : d;
+    } else if (kind == 13) {
+      // None: returns the (single) child value unchanged
+    } else {
+      float b = st[--sp];
+      float a = st[--sp];
+      float r;
+      if (kind == 10) r = (a < b) ? a : b;
+      else if (kind == 11) r = (a > b) ? a : b;
+      else { float nb = -b; r = (a > nb) ? a : nb; }
+      st[sp++] = r;
+    }
+  }
+  return sp > 0 ? st[sp - 1] : 1.0f;
+}
+
+// ------------------------------------------------------------ model state
+struct Level {
+  int d[3];
+  A3 r, z, adiag;
+  std::vector<float> acoef;  // 6 per cell: right, left, top, bottom, front, back
+};
+
+struct Model {
+  orc_params prm;
+  std::vector<float> sdf;
+  int N[3];
+  float mn[3], mx[3], dx[3];
+  A3 vx, vy, vz, tvx, tvy, tvz, wx, wy, wz, lwx, lwy, lwz, pressure, liquidPhi, solidPhi, rhs, p, x;
+  std::vector<uint8_t> isValid;  // (N+1)^3, Grid.h:160, never cleared by Clear()
+  std::vector<float> pos, vel, C, radius;
+  i64 P = 0;
+  std::vector<Level> lv;
+  int max_level = 1;
+  double time = 0.0;
+  std::string err;
+};
+
+inline size_t vid(const Model& m, int x, int y, int z) {
+  return ((size_t)x * (m.N[1] + 1) + y) * (m.N[2] + 1) + z;
+}
+
+// Grid<3> ctor, Grid.cpp:86-98; CGSolver<3> ctor, PressureSolver.cpp:422-447.
+void model_alloc(Model& m) {
+  const int* N = m.N;
+  m.vx.resize(N[0] + 1, N[1], N[2]); m.vy.resize(N[0], N[1] + 1, N[2]); m.vz.resize(N[0], N[1], N[2] + 1);
+  m.tvx.resize(N[0] + 1, N[1], N[2]); m.tvy.resize(N[0], N[1] + 1, N[2]); m.tvz.resize(N[0], N[1], N[2] + 1);
+  m.wx.resize(N[0] + 1, N[1], N[2]); m.wy.resize(N[0], N[1] + 1, N[2]); m.wz.resize(N[0], N[1], N[2] + 1);
+  m.lwx.resize(N[0] + 1, N[1], N[2]); m.lwy.resize(N[0], N[1] + 1, N[2]); m.lwz.resize(N[0], N[1], N[2] + 1);
+  m.pressure.resize(N[0], N[1], N[2]); m.liquidPhi.resize(N[0], N[1], N[2]);
+  m.solidPhi.resize(N[0] + 1, N[1] + 1, N[2] + 1);
+  m.rhs.resize(N[0], N[1], N[2]); m.p.resize(N[0], N[1], N[2]); m.x.resize(N[0], N[1], N[2]);
+  m.isValid.assign((size_t)(N[0] + 1) * (N[1] + 1) * (N[2] + 1), 0);
+  for (int a = 0; a < 3; ++a) {
+    float interval = m.mx[a] - m.mn[a];
+    m.dx[a] = (float)(interval / (float)N[a]);
+  }
+  int L = std::max(1, m.prm.num_level);
+  if (m.prm.precond != 2) L = 1;
+  m.max_level = L;
+  m.lv.resize(L);
+  for (int l = 0; l < L; ++l) {
+    Level& q = m.lv[l];
+    for (int a = 0; a < 3; ++a) q.d[a] = N[a] / (1 << l);
+    q.r.resize(q.d[0], q.d[1], q.d[2]); q.z.resize(q.d[0], q.d[1], q.d[2]);
+    q.adiag.resize(q.d[0], q.d[1], q.d[2]);
+    q.acoef.assign((size_t)q.d[0] * q.d[1] * q.d[2] * 6, 0.f);
+  }
+}
+
+// ------------------------------------------------------------ Grid sampling
+// Grid<3>::GetSolidPhi / GetSolidGradient, Grid.cpp:128-155, 175-192.
+// pinned by: linear-field exactness and edge-clamp cases in tests.
+inline bool in_grid(const Model& m, float x, float y, float z) {
+  return m.mn[0] <= x && m.mn[1] <= y && m.mn[2] <= z && x <= m.mx[0] && y <= m.mx[1] && z <= m.mx[2];
+}
+void solid_gradient(const Model& m, float x, float y, float z, float g[3]);
+float solid_phi_at(const Model& m, float x, float y, float z) {
+  if (!in_grid(m, x, y, z)) {
+    float vx = clampf(x, m.mn[0] + m.dx[0], m.mx[0] - m.dx[0]);
+    float vy = clampf(y, m.mn[1] + m.dx[1], m.mx[1] - m.dx[1]);
+    float vz = clampf(z, m.mn[2] + m.dx[2], m.mx[2] - m.dx[2]);
+    float g[3];
+    solid_gradient(m, vx, vy, vz, g);
+    return solid_phi_at(m, vx, vy, vz) + g[0] * (x - vx) + g[1] * (y - vy) + g[2] * (z - vz);
+  }
+  float lx = (x - m.mn[0]) / m.dx[0];
+  float ly = (y - m.mn[1]) / m.dx[1];
+  float lz = (z - m.mn[2]) / m.dx[2];
+  int xi, yi, zi;
+  float fx, fy, fz;
+  barycentric(lx, 0, m.N[0] + 1, xi, fx);
+  barycentric(ly, 0, m.N[1] + 1, yi, fy);
+  barycentric(lz, 0, m.N[2] + 1, zi, fz);
+  const A3& s = m.solidPhi;
+  return trilerp(s(xi, yi, zi), s(xi + 1, yi, zi), s(xi, yi + 1, zi), s(xi + 1, yi + 1, zi), s(xi, yi, zi + 1),
+                 s(xi + 1, yi, zi + 1), s(xi, yi + 1, zi + 1), s(xi + 1, yi + 1, zi + 1), fx, fy, fz);
+}
+void solid_gradient(const Model& m, float x, float y, float z, float g[3]) {
+  if (!in_grid(m, x, y, z)) {
+    float vx = clampf(x, m.mn[0] + m.dx[0], m.mx[0] - m.dx[0]);
+    float vy = clampf(y, m.mn[1] + m.dx[1], m.mx[1] - m.dx[1]);
+    float vz = clampf(z, m.mn[2] + m.dx[2], m.mx[2] - m.dx[2]);
+    solid_gradient(m, vx, vy, vz, g);
+    return;
+  }
+  const float eps = 1e-4f;
+  float left = solid_phi_at(m, x - eps, y, z);
+  float right = solid_phi_at(m, x + eps, y, z);
+  float bottom = solid_phi_at(m, x, y - eps, z);
+  float top = solid_phi_at(m, x, y + eps, z);
+  float front = solid_phi_at(m, x, y, z + eps);
+  float back = solid_phi_at(m, x, y, z - eps);
+  const float two_eps = (float)(2.0 * (double)eps);
+  float a = (right - left) / two_eps, b = (top - bottom) / two_eps, c = (front - back) / two_eps;
+  float n2 = a * a + (b * b + c * c);  // Eigen normalized(): v / sqrt(squaredNorm) if > 0
+  if (n2 > 0.f) {
+    float n = std::sqrt(n2);
+    a = a / n; b = b / n; c = c / n;
+  }
+  g[0] = a; g[1] = b; g[2] = c;
+}
+
+// ------------------------------------------------------------ stages
+// LiquidModel<3>::ComputeSolidPhi, LiquidModel.cpp:562-611 (edge-based face
+// fractions, Q2).  pinned by: box SDF closed forms; W=0 inside walls, 1 in the
+// open region, fractional on cut edges, checked in tests.
+void compute_solid_phi(Model& m) {
+  const int* N = m.N;
+#pragma omp parallel for
+  for (int xi = 0; xi < N[0] + 1; ++xi)
+    for (int yi = 0; yi < N[1] + 1; ++yi)
+      for (int zi = 0; zi < N[2] + 1; ++zi) {
+        float px = (float)xi * m.dx[0] + m.mn[0];
+        float py = (float)yi * m.dx[1] + m.mn[1];
+        float pz = (float)zi * m.dx[2] + m.mn[2];
+        m.solidPhi(xi, yi, zi) = sdf_eval(m.sdf, px, py, pz);
+      }
+  for (int xi = 0; xi < N[0] + 1; ++xi)
+    for (int yi = 0; yi < N[1]; ++yi)
+      for (int zi = 0; zi < N[2]; ++zi)
+        m.lwx(xi, yi, zi) = clampf(1.0f - fraction_in_region(m.solidPhi(xi, yi, zi), m.solidPhi(xi, yi + 1, zi)), 0.f, 1.f);
+  for (int xi = 0; xi < N[0]; ++xi)
+    for (int yi = 0; yi < N[1] + 1; ++yi)
+      for (int zi = 0; zi < N[2]; ++zi)
+        m.lwy(xi, yi, zi) = clampf(1.0f - fraction_in_region(m.solidPhi(xi, yi, zi), m.solidPhi(xi + 1, yi, zi)), 0.f, 1.f);
+  for (int xi = 0; xi < N[0]; ++xi)
+    for (int yi = 0; yi < N[1]; ++yi)
+      for (int zi = 0; zi < N[2] + 1; ++zi)
+        m.lwz(xi, yi, zi) = clampf(1.0f - fraction_in_region(m.solidPhi(xi, yi, zi), m.solidPhi(xi + 1, yi, zi)), 0.f, 1.f);
+}
+
+// Grid<3>::Clear, Grid.cpp:157-169 (isValid, liquidW, solidPhi untouched).
+void clear_grid(Model& m) {
+  m.pressure.fill(0.f);
+  m.liquidPhi.fill(3 * m.dx[0]);
+  m.vx.fill(0.f); m.wx.fill(0.f); m.tvx.fill(0.f);
+  m.vy.fill(0.f); m.wy.fill(0.f); m.tvy.fill(0.f);
+  m.vz.fill(0.f); m.wz.fill(0.f); m.tvz.fill(0.f);
+}
+
+// LiquidModel<3>::ComputeLiquidPhi, LiquidModel.cpp:539-560.  The min is
+// order-free, so a serial loop gives the reference's result bit for bit.
+// pinned by: brute-force min over all particles on small cases in tests.
+void compute_liquid_phi(Model& m) {
+  if (m.P == 0) return;
+  const int* N = m.N;
+  for (i64 pi = 0; pi < m.P; ++pi) {
+    const float px = m.pos[3 * pi], py = m.pos[3 * pi + 1], pz = m.pos[3 * pi + 2];
+    const float radius = m.radius[pi];
+    int cx = (int)((px - m.mn[0]) / m.dx[0]);
+    int cy = (int)((py - m.mn[1]) / m.dx[1]);
+    int cz = (int)((pz - m.mn[2]) / m.dx[2]);
+    int dX = (int)std::ceil(radius / m.dx[0]);
+    int dY = (int)std::ceil(radius / m.dx[1]);
+    int dZ = (int)std::ceil(radius / m.dx[2]);
+    for (int xi = std::max(0, cx - dX); xi <= std::min(N[0] - 1, cx + dX); ++xi)
+      for (int yi = std::max(0, cy - dY); yi <= std::min(N[1] - 1, cy + dY); ++yi)
+        for (int zi = std::max(0, cz - dZ); zi <= std::min(N[2] - 1, cz + dZ); ++zi) {
+          float cpx = ((float)xi + 0.5f) * m.dx[0] + m.mn[0];
+          float cpy = ((float)yi + 0.5f) * m.dx[1] + m.mn[1];
+          float cpz = ((float)zi + 0.5f) * m.dx[2] + m.mn[2];
+          float phi = norm3(cpx - px, cpy - py, cpz - pz) - radius;
+          float& cur = m.liquidPhi(xi, yi, zi);
+          cur = (cur < phi) ? cur : phi;  // windows.h min(a,b) = (a < b) ? a : b
+        }
+  }
+}
+
+// ---- P2G, LiquidModel.cpp:613-721.
+// pinned by: uniform-field reproduction, FP64 brute force on small particle
+// sets and the LiquidData frame-1 known answer (tests).
+struct P2GContrib {
+  float w, vw;
+};
+// Contribution of particle pi to face (xi,yi,zi) of component a, exactly as
+// LiquidModel.cpp:628-636 (a=0), 657-665 (a=1), 686-694 (a=2).
+inline P2GContrib p2g_term(const Model& m, i64 pi, int a, int xi, int yi, int zi) {
+  const float* pp = &m.pos[3 * pi];
+  float gx = (pp[0] - m.mn[0]) / m.dx[0];
+  float gy = (pp[1] - m.mn[1]) / m.dx[1];
+  float gz = (pp[2] - m.mn[2]) / m.dx[2];
+  float ccx = (a == 0) ? (float)xi : (float)xi + 0.5f;
+  float ccy = (a == 1) ? (float)yi : (float)yi + 0.5f;
+  float ccz = (a == 2) ? (float)zi : (float)zi + 0.5f;
+  float weight = quad_w(gx - ccx) * quad_w(gy - ccy) * quad_w(gz - ccz);
+  float d0 = pp[0] - (ccx * m.dx[0] + m.mn[0]);
+  float d1 = pp[1] - (ccy * m.dx[1] + m.mn[1]);
+  float d2 = pp[2] - (ccz * m.dx[2] + m.mn[2]);
+  const float* col = &m.C[9 * pi + 3 * a];
+  float dot = sum3(col[0] * d0, col[1] * d1, col[2] * d2);
+  return {weight, weight * (m.vel[3 * pi + a] + dot)};
+}
+
+void p2g_particle_order(Model& m) {
+  const int* N = m.N;
+  A3* V[3] = {&m.vx, &m.vy, &m.vz};
+  A3* W[3] = {&m.wx, &m.wy, &m.wz};
+  for (i64 pi = 0; pi < m.P; ++pi) {
+    const float* pp = &m.pos[3 * pi];
+    float gx = (pp[0] - m.mn[0]) / m.dx[0];
+    float gy = (pp[1] - m.mn[1]) / m.dx[1];
+    float gz = (pp[2] - m.mn[2]) / m.dx[2];
+    int cx = (int)gx, cy = (int)gy, cz = (int)gz;
+    float r = m.radius[pi];
+    int dX = (int)std::ceil(r / m.dx[0]), dY = (int)std::ceil(r / m.dx[1]), dZ = (int)std::ceil(r / m.dx[2]);
+    for (int a = 0; a < 3; ++a) {
+      int hi[3] = {N[0] - 1, N[1] - 1, N[2] - 1};
+      hi[a] += 1;
+      for (int ddx = -dX; ddx <= dX; ++ddx)
+        for (int ddy = -dY; ddy <= dY; ++ddy)
+          for (int ddz = -dZ; ddz <= dZ; ++ddz) {
+            int xi = cx + ddx, yi = cy + ddy, zi = cz + ddz;
+            if (xi < 0 || xi > hi[0] || yi < 0 || yi > hi[1] || zi < 0 || zi > hi[2]) continue;
+            P2GContrib t = p2g_term(m, pi, a, xi, yi, zi);
+            (*W[a])(xi, yi, zi) += t.w;
+            (*V[a])(xi, yi, zi) += t.vw;
+          }
+    }
+  }
+}
+
+// Half-cell binning: key of a particle = ((hx*2Ny)+hy)*2Nz+hz with h = floor(2*gridPos)
+// (gridPos = (pos-min)/dx).  Counting sort, stable in particle index.
+struct Bins {
+  int H[3];
+  std::vector<i64> start;  // size H0*H1*H2 + 1
+  std::vector<i64> order;  // particle ids in bin order
+};
+bool build_bins(const Model& m, Bins& b) {
+  for (int a = 0; a < 3; ++a) b.H[a] = 2 * m.N[a];
+  size_t nb = (size_t)b.H[0] * b.H[1] * b.H[2];
+  std::vector<i64> key(m.P);
+  b.start.assign(nb + 1, 0);
+  for (i64 pi = 0; pi < m.P; ++pi) {
+    int h[3];
+    for (int a = 0; a < 3; ++a) {
+      float g = (m.pos[3 * pi + a] - m.mn[a]) / m.dx[a];
+      float g2 = 2.0f * g;
+      if (!(g2 >= 0.f) || !(g2 < (float)b.H[a])) return false;
+      h[a] = (int)g2;
+    }
+    key[pi] = ((i64)h[0] * b.H[1] + h[1]) * b.H[2] + h[2];
+    b.start[key[pi] + 1]++;
+  }
+  for (size_t k = 0; k < nb; ++k) b.start[k + 1] += b.start[k];
+  std::vector<i64> cur(b.start.begin(), b.start.end() - 1);
+  b.order.resize(m.P);
+  for (i64 pi = 0; pi < m.P; ++pi) b.order[cur[key[pi]]++] = pi;
+  return true;
+}
+
+// Per-face gather in bin order.  Face (xi,yi,zi) of component a receives
+// weight from particles whose gridPos lies within 1.5 of the face centre on
+// every axis, i.e. half-cells [2c-3, 2c+2] on the normal axis and [2c-2, 2c+3]
+// on the other two.  Zero-weight taps add exact zeros in the reference, so
+// skipping them changes nothing.
+bool p2g_bin_order(Model& m) {
+  Bins b;
+  if (!build_bins(m, b)) return false;
+  const int* N = m.N;
+  A3* V[3] = {&m.vx, &m.vy, &m.vz};
+  A3* W[3] = {&m.wx, &m.wy, &m.wz};
+  for (int a = 0; a < 3; ++a) {
+    int dim[3] = {N[0], N[1], N[2]};
+    dim[a] += 1;
+#pragma omp parallel for
+    for (int xi = 0; xi < dim[0]; ++xi)
+      for (int yi = 0; yi < dim[1]; ++yi)
+        for (int zi = 0; zi < dim[2]; ++zi) {
+          int c[3] = {xi, yi, zi};
+          int lo[3], hi[3];
+          for (int k = 0; k < 3; ++k) {
+            lo[k] = (k == a) ? 2 * c[k] - 3 : 2 * c[k] - 2;
+            hi[k] = lo[k] + 5;
+            lo[k] = std::max(lo[k], 0);
+            hi[k] = std::min(hi[k], b.H[k] - 1);
+          }
+          float ws = 0.f, vs = 0.f;
+          for (int hx = lo[0]; hx <= hi[0]; ++hx)
+            for (int hy = lo[1]; hy <= hi[1]; ++hy) {
+              size_t k0 = ((size_t)hx * b.H[1] + hy) * b.H[2];
+              for (i64 s = b.start[k0 + lo[2]]; s < b.start[k0 + hi[2] + 1]; ++s) {
+                i64 pi = b.order[s];
+                // the reference also bounds taps to cellIndex +- ceil(r/dx) (LiquidModel.cpp:625)
+                float r = m.radius[pi];
+                int dd[3] = {(int)std::ceil(r / m.dx[0]), (int)std::ceil(r / m.dx[1]), (int)std::ceil(r / m.dx[2])};
+                bool inwin = true;
+                for (int k = 0; k < 3; ++k) {
+                  int ck = (int)((m.pos[3 * pi + k] - m.mn[k]) / m.dx[k]);
+                  if (c[k] < ck - dd[k] || c[k] > ck + dd[k]) inwin = false;
+                }
+                if (!inwin) continue;
+                P2GContrib t = p2g_term(m, pi, a, xi, yi, zi);
+                if (t.w == 0.f) continue;
+                ws += t.w;
+                vs += t.vw;
+              }
+            }
+          (*W[a])(xi, yi, zi) = ws;
+          (*V[a])(xi, yi, zi) = vs;
+        }
+  }
+  return true;
+}
+
+void p2g_normalize(Model& m) {
+  A3* V[3] = {&m.vx, &m.vy, &m.vz};
+  A3* W[3] = {&m.wx, &m.wy, &m.wz};
+  for (int a = 0; a < 3; ++a) {
+    std::vector<float>& v = V[a]->v;
+    const std::vector<float>& w = W[a]->v;
+    for (size_t i = 0; i < v.size(); ++i) v[i] = !is_small(w[i]) ? v[i] / w[i] : 0.0f;
+  }
+}
+
+// LiquidModel<3>::ApplyGravity, LiquidModel.cpp:898-910 (all faces, Q8), plus
+// the optional horizontal tank acceleration a(t) = A sin(2 pi f t) on the
+// interior faces of the chosen axis (same index ranges rotated to that axis).
+void apply_gravity(Model& m, float dt) {
+  const int* N = m.N;
+  float gdt = m.prm.gravity * dt;
+  for (int xi = 1; xi < N[0] - 1; ++xi)
+    for (int yi = 1; yi < N[1]; ++yi)
+      for (int zi = 1; zi < N[2] - 1; ++zi) m.vy(xi, yi, zi) += gdt;
+  int ax = m.prm.tank_axis;
+  if (ax >= 0 && ax <= 2 && m.prm.tank_amp != 0.f) {
+    double acc = (double)m.prm.tank_amp * std::sin(2.0 * M_PI * (double)m.prm.tank_freq * m.time);
+    float adt = (float)acc * dt;
+    A3* V[3] = {&m.vx, &m.vy, &m.vz};
+    int lo[3] = {1, 1, 1}, hi[3] = {N[0] - 2, N[1] - 2, N[2] - 2};
+    hi[ax] = N[ax] - 1;
+    for (int xi = lo[0]; xi <= hi[0]; ++xi)
+      for (int yi = lo[1]; yi <= hi[1]; ++yi)
+        for (int zi = lo[2]; zi <= hi[2]; ++zi) (*V[ax])(xi, yi, zi) += adt;
+  }
+}
+
+// ------------------------------------------------------------ solver
+// CGSolver<3>, PressureSolver.cpp:476-934.
+// pinned by: symmetry / row sums / SPD of A (tests), scipy FP64 direct solves
+// on small grids, and the projection invariants.
+float approx_phi(const Model& m, int xi, int yi, int zi, int level) {  // :476-494
+  if (level == 0) return m.liquidPhi(xi, yi, zi);
+  int f = 1 << level;
+  int bx = f * xi + (1 << (level - 1)) - 1, by = f * yi + (1 << (level - 1)) - 1, bz = f * zi + (1 << (level - 1)) - 1;
+  const A3& p = m.liquidPhi;
+  return trilerp(p(bx, by, bz), p(bx + 1, by, bz), p(bx, by + 1, bz), p(bx + 1, by + 1, bz), p(bx, by, bz + 1),
+                 p(bx + 1, by, bz + 1), p(bx, by + 1, bz + 1), p(bx + 1, by + 1, bz + 1), 0.5f, 0.5f, 0.5f);
+}
+float approx_w(const Model& m, int a, int xi, int yi, int zi, int level) {  // :496-542
+  const A3& w = (a == 0) ? m.lwx : (a == 1) ? m.lwy : m.lwz;
+  if (level == 0) return w(xi, yi, zi);
+  int f = 1 << level, h = (1 << (level - 1)) - 1;
+  if (a == 0) {
+    int bx = f * xi, by = f * yi + h, bz = f * zi + h;
+    return bilerp(w(bx, by, bz), w(bx, by + 1, bz), w(bx, by, bz + 1), w(bx, by + 1, bz + 1), 0.5f, 0.5f);
+  } else if (a == 1) {
+    int bx = f * xi + h, by = f * yi, bz = f * zi + h;
+    return bilerp(w(bx, by, bz), w(bx + 1, by, bz), w(bx, by, bz + 1), w(bx + 1, by, bz + 1), 0.5f, 0.5f);
+  }
+  int bx = f * xi + h, by = f * yi + h, bz = f * zi;
+  return bilerp(w(bx, by, bz), w(bx + 1, by, bz), w(bx, by + 1, bz), w(bx + 1, by + 1, bz), 0.5f, 0.5f);
+}
+
+// BuildLHS, :544-664
+void build_lhs(Model& m, float rho, float dt) {
+  for (int l = 0; l < m.max_level; ++l) {
+    Level& q = m.lv[l];
+    int f = 1 << l;
+    float dxl[3] = {(float)f * m.dx[0], (float)f * m.dx[1], (float)f * m.dx[2]};
+#pragma omp parallel for
+    for (int xi = 0; xi < q.d[0]; ++xi)
+      for (int yi = 0; yi < q.d[1]; ++yi)
+        for (int zi = 0; zi < q.d[2]; ++zi) {
+          float cphi = approx_phi(m, xi, yi, zi, l);
+          if (cphi > 0.0f) continue;
+          float& diag = q.adiag(xi, yi, zi);
+          float* co = &q.acoef[q.adiag.id(xi, yi, zi) * 6];
+          // (face index k, axis, neighbour offset, face index for W)
+          struct F { int axis, s; };
+          const F faces[6] = {{0, +1}, {0, -1}, {1, +1}, {1, -1}, {2, +1}, {2, -1}};
+          for (int k = 0; k < 6; ++k) {
+            int ax = faces[k].axis, s = faces[k].s;
+            int c[3] = {xi, yi, zi};
+            if (s > 0 ? !(c[ax] + 1 < q.d[ax]) : !(c[ax] - 1 >= 0)) continue;
+            int fc[3] = {xi, yi, zi};
+            if (s > 0) fc[ax] += 1;
+            float wv = approx_w(m, ax, fc[0], fc[1], fc[2], l);
+            if (!(wv > 0.0f)) continue;
+            float term = wv * dt / (rho * dxl[ax] * dxl[ax]);
+            int nb[3] = {xi, yi, zi};
+            nb[ax] += s;
+            float phi = approx_phi(m, nb[0], nb[1], nb[2], l);
+            if (phi < 0.0f) {
+              co[k] -= term;
+              diag += term;
+            } else {
+              float frac = max_tenth(fraction_in_region(cphi, phi));
+              diag += term / frac;
+            }
+          }
+        }
+  }
+}
+
+// BuildRHS, :682-698
+void build_rhs(Model& m) {
+  const int* N = m.N;
+#pragma omp parallel for
+  for (int xi = 0; xi < N[0]; ++xi)
+    for (int yi = 0; yi < N[1]; ++yi)
+      for (int zi = 0; zi < N[2]; ++zi) {
+        float& b = m.rhs(xi, yi, zi);
+        b = 0.0f;
+        if (m.liquidPhi(xi, yi, zi) > 0.0f) continue;
+        b -= m.lwx(xi + 1, yi, zi) * m.vx(xi + 1, yi, zi) / m.dx[0];
+        b += m.lwx(xi, yi, zi) * m.vx(xi, yi, zi) / m.dx[0];
+        b -= m.lwy(xi, yi + 1, zi) * m.vy(xi, yi + 1, zi) / m.dx[1];
+        b += m.lwy(xi, yi, zi) * m.vy(xi, yi, zi) / m.dx[1];
+        b -= m.lwz(xi, yi, zi + 1) * m.vz(xi, yi, zi + 1) / m.dx[2];
+        b += m.lwz(xi, yi, zi) * m.vz(xi, yi, zi) / m.dx[2];
+      }
+}
+
+// GetAx, :700-717 (summation order: diag, +x, -x, +y, -y, +z, -z)
+template <class T>
+inline T get_ax(const Level& q, const std::vector<T>& g, int xi, int yi, int zi) {
+  auto I = [&](int x, int y, int z) { return ((size_t)x * q.d[1] + y) * q.d[2] + z; };
+  size_t c = I(xi, yi, zi);
+  const float* co = &q.acoef[c * 6];
+  T res = g[c] * (T)q.adiag.v[c];
+  if (xi + 1 < q.d[0]) res += g[I(xi + 1, yi, zi)] * (T)co[0];
+  if (xi - 1 >= 0) res += g[I(xi - 1, yi, zi)] * (T)co[1];
+  if (yi + 1 < q.d[1]) res += g[I(xi, yi + 1, zi)] * (T)co[2];
+  if (yi - 1 >= 0) res += g[I(xi, yi - 1, zi)] * (T)co[3];
+  if (zi + 1 < q.d[2]) res += g[I(xi, yi, zi + 1)] * (T)co[4];
+  if (zi - 1 >= 0) res += g[I(xi, yi, zi - 1)] * (T)co[5];
+  return res;
+}
+
+// Solver working vectors in precision T (float: the reference path; double:
+// the FP64 oracle of the refined mode, on the same FP32 coefficients).
+template <class T>
+struct SolveVecs {
+  std::vector<std::vector<T>> r, z;
+  std::vector<T> p, x, q;
+};
+
+template <class T>
+void smooth(const Model& m, SolveVecs<T>& v, int l, int phase, int mode) {  // :821-837
+  const Level& q = m.lv[l];
+  const double w = (double)m.prm.omega;
+  std::vector<T>& z = v.z[l];
+  const std::vector<T>& r = v.r[l];
+#pragma omp parallel for
+  for (int xi = 0; xi < q.d[0]; ++xi)
+    for (int yi = 0; yi < q.d[1]; ++yi)
+      for (int zi = 0; zi < q.d[2]; ++zi) {
+        if (((xi + yi + zi) & 1) != phase) continue;
+        size_t c = ((size_t)xi * q.d[1] + yi) * q.d[2] + zi;
+        float ad = q.adiag.v[c];
+        if (is_small(ad)) continue;
+        T term = (r[c] - get_ax(q, z, xi, yi, zi)) / (T)ad;
+        if (mode == 0) {
+          // (1. - w) * z + w * term: double * double + (float w * term) promoted
+          T wt = (T)m.prm.omega * term;
+          z[c] = (T)((1.0 - w) * (double)z[c] + (double)wt);
+        } else {
+          z[c] = z[c] + (T)m.prm.omega * term;
+        }
+      }
+}
+
+template <class T>
+void restrict_(const Model& m, SolveVecs<T>& v, int to) {  // :796-819
+  const Level& qc = m.lv[to];
+  const Level& qf = m.lv[to - 1];
+  std::vector<T>& rc = v.r[to];
+  std::vector<T>& zc = v.z[to];
+  const std::vector<T>& rf = v.r[to - 1];
+  const std::vector<T>& zf = v.z[to - 1];
+#pragma omp parallel for
+  for (int xi = 0; xi < qc.d[0]; ++xi)
+    for (int yi = 0; yi < qc.d[1]; ++yi)
+      for (int zi = 0; zi < qc.d[2]; ++zi) {
+        size_t c = ((size_t)xi * qc.d[1] + yi) * qc.d[2] + zi;
+        T acc = 0;
+        int bx = xi << 1, by = yi << 1, bz = zi << 1;
+        const int ord[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1}, {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+        for (int k = 0; k < 8; ++k) {
+          int x = bx + ord[k][0], y = by + ord[k][1], z = bz + ord[k][2];
+          size_t f = ((size_t)x * qf.d[1] + y) * qf.d[2] + z;
+          // 0.125 (double) * float difference, added to the float accumulator
+          T d = rf[f] - get_ax(qf, zf, x, y, z);
+          acc = (T)((double)acc + 0.125 * (double)d);
+        }
+        rc[c] = acc;
+        zc[c] = 0;
+      }
+}
+
+template <class T>
+void prolong(const Model& m, SolveVecs<T>& v, int to) {  // :774-794
+  const Level& qc = m.lv[to + 1];
+  const Level& qf = m.lv[to];
+  std::vector<T>& zf = v.z[to];
+  const std::vector<T>& zc = v.z[to + 1];
+#pragma omp parallel for
+  for (int xi = 0; xi < qc.d[0]; ++xi)
+    for (int yi = 0; yi < qc.d[1]; ++yi)
+      for (int zi = 0; zi < qc.d[2]; ++zi) {
+        T val = zc[((size_t)xi * qc.d[1] + yi) * qc.d[2] + zi];
+        for (int k = 0; k < 8; ++k) {
+          int x = 2 * xi + ((k >> 2) & 1), y = 2 * yi + ((k >> 1) & 1), z = 2 * zi + (k & 1);
+          zf[((size_t)x * qf.d[1] + y) * qf.d[2] + z] += val;
+        }
+      }
+}
+
+template <class T>
+void apply_precond(const Model& m, SolveVecs<T>& v, int smoother) {  // :719-772
+  const A3& phi = m.liquidPhi;
+  if (m.prm.precond == 0) {
+    for (size_t c = 0; c < phi.v.size(); ++c)
+      if (!(phi.v[c] > 0.0f)) v.z[0][c] = v.r[0][c];
+    return;
+  }
+  if (m.prm.precond == 1) {
+    const Level& q = m.lv[0];
+    for (size_t c = 0; c < phi.v.size(); ++c) {
+      float dn = q.adiag.v[c];
+      if (is_small(dn)) continue;
+      v.z[0][c] = v.r[0][c] / (T)dn;
+    }
+    return;
+  }
+  const int L = m.max_level;
+  if (smoother == 1) std::fill(v.z[0].begin(), v.z[0].end(), (T)0);
+  for (int l = 1; l < L; ++l) {
+    for (int it = 0; it < m.prm.num_pre; ++it) {
+      smooth(m, v, l - 1, 0, smoother);
+      smooth(m, v, l - 1, 1, smoother);
+    }
+    restrict_(m, v, l);
+  }
+  for (int it = 0; it < m.prm.num_final; ++it) {
+    smooth(m, v, L - 1, 0, smoother);
+    smooth(m, v, L - 1, 1, smoother);
+  }
+  if (smoother == 1) {
+    for (int it = 0; it < m.prm.num_final; ++it) {
+      smooth(m, v, L - 1, 1, smoother);
+      smooth(m, v, L - 1, 0, smoother);
+    }
+  }
+  for (int l = L - 2; l >= 0; --l) {
+    prolong(m, v, l);
+    for (int it = 0; it < m.prm.num_post; ++it) {
+      if (smoother == 1) {
+        smooth(m, v, l, 1, smoother);
+        smooth(m, v, l, 0, smoother);
+      } else {
+        smooth(m, v, l, 0, smoother);
+        smooth(m, v, l, 1, smoother);
+      }
+    }
+  }
+}
+
+// Array3::Reduce (Array.h:111-122) in the chosen accumulation precision.
+template <class T>
+T reduce(const std::vector<T>& a, const std::vector<T>& b, int dot_mode) {
+  if (dot_mode == 0) {
+    T s = 0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+  }
+  double s = 0;
+  for (size_t i = 0; i < a.size(); ++i) s += (double)a[i] * (double)b[i];
+  return (T)s;
+}
+
+// CGSolver<3>::Solve core loop, :839-934, on prebuilt A and m.rhs.
+// x0 (optional) is a warm start in the same precision; the reference starts
+// from x = 0 (Init, :666-680).
+template <class T>
+void pcg(Model& m, SolveVecs<T>& v, const std::vector<T>& b, std::vector<T>& xout, int smoother, int dot_mode,
+         int tol_mode, double tol, int max_iter, orc_solve_stats* st) {
+  const A3& phi = m.liquidPhi;
+  const int* N = m.N;
+  const int L = m.max_level;
+  v.r.assign(L, {}); v.z.assign(L, {});
+  for (int l = 0; l < L; ++l) {
+    size_t n = m.lv[l].r.v.size();
+    v.r[l].assign(n, (T)0); v.z[l].assign(n, (T)0);
+  }
+  size_t n0 = phi.v.size();
+  v.p.assign(n0, (T)0);
+  v.x.assign(n0, (T)0);
+  v.q.assign(n0, (T)0);
+  for (size_t c = 0; c < n0; ++c)
+    if (!(phi.v[c] > 0.0f)) v.r[0][c] = b[c];
+  apply_precond(m, v, smoother);
+  for (size_t c = 0; c < n0; ++c)
+    if (!(phi.v[c] > 0.0f)) v.p[c] = v.z[0][c];
+  T oldRZ = reduce(v.r[0], v.z[0], dot_mode);
+  T residual = std::sqrt(reduce(v.r[0], v.r[0], dot_mode));
+  const T res0 = residual;
+  const Level& q0 = m.lv[0];
+  int it = 0;
+  bool conv = false;
+  auto done = [&](T res) {
+    if (tol_mode == 0) return res < (T)tol;
+    return (double)res <= tol * (double)res0;
+  };
+  for (it = 0; it < max_iter; ++it) {
+    if (done(residual)) { conv = true; break; }
+    // denom = sum p.Ap over interior liquid cells (reduction added)
+    double dd = 0;
+    T df = 0;
+    for (int xi = 1; xi < N[0] - 1; ++xi)
+      for (int yi = 1; yi < N[1] - 1; ++yi)
+        for (int zi = 1; zi < N[2] - 1; ++zi) {
+          size_t c = phi.id(xi, yi, zi);
+          if (phi.v[c] > 0.0f) continue;
+          T ap = get_ax(q0, v.p, xi, yi, zi);
+          v.q[c] = ap;
+          if (dot_mode == 0) df += v.p[c] * ap;
+          else dd += (double)v.p[c] * (double)ap;
+        }
+    T denom = (dot_mode == 0) ? df : (T)dd;
+    T alpha = oldRZ / denom;
+    if (std::isnan((double)alpha) || std::isinf((double)alpha)) break;
+    double rd = 0;
+    T rf = 0;
+    for (int xi = 1; xi < N[0] - 1; ++xi)
+      for (int yi = 1; yi < N[1] - 1; ++yi)
+        for (int zi = 1; zi < N[2] - 1; ++zi) {
+          size_t c = phi.id(xi, yi, zi);
+          if (phi.v[c] > 0.0f) continue;
+          v.x[c] += alpha * v.p[c];
+          v.r[0][c] -= alpha * v.q[c];
+          if (dot_mode == 0) rf += v.r[0][c] * v.r[0][c];
+          else rd += (double)v.r[0][c] * (double)v.r[0][c];
+        }
+    residual = std::sqrt((dot_mode == 0) ? rf : (T)rd);
+    apply_precond(m, v, smoother);
+    T newRZ = reduce(v.r[0], v.z[0], dot_mode);
+    T beta = newRZ / oldRZ;
+    if (std::isnan((double)beta) || std::isinf((double)beta)) { ++it; break; }
+    for (size_t c = 0; c < n0; ++c)
+      if (!(phi.v[c] > 0.0f)) v.p[c] = v.z[0][c] + beta * v.p[c];
+    oldRZ = newRZ;
+  }
+  if (!conv && it >= max_iter) conv = done(residual);
+  xout = v.x;
+  if (st) {
+    st->iterations = it;
+    st->converged = conv ? 1 : 0;
+    st->initial_residual = (double)res0;
+    st->final_residual = (double)residual;
+    st->rhs_norm = (double)res0;
+  }
+}
+
+void init_solver(Model& m) {  // Init, :666-680
+  for (int l = 0; l < m.max_level; ++l) {
+    Level& q = m.lv[l];
+    q.r.fill(0.f); q.z.fill(0.f); q.adiag.fill(0.f);
+    std::fill(q.acoef.begin(), q.acoef.end(), 0.f);
+  }
+  m.p.fill(0.f); m.x.fill(0.f); m.rhs.fill(0.f);
+}
+
+int solve_pressure_only(Model& m, float dt, orc_solve_stats* st) {
+  init_solver(m);
+  build_rhs(m);
+  build_lhs(m, m.prm.density, dt);
+  SolveVecs<float> v;
+  std::vector<float> xo;
+  pcg<float>(m, v, m.rhs.v, xo, m.prm.smoother, m.prm.dot_mode, m.prm.tol_mode, (double)m.prm.tol, m.prm.max_iter, st);
+  m.pressure.v = xo;  // m_grid.pressure.CopyFrom(m_x), :933
+  return 0;
+}
+
+// LiquidModel<3>::SolvePressure velocity update, LiquidModel.cpp:834-889
+void velocity_update(Model& m, float dt) {
+  const int* N = m.N;
+  const float rho = m.prm.density;
+#pragma omp parallel for
+  for (int xi = 1; xi < N[0] - 1; ++xi)
+    for (int yi = 1; yi < N[1] - 1; ++yi)
+      for (int zi = 1; zi < N[2] - 1; ++zi) {
+        float cphi = m.liquidPhi(xi, yi, zi);
+        A3* V[3] = {&m.vx, &m.vy, &m.vz};
+        const A3* W[3] = {&m.lwx, &m.lwy, &m.lwz};
+        for (int a = 0; a < 3; ++a) {
+          int nb[3] = {xi, yi, zi};
+          nb[a] -= 1;
+          if ((*W[a])(xi, yi, zi) > 0.0f) {
+            float phi = m.liquidPhi(nb[0], nb[1], nb[2]);
+            if (cphi < 0.0f || phi < 0.0f) {
+              float frac = 1.0f;
+              if (cphi >= 0.0f || phi >= 0.0f) frac = max_tenth(fraction_in_region(cphi, phi));
+              float gp = (m.pressure(xi, yi, zi) - m.pressure(nb[0], nb[1], nb[2])) / (m.dx[a] * frac);
+              (*V[a])(xi, yi, zi) -= gp * dt / rho;
+            }
+          } else {
+            (*V[a])(xi, yi, zi) = 0.0f;
+          }
+        }
+      }
+}
+
+// LiquidModel<3>::Extrapolate, LiquidModel.cpp:937-1063 (sequential, Q6).
+// pinned by: hand-checked small cases (validity chains along the scan order)
+// in tests.
+void extrapolate(Model& m) {
+  if (m.P == 0) return;
+  const int* N = m.N;
+  A3* V[3] = {&m.vx, &m.vy, &m.vz};
+  A3* T[3] = {&m.tvx, &m.tvy, &m.tvz};
+  const A3* W[3] = {&m.lwx, &m.lwy, &m.lwz};
+  for (int a = 0; a < 3; ++a) {
+    int lo[3] = {1, 1, 1}, hi[3] = {N[0] - 2, N[1] - 2, N[2] - 2};
+    hi[a] = N[a] - 1;
+    for (int xi = lo[0]; xi <= hi[0]; ++xi)
+      for (int yi = lo[1]; yi <= hi[1]; ++yi)
+        for (int zi = lo[2]; zi <= hi[2]; ++zi) m.isValid[vid(m, xi, yi, zi)] = ((*W[a])(xi, yi, zi) > 0.0f) ? 1 : 0;
+    for (int iter = 0; iter < 5; ++iter) {
+      A3& v = *V[a];
+      A3& t = *T[a];
+      for (int xi = lo[0]; xi <= hi[0]; ++xi)
+        for (int yi = lo[1]; yi <= hi[1]; ++yi)
+          for (int zi = lo[2]; zi <= hi[2]; ++zi) {
+            t(xi, yi, zi) = v(xi, yi, zi);
+            if (m.isValid[vid(m, xi, yi, zi)]) continue;
+            float sum = 0.0f;
+            int count = 0;
+            for (int ddx = -1; ddx <= 1; ++ddx)
+              for (int ddy = -1; ddy <= 1; ++ddy)
+                for (int ddz = -1; ddz <= 1; ++ddz)
+                  if (m.isValid[vid(m, xi + ddx, yi + ddy, zi + ddz)]) {
+                    sum += v(xi + ddx, yi + ddy, zi + ddz);
+                    ++count;
+                  }
+            if (count > 0) {
+              t(xi, yi, zi) = sum / (float)count;
+              m.isValid[vid(m, xi, yi, zi)] = 1;
+            }
+          }
+      std::swap(V[a]->v, T[a]->v);  // std::swap(tempV, v)
+    }
+  }
+}
+
+// LiquidModel<3>::Grid2Particle, LiquidModel.cpp:723-823.
+// pinned by: uniform field -> exact v; FP64 brute force; frame-1 KAT.
+void g2p(Model& m) {
+  if (m.P == 0) return;
+  const int* N = m.N;
+#pragma omp parallel for
+  for (i64 pi = 0; pi < m.P; ++pi) {
+    const float* pp = &m.pos[3 * pi];
+    float g[3] = {(pp[0] - m.mn[0]) / m.dx[0], (pp[1] - m.mn[1]) / m.dx[1], (pp[2] - m.mn[2]) / m.dx[2]};
+    int ci[3] = {(int)g[0], (int)g[1], (int)g[2]};
+    float r = m.radius[pi];
+    int dd[3] = {(int)std::ceil(r / m.dx[0]), (int)std::ceil(r / m.dx[1]), (int)std::ceil(r / m.dx[2])};
+    float vel[3] = {0.f, 0.f, 0.f};
+    float Cm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const A3* V[3] = {&m.vx, &m.vy, &m.vz};
+    for (int a = 0; a < 3; ++a) {
+      int hi[3] = {N[0] - 1, N[1] - 1, N[2] - 1};
+      hi[a] += 1;
+      for (int ddx = -dd[0]; ddx <= dd[0]; ++ddx)
+        for (int ddy = -dd[1]; ddy <= dd[1]; ++ddy)
+          for (int ddz = -dd[2]; ddz <= dd[2]; ++ddz) {
+            int xi = ci[0] + ddx, yi = ci[1] + ddy, zi = ci[2] + ddz;
+            if (xi < 0 || xi > hi[0] || yi < 0 || yi > hi[1] || zi < 0 || zi > hi[2]) continue;
+            float cc[3] = {(float)xi, (float)yi, (float)zi};
+            for (int k = 0; k < 3; ++k)
+              if (k != a) cc[k] += 0.5f;
+            float wX = quad_w(g[0] - cc[0]), wY = quad_w(g[1] - cc[1]), wZ = quad_w(g[2] - cc[2]);
+            float gX = quad_gw(g[0] - cc[0]), gY = quad_gw(g[1] - cc[1]), gZ = quad_gw(g[2] - cc[2]);
+            float weight = wX * wY * wZ;
+            float gw0 = gX * wY * wZ, gw1 = wX * gY * wZ, gw2 = wX * wY * gZ;
+            float fv = (*V[a])(xi, yi, zi);
+            vel[a] += fv * weight;
+            Cm[3 * a + 0] += fv * gw0;
+            Cm[3 * a + 1] += fv * gw1;
+            Cm[3 * a + 2] += fv * gw2;
+          }
+    }
+    for (int a = 0; a < 3; ++a) m.vel[3 * pi + a] = vel[a];
+    for (int k = 0; k < 9; ++k) m.C[9 * pi + k] = Cm[k];
+  }
+}
+
+// LiquidModel<3>::AdvectParticle, LiquidModel.cpp:912-935 (the velocity
+// reflection is dead code in the reference, Q7).
+void advect(Model& m, float dt) {
+  if (m.P == 0) return;
+#pragma omp parallel for
+  for (i64 pi = 0; pi < m.P; ++pi) {
+    float* pp = &m.pos[3 * pi];
+    const float* v = &m.vel[3 * pi];
+    float p0 = pp[0] + v[0] * dt, p1 = pp[1] + v[1] * dt, p2 = pp[2] + v[2] * dt;
+    float sp = solid_phi_at(m, p0, p1, p2);
+    if (sp < 0.0f) {
+      float g[3];
+      solid_gradient(m, p0, p1, p2, g);
+      float s = sp + kEps;
+      p0 -= s * g[0]; p1 -= s * g[1]; p2 -= s * g[2];
+    }
+    pp[0] = p0; pp[1] = p1; pp[2] = p2;
+  }
+}
+
+// LiquidModel::GetCFL, LiquidModel.h:49-55
+float get_cfl(const Model& m, float* maxv_out) {
+  float maxVel = 0.0f;
+  for (i64 pi = 0; pi < m.P; ++pi) {
+    const float* v = &m.vel[3 * pi];
+    float c = max3(std::fabs(v[0]), std::fabs(v[1]), std::fabs(v[2]));
+    maxVel = (maxVel > c) ? maxVel : c;
+  }
+  if (maxv_out) *maxv_out = maxVel;
+  return clampf(m.dx[0] / maxVel, 0.001f, 0.1f);
+}
+
+// LiquidModel<3>::ComputeDivergence, LiquidModel.cpp:1065-1083 (+ L-inf).
+void divergence(const Model& m, double* mean, double* linf, int* count) {
+  const int* N = m.N;
+  float dv = 0.0f;
+  int cnt = 0;
+  float mx = 0.0f;
+  for (int xi = 0; xi < N[0]; ++xi)
+    for (int yi = 0; yi < N[1]; ++yi)
+      for (int zi = 0; zi < N[2]; ++zi) {
+        if (m.liquidPhi(xi, yi, zi) > 0.0f) continue;
+        ++cnt;
+        float d = m.lwx(xi + 1, yi, zi) * m.vx(xi + 1, yi, zi) - m.lwx(xi, yi, zi) * m.vx(xi, yi, zi);
+        d += m.lwy(xi, yi + 1, zi) * m.vy(xi, yi + 1, zi) - m.lwy(xi, yi, zi) * m.vy(xi, yi, zi);
+        d += m.lwz(xi, yi, zi + 1) * m.vz(xi, yi, zi + 1) - m.lwz(xi, yi, zi) * m.vz(xi, yi, zi);
+        d /= m.dx[0];
+        dv += d;
+        float ad = std::fabs(d);
+        mx = (ad > mx) ? ad : mx;
+      }
+  if (mean) *mean = (double)(dv / (float)cnt);
+  if (linf) *linf = (double)mx;
+  if (count) *count = cnt;
+}
+
+// ------------------------------------------------------------ seeding
+// ParticleSystem<3>::SpawnParticles, ParticleSystem.cpp:44-64 (lattice at
+// spacing 0.5*defaultRadius = dx, truncated at numParticles with the z-loop break).
+void spawn_lattice(Model& m, const float lo[3], const float hi[3], i64 cap) {
+  float interval = (float)(0.5 * (double)(2.0f * m.dx[0]));
+  int d[3];
+  for (int a = 0; a < 3; ++a) d[a] = (int)((hi[a] - lo[a]) / interval);
+  i64 total = (i64)d[0] * d[1] * d[2];
+  i64 n = std::min(cap, total);
+  m.P = n;
+  m.pos.assign(3 * n, 0.f); m.vel.assign(3 * n, 0.f); m.C.assign(9 * n, 0.f);
+  m.radius.assign(n, 2.0f * m.dx[0]);
+  for (int xi = 0; xi < d[0]; ++xi)
+    for (int yi = 0; yi < d[1]; ++yi)
+      for (int zi = 0; zi < d[2]; ++zi) {
+        i64 idx = (i64)xi * d[1] * d[2] + (i64)yi * d[2] + zi;
+        if (idx >= n) break;
+        m.pos[3 * idx] = (float)xi * interval + lo[0];
+        m.pos[3 * idx + 1] = (float)yi * interval + lo[1];
+        m.pos[3 * idx + 2] = (float)zi * interval + lo[2];
+      }
+}
+
+// Jittered block seeding (no reference counterpart): each cell whose centre
+// lies in the block gets k^3 particles, one per sub-cell, at
+// min + ((k*i + s) + u) * (dx/k) with u from a SplitMix64 counter hash, and
+// velocity v0 + sigma * N(0,1) (Box-Muller, FP64) per component.  Particles
+// with SDF <= 0 (inside solids) are dropped.
+inline uint64_t splitmix(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline uint64_t rng_u64(uint64_t seed, uint64_t block, uint64_t cell, uint64_t sub, uint64_t stream) {
+  uint64_t h = splitmix(seed);
+  h = splitmix(h ^ block);
+  h = splitmix(h ^ cell);
+  h = splitmix(h ^ (sub * 16 + stream));
+  return h;
+}
+inline float rng_unit24(uint64_t h) { return (float)(h >> 40) * (1.0f / 16777216.0f); }
+inline double rng_normal(uint64_t h1, uint64_t h2) {
+  double u1 = ((double)(h1 >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+  double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+}
+
+}  // namespace
+
+// ====================================================================== C API
+extern "C" {
+
+void* orc_create(const orc_params* prm) {
+  Model* m = new Model();
+  m->prm = *prm;
+  if (prm->nsdf > 0 && prm->sdf) m->sdf.assign(prm->sdf, prm->sdf + 12 * prm->nsdf);
+  m->prm.sdf = nullptr;
+  for (int a = 0; a < 3; ++a) { m->N[a] = prm->N[a]; m->mn[a] = prm->mn[a]; m->mx[a] = prm->mx[a]; }
+  model_alloc(*m);
+  return m;
+}
+void orc_destroy(void* h) { delete (Model*)h; }
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+int orc_max_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+void orc_grid_info(void* h, int dims[3], float mn[3], float dx[3]) {
+  Model* m = (Model*)h;
+  for (int a = 0; a < 3; ++a) { dims[a] = m->N[a]; mn[a] = m->mn[a]; dx[a] = m->dx[a]; }
+}
+
+static A3* field_ptr(Model* m, int f) {
+  switch (f) {
+    case 0: return &m->vx; case 1: return &m->vy; case 2: return &m->vz;
+    case 3: return &m->tvx; case 4: return &m->tvy; case 5: return &m->tvz;
+    case 6: return &m->lwx; case 7: return &m->lwy; case 8: return &m->lwz;
+    case 9: return &m->pressure; case 10: return &m->liquidPhi; case 11: return &m->solidPhi;
+    case 13: return &m->rhs;
+    case 100: return &m->wx; case 101: return &m->wy; case 102: return &m->wz;
+    default: return nullptr;
+  }
+}
+int orc_field(void* h, int f, void* buf, long long count, int set) {
+  Model* m = (Model*)h;
+  if (f == 12) {
+    if ((size_t)count != m->isValid.size()) return 6;
+    if (set) std::memcpy(m->isValid.data(), buf, count);
+    else std::memcpy(buf, m->isValid.data(), count);
+    return 0;
+  }
+  A3* a = field_ptr(m, f);
+  if (!a || (size_t)count != a->v.size()) return 6;
+  if (set) std::memcpy(a->v.data(), buf, count * 4);
+  else std::memcpy(buf, a->v.data(), count * 4);
+  return 0;
+}
+int orc_set_particles(void* h, const float* pos, const float* vel, const float* C, long long n) {
+  Model* m = (Model*)h;
+  m->P = n;
+  m->pos.assign(pos, pos + 3 * n);
+  if (vel) m->vel.assign(vel, vel + 3 * n); else m->vel.assign(3 * n, 0.f);
+  if (C) m->C.assign(C, C + 9 * n); else m->C.assign(9 * n, 0.f);
+  m->radius.assign(n, 2.0f * m->dx[0]);  // m_defaultRadius = 2 * dx (ParticleSystem.cpp:6-8)
+  return 0;
+}
+long long orc_num_particles(void* h) { return ((Model*)h)->P; }
+int orc_get_particles(void* h, float* pos, float* vel, float* C, long long n) {
+  Model* m = (Model*)h;
+  if (n != m->P) return 6;
+  if (pos) std::memcpy(pos, m->pos.data(), 12 * n);
+  if (vel) std::memcpy(vel, m->vel.data(), 12 * n);
+  if (C) std::memcpy(C, m->C.data(), 36 * n);
+  return 0;
+}
+void orc_set_time(void* h, double t) { ((Model*)h)->time = t; }
+double orc_get_time(void* h) { return ((Model*)h)->time; }
+
+void orc_spawn_lattice(void* h, const float lo[3], const float hi[3], long long cap) {
+  spawn_lattice(*(Model*)h, lo, hi, cap);
+}
+// blocks: 10 floats each: min[3], max[3], vel[3], sigma
+long long orc_seed_blocks(void* h, const float* blocks, int nblocks, int k, unsigned long long seed) {
+  Model* m = (Model*)h;
+  m->pos.clear(); m->vel.clear();
+  for (int b = 0; b < nblocks; ++b) {
+    const float* B = blocks + 10 * b;
+    // cells whose centre lies in [min, max] on every axis
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = m->N[a]; hi[a] = -1;
+      for (int i = 0; i < m->N[a]; ++i) {
+        float c = ((float)i + 0.5f) * m->dx[a] + m->mn[a];
+        if (c >= B[a] && c <= B[3 + a]) { lo[a] = std::min(lo[a], i); hi[a] = i; }
+      }
+    }
+    for (int xi = lo[0]; xi <= hi[0]; ++xi)
+      for (int yi = lo[1]; yi <= hi[1]; ++yi)
+        for (int zi = lo[2]; zi <= hi[2]; ++zi) {
+          uint64_t cell = ((uint64_t)xi * m->N[1] + yi) * m->N[2] + zi;
+          for (int s = 0; s < k * k * k; ++s) {
+            int sx = s / (k * k), sy = (s / k) % k, sz = s % k;
+            int cc[3] = {xi, yi, zi}, ss[3] = {sx, sy, sz};
+            float p[3];
+            for (int a = 0; a < 3; ++a) {
+              float u = rng_unit24(rng_u64(seed, b, cell, s, a));
+              p[a] = m->mn[a] + ((float)(k * cc[a] + ss[a]) + u) * (m->dx[a] / (float)k);
+            }
+            if (!(sdf_eval(m->sdf, p[0], p[1], p[2]) > 0.0f)) continue;
+            float v[3];
+            for (int a = 0; a < 3; ++a) {
+              v[a] = B[6 + a];
+              if (B[9] > 0.f) {
+                double nrm = rng_normal(rng_u64(seed, b, cell, s, 3 + 2 * a), rng_u64(seed, b, cell, s, 4 + 2 * a));
+                v[a] = (float)((double)B[6 + a] + (double)B[9] * nrm);
+              }
+            }
+            for (int a = 0; a < 3; ++a) { m->pos.push_back(p[a]); m->vel.push_back(v[a]); }
+          }
+        }
+  }
+  m->P = (i64)(m->pos.size() / 3);
+  m->C.assign(9 * m->P, 0.f);
+  m->radius.assign(m->P, 2.0f * m->dx[0]);
+  return m->P;
+}
+
+int orc_compute_solid_phi(void* h) { compute_solid_phi(*(Model*)h); return 0; }
+int orc_clear_grid(void* h) { clear_grid(*(Model*)h); return 0; }
+int orc_compute_liquid_phi(void* h) { compute_liquid_phi(*(Model*)h); return 0; }
+int orc_p2g(void* h, int order) {
+  Model& m = *(Model*)h;
+  if (m.P == 0) return 0;
+  if (order == 0) p2g_particle_order(m);
+  else if (!p2g_bin_order(m)) return 7;
+  p2g_normalize(m);
+  return 0;
+}
+int orc_apply_gravity(void* h, float dt) { apply_gravity(*(Model*)h, dt); return 0; }
+int orc_solve_pressure(void* h, float dt, orc_solve_stats* st) {
+  Model& m = *(Model*)h;
+  if (m.P == 0) return 0;
+  return solve_pressure_only(m, dt, st);
+}
+int orc_velocity_update(void* h, float dt) {
+  Model& m = *(Model*)h;
+  if (m.P == 0) return 0;
+  velocity_update(m, dt);
+  return 0;
+}
+int orc_extrapolate(void* h) { extrapolate(*(Model*)h); return 0; }
+int orc_g2p(void* h) { g2p(*(Model*)h); return 0; }
+int orc_advect(void* h, float dt) { advect(*(Model*)h, dt); return 0; }
+float orc_get_cfl(void* h, float* maxv) { return get_cfl(*(Model*)h, maxv); }
+void orc_divergence(void* h, double* mean, double* linf, int* count) { divergence(*(Model*)h, mean, linf, count); }
+float orc_solid_phi_at(void* h, float x, float y, float z) { return solid_phi_at(*(Model*)h, x, y, z); }
+void orc_solid_gradient(void* h, float x, float y, float z, float g[3]) { solid_gradient(*(Model*)h, x, y, z, g); }
+
+// one liquid step: PreStep + Step + PostStep (main.cpp:50-52)
+int orc_step(void* h, float dt, orc_solve_stats* st, double* div_mean, double* div_linf) {
+  Model& m = *(Model*)h;
+  clear_grid(m);
+  compute_liquid_phi(m);
+  int rc = orc_p2g(h, m.prm.p2g_order);
+  if (rc) return rc;
+  apply_gravity(m, dt);
+  if (m.P > 0) {
+    solve_pressure_only(m, dt, st);
+    velocity_update(m, dt);
+  }
+  extrapolate(m);
+  g2p(m);
+  advect(m, dt);
+  divergence(m, div_mean, div_linf, nullptr);
+  m.time += (double)dt;
+  return 0;
+}
+
+// Solver-level access for the refined / FP64 checks: build A from the
+// current liquidPhi / liquidW, then solve A x = b (b given) in FP32 (reference
+// loop) or FP64 (mode 1: symmetric MG, FP64 vectors, FP64 dots).
+int orc_build_system(void* h, float dt) {
+  Model& m = *(Model*)h;
+  init_solver(m);
+  build_rhs(m);
+  build_lhs(m, m.prm.density, dt);
+  return 0;
+}
+int orc_get_level(void* h, int level, float* adiag, float* acoef6, int dims[3]) {
+  Model& m = *(Model*)h;
+  if (level < 0 || level >= m.max_level) return 6;
+  Level& q = m.lv[level];
+  for (int a = 0; a < 3; ++a) dims[a] = q.d[a];
+  if (adiag) std::memcpy(adiag, q.adiag.v.data(), q.adiag.v.size() * 4);
+  if (acoef6) std::memcpy(acoef6, q.acoef.data(), q.acoef.size() * 4);
+  return 0;
+}
+int orc_num_levels(void* h) { return ((Model*)h)->max_level; }
+int orc_solve_fp32(void* h, const float* b, float* x, orc_solve_stats* st) {
+  Model& m = *(Model*)h;
+  SolveVecs<float> v;
+  std::vector<float> bb(b, b + m.liquidPhi.v.size()), xo;
+  pcg<float>(m, v, bb, xo, m.prm.smoother, m.prm.dot_mode, m.prm.tol_mode, (double)m.prm.tol, m.prm.max_iter, st);
+  std::memcpy(x, xo.data(), xo.size() * 4);
+  return 0;
+}
+int orc_solve_fp64(void* h, const double* b, double* x, double rel_tol, int max_iter, orc_solve_stats* st) {
+  Model& m = *(Model*)h;
+  SolveVecs<double> v;
+  std::vector<double> bb(b, b + m.liquidPhi.v.size()), xo;
+  pcg<double>(m, v, bb, xo, 1, 1, 1, rel_tol, max_iter, st);
+  std::memcpy(x, xo.data(), xo.size() * 8);
+  return 0;
+}
+// y = A x at level 0 (GetAx per cell), FP64 arithmetic on the FP32 coefficients
+void orc_apply_a64(void* h, const double* x, double* y) {
+  Model& m = *(Model*)h;
+  const Level& q = m.lv[0];
+  std::vector<double> xv(x, x + q.adiag.v.size());
+#pragma omp parallel for
+  for (int xi = 0; xi < q.d[0]; ++xi)
+    for (int yi = 0; yi < q.d[1]; ++yi)
+      for (int zi = 0; zi < q.d[2]; ++zi) y[q.adiag.id(xi, yi, zi)] = get_ax(q, xv, xi, yi, zi);
+}
+
+}  // extern "C"
+
+// ---- scalar helpers exported for closed-form tests
+extern "C" {
+float orc_quad_w(float d) { return quad_w(d); }
+float orc_quad_gw(float d) { return quad_gw(d); }
+float orc_fraction_in_region(float l, float r) { return fraction_in_region(l, r); }
+float orc_trilerp(const float* v, float f1, float f2, float f3) {
+  return trilerp(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], f1, f2, f3);
+}
+float orc_sdf_eval(const float* prog, int nnodes, float x, float y, float z) {
+  std::vector<float> p(prog, prog + 12 * nnodes);
+  return sdf_eval(p, x, y, z);
+}
+void orc_barycentric(float x, int lo, int hi, int* idx, float* frac) { barycentric(x, lo, hi, *idx, *frac); }
+}

@@ -1,0 +1,64 @@
+"""The oracle against the reference's own recorded run (tests/golden/liquiddata_kat.npz,
+extracted from LiquidData.csv by tests/golden/make_liquiddata_kat.py).
+
+* frame 0 is ParticleSystem<3>::SpawnParticles of Config/PureLiquid3D.json;
+* frame 1: every particle with index % 8 != 0 moved by exactly (0, -g dt^2, 0),
+  dt = 0.01 (main.cpp:32) -- P2G from rest, gravity on all interior faces, zero
+  pressure, G2P partition of unity, forward Euler.  (Class 0 was moved by
+  CorrectVolume in the recording, which the current source has disabled.)
+* frame 2: the centroid height, which CorrectVolume (a symmetric repulsion)
+  leaves unchanged to first order.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import config_file
+from oracle import Oracle
+
+FIX = os.path.join(os.path.dirname(__file__), "golden", "liquiddata_kat.npz")
+
+
+@pytest.fixture(scope="module")
+def kat():
+    return np.load(FIX)
+
+
+@pytest.fixture(scope="module")
+def run3():
+    o = Oracle(config_file("pure_liquid_3d.json"), dot_mode=0, p2g_order=0)
+    o.init()
+    frames = [o.particles()[0].copy()]
+    dt = np.float32(0.01)
+    stats = []
+    for _ in range(2):
+        st, dm, dl = o.step(dt)
+        stats.append(st)
+        frames.append(o.particles()[0].copy())
+        dt = np.float32(o.get_cfl()[0])
+    return frames, stats
+
+
+def test_frame0_is_spawn(kat, run3):
+    f0 = run3[0][0]
+    assert f0.shape == (5000, 3)
+    np.testing.assert_allclose(f0, kat["frame0"], rtol=0, atol=1e-6)
+
+
+def test_frame1_free_fall_known_answer(kat, run3):
+    frames, stats = run3
+    assert stats[0].iterations == 0          # b = 0: the solve exits at once
+    d = frames[1] - frames[0]
+    np.testing.assert_allclose(d, np.tile([0.0, -9.81e-4, 0.0], (5000, 1)), rtol=0, atol=1e-8)
+    moved_by_gravity_only = np.arange(5000) % 8 != 0
+    np.testing.assert_allclose(frames[1][moved_by_gravity_only], kat["frame1"][moved_by_gravity_only],
+                               rtol=0, atol=1e-6)
+
+
+def test_frame2_centroid(kat, run3):
+    frames, stats = run3
+    assert stats[1].iterations > 0 and stats[1].converged
+    assert frames[2][:, 1].mean() == pytest.approx(float(kat["centroid_y"][2]), abs=1e-5)
+    # per-particle agreement up to the CorrectVolume kicks of the recording
+    assert np.abs(frames[2] - kat["frame2"]).max() < 2e-3
