@@ -1,0 +1,144 @@
+"""Liquid-step benchmark on one MI355X.
+
+Workload (default): two-block collision on a 256^3 MAC grid (configs/collision_256.json):
+two 0.3^3 blocks at opposite corners of the unit cube, 8 jittered particles per
+cell, v = +-(2,0,0) + N(0, 0.05^2), solid walls 3 cells thick, MGPCG (3 levels,
+2/2 red-black Gauss-Seidel sweeps, symmetric V-cycle) to 1e-6 relative residual.
+One step = PreStep + Step + PostStep + GetCFL (main.cpp:50-53) run by the library's
+native loop; particles and grid stay resident in HBM for the whole timed region.
+
+Prints one JSON line: steps/s, the live roofline of the dominant kernel (HIP
+events around its launches inside the timed region) and the CPU baseline (the
+oracle, timed on this host for one step of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
+
+
+def algorithmic_bytes(kernel, n0, P):
+    """Bytes per launch counted once per array element the kernel must touch.
+
+    smooth_l0: one colour pass = half of a fused red-black sweep's 28 B/cell
+               (Adiag + 3 face terms = 16 B, r 4 B, z read 4 B + write 4 B);
+    spmv:      p read, q write, 4 coefficient floats -> 24 B/cell;
+    p2g:       one component: 20 B/particle (position, velocity component,
+               affine column) + 4 B per face written (~n0 faces);
+    g2p:       72 B/particle (read position, write position, velocity, C) + 3
+               velocity components read around the particles (~12 B/cell);
+    liquid_phi: 12 B/particle + 4 B/cell.
+    """
+    return {
+        "smooth_l0": 14.0 * n0,
+        "spmv": 24.0 * n0,
+        "p2g": 20.0 * P + 4.0 * n0,
+        "g2p": 72.0 * P + 12.0 * n0,
+        "liquid_phi": 12.0 * P + 4.0 * n0,
+    }[kernel]
+
+
+def cpu_baseline(cfg, steps=1):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    from oracle import Oracle
+
+    threads = os.cpu_count() or 1
+    orc = Oracle(cfg, dot_mode=1, p2g_order=1, threads=threads)
+    orc.init()
+    dt = np.float32(cfg.get("firstDt", 0.01))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        orc.step(dt)
+        dt = np.float32(orc.get_cfl()[0])
+    el = time.perf_counter() - t0
+    return {"value": steps / el, "unit": "steps/s", "cores": int(orc.L.orc_max_threads()), "kind": "port",
+            "sample": "%d step(s) of the same workload from its initial state, FP32 oracle restatement, "
+                      "OpenMP over the host cores (%.1f s)" % (steps, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default=os.path.join(ROOT, "liquidhairinteration_amd", "configs", "collision_256.json"))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import liquidhairinteration_amd as lh
+
+    cfg = json.load(open(args.config))
+    scene = lh.Scene(cfg)
+    model = scene.GetLiquidModel()
+    model.Init()
+    P = model.GetParticleSystem().NumParticles()
+    n0 = 1
+    for d in scene.GetGridDimension():
+        n0 *= d
+    dt = float(cfg.get("firstDt", 0.01))
+    if args.warmup > 0:
+        dt, _ = model.RunSteps(args.warmup, dt)
+    model.profile_enable(True)
+    t0 = time.perf_counter()
+    dt, last = model.RunSteps(args.steps, dt)
+    el = time.perf_counter() - t0
+    prof = {k: model.profile_read(k) for k in model.PROF_KERNELS}
+    model.profile_enable(False)
+
+    # dominant instrumented kernel by total device time in the timed region
+    dom = max(prof, key=lambda k: prof[k][1])
+    launches, tot_ms = prof[dom]
+    avg_s = tot_ms / 1e3 / max(launches, 1)
+    ach = algorithmic_bytes(dom, n0, P) / avg_s / 1e9
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "launches": launches,
+            "avg_launch_us": round(avg_s * 1e6, 2)}
+    kernels = {k: {"launches": v[0], "ms": round(v[1], 3),
+                   "pct_of_timed": round(100.0 * v[1] / (el * 1e3), 1)} for k, v in prof.items()}
+
+    out = {
+        "metric": "liquid steps/s at 256^3",
+        "value": round(args.steps / el, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered particle blocks)",
+        "config": {"workload": os.path.basename(args.config).replace(".json", ""),
+                   "grid": list(scene.GetGridDimension()), "particles": P,
+                   "solver": "MGPCG L=%d pre/post=%d/%d %s, tol %g %s" % (
+                       cfg["SolverParameter"]["numLevel"], cfg["SolverParameter"]["numPreSmooth"],
+                       cfg["SolverParameter"]["numPostSmooth"], cfg["SolverParameter"].get("smoother", "reference"),
+                       cfg["SolverParameter"]["tolerance"], cfg["SolverParameter"].get("toleranceMode", "absolute"))},
+        "roofline": roof,
+        "kernels": kernels,
+        "last_step": {"pcg_iterations": last["solve"]["iterations"], "converged": last["solve"]["converged"],
+                      "solve_ms": round(last["solve"]["solve_ms"], 3), "step_ms": round(last["step_ms"], 3),
+                      "divergence_linf": last["divergence_linf"], "sim_time": last["sim_time"]},
+        "cpu_baseline": None,
+    }
+    scene.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

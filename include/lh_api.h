@@ -134,6 +134,20 @@ int lh_get_particles(lh_ctx* ctx, float* pos, float* vel, float* C, int64_t n);
 /* count = number of elements of the field (uint8 for LH_FIELD_IS_VALID, float otherwise) */
 int lh_get_field(lh_ctx* ctx, int field, void* host, size_t count);
 int lh_set_field(lh_ctx* ctx, int field, const void* host, size_t count);
+/* ---- live kernel timing (HIP events on the context's stream) --------------
+ * No reference counterpart (the reference has no timers; it only
+ * prints residuals under _DEBUG, PressureSolver.cpp:925).  While enabled, the listed kernels are bracketed
+ * by hipEventRecord pairs; lh_profile_read returns launch count and summed ms. */
+enum lh_prof_kernel {
+  LH_PROF_SMOOTH_L0 = 0,   /* red-black smoother sweep on the finest level */
+  LH_PROF_P2G = 1,         /* particle-to-grid gather (one component per launch) */
+  LH_PROF_G2P = 2,         /* grid-to-particle + advection */
+  LH_PROF_LIQUID_PHI = 3,  /* liquid SDF gather */
+  LH_PROF_SPMV = 4,        /* PCG A*p + dot */
+  LH_PROF_COUNT = 5
+};
+int lh_profile_enable(lh_ctx* ctx, int on);
+int lh_profile_read(lh_ctx* ctx, int kernel, int* launches, double* total_ms);
 /* simulated time and the tank-acceleration clock */
 int lh_set_time(lh_ctx* ctx, double t);
 int lh_get_time(const lh_ctx* ctx, double* t);

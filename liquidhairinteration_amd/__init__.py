@@ -113,6 +113,8 @@ def load_library():
         "lh_get_field": ([vp, C.c_int, vp, C.c_size_t], C.c_int),
         "lh_set_field": ([vp, C.c_int, vp, C.c_size_t], C.c_int),
         "lh_set_time": ([vp, C.c_double], C.c_int),
+        "lh_profile_enable": ([vp, C.c_int], C.c_int),
+        "lh_profile_read": ([vp, C.c_int, ip, dp], C.c_int),
         "lh_get_time": ([vp, dp], C.c_int),
     }
     for name, (args, res) in sigs.items():
@@ -327,6 +329,18 @@ class LiquidModel:
         shp = self.field_shape(f)
         a = np.ascontiguousarray(arr, np.uint8 if f == FIELDS["isValid"] else np.float32).reshape(shp)
         self._h.check(self._h.L.lh_set_field(self._h.h, f, a.ctypes.data, a.size))
+
+    # live kernel timing (HIP events on the library's stream)
+    PROF_KERNELS = {"smooth_l0": 0, "p2g": 1, "g2p": 2, "liquid_phi": 3, "spmv": 4}
+
+    def profile_enable(self, on=True):
+        self._h.check(self._h.L.lh_profile_enable(self._h.h, 1 if on else 0))
+
+    def profile_read(self, kernel):
+        k = self.PROF_KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+        n, ms = C.c_int(), C.c_double()
+        self._h.check(self._h.L.lh_profile_read(self._h.h, k, C.byref(n), C.byref(ms)))
+        return int(n.value), float(ms.value)
 
     @property
     def time(self):

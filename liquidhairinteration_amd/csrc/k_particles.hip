@@ -455,6 +455,7 @@ int run_binning(Ctx& c) {
 
 void launch_liquid_phi(Ctx& c) {
   if (c.P == 0 || !c.bins_valid) return;
+  ProfScope ps(c, LH_PROF_LIQUID_PHI);
   hipLaunchKernelGGL(k_liquid_phi, dim3(grid_for(c.ncell)), dim3(kBlock), 0, c.s, c.phi, c.pbuf[c.cur], c.cap,
                      c.bin_start, c.g, c.H[1], c.H[2], c.dW[0], c.dW[1], c.dW[2], c.radius, 3 * c.g.dx[0]);
 }
@@ -479,7 +480,7 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
     q.a = a;
     P2GP qa = q;
     if (!fuse_body_forces || a != 1) qa.gdt = 0.f;
-    // gravity applies only to vy; keep the a==1 interior test for it
+    ProfScope ps(c, LH_PROF_P2G);
     hipLaunchKernelGGL(k_p2g, dim3(grid_for(c.nface[a])), dim3(kBlock), 0, c.s, c.v[a], c.pbuf[c.cur], c.cap,
                        c.bin_start, qa);
   }
@@ -493,6 +494,7 @@ void launch_g2p_advect(Ctx& c, bool do_g2p, bool do_advect, float dt) {
   q.dt = dt;
   for (int k = 0; k < 3; ++k) q.dW[k] = c.dW[k];
   unsigned gp = grid_for((size_t)c.P);
+  ProfScope ps(c, LH_PROF_G2P);
   hipLaunchKernelGGL(k_g2p_advect, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, c.v[0], c.v[1], c.v[2],
                      c.solid_phi, q, c.cfg.mx[0], c.cfg.mx[1], c.cfg.mx[2], do_g2p ? 1 : 0, do_advect ? 1 : 0,
                      c.dmaxv);

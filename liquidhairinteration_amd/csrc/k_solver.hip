@@ -434,6 +434,7 @@ namespace {
 
 void smooth(Ctx& c, int l, int phase, int mode) {
   Level& L = c.lv[l];
+  ProfScope ps(c, l == 0 ? LH_PROF_SMOOTH_L0 : LH_PROF_COUNT);
   hipLaunchKernelGGL(k_smooth, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), phase, c.cfg.omega, mode);
 }
 
@@ -528,7 +529,10 @@ int run_solve(Ctx& c, float dt, lh_solve_stats* st) {
       conv = true;
       break;
     }
-    hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.q, c.p, P0, c.partial);
+    {
+      ProfScope ps(c, LH_PROF_SPMV);
+      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.q, c.p, P0, c.partial);
+    }
     hipLaunchKernelGGL(k_sc_alpha, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dflag + 2);
     hipLaunchKernelGGL(k_cg_update_xr, dim3(nb), dim3(kBlock), 0, c.s, c.x, L0.r, c.p, c.q, L0.cls, c.dsc, L0.d[0],
                        L0.d[1], L0.d[2], c.partial);

@@ -266,6 +266,8 @@ void free_ctx(Ctx& c) {
   if (c.h_flag) hipHostFree(c.h_flag);
   if (c.ev0) hipEventDestroy(c.ev0);
   if (c.ev1) hipEventDestroy(c.ev1);
+  for (int k = 0; k < LH_PROF_COUNT; ++k)
+    for (auto e : c.prof_ev[k]) hipEventDestroy(e);
   if (c.s) hipStreamDestroy(c.s);
 }
 
@@ -834,6 +836,29 @@ int lh_set_field(lh_ctx* ctx, int field, const void* host, size_t count) {
     hipMemcpyAsync(field_dev(c, field), host, bytes, hipMemcpyHostToDevice, c.s);
     return hip_ok(c, hipStreamSynchronize(c.s), "set field") ? LH_OK : LH_ERR_DEVICE;
   });
+}
+
+int lh_profile_enable(lh_ctx* ctx, int on) {
+  if (!ctx) return LH_ERR_ARG;
+  Ctx& c = ctx->c;
+  hipStreamSynchronize(c.s);
+  c.prof = (on != 0);
+  for (int k = 0; k < LH_PROF_COUNT; ++k) c.prof_used[k] = 0;
+  return LH_OK;
+}
+
+int lh_profile_read(lh_ctx* ctx, int kernel, int* launches, double* total_ms) {
+  if (!ctx || kernel < 0 || kernel >= LH_PROF_COUNT) return LH_ERR_ARG;
+  Ctx& c = ctx->c;
+  if (!hip_ok(c, hipStreamSynchronize(c.s), "profile")) return LH_ERR_DEVICE;
+  double tot = 0.0;
+  for (int u = 0; u < c.prof_used[kernel]; ++u) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c.prof_ev[kernel][2 * u], c.prof_ev[kernel][2 * u + 1]) == hipSuccess) tot += ms;
+  }
+  if (launches) *launches = c.prof_used[kernel];
+  if (total_ms) *total_ms = tot;
+  return LH_OK;
 }
 
 int lh_set_time(lh_ctx* ctx, double t) {

@@ -104,7 +104,35 @@ struct Ctx {
   double time = 0.0;
   float last_maxv = 0.f;
   bool solid_ready = false;
+  // live kernel timing: event pairs per instrumented kernel
+  bool prof = false;
+  std::vector<hipEvent_t> prof_ev[LH_PROF_COUNT];
+  int prof_used[LH_PROF_COUNT] = {};
   std::string err;
+};
+
+// brackets one kernel launch with hipEventRecord pairs when c.prof is set
+struct ProfScope {
+  Ctx& c;
+  int k;
+  hipEvent_t e1 = nullptr;
+  ProfScope(Ctx& cc, int kk) : c(cc), k(kk) {
+    if (!c.prof || k < 0 || k >= LH_PROF_COUNT) return;
+    auto& v = c.prof_ev[k];
+    int u = c.prof_used[k];
+    while ((int)v.size() < 2 * (u + 1)) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      v.push_back(e);
+    }
+    hipEventRecord(v[2 * u], c.s);
+    e1 = v[2 * u + 1];
+  }
+  ~ProfScope() {
+    if (!e1) return;
+    hipEventRecord(e1, c.s);
+    c.prof_used[k]++;
+  }
 };
 
 // ---- launchers (k_grid.hip)

@@ -1,0 +1,197 @@
+"""HIP library vs the CPU oracle, stage by stage, on identical inputs (teacher forcing).
+
+Every stage of the liquid step has no order-dependent float reduction on the
+GPU except the PCG dot products, which accumulate in FP64 and round to float
+exactly like the oracle's dot_mode=1.  The expected result is therefore
+bit-exact for every field (np.array_equal treats -0 == +0).
+"""
+import json
+
+import numpy as np
+import pytest
+
+from conftest import config_file
+from oracle import Oracle
+
+lh = pytest.importorskip("liquidhairinteration_amd")
+
+pytestmark = pytest.mark.gpu
+
+F = {"vx": 0, "vy": 1, "vz": 2, "tempVx": 3, "tempVy": 4, "tempVz": 5, "liquidWx": 6, "liquidWy": 7, "liquidWz": 8,
+     "pressure": 9, "liquidPhi": 10, "solidPhi": 11, "isValid": 12, "rhs": 13}
+
+
+def small_dam(N=24, smoother="reference", tank=False):
+    cfg = json.load(open(config_file("dam_break_48.json")))
+    cfg["gridDimension"] = [N, N, N]
+    wall = 0.5 - 3.0 / N
+    cfg["Boundary"]["child"]["halfExtent"] = [wall] * 3
+    cfg["SolverParameter"]["smoother"] = smoother
+    if smoother == "symmetric":
+        cfg["SolverParameter"].update({"smoothingFactor": 1.0, "numLevel": 3, "numPreSmooth": 2, "numPostSmooth": 2})
+    if tank:
+        cfg["TankAcceleration"] = {"amplitude": 3.0, "frequency": 1.0, "axis": 0}
+        cfg["Boundary"] = {"type": "Union", "left": cfg["Boundary"]["child"],
+                           "right": {"type": "Sphere", "center": [0.5, 0.5, 0.5], "radius": 0.15}}
+    return cfg
+
+
+def same(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, what
+    bad = int((a != b).sum())
+    assert bad == 0, "%s: %d of %d elements differ (max |d| %.3e)" % (
+        what, bad, a.size, float(np.abs(a.astype(np.float64) - b.astype(np.float64)).max()))
+
+
+CASES = {
+    "reference_config_64": lambda: json.load(open(config_file("pure_liquid_3d.json"))),
+    "dam24_reference_smoother": lambda: small_dam(24, "reference"),
+    "dam24_symmetric_smoother": lambda: small_dam(24, "symmetric"),
+    "dam24_tank_sphere": lambda: small_dam(24, "symmetric", tank=True),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_stages_bit_exact(case):
+    print("case", case, flush=True)
+    cfg = CASES[case]()
+    orc = Oracle(cfg, dot_mode=1, p2g_order=1)
+    scene = lh.Scene(cfg)
+    m = scene.GetLiquidModel()
+    ps = m.GetParticleSystem()
+    orc.init()
+    m.Init()
+    po, vo, co = orc.particles()
+    pg, vg, cg = ps.get()
+    same(pg, po, "initial positions")
+    same(vg, vo, "initial velocities")
+    for f in ("solidPhi", "liquidWx", "liquidWy", "liquidWz"):
+        same(m.get_field(f), orc.get(F[f]), f)
+    dt = np.float32(cfg.get("firstDt", 0.01))
+    for step in range(3):
+        po, vo, co = orc.particles()
+        ps.set(po, vo, co)
+        m.set_field("isValid", orc.get(F["isValid"]))
+        m.time = orc.time
+        orc.clear_grid()
+        orc.compute_liquid_phi()
+        m.PreStep(dt)
+        same(m.get_field("liquidPhi"), orc.get(F["liquidPhi"]), "liquidPhi step %d" % step)
+        orc.p2g()
+        orc.apply_gravity(dt)
+        m.Particle2Grid()
+        m.ApplyGravity(dt)
+        for f in ("vx", "vy", "vz"):
+            same(m.get_field(f), orc.get(F[f]), "P2G+gravity %s step %d" % (f, step))
+        st_o = orc.solve_pressure(dt)
+        st_g = m.GetSolver().Solve(None, dt)
+        assert st_g["iterations"] == st_o.iterations
+        same(m.get_field("rhs"), orc.get(F["rhs"]), "rhs step %d" % step)
+        same(m.get_field("pressure"), orc.get(F["pressure"]), "pressure step %d" % step)
+        orc.velocity_update(dt)
+        m.VelocityUpdate(dt)
+        for f in ("vx", "vy", "vz"):
+            same(m.get_field(f), orc.get(F[f]), "velocity update %s step %d" % (f, step))
+        orc.extrapolate()
+        m.Extrapolate()
+        for f in ("vx", "vy", "vz", "tempVx", "tempVy", "tempVz", "isValid"):
+            same(m.get_field(f), orc.get(F[f]), "extrapolate %s step %d" % (f, step))
+        orc.g2p()
+        m.Grid2Particle()
+        pg, vg, cg = ps.get()
+        po, vo, co = orc.particles()
+        same(vg, vo, "G2P velocity step %d" % step)
+        same(cg, co, "G2P affine step %d" % step)
+        orc.advect(dt)
+        m.AdvectParticle(dt)
+        same(ps.get()[0], orc.particles()[0], "advected positions step %d" % step)
+        orc.set_time(orc.time + float(dt))
+        dto = np.float32(orc.get_cfl()[0])
+        assert np.float32(m.GetCFL()) == dto
+        dt = dto
+    scene.close()
+
+
+@pytest.mark.parametrize("case", ["dam24_reference_smoother", "dam24_tank_sphere"])
+def test_free_running_steps_bit_exact(case):
+    print("case", case, flush=True)
+    cfg = CASES[case]()
+    orc = Oracle(cfg, dot_mode=1, p2g_order=1)
+    orc.init()
+    scene = lh.Scene(cfg)
+    m = scene.GetLiquidModel()
+    m.Init()
+    dt = np.float32(cfg.get("firstDt", 0.01))
+    dtg = float(dt)
+    for _ in range(6):
+        st, _, _ = orc.step(dt)
+        dt = np.float32(orc.get_cfl()[0])
+        dtg, sst = m.RunSteps(1, dtg)
+        assert sst["solve"]["iterations"] == st.iterations
+        assert np.float32(dtg) == dt
+    pg, vg, cg = m.GetParticleSystem().get()
+    po, vo, co = orc.particles()
+    same(pg, po, "positions after 6 steps")
+    same(vg, vo, "velocities after 6 steps")
+    scene.close()
+
+
+def test_reference_run_known_answer():
+    """LiquidData.csv frame 1 on the GPU: every particle falls by (0, -g dt^2, 0)."""
+    import os
+    kat = np.load(os.path.join(os.path.dirname(__file__), "golden", "liquiddata_kat.npz"))
+    scene = lh.Scene(config_file("pure_liquid_3d.json"))
+    m = scene.GetLiquidModel()
+    m.Init()
+    p0 = m.GetParticleSystem().get()[0]
+    np.testing.assert_allclose(p0, kat["frame0"], rtol=0, atol=1e-6)
+    m.PreStep(0.01)
+    m.Step(0.01)
+    m.PostStep(0.01)
+    p1 = m.GetParticleSystem().get()[0]
+    np.testing.assert_allclose(p1 - p0, np.tile([0.0, -9.81e-4, 0.0], (len(p0), 1)), rtol=0, atol=1e-8)
+    keep = np.arange(len(p0)) % 8 != 0
+    np.testing.assert_allclose(p1[keep], kat["frame1"][keep], rtol=0, atol=1e-6)
+    scene.close()
+
+
+def test_extrapolation_in_place_validity_chain():
+    """Q6: validity is updated in place in (x,y,z) order, values come from the old field.
+
+    Only the x-face (1,1,1) is open and carries v=1; every other face of the x
+    range is closed.  In one sweep the whole range becomes valid through the
+    chain of predecessors, but only faces that see (1,1,1) get a nonzero value:
+    (1,1,2) sees only (1,1,1) -> 1; (1,2,1) sees (1,1,1) and its predecessor
+    (1,1,2), validated earlier in the same sweep with old value 0 -> 1/2.
+    A Jacobi-style sweep would give 1 for both.
+    """
+    cfg = small_dam(8, "reference")
+    scene = lh.Scene(cfg)
+    m = scene.GetLiquidModel()
+    m.Init()
+    orc = Oracle(cfg)
+    orc.init()
+    # Extrapolate returns at once without particles (LiquidModel.cpp:939): give it one
+    one = np.array([[0.5, 0.5, 0.5]], np.float32)
+    m.GetParticleSystem().set(one)
+    orc.set_particles(one)
+    N = 8
+    wx = np.zeros((N + 1, N, N), np.float32)
+    wx[1, 1, 1] = 1.0
+    vx = np.zeros_like(wx)
+    vx[1, 1, 1] = 1.0
+    for tgt in (m,):
+        tgt.set_field("liquidWx", wx)
+        tgt.set_field("vx", vx)
+        tgt.set_field("isValid", np.zeros((N + 1,) * 3, np.uint8))
+    orc.set(F["liquidWx"], wx)
+    orc.set(F["vx"], vx)
+    orc.set(F["isValid"], np.zeros((N + 1,) * 3, np.uint8))
+    m.Extrapolate()
+    orc.extrapolate()
+    g = m.get_field("vx")
+    assert g[1, 1, 2] == 1.0 and g[1, 2, 1] == 0.5
+    same(g, orc.get(F["vx"]), "extrapolated vx")
+    same(m.get_field("isValid"), orc.get(F["isValid"]), "validity")
+    scene.close()
