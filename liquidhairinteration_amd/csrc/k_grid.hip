@@ -186,44 +186,154 @@ __global__ void k_xvalid_init(uint8_t* __restrict__ valid, const float* __restri
   }
 }
 
-__global__ void k_xseed(uint8_t* __restrict__ st, const uint8_t* __restrict__ valid, XRange R) {
-  const int m0 = R.hi[0] - R.lo[0] + 1, m1 = R.hi[1] - R.lo[1] + 1, m2 = R.hi[2] - R.lo[2] + 1;
-  size_t total = (size_t)m0 * m1 * m2;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    int z = R.lo[2] + (int)(i % m2), y = R.lo[1] + (int)((i / m2) % m1), x = R.lo[0] + (int)(i / ((size_t)m1 * m2));
-    size_t f = ((size_t)x * R.d[1] + y) * R.d[2] + z;
-    uint8_t s = 0;
-    if (!valid[vidx(R, x, y, z)]) {
-      s = 1;
-      for (int dx = -1; dx <= 1 && s == 1; ++dx)
-        for (int dy = -1; dy <= 1 && s == 1; ++dy)
-          for (int dz = -1; dz <= 1; ++dz)
-            if (valid[vidx(R, x + dx, y + dy, z + dz)]) { s = 2; break; }
-    }
-    st[f] = s;
+// Row-scan propagation.  One wavefront owns one (x, y) row of the range and
+// walks z in chunks of 64 lanes.  Inside a row the dependency on the z-1
+// predecessor is a chain: a candidate face is validated iff some face at or
+// before it in the row, reached without crossing a non-candidate face, has a
+// "base" reason (a valid neighbour at sweep start, or a validated predecessor in
+// another row).  That is a segmented OR-scan, done exactly with two ballots and
+// a carry between chunks, so one pass settles every in-row chain and rounds are
+// only needed for chains that cross rows (the 12 predecessors in rows x-1 and
+// (x, y-1)).
+__device__ __forceinline__ bool row_chain(bool in_s, bool base, bool& carry) {
+  const int lane = __lane_id();
+  const unsigned long long S = __ballot(in_s);
+  const unsigned long long B = __ballot(in_s && base);
+  const unsigned long long le = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+  const unsigned long long brk = ~S & le;
+  bool v;
+  if (brk) {
+    int j = 63 - __clzll(brk);
+    unsigned long long win = le & ~((j == 63) ? ~0ull : ((1ull << (j + 1)) - 1ull));
+    v = in_s && (B & win) != 0ull;
+  } else {
+    v = in_s && (((B & le) != 0ull) || carry);
   }
+  carry = __shfl(v ? 1 : 0, 63) != 0;
+  return v;
 }
 
-__global__ void k_xprop(uint8_t* __restrict__ st, XRange R, int* __restrict__ changed) {
-  const int m0 = R.hi[0] - R.lo[0] + 1, m1 = R.hi[1] - R.lo[1] + 1, m2 = R.hi[2] - R.lo[2] + 1;
-  size_t total = (size_t)m0 * m1 * m2;
+// round 0: state from the validity at sweep start; base = a valid neighbour among the 26
+__global__ void k_xrow_seed(uint8_t* __restrict__ st, const uint8_t* __restrict__ valid, XRange R,
+                            int* __restrict__ changed) {
+  const int m0 = R.hi[0] - R.lo[0] + 1, m1 = R.hi[1] - R.lo[1] + 1;
+  const int wave = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+  if (wave >= m0 * m1) return;  // whole wavefront exits together
+  const int x = R.lo[0] + wave / m1, y = R.lo[1] + wave % m1;
+  bool carry = false;
   int any = 0;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    int z = R.lo[2] + (int)(i % m2), y = R.lo[1] + (int)((i / m2) % m1), x = R.lo[0] + (int)(i / ((size_t)m1 * m2));
-    size_t f = ((size_t)x * R.d[1] + y) * R.d[2] + z;
-    if (st[f] != 1) continue;
-    bool hit = false;
-    for (int k = 0; k < 13 && !hit; ++k) {
-      int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
-      size_t g = ((size_t)(x + dx) * R.d[1] + (y + dy)) * R.d[2] + (z + dz);
-      if (st[g] == 2) hit = true;
+  for (int z0 = R.lo[2]; z0 <= R.hi[2]; z0 += 64) {
+    const int z = z0 + __lane_id();
+    const bool inr = z <= R.hi[2];
+    bool in_s = false, base = false;
+    if (inr) {
+      in_s = !valid[vidx(R, x, y, z)];
+      if (in_s)
+        for (int dx = -1; dx <= 1 && !base; ++dx)
+          for (int dy = -1; dy <= 1 && !base; ++dy)
+            for (int dz = -1; dz <= 1; ++dz)
+              if (valid[vidx(R, x + dx, y + dy, z + dz)]) { base = true; break; }
     }
-    if (hit) {
-      st[f] = 2;
-      any = 1;
+    bool v = row_chain(in_s, base, carry);
+    if (inr) st[((size_t)x * R.d[1] + y) * R.d[2] + z] = v ? 2 : (in_s ? 1 : 0);
+    any |= v ? 1 : 0;
+  }
+  if (__any(any) && __lane_id() == 0) atomicOr(changed, 1);
+}
+
+// later rounds: base = already validated, or a validated predecessor in another row
+__global__ void k_xrow_prop(uint8_t* __restrict__ st, XRange R, int* __restrict__ changed) {
+  const int m0 = R.hi[0] - R.lo[0] + 1, m1 = R.hi[1] - R.lo[1] + 1;
+  const int wave = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+  if (wave >= m0 * m1) return;
+  const int x = R.lo[0] + wave / m1, y = R.lo[1] + wave % m1;
+  const size_t sx = (size_t)R.d[1] * R.d[2], sy = (size_t)R.d[2];
+  const size_t row = (size_t)x * sx + (size_t)y * sy;
+  bool carry = false;
+  int grew = 0;
+  for (int z0 = R.lo[2]; z0 <= R.hi[2]; z0 += 64) {
+    const int z = z0 + __lane_id();
+    const bool inr = z <= R.hi[2];
+    uint8_t s = inr ? st[row + z] : 0;
+    bool in_s = s != 0, base = (s == 2);
+    if (inr && s == 1) {
+      // predecessors outside this row: (x-1, y-1..y+1, z-1..z+1) and (x, y-1, z-1..z+1)
+      for (int k = 0; k < 12 && !base; ++k) {
+        int dx = (k < 9) ? -1 : 0;
+        int dy = (k < 9) ? (k / 3 - 1) : -1;
+        int dz = (k < 9) ? (k % 3 - 1) : (k - 9 - 1);
+        if (st[row + (ptrdiff_t)dx * sx + (ptrdiff_t)dy * sy + z + dz] == 2) base = true;
+      }
+    }
+    bool v = row_chain(in_s, base, carry);
+    if (inr && v && s == 1) {
+      st[row + z] = 2;
+      grew = 1;
     }
   }
-  if (any) atomicOr(changed, 1);
+  if (__any(grew) && __lane_id() == 0) atomicOr(changed, 1);
+}
+
+// Plane rounds: one 64-lane workgroup owns one x-plane and visits its rows in
+// increasing y, so a chain that moves through rows of the same plane (offsets
+// (0,-1,-1..1)) is settled in a single pass, as in the sequential sweep.  The
+// plane's previous row and the x-1 plane's rows y-1..y+1 are kept in LDS.
+// Only chains that cross planes need further rounds.
+constexpr int kPlaneRow = 2064;  // LDS row capacity (face rows up to 2063 long)
+
+__global__ void __launch_bounds__(64) k_xplane_prop(uint8_t* __restrict__ st, XRange R, int* __restrict__ changed) {
+  __shared__ uint8_t rows[5][kPlaneRow];  // [0..2] plane x-1 rows y-1, y, y+1; [3] prev row; [4] current row
+  const int x = R.lo[0] + blockIdx.x;
+  const int lane = threadIdx.x;
+  const int D2 = R.d[2];
+  const size_t sx = (size_t)R.d[1] * R.d[2], sy = (size_t)R.d[2];
+  // LDS index = z + 1 (index 0 stands for z = -1, never a candidate)
+  int ia = 0, ib = 1, ic = 2, ip = 3, iq = 4;
+  for (int k = lane; k < 5 * kPlaneRow; k += 64) (&rows[0][0])[k] = 0;
+  __syncthreads();
+  auto load_row = [&](int buf, int xx, int yy) {
+    const uint8_t* src = st + (size_t)xx * sx + (size_t)yy * sy;
+    for (int z = lane; z < D2; z += 64) rows[buf][z + 1] = src[z];
+  };
+  load_row(ia, x - 1, R.lo[1] - 1);
+  load_row(ib, x - 1, R.lo[1]);
+  load_row(ip, x, R.lo[1] - 1);
+  int grew = 0;
+  for (int y = R.lo[1]; y <= R.hi[1]; ++y) {
+    load_row(ic, x - 1, y + 1);
+    __syncthreads();
+    bool carry = false;
+    const size_t row = (size_t)x * sx + (size_t)y * sy;
+    for (int z0 = R.lo[2]; z0 <= R.hi[2]; z0 += 64) {
+      const int z = z0 + lane;
+      const bool inr = z <= R.hi[2];
+      uint8_t s = inr ? st[row + z] : 0;
+      bool in_s = s != 0, base = (s == 2);
+      if (inr && s == 1) {
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz) {
+          base = base || rows[ia][z + dz] == 2 || rows[ib][z + dz] == 2 || rows[ic][z + dz] == 2 ||
+                 rows[ip][z + dz] == 2;
+        }
+      }
+      bool v = row_chain(in_s, base, carry);
+      uint8_t ns = inr ? (v ? (uint8_t)2 : s) : (uint8_t)0;
+      if (z + 1 < kPlaneRow) rows[iq][z + 1] = ns;
+      if (inr && v && s == 1) {
+        st[row + z] = 2;
+        grew = 1;
+      }
+    }
+    __syncthreads();
+    int t = ip;  // current row becomes the previous row; the x-1 window slides up
+    ip = iq;
+    iq = t;
+    t = ia;
+    ia = ib;
+    ib = ic;
+    ic = t;
+  }
+  if (__any(grew) && lane == 0) atomicOr(changed, 1);
 }
 
 __global__ void k_xvalue(float* __restrict__ t, const float* __restrict__ v, const uint8_t* __restrict__ st,
@@ -374,6 +484,7 @@ void launch_velocity_update(Ctx& c, float dt) {
 int run_extrapolate(Ctx& c) {
   if (c.P == 0) return 0;
   int rounds = 0;
+  constexpr int kBatch = 2;  // row-parallel rounds launched before the first convergence check
   for (int a = 0; a < 3; ++a) {
     XRange R;
     for (int k = 0; k < 3; ++k) {
@@ -385,19 +496,36 @@ int run_extrapolate(Ctx& c) {
     size_t m = (size_t)std::max(R.hi[0] - R.lo[0] + 1, 0) * std::max(R.hi[1] - R.lo[1] + 1, 0) *
                std::max(R.hi[2] - R.lo[2] + 1, 0);
     if (!m) continue;
-    unsigned gb = grid_for(m);
+    const size_t rows = (size_t)(R.hi[0] - R.lo[0] + 1) * (R.hi[1] - R.lo[1] + 1);
+    const unsigned gb = grid_for(m);
+    const unsigned gr = (unsigned)div_up(rows * 64, kBlock);
     hipLaunchKernelGGL(k_xvalid_init, dim3(gb), dim3(kBlock), 0, c.s, c.is_valid, c.lw[a], R);
+    // faces outside the range keep state 0 for the whole component
+    launch_fill_u8(c, c.xstate, c.nface[a], 0);
     for (int iter = 0; iter < 5; ++iter) {
-      launch_fill_u8(c, c.xstate, c.nface[a], 0);
-      hipLaunchKernelGGL(k_xseed, dim3(gb), dim3(kBlock), 0, c.s, c.xstate, c.is_valid, R);
+      hipLaunchKernelGGL(k_xrow_seed, dim3(gr), dim3(kBlock), 0, c.s, c.xstate, c.is_valid, R, c.dflag + 8);
+      // a few cheap row-parallel rounds settle the common short cross-row chains;
+      // if the last one still changed something, plane rounds (in-plane chains
+      // settled per pass) run until a round changes nothing
+      bool plane_mode = false;
       while (true) {
-        *c.h_flag = 0;
-        if (!hip_ok(c, hipMemsetAsync(c.dflag, 0, sizeof(int), c.s), "memset")) return -1;
-        hipLaunchKernelGGL(k_xprop, dim3(gb), dim3(kBlock), 0, c.s, c.xstate, R, c.dflag);
-        if (!hip_ok(c, hipMemcpyAsync(c.h_flag, c.dflag, sizeof(int), hipMemcpyDeviceToHost, c.s), "copy")) return -1;
+        if (!hip_ok(c, hipMemsetAsync(c.dflag + 8, 0, kBatch * sizeof(int), c.s), "memset")) return -1;
+        int launched = 0;
+        if (!plane_mode) {
+          for (int r = 0; r < kBatch; ++r)
+            hipLaunchKernelGGL(k_xrow_prop, dim3(gr), dim3(kBlock), 0, c.s, c.xstate, R, c.dflag + 8 + r);
+          launched = kBatch;
+        } else {
+          hipLaunchKernelGGL(k_xplane_prop, dim3(R.hi[0] - R.lo[0] + 1), dim3(64), 0, c.s, c.xstate, R, c.dflag + 8);
+          launched = 1;
+        }
+        if (!hip_ok(c, hipMemcpyAsync(c.h_flag + 8, c.dflag + 8, kBatch * sizeof(int), hipMemcpyDeviceToHost, c.s),
+                    "copy"))
+          return -1;
         if (!hip_ok(c, hipStreamSynchronize(c.s), "extrapolate")) return -1;
-        ++rounds;
-        if (*c.h_flag == 0) break;
+        rounds += launched;
+        if (c.h_flag[8 + launched - 1] == 0) break;
+        if (R.d[2] + 2 < kPlaneRow) plane_mode = true;
       }
       hipLaunchKernelGGL(k_xvalue, dim3(gb), dim3(kBlock), 0, c.s, c.tv[a], c.v[a], c.xstate, c.is_valid, R);
       hipLaunchKernelGGL(k_xcommit, dim3(gb), dim3(kBlock), 0, c.s, c.is_valid, c.xstate, R);

@@ -147,37 +147,6 @@ __global__ void k_bin_permute(const float* __restrict__ src, float* __restrict__
   }
 }
 
-// ------------------------------------------------------------------ liquid phi
-__global__ void k_liquid_phi(float* __restrict__ phi, const float* __restrict__ pb, int64_t cap,
-                             const int* __restrict__ start, GridP g, int H1, int H2, int dW0, int dW1, int dW2,
-                             float radius, float init) {
-  const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
-  size_t total = (size_t)N0 * N1 * N2;
-  const float* px = pb + (size_t)PA_X * cap;
-  const float* py = px + cap;
-  const float* pz = py + cap;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    int zi = (int)(i % N2), yi = (int)((i / N2) % N1), xi = (int)(i / ((size_t)N1 * N2));
-    float cpx = ((float)xi + 0.5f) * g.dx[0] + g.mn[0];
-    float cpy = ((float)yi + 0.5f) * g.dx[1] + g.mn[1];
-    float cpz = ((float)zi + 0.5f) * g.dx[2] + g.mn[2];
-    int hx0 = max(0, 2 * (xi - dW0)), hx1 = min(2 * N0 - 1, 2 * (xi + dW0) + 1);
-    int hy0 = max(0, 2 * (yi - dW1)), hy1 = min(2 * N1 - 1, 2 * (yi + dW1) + 1);
-    int hz0 = max(0, 2 * (zi - dW2)), hz1 = min(2 * N2 - 1, 2 * (zi + dW2) + 1);
-    float best = init;
-    for (int hx = hx0; hx <= hx1; ++hx)
-      for (int hy = hy0; hy <= hy1; ++hy) {
-        size_t k0 = ((size_t)hx * H1 + hy) * H2;
-        int s0 = start[k0 + hz0], s1 = start[k0 + hz1 + 1];
-        for (int s = s0; s < s1; ++s) {
-          float ph = norm3(cpx - px[s], cpy - py[s], cpz - pz[s]) - radius;
-          best = (best < ph) ? best : ph;
-        }
-      }
-    phi[i] = best;
-  }
-}
-
 // ------------------------------------------------------------------ P2G
 struct P2GP {
   GridP g;
@@ -188,25 +157,62 @@ struct P2GP {
   float adt;         // tank acceleration * dt on interior faces of tank_axis
 };
 
-__global__ void k_p2g(float* __restrict__ vout, const float* __restrict__ pb, int64_t cap,
-                      const int* __restrict__ start, P2GP q) {
+// Tiles of 1 x 4 x 64 faces (or cells) per 256-thread workgroup: each
+// wavefront owns 64 consecutive z of one (x, y) row, so its lanes walk the same
+// bin rows and read overlapping, nearly contiguous particle windows.  A tile
+// that no particle can reach writes its closed-form value and leaves; the test
+// reads the two end offsets of every (hx, hy) bin row of the tile's reach.
+constexpr int kTX = 1, kTY = 4, kTZ = 64;
+
+__device__ bool tile_has_particles(const int* __restrict__ start, const int lo[3], const int hi[3], int H1, int H2) {
+  const int ny = hi[1] - lo[1] + 1;
+  const int pairs = (hi[0] - lo[0] + 1) * ny;
+  int cnt = 0;
+  for (int pr = threadIdx.x; pr < pairs; pr += blockDim.x) {
+    int hx = lo[0] + pr / ny, hy = lo[1] + pr % ny;
+    size_t k0 = ((size_t)hx * H1 + hy) * H2;
+    cnt += start[k0 + hi[2] + 1] - start[k0 + lo[2]];
+  }
+  return __syncthreads_or(cnt != 0) != 0;
+}
+
+__device__ __forceinline__ void tile_coords(int d0, int d1, int d2, int c[3], int c0[3]) {
+  const int nty = (d1 + kTY - 1) / kTY, ntz = (d2 + kTZ - 1) / kTZ;
+  const int bz = blockIdx.x % ntz, by = (blockIdx.x / ntz) % nty, bx = blockIdx.x / (ntz * nty);
+  c0[0] = bx * kTX;
+  c0[1] = by * kTY;
+  c0[2] = bz * kTZ;
+  c[0] = c0[0];
+  c[1] = c0[1] + (threadIdx.x >> 6);
+  c[2] = c0[2] + (threadIdx.x & 63);
+}
+
+__global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const float* __restrict__ pb, int64_t cap,
+                                              const int* __restrict__ start, P2GP q) {
   const int a = q.a;
   const int d0 = q.g.N[0] + (a == 0), d1 = q.g.N[1] + (a == 1), d2 = q.g.N[2] + (a == 2);
-  size_t total = (size_t)d0 * d1 * d2;
-  const float* px = pb + (size_t)PA_X * cap;
-  const float* py = px + cap;
-  const float* pz = py + cap;
-  const float* pv = pb + (size_t)(PA_VX + a) * cap;
-  const float* c0 = pb + (size_t)(PA_C + 3 * a) * cap;  // C.col(a)
-  const float* c1 = c0 + cap;
-  const float* c2 = c1 + cap;
-  const float mn0 = q.g.mn[0], mn1 = q.g.mn[1], mn2 = q.g.mn[2];
-  const float dx0 = q.g.dx[0], dx1 = q.g.dx[1], dx2 = q.g.dx[2];
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    int c[3];
-    c[2] = (int)(i % d2);
-    c[1] = (int)((i / d2) % d1);
-    c[0] = (int)(i / ((size_t)d1 * d2));
+  int c[3], c0[3];
+  tile_coords(d0, d1, d2, c, c0);
+  const int T[3] = {kTX, kTY, kTZ};
+  int blo[3], bhi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    blo[k] = max((k == a) ? 2 * c0[k] - 3 : 2 * c0[k] - 2, 0);
+    bhi[k] = min(2 * (c0[k] + T[k] - 1) + ((k == a) ? 2 : 3), q.H[k] - 1);
+  }
+  const bool any = tile_has_particles(start, blo, bhi, q.H[1], q.H[2]);
+  if (c[0] >= d0 || c[1] >= d1 || c[2] >= d2) return;
+  float ws = 0.f, vs = 0.f;
+  if (any) {
+    const float* px = pb + (size_t)PA_X * cap;
+    const float* py = px + cap;
+    const float* pz = py + cap;
+    const float* pv = pb + (size_t)(PA_VX + a) * cap;
+    const float* cc0 = pb + (size_t)(PA_C + 3 * a) * cap;  // C.col(a)
+    const float* cc1 = cc0 + cap;
+    const float* cc2 = cc1 + cap;
+    const float mn0 = q.g.mn[0], mn1 = q.g.mn[1], mn2 = q.g.mn[2];
+    const float dx0 = q.g.dx[0], dx1 = q.g.dx[1], dx2 = q.g.dx[2];
     int lo[3], hi[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -218,7 +224,6 @@ __global__ void k_p2g(float* __restrict__ vout, const float* __restrict__ pb, in
     const float ccy = (a == 1) ? (float)c[1] : (float)c[1] + 0.5f;
     const float ccz = (a == 2) ? (float)c[2] : (float)c[2] + 0.5f;
     const float fx = ccx * dx0 + mn0, fy = ccy * dx1 + mn1, fz = ccz * dx2 + mn2;
-    float ws = 0.f, vs = 0.f;
     for (int hx = lo[0]; hx <= hi[0]; ++hx)
       for (int hy = lo[1]; hy <= hi[1]; ++hy) {
         size_t k0 = ((size_t)hx * q.H[1] + hy) * q.H[2];
@@ -226,30 +231,90 @@ __global__ void k_p2g(float* __restrict__ vout, const float* __restrict__ pb, in
         for (int s = s0; s < s1; ++s) {
           float ppx = px[s], ppy = py[s], ppz = pz[s];
           float gx = (ppx - mn0) / dx0, gy = (ppy - mn1) / dx1, gz = (ppz - mn2) / dx2;
-          float w = quad_w(gx - ccx) * quad_w(gy - ccy) * quad_w(gz - ccz);
-          if (w == 0.f) continue;
+          float wgt = quad_w(gx - ccx) * quad_w(gy - ccy) * quad_w(gz - ccz);
+          if (wgt == 0.f) continue;
           float d0v = ppx - fx, d1v = ppy - fy, d2v = ppz - fz;
-          float dot = sum3(c0[s] * d0v, c1[s] * d1v, c2[s] * d2v);
-          ws += w;
-          vs += w * (pv[s] + dot);
+          float dot = sum3(cc0[s] * d0v, cc1[s] * d1v, cc2[s] * d2v);
+          ws += wgt;
+          vs += wgt * (pv[s] + dot);
         }
       }
-    float v = !is_small(ws) ? vs / ws : 0.0f;
-    // interior ranges of ApplyGravity (a == 1) and of the tank acceleration
-    if (a == 1 || a == q.tank_axis) {
-      bool in = true;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        int h = (k == a) ? q.g.N[k] - 1 : q.g.N[k] - 2;
-        if (c[k] < 1 || c[k] > h) in = false;
-      }
-      if (in) {
-        if (a == 1) v += q.gdt;
-        if (a == q.tank_axis) v += q.adt;
-      }
-    }
-    vout[i] = v;
   }
+  float v = !is_small(ws) ? vs / ws : 0.0f;
+  // interior ranges of ApplyGravity (a == 1) and of the tank acceleration
+  if (a == 1 || a == q.tank_axis) {
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      int h = (k == a) ? q.g.N[k] - 1 : q.g.N[k] - 2;
+      if (c[k] < 1 || c[k] > h) in = false;
+    }
+    if (in) {
+      if (a == 1) v += q.gdt;
+      if (a == q.tank_axis) v += q.adt;
+    }
+  }
+  vout[((size_t)c[0] * d1 + c[1]) * d2 + c[2]] = v;
+}
+
+// ------------------------------------------------------------------ liquid phi
+// min over particles in cells c +- dW of |centre - x_p| - r.  The min is
+// order-free, so bin rows are visited nearest first (the cell's own 2 x 2 rows)
+// and a row is skipped when its distance lower bound already exceeds the
+// current min by a margin far above the float error of the distance: skipped
+// particles cannot change the result.
+__global__ void __launch_bounds__(256) k_liquid_phi(float* __restrict__ phi, const float* __restrict__ pb,
+                                                     int64_t cap, const int* __restrict__ start, GridP g, int H0,
+                                                     int H1, int H2, int dW0, int dW1, int dW2, float radius,
+                                                     float init) {
+  const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+  int c[3], c0[3];
+  tile_coords(N0, N1, N2, c, c0);
+  const int T[3] = {kTX, kTY, kTZ};
+  const int dW[3] = {dW0, dW1, dW2};
+  const int H[3] = {H0, H1, H2};
+  int blo[3], bhi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    blo[k] = max(0, 2 * (c0[k] - dW[k]));
+    bhi[k] = min(H[k] - 1, 2 * (c0[k] + T[k] - 1 + dW[k]) + 1);
+  }
+  const bool any = tile_has_particles(start, blo, bhi, H1, H2);
+  const int xi = c[0], yi = c[1], zi = c[2];
+  if (xi >= N0 || yi >= N1 || zi >= N2) return;
+  float best = init;
+  if (any) {
+    const float* px = pb + (size_t)PA_X * cap;
+    const float* py = px + cap;
+    const float* pz = py + cap;
+    const float cpx = ((float)xi + 0.5f) * g.dx[0] + g.mn[0];
+    const float cpy = ((float)yi + 0.5f) * g.dx[1] + g.mn[1];
+    const float cpz = ((float)zi + 0.5f) * g.dx[2] + g.mn[2];
+    const int hx0 = max(0, 2 * (xi - dW0)), hx1 = min(H0 - 1, 2 * (xi + dW0) + 1);
+    const int hy0 = max(0, 2 * (yi - dW1)), hy1 = min(H1 - 1, 2 * (yi + dW1) + 1);
+    const int hz0 = max(0, 2 * (zi - dW2)), hz1 = min(H2 - 1, 2 * (zi + dW2) + 1);
+    // bound slack: 1e-3 dx plus the float error of world coordinates
+    const float margin = 1e-3f * g.dx[0] + 1e-6f * (fabsf(cpx) + fabsf(cpy) + fabsf(cpz));
+    for (int pass = 0; pass < 2; ++pass)
+      for (int hx = hx0; hx <= hx1; ++hx)
+        for (int hy = hy0; hy <= hy1; ++hy) {
+          const bool own = (hx >> 1) == xi && (hy >> 1) == yi;
+          if (own != (pass == 0)) continue;
+          if (pass == 1) {
+            // gap from the cell centre (grid units x+0.5) to half-cell [h/2, (h+1)/2] per axis
+            float gx = fmaxf(0.f, fmaxf(0.5f * hx - (xi + 0.5f), (xi + 0.5f) - 0.5f * (hx + 1))) * g.dx[0];
+            float gy = fmaxf(0.f, fmaxf(0.5f * hy - (yi + 0.5f), (yi + 0.5f) - 0.5f * (hy + 1))) * g.dx[1];
+            if (sqrtf(gx * gx + gy * gy) - radius > best + margin) continue;
+          }
+          size_t k0 = ((size_t)hx * H1 + hy) * H2;
+          int s0 = start[k0 + hz0], s1 = start[k0 + hz1 + 1];
+          for (int s = s0; s < s1; ++s) {
+            float ph = norm3(cpx - px[s], cpy - py[s], cpz - pz[s]) - radius;
+            best = (best < ph) ? best : ph;
+          }
+        }
+  }
+  phi[((size_t)xi * N1 + yi) * N2 + zi] = best;
 }
 
 // ------------------------------------------------------------------ G2P + advect
@@ -456,8 +521,9 @@ int run_binning(Ctx& c) {
 void launch_liquid_phi(Ctx& c) {
   if (c.P == 0 || !c.bins_valid) return;
   ProfScope ps(c, LH_PROF_LIQUID_PHI);
-  hipLaunchKernelGGL(k_liquid_phi, dim3(grid_for(c.ncell)), dim3(kBlock), 0, c.s, c.phi, c.pbuf[c.cur], c.cap,
-                     c.bin_start, c.g, c.H[1], c.H[2], c.dW[0], c.dW[1], c.dW[2], c.radius, 3 * c.g.dx[0]);
+  unsigned nt = (unsigned)(div_up(c.g.N[0], kTX) * div_up(c.g.N[1], kTY) * div_up(c.g.N[2], kTZ));
+  hipLaunchKernelGGL(k_liquid_phi, dim3(nt), dim3(kBlock), 0, c.s, c.phi, c.pbuf[c.cur], c.cap, c.bin_start, c.g,
+                     c.H[0], c.H[1], c.H[2], c.dW[0], c.dW[1], c.dW[2], c.radius, 3 * c.g.dx[0]);
 }
 
 void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
@@ -481,8 +547,9 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
     P2GP qa = q;
     if (!fuse_body_forces || a != 1) qa.gdt = 0.f;
     ProfScope ps(c, LH_PROF_P2G);
-    hipLaunchKernelGGL(k_p2g, dim3(grid_for(c.nface[a])), dim3(kBlock), 0, c.s, c.v[a], c.pbuf[c.cur], c.cap,
-                       c.bin_start, qa);
+    unsigned nt = (unsigned)(div_up(c.g.N[0] + (a == 0), kTX) * div_up(c.g.N[1] + (a == 1), kTY) *
+                             div_up(c.g.N[2] + (a == 2), kTZ));
+    hipLaunchKernelGGL(k_p2g, dim3(nt), dim3(kBlock), 0, c.s, c.v[a], c.pbuf[c.cur], c.cap, c.bin_start, qa);
   }
 }
 
