@@ -363,6 +363,87 @@ __global__ void k_xvalue(float* __restrict__ t, const float* __restrict__ v, con
   }
 }
 
+// recording variant of k_xvalue: also appends (face, neighbour mask) of every
+// validated face to the sweep's list (list order is irrelevant: faces are independent)
+__global__ void k_xvalue_rec(float* __restrict__ t, const float* __restrict__ v, const uint8_t* __restrict__ st,
+                             const uint8_t* __restrict__ valid, XRange R, uint32_t* __restrict__ list,
+                             int* __restrict__ counter) {
+  const int m0 = R.hi[0] - R.lo[0] + 1, m1 = R.hi[1] - R.lo[1] + 1, m2 = R.hi[2] - R.lo[2] + 1;
+  size_t total = (size_t)m0 * m1 * m2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    int z = R.lo[2] + (int)(i % m2), y = R.lo[1] + (int)((i / m2) % m1), x = R.lo[0] + (int)(i / ((size_t)m1 * m2));
+    size_t f = ((size_t)x * R.d[1] + y) * R.d[2] + z;
+    float out = v[f];
+    if (st[f] == 2) {
+      float sum = 0.0f;
+      int count = 0;
+      uint32_t mask = 0;
+      for (int k = 0; k < 27; ++k) {
+        int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
+        size_t g = ((size_t)(x + dx) * R.d[1] + (y + dy)) * R.d[2] + (z + dz);
+        bool ok = valid[vidx(R, x + dx, y + dy, z + dz)] != 0;
+        if (!ok && k < 13) ok = (st[g] == 2);
+        if (ok) {
+          sum += v[g];
+          ++count;
+          mask |= 1u << k;
+        }
+      }
+      out = sum / (float)count;
+      int slot = atomicAdd(counter, 1);
+      list[2 * (size_t)slot] = (uint32_t)f;
+      list[2 * (size_t)slot + 1] = mask;
+    }
+    t[f] = out;
+  }
+}
+
+// replay: new value of each listed face from its recorded neighbour set, in the
+// reference's 27-offset order
+__global__ void k_xreplay(float* __restrict__ t, const float* __restrict__ v, const uint32_t* __restrict__ list,
+                          int n, XRange R) {
+  const size_t sx = (size_t)R.d[1] * R.d[2], sy = (size_t)R.d[2];
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    size_t f = list[2 * (size_t)e];
+    uint32_t mask = list[2 * (size_t)e + 1];
+    float sum = 0.0f;
+    int count = 0;
+    for (int k = 0; k < 27; ++k) {
+      if (!(mask & (1u << k))) continue;
+      ptrdiff_t off = (ptrdiff_t)(k / 9 - 1) * (ptrdiff_t)sx + (ptrdiff_t)((k / 3) % 3 - 1) * (ptrdiff_t)sy + (k % 3 - 1);
+      sum += v[(ptrdiff_t)f + off];
+      ++count;
+    }
+    t[f] = sum / (float)count;
+  }
+}
+
+__global__ void k_xcopy_list(float* __restrict__ t, const float* __restrict__ v, const uint32_t* __restrict__ list,
+                             int n) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    size_t f = list[2 * (size_t)e];
+    t[f] = v[f];
+  }
+}
+
+__global__ void k_xcopy_range(float* __restrict__ t, const float* __restrict__ v, XRange R) {
+  const int m0 = R.hi[0] - R.lo[0] + 1, m1 = R.hi[1] - R.lo[1] + 1, m2 = R.hi[2] - R.lo[2] + 1;
+  size_t total = (size_t)m0 * m1 * m2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    int z = R.lo[2] + (int)(i % m2), y = R.lo[1] + (int)((i / m2) % m1), x = R.lo[0] + (int)(i / ((size_t)m1 * m2));
+    size_t f = ((size_t)x * R.d[1] + y) * R.d[2] + z;
+    t[f] = v[f];
+  }
+}
+
+__global__ void k_cmp_u8(const uint8_t* __restrict__ a, const uint8_t* __restrict__ b, size_t n,
+                         int* __restrict__ differ) {
+  int d = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    d |= (a[i] != b[i]) ? 1 : 0;
+  if (__any(d) && (threadIdx.x & 63) == 0) atomicOr(differ, 1);
+}
+
 __global__ void k_xcommit(uint8_t* __restrict__ valid, const uint8_t* __restrict__ st, XRange R) {
   const int m0 = R.hi[0] - R.lo[0] + 1, m1 = R.hi[1] - R.lo[1] + 1, m2 = R.hi[2] - R.lo[2] + 1;
   size_t total = (size_t)m0 * m1 * m2;
@@ -481,20 +562,50 @@ void launch_velocity_update(Ctx& c, float dt) {
                      c.lw[1], c.lw[2], c.phi, c.pressure, c.g, dt, c.cfg.density);
 }
 
-int run_extrapolate(Ctx& c) {
-  if (c.P == 0) return 0;
+namespace {
+
+XRange xrange_of(const Ctx& c, int a) {
+  XRange R;
+  for (int k = 0; k < 3; ++k) {
+    R.d[k] = c.g.N[k] + (k == a);
+    R.lo[k] = 1;
+    R.hi[k] = (k == a) ? c.g.N[k] - 1 : c.g.N[k] - 2;
+    R.vd[k] = c.g.N[k] + 1;
+  }
+  return R;
+}
+
+size_t xrange_size(const XRange& R) {
+  return (size_t)std::max(R.hi[0] - R.lo[0] + 1, 0) * std::max(R.hi[1] - R.lo[1] + 1, 0) *
+         std::max(R.hi[2] - R.lo[2] + 1, 0);
+}
+
+bool ensure_xcache(Ctx& c) {
+  if (c.xv_snap) return true;
+  if (hipMalloc((void**)&c.xv_snap, c.nnode) != hipSuccess || hipMalloc((void**)&c.xv_final, c.nnode) != hipSuccess ||
+      hipMalloc((void**)&c.xcount, 32 * sizeof(int)) != hipSuccess) {
+    c.err = "hipMalloc failed (extrapolation schedule)";
+    return false;
+  }
+  for (int a = 0; a < 3; ++a) {
+    c.xlist_cap[a] = std::max<size_t>(xrange_size(xrange_of(c, a)), 1);
+    if (hipMalloc((void**)&c.xlist[a], 2 * sizeof(uint32_t) * c.xlist_cap[a]) != hipSuccess) {
+      c.err = "hipMalloc failed (extrapolation schedule)";
+      return false;
+    }
+  }
+  return true;
+}
+
+// the exact algorithm (seed, propagation rounds, values), recording the schedule
+int extrapolate_record(Ctx& c) {
   int rounds = 0;
   constexpr int kBatch = 2;  // row-parallel rounds launched before the first convergence check
+  hipMemcpyAsync(c.xv_snap, c.is_valid, c.nnode, hipMemcpyDeviceToDevice, c.s);
+  hipMemsetAsync(c.xcount, 0, 32 * sizeof(int), c.s);
   for (int a = 0; a < 3; ++a) {
-    XRange R;
-    for (int k = 0; k < 3; ++k) {
-      R.d[k] = c.g.N[k] + (k == a);
-      R.lo[k] = 1;
-      R.hi[k] = (k == a) ? c.g.N[k] - 1 : c.g.N[k] - 2;
-      R.vd[k] = c.g.N[k] + 1;
-    }
-    size_t m = (size_t)std::max(R.hi[0] - R.lo[0] + 1, 0) * std::max(R.hi[1] - R.lo[1] + 1, 0) *
-               std::max(R.hi[2] - R.lo[2] + 1, 0);
+    XRange R = xrange_of(c, a);
+    size_t m = xrange_size(R);
     if (!m) continue;
     const size_t rows = (size_t)(R.hi[0] - R.lo[0] + 1) * (R.hi[1] - R.lo[1] + 1);
     const unsigned gb = grid_for(m);
@@ -527,12 +638,71 @@ int run_extrapolate(Ctx& c) {
         if (c.h_flag[8 + launched - 1] == 0) break;
         if (R.d[2] + 2 < kPlaneRow) plane_mode = true;
       }
-      hipLaunchKernelGGL(k_xvalue, dim3(gb), dim3(kBlock), 0, c.s, c.tv[a], c.v[a], c.xstate, c.is_valid, R);
+      // one running counter per component; the value after each sweep is kept as
+      // that sweep's end offset in the component's list
+      hipLaunchKernelGGL(k_xvalue_rec, dim3(gb), dim3(kBlock), 0, c.s, c.tv[a], c.v[a], c.xstate, c.is_valid, R,
+                         c.xlist[a], c.xcount + a);
+      hipMemcpyAsync(c.xcount + 3 + a * 5 + iter, c.xcount + a, sizeof(int), hipMemcpyDeviceToDevice, c.s);
       hipLaunchKernelGGL(k_xcommit, dim3(gb), dim3(kBlock), 0, c.s, c.is_valid, c.xstate, R);
       std::swap(c.v[a], c.tv[a]);  // std::swap(m_grid.tempV, m_grid.v)
     }
   }
+  hipMemcpyAsync(c.xv_final, c.is_valid, c.nnode, hipMemcpyDeviceToDevice, c.s);
+  int ends[15];
+  hipMemcpyAsync(c.h_flag, c.xcount + 3, 15 * sizeof(int), hipMemcpyDeviceToHost, c.s);
+  if (!hip_ok(c, hipStreamSynchronize(c.s), "extrapolate")) return -1;
+  std::memcpy(ends, c.h_flag, sizeof(ends));
+  for (int k = 0; k < 15; ++k) c.xcnt[k] = ends[k] - ((k % 5) ? ends[k - 1] : 0);
+  c.xcache_ok = true;
   return rounds;
+}
+
+// values only, from the recorded schedule: sweep k copies the faces sweep k-1
+// changed (the buffers agree everywhere else in the range) and writes its own
+void extrapolate_replay(Ctx& c) {
+  for (int a = 0; a < 3; ++a) {
+    XRange R = xrange_of(c, a);
+    size_t m = xrange_size(R);
+    if (!m) continue;
+    size_t off = 0, prev_off = 0;
+    int prev_n = 0;
+    for (int iter = 0; iter < 5; ++iter) {
+      int n = c.xcnt[a * 5 + iter];
+      if (iter == 0) {
+        hipLaunchKernelGGL(k_xcopy_range, dim3(grid_for(m)), dim3(kBlock), 0, c.s, c.tv[a], c.v[a], R);
+      } else if (prev_n > 0) {
+        hipLaunchKernelGGL(k_xcopy_list, dim3(grid_for(prev_n)), dim3(kBlock), 0, c.s, c.tv[a], c.v[a],
+                           c.xlist[a] + 2 * prev_off, prev_n);
+      }
+      if (n > 0)
+        hipLaunchKernelGGL(k_xreplay, dim3(grid_for(n)), dim3(kBlock), 0, c.s, c.tv[a], c.v[a], c.xlist[a] + 2 * off,
+                           n, R);
+      std::swap(c.v[a], c.tv[a]);
+      prev_off = off;
+      prev_n = n;
+      off += (size_t)n;
+    }
+  }
+  hipMemcpyAsync(c.is_valid, c.xv_final, c.nnode, hipMemcpyDeviceToDevice, c.s);
+}
+
+}  // namespace
+
+int run_extrapolate(Ctx& c) {
+  if (c.P == 0) return 0;
+  if (!ensure_xcache(c)) return -1;
+  if (c.xcache_ok) {
+    hipMemsetAsync(c.dflag + 12, 0, sizeof(int), c.s);
+    hipLaunchKernelGGL(k_cmp_u8, dim3(grid_for(c.nnode)), dim3(kBlock), 0, c.s, c.is_valid, c.xv_snap, c.nnode,
+                       c.dflag + 12);
+    hipMemcpyAsync(c.h_flag + 12, c.dflag + 12, sizeof(int), hipMemcpyDeviceToHost, c.s);
+    if (!hip_ok(c, hipStreamSynchronize(c.s), "extrapolate")) return -1;
+    if (c.h_flag[12] == 0) {
+      extrapolate_replay(c);
+      return 0;
+    }
+  }
+  return extrapolate_record(c);
 }
 
 void launch_divergence(Ctx& c, double* h_mean, double* h_linf, int* h_count) {

@@ -190,8 +190,8 @@ __global__ void k_build_rhs(float* __restrict__ b, const float* __restrict__ phi
 }
 
 // ---------------------------------------------------------------- V-cycle
-// Smooth, PressureSolver.cpp:821-837.  mode 0: (1. - w) * z + w * term with the
-// reference's double/float promotion; mode 1: z + w * term.
+// Smooth, PressureSolver.cpp:821-837: mode 0 is (1. - w) * z + w * term with the
+// reference's double/float promotion (quirk Q4); mode 1 is z + w * term.
 __global__ void k_smooth(float* __restrict__ z, const float* __restrict__ r, LvP L, int phase, float w, int mode) {
   const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
   size_t total = (size_t)d0 * d1 * d2;
@@ -207,6 +207,99 @@ __global__ void k_smooth(float* __restrict__ z, const float* __restrict__ r, LvP
     } else {
       z[i] = z[i] + w * term;
     }
+  }
+}
+
+// One red-black sweep (both colours) in one kernel: Smooth(l, first) followed by
+// Smooth(l, 1 - first).  The first colour uses the other colour's old values and
+// the second colour uses the first colour's new values, exactly as two separate
+// colour passes.  A workgroup owns an 8 x 8 x 64 tile: it stages the old z of
+// the tile plus a 2-cell halo in LDS, updates the first colour on the tile plus
+// a 1-cell halo (the halo cells are recomputed by the neighbouring tiles too,
+// with identical results), then the second colour on the tile, and writes the
+// tile.  z is read from zin and written to zout (ping-pong) so tiles never see
+// each other's writes; every cell of the tile is written.  Each array element is
+// fetched from HBM about once per sweep instead of once per colour.
+constexpr int kSwX = 8, kSwY = 8, kSwZ = 64;
+constexpr int kSwHX = kSwX + 4, kSwHY = kSwY + 4, kSwHZ = kSwZ + 4;
+
+template <int MODE>
+__device__ __forceinline__ float relax(float zc, float r, float ax, float ad, float w) {
+  float term = (r - ax) / ad;
+  if (MODE == 0) {
+    float wt = w * term;
+    return (float)((1.0 - (double)w) * (double)zc + (double)wt);
+  }
+  return zc + w * term;
+}
+
+// GetAx with the 7 z values supplied (same order and conditions as get_ax)
+__device__ __forceinline__ float get_ax7(const LvP& L, size_t c, int x, int y, int w, float zc, float zxp, float zxm,
+                                        float zyp, float zym, float zzp, float zzm) {
+  float res = zc * L.adiag[c];
+  if (L.cls[c] == 0) return res;
+  const size_t sx = (size_t)L.d[1] * L.d[2], sy = (size_t)L.d[2];
+  if (x + 1 < L.d[0] && L.cls[c + sx] == 2) res += zxp * (-L.t0[c]);
+  if (x - 1 >= 0 && L.cls[c - sx] == 2) res += zxm * (-L.t0[c - sx]);
+  if (y + 1 < L.d[1] && L.cls[c + sy] == 2) res += zyp * (-L.t1[c]);
+  if (y - 1 >= 0 && L.cls[c - sy] == 2) res += zym * (-L.t1[c - sy]);
+  if (w + 1 < L.d[2] && L.cls[c + 1] == 2) res += zzp * (-L.t2[c]);
+  if (w - 1 >= 0 && L.cls[c - 1] == 2) res += zzm * (-L.t2[c - 1]);
+  return res;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_rbgs_sweep(float* __restrict__ zout, const float* __restrict__ zin,
+                                                     const float* __restrict__ r, LvP L, int first, float w) {
+  __shared__ float sz[kSwHX * kSwHY * kSwHZ];
+  const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
+  const int nby = (d1 + kSwY - 1) / kSwY, nbz = (d2 + kSwZ - 1) / kSwZ;
+  const int bz = blockIdx.x % nbz, by = (blockIdx.x / nbz) % nby, bx = blockIdx.x / (nbz * nby);
+  const int x0 = bx * kSwX, y0 = by * kSwY, z0 = bz * kSwZ;
+  const size_t sx = (size_t)d1 * d2, sy = (size_t)d2;
+  const int tid = threadIdx.x;
+  constexpr int PX = kSwHY * kSwHZ;  // LDS x-stride
+  for (int i = tid; i < kSwHX * kSwHY * kSwHZ; i += 256) {
+    int xx = i / PX, rem = i - xx * PX, yy = rem / kSwHZ, zz = rem - yy * kSwHZ;
+    int x = x0 - 2 + xx, y = y0 - 2 + yy, zc = z0 - 2 + zz;
+    float v = 0.f;
+    if (x >= 0 && x < d0 && y >= 0 && y < d1 && zc >= 0 && zc < d2) v = zin[(size_t)x * sx + (size_t)y * sy + zc];
+    sz[i] = v;
+  }
+  __syncthreads();
+  // first colour on the tile plus a 1-cell halo (reads only second-colour neighbours)
+  constexpr int RX = kSwX + 2, RY = kSwY + 2, RZ = kSwZ + 2;
+  for (int i = tid; i < RX * RY * RZ; i += 256) {
+    int xx = i / (RY * RZ), rem = i - xx * (RY * RZ), yy = rem / RZ, zz = rem - yy * RZ;
+    int x = x0 - 1 + xx, y = y0 - 1 + yy, zc = z0 - 1 + zz;
+    if (x < 0 || x >= d0 || y < 0 || y >= d1 || zc < 0 || zc >= d2) continue;
+    if (((x + y + zc) & 1) != first) continue;
+    size_t c = (size_t)x * sx + (size_t)y * sy + zc;
+    float ad = L.adiag[c];
+    if (is_small(ad)) continue;
+    int k = (xx + 1) * PX + (yy + 1) * kSwHZ + (zz + 1);
+    float ax = get_ax7(L, c, x, y, zc, sz[k], sz[k + PX], sz[k - PX], sz[k + kSwHZ], sz[k - kSwHZ], sz[k + 1],
+                       sz[k - 1]);
+    sz[k] = relax<MODE>(sz[k], r[c], ax, ad, w);
+  }
+  __syncthreads();
+  // second colour on the tile; write the whole tile
+  for (int i = tid; i < kSwX * kSwY * kSwZ; i += 256) {
+    int xx = i / (kSwY * kSwZ), rem = i - xx * (kSwY * kSwZ), yy = rem / kSwZ, zz = rem - yy * kSwZ;
+    int x = x0 + xx, y = y0 + yy, zc = z0 + zz;
+    if (x >= d0 || y >= d1 || zc >= d2) continue;
+    size_t c = (size_t)x * sx + (size_t)y * sy + zc;
+    int k = (xx + 2) * PX + (yy + 2) * kSwHZ + (zz + 2);
+    float zv = sz[k];
+    if (((x + y + zc) & 1) != first) {
+      float ad = L.adiag[c];
+      if (!is_small(ad)) {
+        float ax = get_ax7(L, c, x, y, zc, zv, sz[k + PX], sz[k - PX], sz[k + kSwHZ], sz[k - kSwHZ], sz[k + 1],
+                           sz[k - 1]);
+        zv = relax<MODE>(zv, r[c], ax, ad, w);
+      }
+    }
+    zout[c] = zv;
   }
 }
 
@@ -432,10 +525,24 @@ void launch_build_lhs(Ctx& c, float dt) {
 
 namespace {
 
-void smooth(Ctx& c, int l, int phase, int mode) {
+// one red-black sweep on level l: Smooth(l, first) then Smooth(l, 1 - first).
+// Large levels use the fused tile kernel; small ones (launch-bound) two colour passes.
+void sweep(Ctx& c, int l, int first, int mode) {
   Level& L = c.lv[l];
   ProfScope ps(c, l == 0 ? LH_PROF_SMOOTH_L0 : LH_PROF_COUNT);
-  hipLaunchKernelGGL(k_smooth, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), phase, c.cfg.omega, mode);
+  if ((long long)L.n < c.cfg.fused_sweep_min_cells) {
+    hipLaunchKernelGGL(k_smooth, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), first, c.cfg.omega,
+                       mode);
+    hipLaunchKernelGGL(k_smooth, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), 1 - first,
+                       c.cfg.omega, mode);
+    return;
+  }
+  unsigned nb = (unsigned)(div_up(L.d[0], kSwX) * div_up(L.d[1], kSwY) * div_up(L.d[2], kSwZ));
+  if (mode == 0)
+    hipLaunchKernelGGL(k_rbgs_sweep<0>, dim3(nb), dim3(256), 0, c.s, L.z2, L.z, L.r, lvp(L), first, c.cfg.omega);
+  else
+    hipLaunchKernelGGL(k_rbgs_sweep<1>, dim3(nb), dim3(256), 0, c.s, L.z2, L.z, L.r, lvp(L), first, c.cfg.omega);
+  std::swap(L.z, L.z2);
 }
 
 // ApplyPreconditioner, PressureSolver.cpp:719-772 (symmetric mode: z[0] = 0 first,
@@ -452,38 +559,21 @@ void apply_precond(Ctx& c) {
   const int mode = sym ? 1 : 0;
   if (sym) launch_fill_f32(c, L0.z, L0.n, 0.f);
   for (int l = 1; l < L; ++l) {
-    for (int it = 0; it < c.cfg.num_pre; ++it) {
-      smooth(c, l - 1, 0, mode);
-      smooth(c, l - 1, 1, mode);
-    }
+    for (int it = 0; it < c.cfg.num_pre; ++it) sweep(c, l - 1, 0, mode);
     Level& Lc = c.lv[l];
     Level& Lf = c.lv[l - 1];
     hipLaunchKernelGGL(k_restrict, dim3(grid_for(Lc.n)), dim3(kBlock), 0, c.s, Lc.r, Lc.z, Lf.r, Lf.z, lvp(Lf),
                        Lc.d[0], Lc.d[1], Lc.d[2]);
   }
-  for (int it = 0; it < c.cfg.num_final; ++it) {
-    smooth(c, L - 1, 0, mode);
-    smooth(c, L - 1, 1, mode);
-  }
+  for (int it = 0; it < c.cfg.num_final; ++it) sweep(c, L - 1, 0, mode);
   if (sym)
-    for (int it = 0; it < c.cfg.num_final; ++it) {
-      smooth(c, L - 1, 1, mode);
-      smooth(c, L - 1, 0, mode);
-    }
+    for (int it = 0; it < c.cfg.num_final; ++it) sweep(c, L - 1, 1, mode);
   for (int l = L - 2; l >= 0; --l) {
     Level& Lc = c.lv[l + 1];
     Level& Lf = c.lv[l];
     hipLaunchKernelGGL(k_prolong, dim3(grid_for(Lc.n)), dim3(kBlock), 0, c.s, Lf.z, Lc.z, Lf.d[1], Lf.d[2], Lc.d[0],
                        Lc.d[1], Lc.d[2]);
-    for (int it = 0; it < c.cfg.num_post; ++it) {
-      if (sym) {
-        smooth(c, l, 1, mode);
-        smooth(c, l, 0, mode);
-      } else {
-        smooth(c, l, 0, mode);
-        smooth(c, l, 1, mode);
-      }
-    }
+    for (int it = 0; it < c.cfg.num_post; ++it) sweep(c, l, sym ? 1 : 0, mode);
   }
 }
 

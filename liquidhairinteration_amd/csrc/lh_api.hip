@@ -83,6 +83,7 @@ Config parse_config(const std::string& text) {
   if (sm == "reference") c.smoother = LH_SMOOTHER_REFERENCE;
   else if (sm == "symmetric") c.smoother = LH_SMOOTHER_SYMMETRIC;
   else throw std::runtime_error("Unknown smoother " + sm);
+  c.fused_sweep_min_cells = (long long)s.get("fusedSweepMinCells", (double)(1LL << 62));
   // LiquidParameter (SceneParameter.h:55-63)
   const Json& lq = j.at("LiquidParameter");
   c.density = lq.at("density").as_float();
@@ -210,7 +211,8 @@ int alloc_ctx(Ctx& c) {
     }
     q.n = (size_t)q.d[0] * q.d[1] * q.d[2];
     ok = ok && dalloc(c, q.adiag, q.n) && dalloc(c, q.t[0], q.n) && dalloc(c, q.t[1], q.n) &&
-         dalloc(c, q.t[2], q.n) && dalloc(c, q.cls, q.n) && dalloc(c, q.r, q.n) && dalloc(c, q.z, q.n);
+         dalloc(c, q.t[2], q.n) && dalloc(c, q.cls, q.n) && dalloc(c, q.r, q.n) && dalloc(c, q.z, q.n) &&
+         dalloc(c, q.z2, q.n);
   }
   c.npartial = 1024;
   ok = ok && dalloc(c, c.p, c.ncell) && dalloc(c, c.x, c.ncell) && dalloc(c, c.q, c.ncell) &&
@@ -259,9 +261,10 @@ void free_ctx(Ctx& c) {
   F(c.bin_count); F(c.bin_start); F(c.scan_tmp); F(c.pkey); F(c.pslot);
   F(c.pbuf[0]); F(c.pbuf[1]);
   for (auto& q : c.lv) {
-    F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z);
+    F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2);
   }
   F(c.p); F(c.x); F(c.q); F(c.partial); F(c.dsc); F(c.dsc64); F(c.dmaxv); F(c.dflag);
+  F(c.xv_snap); F(c.xv_final); F(c.xcount); F(c.xlist[0]); F(c.xlist[1]); F(c.xlist[2]);
   if (c.h_pin) hipHostFree(c.h_pin);
   if (c.h_flag) hipHostFree(c.h_flag);
   if (c.ev0) hipEventDestroy(c.ev0);
@@ -599,6 +602,7 @@ int lh_field_shape(const lh_ctx* ctx, int field, int shape[3]) {
 int lh_compute_solid_phi(lh_ctx* ctx) {
   return guarded(ctx, [](Ctx& c) {
     launch_solid_phi(c);
+    c.xcache_ok = false;  // liquidW changed: the Extrapolate schedule must be re-recorded
     c.solid_ready = true;
     return hip_ok(c, hipStreamSynchronize(c.s), "solid phi") ? LH_OK : LH_ERR_DEVICE;
   });
@@ -608,6 +612,7 @@ int lh_init(lh_ctx* ctx) {
   return guarded(ctx, [](Ctx& c) {
     // LiquidModel<3>::Init: ComputeSolidPhi, then spawn (LiquidModel.cpp:506-512)
     launch_solid_phi(c);
+    c.xcache_ok = false;  // liquidW changed: the Extrapolate schedule must be re-recorded
     c.solid_ready = true;
     std::vector<float> pos, vel;
     if (!c.cfg.blocks.empty()) seed_blocks(c, pos, vel);
@@ -834,6 +839,7 @@ int lh_set_field(lh_ctx* ctx, int field, const void* host, size_t count) {
     }
     size_t bytes = n * (field == LH_FIELD_IS_VALID ? 1 : 4);
     hipMemcpyAsync(field_dev(c, field), host, bytes, hipMemcpyHostToDevice, c.s);
+    if (field >= LH_FIELD_LIQUID_WX && field <= LH_FIELD_LIQUID_WZ) c.xcache_ok = false;
     return hip_ok(c, hipStreamSynchronize(c.s), "set field") ? LH_OK : LH_ERR_DEVICE;
   });
 }

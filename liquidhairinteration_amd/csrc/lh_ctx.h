@@ -27,6 +27,8 @@ struct Config {
   int num_level = 1, num_pre = 5, num_post = 5, num_final = 10;
   float omega = 0.5f;
   int smoother = LH_SMOOTHER_REFERENCE;
+  // levels with at least this many cells use the fused one-kernel red-black sweep
+  long long fused_sweep_min_cells = 1LL << 62;  // off: two colour passes measured faster at 256^3
   // mixed-precision refinement (FP32 inner MGPCG under an FP64 residual loop)
   int refined = 0;
   double refined_tol = 1e-9;
@@ -57,6 +59,7 @@ struct Level {
   uint8_t* cls = nullptr;                     // 0: phi > 0, 1: phi == 0 (or NaN), 2: phi < 0
   float* r = nullptr;
   float* z = nullptr;
+  float* z2 = nullptr;  // ping-pong partner of z for the fused red-black sweep
 };
 
 // attributes of one particle in the SoA particle buffer (kParticleAttrs x cap)
@@ -104,6 +107,17 @@ struct Ctx {
   double time = 0.0;
   float last_maxv = 0.f;
   bool solid_ready = false;
+  // Extrapolate schedule cache: which faces each sweep validates, and from which
+  // neighbours, is a function of isValid at the start of Extrapolate and of
+  // liquidW only; once recorded it is replayed (values only) whenever isValid
+  // at the start is bit-identical to the recorded snapshot.
+  uint8_t* xv_snap = nullptr;   // isValid when the schedule was recorded
+  uint8_t* xv_final = nullptr;  // isValid at the end of that Extrapolate
+  uint32_t* xlist[3] = {nullptr, nullptr, nullptr};  // (face, 27-bit neighbour mask) per validated face
+  size_t xlist_cap[3] = {0, 0, 0};
+  int* xcount = nullptr;        // device counters, component * 5 + sweep
+  int xcnt[15] = {};
+  bool xcache_ok = false;
   // live kernel timing: event pairs per instrumented kernel
   bool prof = false;
   std::vector<hipEvent_t> prof_ev[LH_PROF_COUNT];

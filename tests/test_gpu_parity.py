@@ -21,12 +21,14 @@ F = {"vx": 0, "vy": 1, "vz": 2, "tempVx": 3, "tempVy": 4, "tempVz": 5, "liquidWx
      "pressure": 9, "liquidPhi": 10, "solidPhi": 11, "isValid": 12, "rhs": 13}
 
 
-def small_dam(N=24, smoother="reference", tank=False):
+def small_dam(N=24, smoother="reference", tank=False, fused=False):
     cfg = json.load(open(config_file("dam_break_48.json")))
     cfg["gridDimension"] = [N, N, N]
     wall = 0.5 - 3.0 / N
     cfg["Boundary"]["child"]["halfExtent"] = [wall] * 3
     cfg["SolverParameter"]["smoother"] = smoother
+    # 0: every level uses the one-kernel red-black sweep (default: two colour passes)
+    cfg["SolverParameter"]["fusedSweepMinCells"] = 0 if fused else 1 << 40
     if smoother == "symmetric":
         cfg["SolverParameter"].update({"smoothingFactor": 1.0, "numLevel": 3, "numPreSmooth": 2, "numPostSmooth": 2})
     if tank:
@@ -49,6 +51,8 @@ CASES = {
     "dam24_reference_smoother": lambda: small_dam(24, "reference"),
     "dam24_symmetric_smoother": lambda: small_dam(24, "symmetric"),
     "dam24_tank_sphere": lambda: small_dam(24, "symmetric", tank=True),
+    "dam24_reference_fused_sweep": lambda: small_dam(24, "reference", fused=True),
+    "dam40_symmetric_fused_sweep": lambda: small_dam(40, "symmetric", fused=True),
 }
 
 
@@ -113,7 +117,7 @@ def test_stages_bit_exact(case):
     scene.close()
 
 
-@pytest.mark.parametrize("case", ["dam24_reference_smoother", "dam24_tank_sphere"])
+@pytest.mark.parametrize("case", ["dam24_reference_smoother", "dam24_tank_sphere", "dam40_symmetric_fused_sweep"])
 def test_free_running_steps_bit_exact(case):
     print("case", case, flush=True)
     cfg = CASES[case]()
