@@ -148,6 +148,14 @@ enum lh_prof_kernel {
 };
 int lh_profile_enable(lh_ctx* ctx, int on);
 int lh_profile_read(lh_ctx* ctx, int kernel, int* launches, double* total_ms);
+/* ---- trajectory files ------------------------------------------------------
+ * ParticleSystem<3>::WriteToFile (ParticleSystem.cpp:66-84): a line with the
+ * particle count, then one line "x,y,z,x,y,z,...," with the default ostream
+ * float format (6 significant digits); append != 0 opens in append mode.
+ * LoadFromLine (ParticleSystem.cpp:86-101): reads positions from such a line
+ * into the first min(n, count) particles (missing tokens are skipped). */
+int lh_write_positions(lh_ctx* ctx, const char* path, int append);
+int lh_load_positions_line(lh_ctx* ctx, const char* line);
 /* simulated time and the tank-acceleration clock */
 int lh_set_time(lh_ctx* ctx, double t);
 int lh_get_time(const lh_ctx* ctx, double* t);

@@ -867,6 +867,47 @@ int lh_profile_read(lh_ctx* ctx, int kernel, int* launches, double* total_ms) {
   return LH_OK;
 }
 
+int lh_write_positions(lh_ctx* ctx, const char* path, int append) {
+  if (!ctx || !path) return LH_ERR_ARG;
+  const int64_t n = ctx->c.P;
+  std::vector<float> pos((size_t)3 * n);
+  int rc = lh_get_particles(ctx, pos.data(), nullptr, nullptr, n);
+  if (rc) return rc;
+  std::ofstream w(path, append ? (std::ios::out | std::ios::app) : (std::ios::out | std::ios::trunc));
+  if (!w.is_open()) {
+    ctx->c.err = std::string("Failed to open file: ") + path;  // ParticleSystem.cpp:72-74
+    return LH_ERR_ARG;
+  }
+  w << n << "\n";
+  for (size_t i = 0; i < (size_t)3 * n; ++i) w << pos[i] << ",";
+  w << "\n";
+  return w.good() ? LH_OK : LH_ERR_ARG;
+}
+
+int lh_load_positions_line(lh_ctx* ctx, const char* line) {
+  if (!ctx || !line) return LH_ERR_ARG;
+  const int64_t n = ctx->c.P;
+  std::vector<float> pos((size_t)3 * n), vel((size_t)3 * n), C((size_t)9 * n);
+  int rc = lh_get_particles(ctx, pos.data(), vel.data(), C.data(), n);
+  if (rc) return rc;
+  // ParticleSystem<DIM>::LoadFromLine: tokens separated by ',', the first one
+  // skipped, then x,y,z per particle; a missing token leaves the value unchanged
+  std::istringstream ss(line);
+  std::string tok;
+  std::getline(ss, tok, ',');
+  for (int64_t i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      if (!std::getline(ss, tok, ',')) continue;
+      try {
+        pos[3 * i + a] = std::stof(tok);
+      } catch (const std::exception&) {
+        ctx->c.err = "Failed to recognize line";
+        return LH_ERR_ARG;
+      }
+    }
+  return lh_set_particles(ctx, pos.data(), vel.data(), C.data(), n);
+}
+
 int lh_set_time(lh_ctx* ctx, double t) {
   if (!ctx) return LH_ERR_ARG;
   ctx->c.time = t;
