@@ -116,6 +116,8 @@ def load_library():
         "lh_profile_enable": ([vp, C.c_int], C.c_int),
         "lh_profile_read": ([vp, C.c_int, ip, dp], C.c_int),
         "lh_get_time": ([vp, dp], C.c_int),
+        "lh_write_positions": ([vp, C.c_char_p, C.c_int], C.c_int),
+        "lh_load_positions_line": ([vp, C.c_char_p], C.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -199,6 +201,14 @@ class ParticleSystem:
         """(n, 3, 3) with [i, row, col] (the Eigen column-major matrix of each particle)."""
         Cm = self.get()[2]
         return Cm.reshape(-1, 3, 3).transpose(0, 2, 1)
+
+    def WriteToFile(self, filename, isAppend=False):
+        """ParticleSystem::WriteToFile (ParticleSystem.cpp:66-84): count line, then 'x,y,z,...,'."""
+        self._h.check(self._h.L.lh_write_positions(self._h.h, os.fspath(filename).encode(), 1 if isAppend else 0))
+
+    def LoadFromLine(self, line):
+        """ParticleSystem::LoadFromLine (ParticleSystem.cpp:86-101), first token skipped as in the reference."""
+        self._h.check(self._h.L.lh_load_positions_line(self._h.h, line.encode()))
 
 
 class CGSolver:
