@@ -125,6 +125,20 @@ int lh_g2p(lh_ctx* ctx);                             /* LiquidModel.cpp:723-823 
 int lh_advect(lh_ctx* ctx, float dt);                /* LiquidModel.cpp:912-935 */
 int lh_divergence(lh_ctx* ctx, double* mean, double* linf, int* count); /* LiquidModel.cpp:1065-1083 */
 
+/* ---- solver-level access (CGSolver<3>::BuildRHS / BuildLHS / Solve split,
+ * PressureSolver.cpp:682-698, 544-664, 839-934) ---------------------------- */
+/* BuildRHS + BuildLHS (all levels) from the current liquidPhi / liquidW / v */
+int lh_build_system(lh_ctx* ctx, float dt);
+/* solve A x = b on the built operator in the configured precision (FP32 MGPCG,
+ * or FP32 MGPCG under the FP64 refinement loop with "precision": "refined");
+ * b == NULL uses the built right-hand side; x0 (optional) is an initial guess,
+ * dropped when ||b - A x0|| > ||b||.  x is written to the pressure field. */
+int lh_solve_rhs(lh_ctx* ctx, const float* b, const float* x0, lh_solve_stats* stats);
+/* the last solution in FP64 (refined mode; the FP32 pressure widened otherwise) */
+int lh_get_solution64(lh_ctx* ctx, double* host, size_t count);
+/* initial guess used by the following pressure solves (NULL removes it) */
+int lh_set_warm_start(lh_ctx* ctx, const float* x0, size_t count);
+
 /* ---- data ---------------------------------------------------------------- */
 int lh_num_particles(const lh_ctx* ctx, int64_t* n);
 /* ParticleSystem<3>::Resize + SetPosition/SetVelocity/SetAffineMatrix

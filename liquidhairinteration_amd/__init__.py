@@ -116,6 +116,10 @@ def load_library():
         "lh_profile_enable": ([vp, C.c_int], C.c_int),
         "lh_profile_read": ([vp, C.c_int, ip, dp], C.c_int),
         "lh_get_time": ([vp, dp], C.c_int),
+        "lh_build_system": ([vp, C.c_float], C.c_int),
+        "lh_solve_rhs": ([vp, fp, fp, C.POINTER(SolveStats)], C.c_int),
+        "lh_get_solution64": ([vp, dp, C.c_size_t], C.c_int),
+        "lh_set_warm_start": ([vp, fp, C.c_size_t], C.c_int),
         "lh_write_positions": ([vp, C.c_char_p, C.c_int], C.c_int),
         "lh_load_positions_line": ([vp, C.c_char_p], C.c_int),
     }
@@ -231,6 +235,33 @@ class CGSolver:
         if rc not in (LH_OK, 3):
             self._h.check(rc)
         return self.last_stats
+
+    # split access: BuildRHS + BuildLHS, then solves on that operator
+    def BuildSystem(self, dt):
+        self._h.check(self._h.L.lh_build_system(self._h.h, float(dt)))
+
+    def SolveRHS(self, b=None, x0=None):
+        """Solve A x = b (b=None: the built RHS) from x0 (optional); x -> pressure. Returns stats."""
+        b = None if b is None else np.ascontiguousarray(b, np.float32)
+        x0 = None if x0 is None else np.ascontiguousarray(x0, np.float32)
+        st = SolveStats()
+        rc = self._h.L.lh_solve_rhs(self._h.h, _fptr(b), _fptr(x0), C.byref(st))
+        self.last_stats = st.as_dict()
+        if rc not in (LH_OK, 3):
+            self._h.check(rc)
+        return self.last_stats
+
+    def GetSolution64(self, shape):
+        x = np.empty(shape, np.float64)
+        self._h.check(self._h.L.lh_get_solution64(self._h.h, x.ctypes.data_as(C.POINTER(C.c_double)), x.size))
+        return x
+
+    def SetWarmStart(self, x0):
+        if x0 is None:
+            self._h.check(self._h.L.lh_set_warm_start(self._h.h, None, 0))
+        else:
+            x0 = np.ascontiguousarray(x0, np.float32)
+            self._h.check(self._h.L.lh_set_warm_start(self._h.h, _fptr(x0), x0.size))
 
 
 class LiquidModel:

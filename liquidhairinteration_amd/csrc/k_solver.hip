@@ -579,24 +579,146 @@ void apply_precond(Ctx& c) {
 
 }  // namespace
 
-// CGSolver<3>::Solve, PressureSolver.cpp:839-934: Init, BuildRHS, BuildLHS, PCG
-// from x = 0, pressure = x.
-int run_solve(Ctx& c, float dt, lh_solve_stats* st) {
-  lh_solve_stats S;
-  std::memset(&S, 0, sizeof(S));
-  hipEventRecord(c.ev0, c.s);
-  // Init (:666-680): r, z of every level and x, p are zeroed
+namespace {
+
+// ---------------------------------------------------------------- warm start
+// x = x0 on interior liquid cells (the cells the PCG updates, :875-877), 0 elsewhere
+__global__ void k_cg_set_x0(float* __restrict__ x, float* __restrict__ p, float* __restrict__ z,
+                            const float* __restrict__ x0, const uint8_t* __restrict__ cls, int d0, int d1, int d2) {
+  size_t total = (size_t)d0 * d1 * d2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    int zi = (int)(i % d2), yi = (int)((i / d2) % d1), xi = (int)(i / ((size_t)d1 * d2));
+    bool in = xi >= 1 && xi <= d0 - 2 && yi >= 1 && yi <= d1 - 2 && zi >= 1 && zi <= d2 - 2 && cls[i] != 0;
+    x[i] = in ? x0[i] : 0.0f;
+    p[i] = 0.0f;
+    z[i] = 0.0f;
+  }
+}
+// r = b - A x on liquid cells (0 elsewhere); partials of |b|^2 and |r|^2 over liquid cells
+__global__ void k_cg_warm_residual(float* __restrict__ r, const float* __restrict__ x, const float* __restrict__ b,
+                                   LvP L, double* __restrict__ partial) {
+  const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
+  size_t total = (size_t)d0 * d1 * d2;
+  double v[2] = {0.0, 0.0};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    float rv = 0.0f;
+    if (L.cls[i] != 0) {
+      int zi = (int)(i % d2), yi = (int)((i / d2) % d1), xi = (int)(i / ((size_t)d1 * d2));
+      rv = b[i] - get_ax(L, x, xi, yi, zi, i);
+      v[0] += (double)b[i] * (double)b[i];
+      v[1] += (double)rv * (double)rv;
+    }
+    r[i] = rv;
+  }
+  block_partials<2>(v, partial);
+}
+__global__ void k_sc_norms(const double* __restrict__ part, int nb, double* __restrict__ sc64) {
+  double bb = sum_partials_block(part, nb);
+  double rr = sum_partials_block(part + nb, nb);
+  if (threadIdx.x == 0) {
+    sc64[4] = bb;
+    sc64[5] = rr;
+  }
+}
+
+// ---------------------------------------------------------------- FP64 refinement
+__device__ __forceinline__ double get_ax64(const LvP& L, const double* __restrict__ z, int x, int y, int w, size_t c) {
+  double res = z[c] * (double)L.adiag[c];
+  if (L.cls[c] == 0) return res;
+  const size_t sx = (size_t)L.d[1] * L.d[2], sy = (size_t)L.d[2];
+  if (x + 1 < L.d[0] && L.cls[c + sx] == 2) res += z[c + sx] * (double)(-L.t0[c]);
+  if (x - 1 >= 0 && L.cls[c - sx] == 2) res += z[c - sx] * (double)(-L.t0[c - sx]);
+  if (y + 1 < L.d[1] && L.cls[c + sy] == 2) res += z[c + sy] * (double)(-L.t1[c]);
+  if (y - 1 >= 0 && L.cls[c - sy] == 2) res += z[c - sy] * (double)(-L.t1[c - sy]);
+  if (w + 1 < L.d[2] && L.cls[c + 1] == 2) res += z[c + 1] * (double)(-L.t2[c]);
+  if (w - 1 >= 0 && L.cls[c - 1] == 2) res += z[c - 1] * (double)(-L.t2[c - 1]);
+  return res;
+}
+__device__ __forceinline__ bool interior_liquid(const LvP& L, size_t i, int& xi, int& yi, int& zi) {
+  const int d1 = L.d[1], d2 = L.d[2];
+  zi = (int)(i % d2);
+  yi = (int)((i / d2) % d1);
+  xi = (int)(i / ((size_t)d1 * d2));
+  return xi >= 1 && xi <= L.d[0] - 2 && yi >= 1 && yi <= d1 - 2 && zi >= 1 && zi <= d2 - 2 && L.cls[i] != 0;
+}
+// r64 = b - A x64 on interior liquid cells (0 elsewhere); partials of |b|^2, |r64|^2 there
+__global__ void k_ir_residual(double* __restrict__ r64, const double* __restrict__ x64, const float* __restrict__ b,
+                              LvP L, double* __restrict__ partial) {
+  size_t total = (size_t)L.d[0] * L.d[1] * L.d[2];
+  double v[2] = {0.0, 0.0};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    int xi, yi, zi;
+    double rv = 0.0;
+    if (interior_liquid(L, i, xi, yi, zi)) {
+      rv = (double)b[i] - get_ax64(L, x64, xi, yi, zi, i);
+      v[0] += (double)b[i] * (double)b[i];
+      v[1] += rv * rv;
+    }
+    r64[i] = rv;
+  }
+  block_partials<2>(v, partial);
+}
+// FP32 right-hand side of the correction equation, scaled to unit norm
+__global__ void k_ir_rhs(float* __restrict__ rf, const double* __restrict__ r64, const double* __restrict__ sc64,
+                         size_t n) {
+  const double inv = (sc64[5] > 0.0) ? 1.0 / sqrt(sc64[5]) : 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    rf[i] = (float)(r64[i] * inv);
+}
+__global__ void k_ir_update(double* __restrict__ x64, const float* __restrict__ e, const double* __restrict__ sc64,
+                            size_t n) {
+  const double s = sqrt(sc64[5]);
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x64[i] += s * (double)e[i];
+}
+__global__ void k_ir_to_f32(float* __restrict__ out, const double* __restrict__ x64, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (float)x64[i];
+}
+__global__ void k_f32_to_f64(double* __restrict__ out, const float* __restrict__ in, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (double)in[i];
+}
+
+}  // namespace
+
+// Init (:666-680: r and z of every level zeroed), BuildRHS (:682-698), BuildLHS (:544-664)
+void build_system(Ctx& c, float dt) {
   for (auto& L : c.lv) {
     launch_fill_f32(c, L.r, L.n, 0.f);
     launch_fill_f32(c, L.z, L.n, 0.f);
   }
   launch_build_rhs(c);
   launch_build_lhs(c, dt);
+}
+
+// The PCG loop of CGSolver<3>::Solve (PressureSolver.cpp:845-931) on the operator
+// already built, right-hand side b (device, Array3 layout), optional initial
+// guess x0 (device).  With x0, the PCG starts from x0 restricted to the cells it
+// updates and falls back to x = 0 when ||b - A x0|| > ||b||.  The stop rule is
+// the configured one; in relative mode it is ||r|| <= tol * ||b||.  Result in c.x.
+int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, int max_iter, lh_solve_stats& S) {
   Level& L0 = c.lv[0];
   const size_t n = L0.n;
   const int nb = c.npartial;
+  LvP P0 = lvp(L0);
   hipMemsetAsync(c.dflag + 2, 0, sizeof(int), c.s);
-  hipLaunchKernelGGL(k_cg_init, dim3(grid_for(n)), dim3(kBlock), 0, c.s, L0.r, c.x, c.p, L0.z, c.rhs, L0.cls, n);
+  for (size_t l = 1; l < c.lv.size(); ++l) launch_fill_f32(c, c.lv[l].z, c.lv[l].n, 0.f);
+  double bnorm = -1.0;
+  S.warm_start_used = 0;
+  if (x0) {
+    hipLaunchKernelGGL(k_cg_set_x0, dim3(grid_for(n)), dim3(kBlock), 0, c.s, c.x, c.p, L0.z, x0, L0.cls, L0.d[0],
+                       L0.d[1], L0.d[2]);
+    hipLaunchKernelGGL(k_cg_warm_residual, dim3(nb), dim3(kBlock), 0, c.s, L0.r, c.x, b, P0, c.partial);
+    hipLaunchKernelGGL(k_sc_norms, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc64);
+    hipMemcpyAsync(c.h_pin + 16, c.dsc64 + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, c.s);
+    if (!hip_ok(c, hipStreamSynchronize(c.s), "warm start")) return LH_ERR_DEVICE;
+    bnorm = (double)sqrtf((float)c.h_pin[16]);
+    double r0 = (double)sqrtf((float)c.h_pin[17]);
+    S.warm_start_used = (r0 <= bnorm) ? 1 : 0;  // fallback: the zero guess is better
+  }
+  if (!S.warm_start_used)
+    hipLaunchKernelGGL(k_cg_init, dim3(grid_for(n)), dim3(kBlock), 0, c.s, L0.r, c.x, c.p, L0.z, b, L0.cls, n);
   apply_precond(c);
   hipLaunchKernelGGL(k_cg_first, dim3(nb), dim3(kBlock), 0, c.s, c.p, L0.z, L0.r, L0.cls, n, c.partial);
   hipLaunchKernelGGL(k_sc_first, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dsc64);
@@ -606,15 +728,14 @@ int run_solve(Ctx& c, float dt, lh_solve_stats* st) {
   std::memcpy(hsc, c.h_pin, sizeof(hsc));
   float residual = hsc[4];
   const float res0 = residual;
-  const double tol = (double)c.cfg.tol;
+  if (bnorm < 0.0) bnorm = (double)res0;  // cold start: ||r0|| = ||b|| over liquid cells
   auto done = [&](float res) {
-    if (c.cfg.tol_mode == LH_TOL_ABSOLUTE) return res < c.cfg.tol;
-    return (double)res <= tol * (double)res0;
+    if (tol_mode == LH_TOL_ABSOLUTE) return res < tol;
+    return (double)res <= (double)tol * bnorm;
   };
-  LvP P0 = lvp(L0);
   int it = 0;
   bool conv = false, nan = false;
-  for (it = 0; it < c.cfg.max_iter; ++it) {
+  for (it = 0; it < max_iter; ++it) {
     if (done(residual)) {
       conv = true;
       break;
@@ -642,20 +763,12 @@ int run_solve(Ctx& c, float dt, lh_solve_stats* st) {
       break;
     }
   }
-  if (!conv && !nan && it >= c.cfg.max_iter) conv = done(residual);
-  // m_grid.pressure.CopyFrom(m_x), :933
-  hipMemcpyAsync(c.pressure, c.x, n * sizeof(float), hipMemcpyDeviceToDevice, c.s);
-  hipEventRecord(c.ev1, c.s);
-  if (!hip_ok(c, hipEventSynchronize(c.ev1), "solve")) return LH_ERR_DEVICE;
-  float ms = 0.f;
-  hipEventElapsedTime(&ms, c.ev0, c.ev1);
-  S.iterations = it;
+  if (!conv && !nan && it >= max_iter) conv = done(residual);
+  S.iterations += it;
   S.converged = conv ? 1 : 0;
   S.initial_residual = res0;
   S.final_residual = residual;
-  S.rhs_norm = res0;
-  S.solve_ms = ms;
-  if (st) *st = S;
+  S.rhs_norm = bnorm;
   if (nan) {
     c.err = "PCG produced a NaN/Inf alpha or beta";
     return LH_ERR_NAN;
@@ -665,6 +778,99 @@ int run_solve(Ctx& c, float dt, lh_solve_stats* st) {
     return LH_ERR_NOT_CONVERGED;
   }
   return LH_OK;
+}
+
+namespace {
+
+bool ensure_fp64(Ctx& c) {
+  if (c.x64) return true;
+  if (hipMalloc((void**)&c.x64, c.ncell * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c.r64, c.ncell * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c.rf, c.ncell * sizeof(float)) != hipSuccess) {
+    c.err = "hipMalloc failed (FP64 refinement buffers)";
+    return false;
+  }
+  return true;
+}
+
+// Mixed-precision iterative refinement: the true residual r = b - A x is formed
+// in FP64 (FP64 arithmetic on the FP32 operator) over the cells the PCG updates;
+// each correction A e = r / ||r|| is solved by the FP32 MGPCG to the configured
+// tolerance and accumulated into the FP64 solution, until ||r|| <= tol64 ||b||.
+int refine_solve(Ctx& c, const float* b, const float* x0, lh_solve_stats& S) {
+  if (!ensure_fp64(c)) return LH_ERR_OOM;
+  Level& L0 = c.lv[0];
+  const size_t n = L0.n;
+  const int nb = c.npartial;
+  LvP P0 = lvp(L0);
+  if (x0) {
+    hipLaunchKernelGGL(k_cg_set_x0, dim3(grid_for(n)), dim3(kBlock), 0, c.s, c.x, c.p, L0.z, x0, L0.cls, L0.d[0],
+                       L0.d[1], L0.d[2]);
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_for(n)), dim3(kBlock), 0, c.s, c.x64, c.x, n);
+  } else {
+    hipMemsetAsync(c.x64, 0, n * sizeof(double), c.s);
+  }
+  int rc = LH_OK;
+  for (int outer = 0;; ++outer) {
+    hipLaunchKernelGGL(k_ir_residual, dim3(nb), dim3(kBlock), 0, c.s, c.r64, c.x64, b, P0, c.partial);
+    hipLaunchKernelGGL(k_sc_norms, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc64);
+    hipMemcpyAsync(c.h_pin + 16, c.dsc64 + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, c.s);
+    if (!hip_ok(c, hipStreamSynchronize(c.s), "refinement")) return LH_ERR_DEVICE;
+    const double bn = std::sqrt(c.h_pin[16]), rn = std::sqrt(c.h_pin[17]);
+    S.true_residual = (bn > 0.0) ? rn / bn : 0.0;
+    S.outer_iterations = outer;
+    if (outer == 0 && x0 && rn > bn) {
+      // the warm start is worse than the zero guess: restart from x = 0
+      hipMemsetAsync(c.x64, 0, n * sizeof(double), c.s);
+      x0 = nullptr;
+      continue;
+    }
+    if (rn <= c.cfg.refined_tol * bn || bn == 0.0) break;
+    if (outer >= c.cfg.max_outer) {
+      c.err = "FP64 refinement did not reach refinedTolerance";
+      rc = LH_ERR_NOT_CONVERGED;
+      break;
+    }
+    hipLaunchKernelGGL(k_ir_rhs, dim3(grid_for(n)), dim3(kBlock), 0, c.s, c.rf, c.r64, c.dsc64, n);
+    int irc = pcg_solve(c, c.rf, nullptr, c.cfg.tol, LH_TOL_RELATIVE, c.cfg.max_iter, S);
+    if (irc == LH_ERR_DEVICE || irc == LH_ERR_NAN) return irc;
+    hipLaunchKernelGGL(k_ir_update, dim3(grid_for(n)), dim3(kBlock), 0, c.s, c.x64, c.x, c.dsc64, n);
+  }
+  S.warm_start_used = x0 ? 1 : 0;
+  S.converged = (rc == LH_OK) ? 1 : 0;
+  S.rhs_norm = std::sqrt(c.h_pin[16]);
+  hipLaunchKernelGGL(k_ir_to_f32, dim3(grid_for(n)), dim3(kBlock), 0, c.s, c.x, c.x64, n);
+  return rc;
+}
+
+}  // namespace
+
+// solve A x = b on the built operator in the configured precision; x -> pressure
+int solve_built(Ctx& c, const float* b, const float* x0, lh_solve_stats* st, bool record_start) {
+  lh_solve_stats S;
+  std::memset(&S, 0, sizeof(S));
+  if (record_start) hipEventRecord(c.ev0, c.s);
+  int rc;
+  if (c.cfg.refined) rc = refine_solve(c, b, x0, S);
+  else rc = pcg_solve(c, b, x0, c.cfg.tol, c.cfg.tol_mode, c.cfg.max_iter, S);
+  if (rc == LH_ERR_DEVICE) return rc;
+  // m_grid.pressure.CopyFrom(m_x), :933
+  hipMemcpyAsync(c.pressure, c.x, c.ncell * sizeof(float), hipMemcpyDeviceToDevice, c.s);
+  hipEventRecord(c.ev1, c.s);
+  if (!hip_ok(c, hipEventSynchronize(c.ev1), "solve")) return LH_ERR_DEVICE;
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c.ev0, c.ev1);
+  S.solve_ms = ms;
+  if (st) *st = S;
+  return rc;
+}
+
+// CGSolver<3>::Solve, PressureSolver.cpp:839-934: Init, BuildRHS, BuildLHS, PCG
+// from x = 0 (or the warm start, when one is installed), pressure = x.
+int run_solve(Ctx& c, float dt, lh_solve_stats* st) {
+  hipEventRecord(c.ev0, c.s);
+  build_system(c, dt);
+  return solve_built(c, c.rhs, c.warm_x0, st, false);
 }
 
 }  // namespace lh

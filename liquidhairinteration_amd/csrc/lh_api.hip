@@ -264,6 +264,7 @@ void free_ctx(Ctx& c) {
     F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2);
   }
   F(c.p); F(c.x); F(c.q); F(c.partial); F(c.dsc); F(c.dsc64); F(c.dmaxv); F(c.dflag);
+  F(c.x64); F(c.r64); F(c.rf); F(c.b_user); F(c.x0_user);
   F(c.xv_snap); F(c.xv_final); F(c.xcount); F(c.xlist[0]); F(c.xlist[1]); F(c.xlist[2]);
   if (c.h_pin) hipHostFree(c.h_pin);
   if (c.h_flag) hipHostFree(c.h_flag);
@@ -865,6 +866,77 @@ int lh_profile_read(lh_ctx* ctx, int kernel, int* launches, double* total_ms) {
   if (launches) *launches = c.prof_used[kernel];
   if (total_ms) *total_ms = tot;
   return LH_OK;
+}
+
+int lh_build_system(lh_ctx* ctx, float dt) {
+  return guarded(ctx, [dt](Ctx& c) {
+    build_system(c, dt);
+    return hip_ok(c, hipStreamSynchronize(c.s), "build system") ? LH_OK : LH_ERR_DEVICE;
+  });
+}
+
+namespace {
+int upload_cells(Ctx& c, float*& dst, const float* host) {
+  if (!dst && hipMalloc((void**)&dst, c.ncell * sizeof(float)) != hipSuccess) {
+    c.err = "hipMalloc failed";
+    return LH_ERR_OOM;
+  }
+  hipMemcpyAsync(dst, host, c.ncell * sizeof(float), hipMemcpyHostToDevice, c.s);
+  return LH_OK;
+}
+}  // namespace
+
+int lh_solve_rhs(lh_ctx* ctx, const float* b, const float* x0, lh_solve_stats* stats) {
+  return guarded(ctx, [=](Ctx& c) {
+    const float* db = c.rhs;
+    const float* dx0 = nullptr;
+    if (b) {
+      int rc = upload_cells(c, c.b_user, b);
+      if (rc) return rc;
+      db = c.b_user;
+    }
+    if (x0) {
+      int rc = upload_cells(c, c.x0_user, x0);
+      if (rc) return rc;
+      dx0 = c.x0_user;
+    }
+    return solve_built(c, db, dx0, stats);
+  });
+}
+
+int lh_get_solution64(lh_ctx* ctx, double* host, size_t count) {
+  return guarded(ctx, [=](Ctx& c) {
+    if (!host || count != c.ncell) {
+      c.err = "lh_get_solution64: bad buffer";
+      return LH_ERR_ARG;
+    }
+    if (c.cfg.refined && c.x64) {
+      hipMemcpyAsync(host, c.x64, count * sizeof(double), hipMemcpyDeviceToHost, c.s);
+      return hip_ok(c, hipStreamSynchronize(c.s), "get solution") ? LH_OK : LH_ERR_DEVICE;
+    }
+    std::vector<float> tmp(count);
+    hipMemcpyAsync(tmp.data(), c.pressure, count * sizeof(float), hipMemcpyDeviceToHost, c.s);
+    if (!hip_ok(c, hipStreamSynchronize(c.s), "get solution")) return LH_ERR_DEVICE;
+    for (size_t i = 0; i < count; ++i) host[i] = tmp[i];
+    return LH_OK;
+  });
+}
+
+int lh_set_warm_start(lh_ctx* ctx, const float* x0, size_t count) {
+  return guarded(ctx, [=](Ctx& c) {
+    if (!x0) {
+      c.warm_x0 = nullptr;
+      return LH_OK;
+    }
+    if (count != c.ncell) {
+      c.err = "lh_set_warm_start: bad element count";
+      return LH_ERR_ARG;
+    }
+    int rc = upload_cells(c, c.x0_user, x0);
+    if (rc) return rc;
+    c.warm_x0 = c.x0_user;
+    return hip_ok(c, hipStreamSynchronize(c.s), "warm start") ? LH_OK : LH_ERR_DEVICE;
+  });
 }
 
 int lh_write_positions(lh_ctx* ctx, const char* path, int append) {

@@ -96,6 +96,11 @@ struct Ctx {
   // solver (CGSolver<3>)
   std::vector<Level> lv;
   float *p = nullptr, *x = nullptr, *q = nullptr;
+  double *x64 = nullptr, *r64 = nullptr;  // FP64 refinement solution and true residual
+  float* rf = nullptr;                     // FP32 right-hand side of a refinement correction
+  float* b_user = nullptr;                 // right-hand side handed in through lh_solve_rhs
+  float* x0_user = nullptr;                // initial guess handed in through lh_solve_rhs / lh_set_warm_start
+  float* warm_x0 = nullptr;                // installed initial guess for the next pressure solve (or null)
   double* partial = nullptr;  // per-block FP64 partial sums
   int npartial = 0;
   float* dsc = nullptr;       // device scalars: [0] oldRZ [1] newRZ [2] alpha [3] beta [4] residual ...
@@ -166,6 +171,9 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt);
 void launch_g2p_advect(Ctx& c, bool do_g2p, bool do_advect, float dt);
 // ---- launchers (k_solver.hip)
 int run_solve(Ctx& c, float dt, lh_solve_stats* st);  // CGSolver<3>::Solve
+void build_system(Ctx& c, float dt);                  // Init + BuildRHS + BuildLHS
+// solve A x = b on the built operator (configured precision), x -> pressure
+int solve_built(Ctx& c, const float* b, const float* x0, lh_solve_stats* st, bool record_start = true);
 void launch_build_rhs(Ctx& c);
 void launch_build_lhs(Ctx& c, float dt);
 

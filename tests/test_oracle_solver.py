@@ -1,0 +1,95 @@
+"""The oracle's pressure operator and PCG against an independent sparse direct solve.
+
+A is assembled from the oracle's level-0 Adiag / Acoef (BuildLHS,
+PressureSolver.cpp:544-664) over the cells the PCG updates (interior liquid
+cells); scipy's spsolve in FP64 gives the exact solution of that system.
+"""
+import json
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import config_file
+from oracle import Oracle
+
+
+def advanced(N=16, smoother="symmetric", steps=3):
+    cfg = json.load(open(config_file("dam_break_48.json")))
+    cfg["gridDimension"] = [N, N, N]
+    cfg["Boundary"]["child"]["halfExtent"] = [0.5 - 3.0 / N] * 3
+    cfg["SolverParameter"].update({"smoother": smoother, "numLevel": 2, "maxIterations": 400})
+    orc = Oracle(cfg, dot_mode=1, p2g_order=1)
+    orc.init()
+    dt = np.float32(0.01)
+    for _ in range(steps):
+        orc.step(dt)
+        dt = np.float32(orc.get_cfl()[0])
+    orc.clear_grid()
+    orc.compute_liquid_phi()
+    orc.p2g()
+    orc.apply_gravity(dt)
+    orc.build_system(dt)
+    return orc
+
+
+def assemble(orc):
+    ad, co = orc.level(0)
+    N = ad.shape
+    phi = orc.get(10)
+    idx = np.arange(ad.size).reshape(N)
+    inter = np.zeros(N, bool)
+    inter[1:-1, 1:-1, 1:-1] = True
+    # liquid cells the PCG updates; rows with a zero diagonal (cells whose faces are all
+    # closed, inside the walls) have b = 0 and stay at x = 0, so they are left out
+    rows = np.flatnonzero((phi.reshape(-1) <= 0) & inter.reshape(-1) & (ad.reshape(-1) != 0))
+    pos = -np.ones(ad.size, np.int64)
+    pos[rows] = np.arange(len(rows))
+    I, J, V = [], [], []
+    offs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    for r in rows:
+        x, y, z = np.unravel_index(r, N)
+        I.append(pos[r]); J.append(pos[r]); V.append(float(ad[x, y, z]))
+        for k, (dx, dy, dz) in enumerate(offs):
+            c = float(co[x, y, z, k])
+            if c == 0.0:
+                continue
+            nb = idx[x + dx, y + dy, z + dz]
+            assert pos[nb] >= 0, "coupling to a cell outside the solved set"
+            I.append(pos[r]); J.append(pos[nb]); V.append(c)
+    A = sp.csr_matrix((V, (I, J)), shape=(len(rows), len(rows)))
+    return A, rows
+
+
+def test_operator_is_symmetric_and_diagonally_dominant():
+    orc = advanced()
+    A, rows = assemble(orc)
+    assert len(rows) > 100
+    assert abs(A - A.T).max() == 0.0              # symmetric (no cell has phi == 0 here)
+    d = A.diagonal()
+    off = np.asarray(abs(A).sum(axis=1)).ravel() - np.abs(d)
+    assert np.all(d >= off * (1 - 1e-6))          # weakly diagonally dominant -> SPD with a Dirichlet row
+    assert np.any(d > off * (1 + 1e-3))           # free-surface (ghost fluid) rows are strictly dominant
+
+
+@pytest.mark.parametrize("smoother", ["symmetric", "reference"])
+def test_pcg_matches_sparse_direct_solve(smoother):
+    orc = advanced(smoother=smoother)
+    A, rows = assemble(orc)
+    b = orc.get(13).reshape(-1)
+    x_direct = spla.spsolve(A.tocsc(), b[rows].astype(np.float64))
+    x64, st64 = orc.solve_fp64(b.astype(np.float64), rel_tol=1e-13, max_iter=1000)
+    assert st64.converged
+    e64 = np.linalg.norm(x64.reshape(-1)[rows] - x_direct) / np.linalg.norm(x_direct)
+    assert e64 < 1e-9
+    x32, st32 = orc.solve_fp32(b)
+    assert st32.converged
+    e32 = np.linalg.norm(x32.reshape(-1)[rows].astype(np.float64) - x_direct) / np.linalg.norm(x_direct)
+    assert e32 < 1e-4
+    # x stays zero outside the interior liquid cells (PressureSolver.cpp:875-877, 891-893)
+    phi = orc.get(10)
+    upd = np.zeros(phi.shape, bool)
+    upd[1:-1, 1:-1, 1:-1] = True
+    upd &= phi <= 0
+    assert np.all(x32.reshape(phi.shape)[~upd] == 0)
