@@ -8,8 +8,10 @@ One step = PreStep + Step + PostStep + GetCFL (main.cpp:50-53) run by the librar
 native loop; particles and grid stay resident in HBM for the whole timed region.
 
 Prints one JSON line: steps/s, the live roofline of the dominant kernel (HIP
-events around its launches inside the timed region) and the CPU baseline (the
-oracle, timed on this host for one step of the same workload).
+events around its launches inside the timed region), the CPU baseline (the
+oracle, timed on this host for one step of the same workload) and, under
+"warm_start", ms and iterations per solve to 1e-6 relative residual of the learned
+initial guess versus plain MGPCG on identical systems of held-out 128^3 scenes.
 """
 import argparse
 import json
@@ -63,6 +65,26 @@ def cpu_baseline(cfg, steps=1):
                       "OpenMP over the host cores (%.1f s)" % (steps, el)}
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA peak
+
+
+def warm_start_eval(grid, scenes, steps):
+    """ms per solve to 1e-6 relative residual, learned vs plain MGPCG, on held-out scenes"""
+    from liquidhairinteration_amd import warmstart as W
+
+    if not os.path.exists(W.WEIGHTS):
+        return None
+    s, _ = W.evaluate(W.WEIGHTS, n=grid, scenes=scenes, steps=steps, seed=12345, log=lambda *a: None)
+    out = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()}
+    out["solver"] = "MGPCG L=3 pre/post=2/2 symmetric, tol 1e-06 relative, unit-cube walls"
+    out["scenes"] = "%d held-out dam/collision scenes x %d steps (first step of each excluded)" % (scenes, steps)
+    if s.get("conv_d0_tflops"):
+        out["mfma"] = {"kernel": "conv 48->16 at full resolution (the largest layer)", "bound": "mfma",
+                       "achieved": round(s["conv_d0_tflops"], 1), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": round(s["conv_d0_tflops"] / MFMA_BF16_PEAK_TFLOPS, 4)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +93,10 @@ def main():
     ap.add_argument("--config", default=os.path.join(ROOT, "liquidhairinteration_amd", "configs", "collision_256.json"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-warm-start", action="store_true", help="skip the learned-vs-plain solve comparison")
+    ap.add_argument("--ws-grid", type=int, default=128)
+    ap.add_argument("--ws-scenes", type=int, default=2)
+    ap.add_argument("--ws-steps", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
@@ -94,7 +120,7 @@ def main():
     t0 = time.perf_counter()
     dt, last = model.RunSteps(args.steps, dt)
     el = time.perf_counter() - t0
-    prof = {k: model.profile_read(k) for k in model.PROF_KERNELS}
+    prof = {k: model.profile_read(k) for k in ("smooth_l0", "p2g", "g2p", "liquid_phi", "spmv")}
     model.profile_enable(False)
 
     # dominant instrumented kernel by total device time in the timed region
@@ -109,7 +135,7 @@ def main():
                    "pct_of_timed": round(100.0 * v[1] / (el * 1e3), 1)} for k, v in prof.items()}
 
     out = {
-        "metric": "liquid steps/s at 256^3",
+        "metric": "liquid steps/s at 256^3; ms per solve to 1e-6 residual, learned vs plain MGPCG",
         "value": round(args.steps / el, 4),
         "unit": "steps/s",
         "n_gpus": 1,
@@ -126,7 +152,10 @@ def main():
                    "solver": "MGPCG L=%d pre/post=%d/%d %s, tol %g %s" % (
                        cfg["SolverParameter"]["numLevel"], cfg["SolverParameter"]["numPreSmooth"],
                        cfg["SolverParameter"]["numPostSmooth"], cfg["SolverParameter"].get("smoother", "reference"),
-                       cfg["SolverParameter"]["tolerance"], cfg["SolverParameter"].get("toleranceMode", "absolute"))},
+                       cfg["SolverParameter"]["tolerance"], cfg["SolverParameter"].get("toleranceMode", "absolute")),
+                   "precision": cfg.get("precision", "fp32") + (
+                       " (FP32 MGPCG inside an FP64 residual loop to %g)" % cfg.get("refinedTolerance", 1e-9)
+                       if cfg.get("precision") == "refined" else "")},
         "roofline": roof,
         "kernels": kernels,
         "last_step": {"pcg_iterations": last["solve"]["iterations"], "converged": last["solve"]["converged"],
@@ -135,6 +164,8 @@ def main():
         "cpu_baseline": None,
     }
     scene.close()
+    if not args.no_warm_start:
+        out["warm_start"] = warm_start_eval(args.ws_grid, args.ws_scenes, args.ws_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
     print(json.dumps(out))

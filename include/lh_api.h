@@ -139,6 +139,45 @@ int lh_get_solution64(lh_ctx* ctx, double* host, size_t count);
 /* initial guess used by the following pressure solves (NULL removes it) */
 int lh_set_warm_start(lh_ctx* ctx, const float* x0, size_t count);
 
+/* ---- learned initial guess --------------------------------------------------
+ * No reference counterpart: the reference's CGSolver<3>::Init starts every solve
+ * from x = 0 (PressureSolver.cpp:666-680).  A 3D convolutional encoder-decoder
+ * (channels 16/32/64/96, 3x3x3 kernels, BF16 weights and activations, FP32
+ * accumulation) maps (divergence / rms, cell type +1 liquid / 0 air / -1 solid)
+ * on the level-0 grid to a pressure guess; the guess is restricted to the cells
+ * the PCG updates and scaled by alpha = (x0.b)/(x0.A x0), and the PCG still runs
+ * to the configured tolerance from it, falling back to x = 0 whenever
+ * ||b - A x0|| > ||b||.  Grid dimensions must be multiples of 8.
+ * Tensors handed across this ABI are channels-last: in[v * 2 + ch] with v the
+ * Array3 index (z fastest) of dims[0] x dims[1] x dims[2]. */
+int lh_warmstart_init(lh_ctx* ctx, uint64_t seed);      /* fresh He-initialised network */
+int lh_warmstart_load(lh_ctx* ctx, const char* path);
+int lh_warmstart_save(lh_ctx* ctx, const char* path);
+/* on != 0: every following pressure solve starts from the network's guess */
+int lh_warmstart_enable(lh_ctx* ctx, int on);
+/* number of parameters; get/set them (per layer W[cout][27][cin] with tap index
+ * (dx+1)*9 + (dy+1)*3 + (dz+1), then bias[cout]; layers E0 E1 E2 E3 D2 D1 D0 OUT) */
+int lh_warmstart_num_params(lh_ctx* ctx, int64_t* n);
+int lh_warmstart_get_params(lh_ctx* ctx, float* w, int64_t n);
+int lh_warmstart_set_params(lh_ctx* ctx, const float* w, int64_t n);
+/* target scale: the training target is p / rms(p over interior liquid cells) / scale */
+int lh_warmstart_set_target_scale(lh_ctx* ctx, float scale);
+/* network output y[v] for the input in[v * 2 + ch] */
+int lh_warmstart_forward(lh_ctx* ctx, const int dims[3], const float* in, float* y);
+/* loss = sum mask (y - target)^2 / sum mask and its gradient (parameter layout
+ * above; grad may be NULL); with lr > 0 also one Adam step (beta 0.9 / 0.999) */
+int lh_warmstart_step(lh_ctx* ctx, const int dims[3], const float* in, const float* target, const float* mask,
+                      float lr, float* grad, float* loss);
+/* training loss on the simulator's systems: 0 (default) masked MSE against
+ * p / rms(p); 1 energy norm: 1 - (y.b)^2 / ((y.Ay)(x*.b)), the squared A-norm error of
+ * the energy-scaled guess relative to that of the zero guess */
+int lh_warmstart_set_loss(lh_ctx* ctx, int mode);
+/* loss and gradient (grad may be NULL) on the last built system and its solved
+ * pressure; with lr > 0 also one Adam step */
+int lh_warmstart_train(lh_ctx* ctx, float lr, float* grad, float* loss);
+/* the last built system's training pair: in (ncell x 2, BF16-rounded), target, mask */
+int lh_warmstart_sample(lh_ctx* ctx, float* in, float* target, float* mask);
+
 /* ---- data ---------------------------------------------------------------- */
 int lh_num_particles(const lh_ctx* ctx, int64_t* n);
 /* ParticleSystem<3>::Resize + SetPosition/SetVelocity/SetAffineMatrix
@@ -158,7 +197,10 @@ enum lh_prof_kernel {
   LH_PROF_G2P = 2,         /* grid-to-particle + advection */
   LH_PROF_LIQUID_PHI = 3,  /* liquid SDF gather */
   LH_PROF_SPMV = 4,        /* PCG A*p + dot */
-  LH_PROF_COUNT = 5
+  LH_PROF_NN_FWD = 5,      /* warm-start network forward (all layers) */
+  LH_PROF_NN_BWD = 6,      /* warm-start network backward (all layers, gradients only) */
+  LH_PROF_CONV_D0 = 7,     /* the network's largest convolution: 48 -> 16 channels at full resolution */
+  LH_PROF_COUNT = 8
 };
 int lh_profile_enable(lh_ctx* ctx, int on);
 int lh_profile_read(lh_ctx* ctx, int kernel, int* launches, double* total_ms);

@@ -122,6 +122,19 @@ def load_library():
         "lh_set_warm_start": ([vp, fp, C.c_size_t], C.c_int),
         "lh_write_positions": ([vp, C.c_char_p, C.c_int], C.c_int),
         "lh_load_positions_line": ([vp, C.c_char_p], C.c_int),
+        "lh_warmstart_init": ([vp, C.c_uint64], C.c_int),
+        "lh_warmstart_load": ([vp, C.c_char_p], C.c_int),
+        "lh_warmstart_save": ([vp, C.c_char_p], C.c_int),
+        "lh_warmstart_enable": ([vp, C.c_int], C.c_int),
+        "lh_warmstart_num_params": ([vp, C.POINTER(C.c_int64)], C.c_int),
+        "lh_warmstart_get_params": ([vp, fp, C.c_int64], C.c_int),
+        "lh_warmstart_set_params": ([vp, fp, C.c_int64], C.c_int),
+        "lh_warmstart_set_target_scale": ([vp, C.c_float], C.c_int),
+        "lh_warmstart_forward": ([vp, ip, fp, fp], C.c_int),
+        "lh_warmstart_step": ([vp, ip, fp, fp, fp, C.c_float, fp, fp], C.c_int),
+        "lh_warmstart_train": ([vp, C.c_float, fp, fp], C.c_int),
+        "lh_warmstart_set_loss": ([vp, C.c_int], C.c_int),
+        "lh_warmstart_sample": ([vp, fp, fp, fp], C.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -264,6 +277,91 @@ class CGSolver:
             self._h.check(self._h.L.lh_set_warm_start(self._h.h, _fptr(x0), x0.size))
 
 
+class WarmStart:
+    """Learned initial guess of the pressure solve (no reference counterpart: the
+    reference's CGSolver<3>::Init starts from x = 0, PressureSolver.cpp:666-680).
+
+    A 3D convolutional encoder-decoder (16/32/64/96 channels, 3x3x3 kernels, BF16,
+    FP32 accumulation, trained and run by the library's own HIP kernels).  Arrays are
+    channels-last numpy arrays of the grid shape: inputs [d0, d1, d2, 2], outputs
+    [d0, d1, d2].
+    """
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def init(self, seed=0):
+        self._h.check(self._h.L.lh_warmstart_init(self._h.h, int(seed)))
+
+    def load(self, path):
+        self._h.check(self._h.L.lh_warmstart_load(self._h.h, os.fspath(path).encode()))
+
+    def save(self, path):
+        self._h.check(self._h.L.lh_warmstart_save(self._h.h, os.fspath(path).encode()))
+
+    def enable(self, on=True):
+        """use the network's guess in every following pressure solve"""
+        self._h.check(self._h.L.lh_warmstart_enable(self._h.h, 1 if on else 0))
+
+    def num_params(self):
+        n = C.c_int64()
+        self._h.check(self._h.L.lh_warmstart_num_params(self._h.h, C.byref(n)))
+        return int(n.value)
+
+    def get_params(self):
+        w = np.empty(self.num_params(), np.float32)
+        self._h.check(self._h.L.lh_warmstart_get_params(self._h.h, _fptr(w), w.size))
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._h.check(self._h.L.lh_warmstart_set_params(self._h.h, _fptr(w), w.size))
+
+    def set_target_scale(self, s):
+        self._h.check(self._h.L.lh_warmstart_set_target_scale(self._h.h, float(s)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        dims = (C.c_int * 3)(*x.shape[:3])
+        y = np.empty(x.shape[:3], np.float32)
+        self._h.check(self._h.L.lh_warmstart_forward(self._h.h, dims, _fptr(x), _fptr(y)))
+        return y
+
+    def step(self, x, target, mask, lr=0.0, want_grad=True):
+        """loss and gradient for (x, target, mask); with lr > 0 also one Adam step"""
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        m = np.ascontiguousarray(mask, np.float32)
+        dims = (C.c_int * 3)(*x.shape[:3])
+        g = np.empty(self.num_params(), np.float32) if want_grad else None
+        loss = C.c_float()
+        self._h.check(self._h.L.lh_warmstart_step(self._h.h, dims, _fptr(x), _fptr(t), _fptr(m), float(lr), _fptr(g),
+                                                  C.byref(loss)))
+        return float(loss.value), g
+
+    LOSSES = {"mse": 0, "energy": 1}
+
+    def set_loss(self, mode):
+        """'mse' (p / rms(p) target) or 'energy' (A-norm error of the energy-scaled guess)"""
+        self._h.check(self._h.L.lh_warmstart_set_loss(self._h.h, self.LOSSES.get(mode, mode)))
+
+    def train(self, lr, want_grad=False):
+        """loss (and gradient) on the last built system and its solved pressure; with lr > 0
+        also one Adam step.  Returns the loss, or (loss, grad) with want_grad."""
+        loss = C.c_float()
+        g = np.empty(self.num_params(), np.float32) if want_grad else None
+        self._h.check(self._h.L.lh_warmstart_train(self._h.h, float(lr), _fptr(g), C.byref(loss)))
+        return (float(loss.value), g) if want_grad else float(loss.value)
+
+    def sample(self, shape):
+        """(input [.., 2], target, mask) of the last built system"""
+        x = np.empty(tuple(shape) + (2,), np.float32)
+        t = np.empty(shape, np.float32)
+        m = np.empty(shape, np.float32)
+        self._h.check(self._h.L.lh_warmstart_sample(self._h.h, _fptr(x), _fptr(t), _fptr(m)))
+        return x, t, m
+
+
 class LiquidModel:
     """LiquidModel<3> (LiquidModel.h:11-82) on the GPU."""
 
@@ -271,7 +369,11 @@ class LiquidModel:
         self._h = handle
         self._ps = ParticleSystem(handle)
         self._solver = CGSolver(handle)
+        self._warm = WarmStart(handle)
         self.last_step = None
+
+    def GetWarmStart(self):
+        return self._warm
 
     # LiquidModel.cpp:506-537
     def Init(self):
@@ -372,7 +474,8 @@ class LiquidModel:
         self._h.check(self._h.L.lh_set_field(self._h.h, f, a.ctypes.data, a.size))
 
     # live kernel timing (HIP events on the library's stream)
-    PROF_KERNELS = {"smooth_l0": 0, "p2g": 1, "g2p": 2, "liquid_phi": 3, "spmv": 4}
+    PROF_KERNELS = {"smooth_l0": 0, "p2g": 1, "g2p": 2, "liquid_phi": 3, "spmv": 4, "nn_fwd": 5, "nn_bwd": 6,
+                    "conv_d0": 7}
 
     def profile_enable(self, on=True):
         self._h.check(self._h.L.lh_profile_enable(self._h.h, 1 if on else 0))

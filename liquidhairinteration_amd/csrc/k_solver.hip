@@ -24,6 +24,7 @@
 #include <cstring>
 
 #include "lh_ctx.h"
+#include "lh_nn.h"
 #include "lh_math.h"
 
 namespace lh {
@@ -865,12 +866,99 @@ int solve_built(Ctx& c, const float* b, const float* x0, lh_solve_stats* st, boo
   return rc;
 }
 
+namespace {
+// partials of (x . b, x . A x) over the level-0 liquid cells (x is zero outside the PCG set)
+__global__ void k_ritz_partials(const float* __restrict__ x, const float* __restrict__ b, LvP L,
+                                double* __restrict__ partial) {
+  const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
+  size_t total = (size_t)d0 * d1 * d2;
+  double v[2] = {0.0, 0.0};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    if (L.cls[i] == 0 || x[i] == 0.0f) continue;
+    int zi = (int)(i % d2), yi = (int)((i / d2) % d1), xi = (int)(i / ((size_t)d1 * d2));
+    const double ax = (double)get_ax(L, x, xi, yi, zi, i);
+    v[0] += (double)x[i] * (double)b[i];
+    v[1] += (double)x[i] * ax;
+  }
+  block_partials<2>(v, partial);
+}
+__global__ void k_ritz_final(const double* __restrict__ part, int nb, double* __restrict__ sc64) {
+  double xb = sum_partials_block(part, nb);
+  double xax = sum_partials_block(part + nb, nb);
+  if (threadIdx.x == 0) sc64[6] = (xax > 0.0) ? xb / xax : 0.0;
+}
+__global__ void k_ritz_scale(float* __restrict__ x, const double* __restrict__ sc64, size_t n) {
+  const float a = (float)sc64[6];
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] *= a;
+}
+}  // namespace
+
+namespace {
+// ay = A y on liquid cells; partials of (y . b, y . A y, xs . b) over liquid cells
+__global__ void k_energy_partials(const float* __restrict__ y, const float* __restrict__ b,
+                                  const float* __restrict__ xs, LvP L, float* __restrict__ ay,
+                                  double* __restrict__ partial) {
+  const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
+  size_t total = (size_t)d0 * d1 * d2;
+  double v[3] = {0.0, 0.0, 0.0};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    float a = 0.0f;
+    if (L.cls[i] != 0) {
+      int zi = (int)(i % d2), yi = (int)((i / d2) % d1), xi = (int)(i / ((size_t)d1 * d2));
+      a = get_ax(L, y, xi, yi, zi, i);
+      v[0] += (double)y[i] * (double)b[i];
+      v[1] += (double)y[i] * (double)a;
+      v[2] += (double)xs[i] * (double)b[i];
+    }
+    ay[i] = a;
+  }
+  block_partials<3>(v, partial);
+}
+__global__ void k_energy_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  double s0 = sum_partials_block(part, nb);
+  double s1 = sum_partials_block(part + nb, nb);
+  double s2 = sum_partials_block(part + 2 * nb, nb);
+  if (threadIdx.x == 0) {
+    out[0] = s0;
+    out[1] = s1;
+    out[2] = s2;
+  }
+}
+}  // namespace
+
+// ay = A y and out[0..2] = (y . b, y . A y, xs . b) on the built level-0 operator (device)
+void launch_energy_terms(Ctx& c, const float* y, const float* b, const float* xs, float* ay, double* out) {
+  Level& L0 = c.lv[0];
+  const int nb = c.npartial;
+  hipLaunchKernelGGL(k_energy_partials, dim3(nb), dim3(kBlock), 0, c.s, y, b, xs, lvp(L0), ay, c.partial);
+  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, out);
+}
+
+// x0 <- alpha x0, alpha = (x0 . b) / (x0 . A x0): the multiple of the guess closest to
+// the solution in the A-norm, so ||x* - alpha x0||_A <= ||x*||_A (the zero guess's error)
+void launch_warm_ritz(Ctx& c, float* x0, const float* b) {
+  Level& L0 = c.lv[0];
+  const int nb = c.npartial;
+  hipLaunchKernelGGL(k_ritz_partials, dim3(nb), dim3(kBlock), 0, c.s, x0, b, lvp(L0), c.partial);
+  hipLaunchKernelGGL(k_ritz_final, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc64);
+  hipLaunchKernelGGL(k_ritz_scale, dim3(grid_for(L0.n)), dim3(kBlock), 0, c.s, x0, c.dsc64, L0.n);
+}
+
 // CGSolver<3>::Solve, PressureSolver.cpp:839-934: Init, BuildRHS, BuildLHS, PCG
-// from x = 0 (or the warm start, when one is installed), pressure = x.
+// from x = 0 (or the installed / learned initial guess), pressure = x.
 int run_solve(Ctx& c, float dt, lh_solve_stats* st) {
   hipEventRecord(c.ev0, c.s);
   build_system(c, dt);
-  return solve_built(c, c.rhs, c.warm_x0, st, false);
+  c.last_dt = dt;
+  const float* x0 = c.warm_x0;
+  if (c.warm_learned && c.net) {
+    float* g = nullptr;
+    int rc = nn_predict(c, c.rhs, &g);
+    if (rc) return rc;
+    x0 = g;
+  }
+  return solve_built(c, c.rhs, x0, st, false);
 }
 
 }  // namespace lh

@@ -15,6 +15,7 @@
 #include "lh_ctx.h"
 #include "lh_json.h"
 #include "lh_math.h"
+#include "lh_nn.h"
 
 using lh::Ctx;
 
@@ -266,6 +267,8 @@ void free_ctx(Ctx& c) {
   F(c.p); F(c.x); F(c.q); F(c.partial); F(c.dsc); F(c.dsc64); F(c.dmaxv); F(c.dflag);
   F(c.x64); F(c.r64); F(c.rf); F(c.b_user); F(c.x0_user);
   F(c.xv_snap); F(c.xv_final); F(c.xcount); F(c.xlist[0]); F(c.xlist[1]); F(c.xlist[2]);
+  nn_free(c.net);
+  c.net = nullptr;
   if (c.h_pin) hipHostFree(c.h_pin);
   if (c.h_flag) hipHostFree(c.h_flag);
   if (c.ev0) hipEventDestroy(c.ev0);
@@ -501,6 +504,7 @@ int do_step(Ctx& c, float dt, lh_solve_stats* st) {
   launch_g2p_advect(c, true, true, dt);  // Grid2Particle + AdvectParticle (+ max|v|)
   c.bins_valid = false;
   c.time += (double)dt;
+  c.last_solve = local;
   int rc = check_domain(c);
   if (rc) return rc;
   rc = read_maxv(c);
@@ -721,6 +725,7 @@ int lh_post_step(lh_ctx* ctx, float dt, lh_step_stats* stats) {
   (void)dt;
   return guarded(ctx, [stats](Ctx& c) {
     if (stats) {
+      stats->solve = c.last_solve;
       launch_divergence(c, &stats->divergence_mean, &stats->divergence_linf, &stats->fluid_cells);
       stats->max_velocity = c.last_maxv;
       stats->cfl_dt = cfl_from(c);
@@ -871,6 +876,7 @@ int lh_profile_read(lh_ctx* ctx, int kernel, int* launches, double* total_ms) {
 int lh_build_system(lh_ctx* ctx, float dt) {
   return guarded(ctx, [dt](Ctx& c) {
     build_system(c, dt);
+    c.last_dt = dt;
     return hip_ok(c, hipStreamSynchronize(c.s), "build system") ? LH_OK : LH_ERR_DEVICE;
   });
 }
@@ -899,6 +905,13 @@ int lh_solve_rhs(lh_ctx* ctx, const float* b, const float* x0, lh_solve_stats* s
       int rc = upload_cells(c, c.x0_user, x0);
       if (rc) return rc;
       dx0 = c.x0_user;
+    } else if (c.warm_learned && c.net) {
+      // the learned guess: inference, scaling and r0 are inside the timed solve
+      hipEventRecord(c.ev0, c.s);
+      float* g = nullptr;
+      int rc = nn_predict(c, db, &g);
+      if (rc) return rc;
+      return solve_built(c, db, g, stats, false);
     }
     return solve_built(c, db, dx0, stats);
   });
@@ -937,6 +950,83 @@ int lh_set_warm_start(lh_ctx* ctx, const float* x0, size_t count) {
     c.warm_x0 = c.x0_user;
     return hip_ok(c, hipStreamSynchronize(c.s), "warm start") ? LH_OK : LH_ERR_DEVICE;
   });
+}
+
+int lh_warmstart_init(lh_ctx* ctx, uint64_t seed) {
+  return guarded(ctx, [=](Ctx& c) { return nn_create(c, seed); });
+}
+int lh_warmstart_load(lh_ctx* ctx, const char* path) {
+  if (!path) return LH_ERR_ARG;
+  return guarded(ctx, [=](Ctx& c) { return nn_load(c, path); });
+}
+int lh_warmstart_save(lh_ctx* ctx, const char* path) {
+  if (!path) return LH_ERR_ARG;
+  return guarded(ctx, [=](Ctx& c) { return nn_save(c, path); });
+}
+int lh_warmstart_enable(lh_ctx* ctx, int on) {
+  return guarded(ctx, [=](Ctx& c) {
+    if (on && !c.net) {
+      c.err = "no warm-start network: call lh_warmstart_init or lh_warmstart_load first";
+      return LH_ERR_ARG;
+    }
+    c.warm_learned = (on != 0);
+    return LH_OK;
+  });
+}
+int lh_warmstart_num_params(lh_ctx* ctx, int64_t* n) {
+  return guarded(ctx, [=](Ctx& c) {
+    if (!c.net || !n) {
+      c.err = "no warm-start network";
+      return LH_ERR_ARG;
+    }
+    *n = (int64_t)c.net->nlog;
+    return LH_OK;
+  });
+}
+int lh_warmstart_get_params(lh_ctx* ctx, float* w, int64_t n) {
+  return guarded(ctx, [=](Ctx& c) { return nn_get_params(c, w, n, false); });
+}
+int lh_warmstart_set_params(lh_ctx* ctx, const float* w, int64_t n) {
+  return guarded(ctx, [=](Ctx& c) { return nn_set_params(c, w, n); });
+}
+int lh_warmstart_set_target_scale(lh_ctx* ctx, float scale) {
+  return guarded(ctx, [=](Ctx& c) {
+    if (!c.net || !(scale > 0.f)) {
+      c.err = "lh_warmstart_set_target_scale: no network or scale <= 0";
+      return LH_ERR_ARG;
+    }
+    c.net->target_scale = scale;
+    return LH_OK;
+  });
+}
+int lh_warmstart_forward(lh_ctx* ctx, const int dims[3], const float* in, float* y) {
+  if (!dims) return LH_ERR_ARG;
+  return guarded(ctx, [=](Ctx& c) { return nn_forward_host(c, dims, in, y); });
+}
+int lh_warmstart_step(lh_ctx* ctx, const int dims[3], const float* in, const float* target, const float* mask,
+                      float lr, float* grad, float* loss) {
+  if (!dims) return LH_ERR_ARG;
+  return guarded(ctx, [=](Ctx& c) { return nn_step_host(c, dims, in, target, mask, lr, grad, loss); });
+}
+int lh_warmstart_set_loss(lh_ctx* ctx, int mode) {
+  return guarded(ctx, [=](Ctx& c) {
+    if (!c.net || (mode != 0 && mode != 1)) {
+      c.err = "lh_warmstart_set_loss: no network or mode not 0 / 1";
+      return LH_ERR_ARG;
+    }
+    c.net->loss_mode = mode;
+    return LH_OK;
+  });
+}
+int lh_warmstart_train(lh_ctx* ctx, float lr, float* grad, float* loss) {
+  return guarded(ctx, [=](Ctx& c) {
+    int rc = nn_train_last_solve(c, lr, loss);
+    if (rc == LH_OK && grad) rc = nn_get_params(c, grad, (int64_t)c.net->nlog, true);
+    return rc;
+  });
+}
+int lh_warmstart_sample(lh_ctx* ctx, float* in, float* target, float* mask) {
+  return guarded(ctx, [=](Ctx& c) { return nn_sample_last_solve(c, in, target, mask); });
 }
 
 int lh_write_positions(lh_ctx* ctx, const char* path, int append) {

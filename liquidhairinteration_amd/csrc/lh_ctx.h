@@ -11,6 +11,8 @@
 
 namespace lh {
 
+struct Net;  // learned initial guess (lh_nn.h)
+
 // Scene configuration: the reference's four parameter structs
 // (SceneParameter.h:9-152) plus the build's optional keys.
 struct Config {
@@ -101,6 +103,10 @@ struct Ctx {
   float* b_user = nullptr;                 // right-hand side handed in through lh_solve_rhs
   float* x0_user = nullptr;                // initial guess handed in through lh_solve_rhs / lh_set_warm_start
   float* warm_x0 = nullptr;                // installed initial guess for the next pressure solve (or null)
+  Net* net = nullptr;                      // learned initial guess network (or null)
+  bool warm_learned = false;               // use the network's guess in every pressure solve
+  float last_dt = 0.f;                     // dt of the last built system
+  lh_solve_stats last_solve = {};          // statistics of the last step's pressure solve
   double* partial = nullptr;  // per-block FP64 partial sums
   int npartial = 0;
   float* dsc = nullptr;       // device scalars: [0] oldRZ [1] newRZ [2] alpha [3] beta [4] residual ...

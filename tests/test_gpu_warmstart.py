@@ -1,0 +1,260 @@
+"""GPU tests of the learned initial guess (k_nn.hip) through the C ABI.
+
+Forward pass and the full parameter gradient of the HIP network (BF16 MFMA
+convolutions, FP32 accumulation) against the CPU oracle (oracle/nn_oracle.py),
+which rounds to BF16 at the same points and sums in FP64.  Tolerances: the two
+differ only where FP32 accumulation order moves a value across a BF16 rounding
+boundary, so the output agrees to 2e-3 and the gradient of every layer to 2e-2
+(relative L2).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import liquidhairinteration_amd as lh
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import nn_oracle as nno  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(lh.__file__), "weights", "warmstart.lhws")
+
+
+def scene_cfg(n=16):
+    """dam break at n^3 (walls 3 cells in), 3-level symmetric MGPCG to 1e-6 relative"""
+    cfg = json.load(open(lh.config_path("dam_break_48")))
+    cfg["gridDimension"] = [n, n, n]
+    cfg["Boundary"]["child"]["halfExtent"] = [0.5 - 3.0 / n] * 3
+    cfg["SolverParameter"].update({"smoother": "symmetric", "smoothingFactor": 1.0, "numLevel": 3,
+                                   "numPreSmooth": 2, "numPostSmooth": 2})
+    return cfg
+
+
+def scene(n=16):
+    return lh.Scene(scene_cfg(n))
+
+
+def rand_params(seed):
+    rng = np.random.default_rng(seed)
+    L = nno.unflatten(nno.init_params(seed))
+    return nno.flatten([(w, rng.normal(0, 0.05, b.shape)) for w, b in L]).astype(np.float32)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def layer_slices():
+    offs = np.cumsum([0] + [co * 27 * ci + co for ci, co in zip(nno.CIN, nno.COUT)])
+    return [slice(a, b) for a, b in zip(offs[:-1], offs[1:])]
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16), (8, 16, 24)])
+def test_forward_matches_oracle(dims):
+    print("warmstart forward", dims, flush=True)
+    sc = scene()
+    ws = sc.GetLiquidModel().GetWarmStart()
+    ws.init(1)
+    assert ws.num_params() == nno.num_params()
+    theta = rand_params(7)
+    ws.set_params(theta)
+    np.testing.assert_array_equal(ws.get_params(), theta)
+    rng = np.random.default_rng(8)
+    x = rng.normal(size=dims + (2,)).astype(np.float32)
+    x[..., 1] = rng.integers(-1, 2, dims)
+    y = ws.forward(x)
+    yo, _ = nno.forward(theta.astype(np.float64), x, rnd=True)
+    e = rel(y, yo)
+    print("forward rel L2", e)
+    assert e < 2e-3
+
+
+def test_gradient_matches_oracle():
+    print("warmstart gradient 16^3", flush=True)
+    dims = (16, 16, 16)
+    sc = scene()
+    ws = sc.GetLiquidModel().GetWarmStart()
+    ws.init(2)
+    theta = rand_params(9)
+    ws.set_params(theta)
+    rng = np.random.default_rng(10)
+    x = rng.normal(size=dims + (2,)).astype(np.float32)
+    x[..., 1] = rng.integers(-1, 2, dims)
+    t = rng.normal(size=dims).astype(np.float32)
+    m = (rng.random(dims) < 0.6).astype(np.float32)
+    loss, g = ws.step(x, t, m, lr=0.0)
+    lo, go, _ = nno.loss_grad(theta.astype(np.float64), x, t, m, rnd=True)
+    assert abs(loss - lo) <= 1e-3 * abs(lo)
+    for name, s in zip(nno.NAMES, layer_slices()):
+        e = rel(g[s], go[s])
+        print("grad", name, "rel L2", e)
+        assert e < 2e-2, name
+    # lr = 0 leaves the parameters unchanged
+    np.testing.assert_array_equal(ws.get_params(), theta)
+
+
+def test_adam_reduces_loss():
+    print("warmstart adam 16^3", flush=True)
+    dims = (16, 16, 16)
+    sc = scene()
+    ws = sc.GetLiquidModel().GetWarmStart()
+    ws.init(3)
+    rng = np.random.default_rng(11)
+    x = rng.normal(size=dims + (2,)).astype(np.float32)
+    t = np.tanh(x[..., 0]).astype(np.float32)
+    m = np.ones(dims, np.float32)
+    l0, _ = ws.step(x, t, m, lr=0.0, want_grad=False)
+    for _ in range(60):
+        l1, _ = ws.step(x, t, m, lr=2e-3, want_grad=False)
+    l1, _ = ws.step(x, t, m, lr=0.0, want_grad=False)
+    print("loss", l0, "->", l1)
+    assert l1 < 0.5 * l0
+
+
+def test_training_pair_from_simulation(tmp_path):
+    print("warmstart sample 32^3 dam", flush=True)
+    n = 32
+    sc = scene(n)
+    model = sc.GetLiquidModel()
+    model.Init()
+    ws = model.GetWarmStart()
+    ws.init(4)
+    dt = 0.01
+    for _ in range(3):
+        model.PreStep(dt)
+        model.Step(dt)
+        model.PostStep(dt)
+        dt = model.GetCFL()
+    x, t, m = ws.sample((n, n, n))
+    phi = model.get_field("liquidPhi")
+    rhs = model.get_field("rhs")
+    liquid = phi <= 0
+    # cell types
+    assert set(np.unique(x[..., 1])) <= {-1.0, 0.0, 1.0}
+    np.testing.assert_array_equal(x[..., 1] == 1.0, liquid)
+    # the wall cells (outside the 3-cell inset walls) are solid
+    assert (x[0, :, :, 1] == -1.0).all() and (x[-1, :, :, 1] == -1.0).all()
+    # divergence channel: b / rms(b over liquid), BF16
+    s = np.sqrt(np.mean(rhs[liquid].astype(np.float64) ** 2))
+    np.testing.assert_allclose(x[..., 0][liquid], rhs[liquid] / s, rtol=2.0 ** -8, atol=1e-30)
+    # mask: interior liquid cells; target zero outside it
+    inner = np.zeros_like(liquid)
+    inner[1:-1, 1:-1, 1:-1] = True
+    np.testing.assert_array_equal(m == 1.0, liquid & inner)
+    assert (t[m == 0] == 0).all()
+    # target = p * dt / (rho dx^2) / rms(b): one common factor times the solved pressure
+    p = model.get_field("pressure")
+    assert np.isfinite(t).all()
+    nz = (m == 1) & (p != 0)
+    ratio = t[nz].astype(np.float64) / p[nz]
+    assert ratio.size > 0 and np.allclose(ratio, ratio[0], rtol=1e-5)
+    # save / load round trip
+    path = str(tmp_path / "w.lhws")
+    ws.save(path)
+    w0 = ws.get_params()
+    ws.init(5)
+    ws.load(path)
+    np.testing.assert_array_equal(ws.get_params(), w0)
+
+
+def test_learned_guess_never_breaks_the_solve():
+    """an untrained network: the solve still reaches the tolerance from the (scaled) guess or
+    falls back to x = 0, and the pressure matches the plain solve to the tolerance"""
+    print("warmstart untrained 32^3", flush=True)
+    n = 32
+    res = []
+    for learned in (False, True):
+        sc = scene(n)
+        model = sc.GetLiquidModel()
+        model.Init()
+        if learned:
+            ws = model.GetWarmStart()
+            ws.init(6)
+            ws.enable(True)
+        dt = 0.01
+        for _ in range(4):
+            model.PreStep(dt)
+            model.Step(dt)
+            model.PostStep(dt)
+            dt = model.GetCFL()
+        p = model.get_field("pressure")
+        res.append((p, model.last_step))
+    (p0, s0), (p1, s1) = res
+    assert s1["solve"]["converged"] == 1
+    assert s1["solve"]["final_residual"] <= 1e-6 * s1["solve"]["rhs_norm"] * 1.0001
+    # both solutions solve the same system to 1e-6 relative residual
+    assert rel(p1, p0) < 1e-3
+
+
+def test_energy_loss():
+    """energy loss on a simulator system: 0 <= L <= 1; its derivative w.r.t. the output bias
+    (an FP32 parameter, y -> y + delta on every cell) matches a central difference; Adam on
+    the fixed system lowers it"""
+    print("warmstart energy loss 32^3", flush=True)
+    n = 32
+    sc = scene(n)
+    model = sc.GetLiquidModel()
+    model.Init()
+    ws = model.GetWarmStart()
+    ws.init(8)
+    ws.set_loss("energy")
+    dt = 0.01
+    for _ in range(3):
+        model.PreStep(dt)
+        model.Step(dt)
+        model.PostStep(dt)
+        dt = model.GetCFL()
+    L0, g = ws.train(0.0, want_grad=True)
+    assert 0.0 <= L0 <= 1.0
+    theta = ws.get_params()
+    ib = theta.size - 1  # bias of the output layer
+    y = ws.forward(ws.sample((n, n, n))[0])
+    h = 1e-3 * float(np.sqrt(np.mean(y.astype(np.float64) ** 2)))
+    lp, lm = [], []
+    for s, out in ((h, lp), (-h, lm)):
+        th = theta.copy()
+        th[ib] += s
+        ws.set_params(th)
+        out.append(ws.train(0.0))
+    fd = (lp[0] - lm[0]) / (2 * h)
+    print("energy loss", L0, "d/d bias", g[ib], "central difference", fd)
+    assert abs(fd - g[ib]) <= 0.05 * abs(g[ib]) + 1e-6
+    ws.set_params(theta)
+    for _ in range(30):
+        ws.train(1e-3)
+    L1 = ws.train(0.0)
+    print("energy loss after 30 Adam steps", L1)
+    assert L1 < L0
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="trained weights not present")
+def test_trained_network_gain():
+    """the committed network (trained by the library's own kernels on 20k simulator systems)
+    against the zero guess, on held-out scenes (seeds never used in training):
+    * the energy error of its scaled guess is a fraction of the zero guess's (L = 1);
+    * the PCG from it needs fewer iterations on average, and every solve converges."""
+    from liquidhairinteration_amd import warmstart as W
+    print("warmstart trained gain 64^3", flush=True)
+    summary, r = W.evaluate(WEIGHTS, n=64, scenes=4, steps=12, seed=777)
+    print(summary)
+    assert summary["learned_iterations"] < summary["plain_iterations"]
+    assert summary["warm_start_used_fraction"] >= 0.75
+    # energy error of the learned guess on held-out systems
+    sim = W.Simulator(64)
+    sim.ws.load(WEIGHTS)
+    sim.ws.set_loss("energy")
+    rng = np.random.default_rng(4242)
+    losses = []
+    while len(losses) < 20:
+        if not sim.reset(rng, "dam" if len(losses) % 2 == 0 else "collision"):
+            continue
+        for _ in range(5):
+            sim.step()
+            losses.append(sim.ws.train(0.0))
+    print("held-out energy error ratio (zero guess = 1): mean %.3f median %.3f" % (np.mean(losses),
+                                                                                  np.median(losses)))
+    assert np.median(losses) < 0.5
