@@ -28,8 +28,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
 def algorithmic_bytes(kernel, n0, P):
     """Bytes per launch counted once per array element the kernel must touch.
 
-    smooth_l0: one fused red-black sweep (both colours) = 28 B/cell
-               (Adiag + 3 face terms = 16 B, r 4 B, z read 4 B + write 4 B);
+    smooth_l0: one colour pass = half of a red-black sweep, whose minimum is 28 B/cell
+               (Adiag + 3 face terms = 16 B, r 4 B, z read 4 B + write 4 B) -> 14 B/cell
+               (the fused one-launch sweep, when selected, counts 28 B/cell);
     spmv:      p read, q write, 4 coefficient floats -> 24 B/cell;
     p2g:       one component: 20 B/particle (position, velocity component,
                affine column) + 4 B per face written (~n0 faces);
@@ -38,7 +39,7 @@ def algorithmic_bytes(kernel, n0, P):
     liquid_phi: 12 B/particle + 4 B/cell.
     """
     return {
-        "smooth_l0": 28.0 * n0,
+        "smooth_l0": 14.0 * n0,
         "spmv": 24.0 * n0,
         "p2g": 20.0 * P + 4.0 * n0,
         "g2p": 72.0 * P + 12.0 * n0,

@@ -192,7 +192,7 @@ int lh_set_field(lh_ctx* ctx, int field, const void* host, size_t count);
  * prints residuals under _DEBUG, PressureSolver.cpp:925).  While enabled, the listed kernels are bracketed
  * by hipEventRecord pairs; lh_profile_read returns launch count and summed ms. */
 enum lh_prof_kernel {
-  LH_PROF_SMOOTH_L0 = 0,   /* red-black smoother sweep on the finest level */
+  LH_PROF_SMOOTH_L0 = 0,   /* red-black smoother on the finest level: one colour pass (one fused sweep when fused) */
   LH_PROF_P2G = 1,         /* particle-to-grid gather (one component per launch) */
   LH_PROF_G2P = 2,         /* grid-to-particle + advection */
   LH_PROF_LIQUID_PHI = 3,  /* liquid SDF gather */

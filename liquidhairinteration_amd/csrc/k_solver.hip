@@ -193,6 +193,8 @@ __global__ void k_build_rhs(float* __restrict__ b, const float* __restrict__ phi
 // ---------------------------------------------------------------- V-cycle
 // Smooth, PressureSolver.cpp:821-837: mode 0 is (1. - w) * z + w * term with the
 // reference's double/float promotion (quirk Q4); mode 1 is z + w * term.
+// one colour pass; kLevel0 only gives the finest level its own symbol in kernel traces
+template <bool kLevel0>
 __global__ void k_smooth(float* __restrict__ z, const float* __restrict__ r, LvP L, int phase, float w, int mode) {
   const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
   size_t total = (size_t)d0 * d1 * d2;
@@ -530,14 +532,21 @@ namespace {
 // Large levels use the fused tile kernel; small ones (launch-bound) two colour passes.
 void sweep(Ctx& c, int l, int first, int mode) {
   Level& L = c.lv[l];
-  ProfScope ps(c, l == 0 ? LH_PROF_SMOOTH_L0 : LH_PROF_COUNT);
   if ((long long)L.n < c.cfg.fused_sweep_min_cells) {
-    hipLaunchKernelGGL(k_smooth, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), first, c.cfg.omega,
-                       mode);
-    hipLaunchKernelGGL(k_smooth, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), 1 - first,
-                       c.cfg.omega, mode);
+    for (int k = 0; k < 2; ++k) {
+      const int phase = k == 0 ? first : 1 - first;
+      if (l == 0) {
+        ProfScope ps(c, LH_PROF_SMOOTH_L0);
+        hipLaunchKernelGGL(k_smooth<true>, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), phase,
+                           c.cfg.omega, mode);
+      } else {
+        hipLaunchKernelGGL(k_smooth<false>, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), phase,
+                           c.cfg.omega, mode);
+      }
+    }
     return;
   }
+  ProfScope ps(c, l == 0 ? LH_PROF_SMOOTH_L0 : LH_PROF_COUNT);
   unsigned nb = (unsigned)(div_up(L.d[0], kSwX) * div_up(L.d[1], kSwY) * div_up(L.d[2], kSwZ));
   if (mode == 0)
     hipLaunchKernelGGL(k_rbgs_sweep<0>, dim3(nb), dim3(256), 0, c.s, L.z2, L.z, L.r, lvp(L), first, c.cfg.omega);
@@ -833,7 +842,12 @@ int refine_solve(Ctx& c, const float* b, const float* x0, lh_solve_stats& S) {
       break;
     }
     hipLaunchKernelGGL(k_ir_rhs, dim3(grid_for(n)), dim3(kBlock), 0, c.s, c.rf, c.r64, c.dsc64, n);
-    int irc = pcg_solve(c, c.rf, nullptr, c.cfg.tol, LH_TOL_RELATIVE, c.cfg.max_iter, S);
+    // inner tolerance: the correction only has to bring ||r|| under tol64 ||b||, so it is
+    // solved to max(tol, 0.25 tol64 ||b|| / ||r||) relative (at most 0.1), never tighter than
+    // the configured FP32 tolerance
+    double inner = std::max((double)c.cfg.tol, 0.25 * c.cfg.refined_tol * bn / rn);
+    inner = std::min(inner, 0.1);
+    int irc = pcg_solve(c, c.rf, nullptr, (float)inner, LH_TOL_RELATIVE, c.cfg.max_iter, S);
     if (irc == LH_ERR_DEVICE || irc == LH_ERR_NAN) return irc;
     hipLaunchKernelGGL(k_ir_update, dim3(grid_for(n)), dim3(kBlock), 0, c.s, c.x64, c.x, c.dsc64, n);
   }
