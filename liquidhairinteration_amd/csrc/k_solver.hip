@@ -72,6 +72,65 @@ __device__ __forceinline__ float get_ax(const LvP& L, const float* __restrict__ 
   return res;
 }
 
+// ---------------------------------------------------------------- active z-rows
+// The kernels below visit only z-rows (x, y) that hold a liquid cell at their level:
+// every cell they update has cls != 0, and every array they reduce is zero on cells
+// with cls == 0 (r is zeroed there, adiag is 0 there), so skipping the other rows
+// changes no value.  R = kBlock / d2 rows per block pass (one row, strided, if d2 >= kBlock).
+struct RowIter {
+  int R, sub, z0, zs;
+  bool on;
+};
+__device__ __forceinline__ RowIter row_iter(int d2) {
+  RowIter it;
+  if (d2 >= kBlock) {
+    it.R = 1;
+    it.sub = 0;
+    it.z0 = threadIdx.x;
+    it.zs = kBlock;
+  } else {
+    it.R = kBlock / d2;
+    it.sub = threadIdx.x / d2;
+    it.z0 = threadIdx.x - it.sub * d2;
+    it.zs = d2;
+  }
+  it.on = it.sub < it.R;
+  return it;
+}
+
+// flag[row] = 1 when the z-row holds a cell with cls != 0; one wave per row
+__global__ void k_row_flags(const uint8_t* __restrict__ cls, int* __restrict__ flag, int nrow, int d2) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (row >= nrow) return;
+  bool any = false;
+  for (int z = lane; z < d2; z += 64) any = any || cls[(size_t)row * d2 + z] != 0;
+  const bool hit = __ballot(any) != 0;
+  if (lane == 0) flag[row] = hit ? 1 : 0;
+}
+// ascending list of the flagged rows (one block of 1024 threads; deterministic order)
+__global__ __launch_bounds__(1024) void k_row_compact(const int* __restrict__ flag, int nrow, int* __restrict__ rows,
+                                                      int* __restrict__ nrows) {
+  __shared__ int s[1024];
+  const int t = threadIdx.x;
+  const int chunk = (nrow + 1023) / 1024;
+  const int b = t * chunk, e = min(nrow, b + chunk);
+  int cnt = 0;
+  for (int i = b; i < e; ++i) cnt += flag[i];
+  s[t] = cnt;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = (t >= off) ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  int pos = s[t] - cnt;
+  for (int i = b; i < e; ++i)
+    if (flag[i]) rows[pos++] = i;
+  if (t == 1023) *nrows = s[1023];
+}
+
 // ---------------------------------------------------------------- assembly
 struct FineP {
   int N[3];
@@ -193,22 +252,36 @@ __global__ void k_build_rhs(float* __restrict__ b, const float* __restrict__ phi
 // ---------------------------------------------------------------- V-cycle
 // Smooth, PressureSolver.cpp:821-837: mode 0 is (1. - w) * z + w * term with the
 // reference's double/float promotion (quirk Q4); mode 1 is z + w * term.
-// one colour pass; kLevel0 only gives the finest level its own symbol in kernel traces
+// one colour pass over the active rows; kLevel0 only gives the finest level its own
+// symbol in kernel traces.  Thread -> (row, z of the pass's colour in that row).
 template <bool kLevel0>
-__global__ void k_smooth(float* __restrict__ z, const float* __restrict__ r, LvP L, int phase, float w, int mode) {
-  const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
-  size_t total = (size_t)d0 * d1 * d2;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    int zi = (int)(i % d2), yi = (int)((i / d2) % d1), xi = (int)(i / ((size_t)d1 * d2));
-    if (((xi + yi + zi) & 1) != phase) continue;
-    float ad = L.adiag[i];
-    if (is_small(ad)) continue;
-    float term = (r[i] - get_ax(L, z, xi, yi, zi, i)) / ad;
-    if (mode == 0) {
-      float wt = w * term;
-      z[i] = (float)((1.0 - (double)w) * (double)z[i] + (double)wt);
-    } else {
-      z[i] = z[i] + w * term;
+__global__ __launch_bounds__(kBlock) void k_smooth(float* __restrict__ z, const float* __restrict__ r, LvP L,
+                                                   const int* __restrict__ rows, const int* __restrict__ nrows,
+                                                   int phase, float w, int mode) {
+  const int d1 = L.d[1], d2 = L.d[2];
+  const int H = (d2 + 1) >> 1;  // cells of one colour per row (at most)
+  const int R = H >= kBlock ? 1 : kBlock / H;
+  const int sub = H >= kBlock ? 0 : (int)threadIdx.x / H;
+  if (sub >= R) return;
+  const int h0 = (int)threadIdx.x - sub * (H >= kBlock ? 0 : H), hs = H >= kBlock ? kBlock : H;
+  const int nr = *nrows;
+  for (int k = blockIdx.x * R + sub; k < nr; k += gridDim.x * R) {
+    const int row = rows[k];
+    const int xi = row / d1, yi = row - xi * d1;
+    const int par = (xi + yi + phase) & 1;
+    for (int h = h0; h < H; h += hs) {
+      const int zi = 2 * h + par;
+      if (zi >= d2) continue;
+      const size_t i = (size_t)row * d2 + zi;
+      float ad = L.adiag[i];
+      if (is_small(ad)) continue;
+      float term = (r[i] - get_ax(L, z, xi, yi, zi, i)) / ad;
+      if (mode == 0) {
+        float wt = w * term;
+        z[i] = (float)((1.0 - (double)w) * (double)z[i] + (double)wt);
+      } else {
+        z[i] = z[i] + w * term;
+      }
     }
   }
 }
@@ -386,68 +459,109 @@ __global__ void k_cg_init(float* __restrict__ r, float* __restrict__ x, float* _
   }
 }
 
-// p = z on liquid cells; partials of r.z and r.r over all cells (:856-867)
+// p = z on liquid cells; partials of r.z and r.r (:856-867; r is zero off the liquid cells)
 __global__ void k_cg_first(float* __restrict__ p, const float* __restrict__ z, const float* __restrict__ r,
-                           const uint8_t* __restrict__ cls, size_t n, double* __restrict__ partial) {
+                           const uint8_t* __restrict__ cls, const int* __restrict__ rows,
+                           const int* __restrict__ nrows, int d2, double* __restrict__ partial) {
   double v[2] = {0.0, 0.0};
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    float zi = z[i], ri = r[i];
-    if (cls[i] != 0) p[i] = zi;
-    v[0] += (double)ri * (double)zi;
-    v[1] += (double)ri * (double)ri;
-  }
+  const RowIter it = row_iter(d2);
+  const int nr = *nrows;
+  if (it.on)
+    for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
+      const size_t base = (size_t)rows[k] * d2;
+      for (int zi = it.z0; zi < d2; zi += it.zs) {
+        const size_t i = base + zi;
+        float zv = z[i], rv = r[i];
+        if (cls[i] != 0) p[i] = zv;
+        v[0] += (double)rv * (double)zv;
+        v[1] += (double)rv * (double)rv;
+      }
+    }
   block_partials<2>(v, partial);
 }
 
 // q = A p and partial p.q over interior liquid cells (:873-882)
-__global__ void k_cg_spmv(float* __restrict__ q, const float* __restrict__ p, LvP L, double* __restrict__ partial) {
+__global__ void k_cg_spmv(float* __restrict__ q, const float* __restrict__ p, LvP L, const int* __restrict__ rows,
+                          const int* __restrict__ nrows, double* __restrict__ partial) {
   const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
-  size_t total = (size_t)d0 * d1 * d2;
   double v[1] = {0.0};
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    int zi = (int)(i % d2), yi = (int)((i / d2) % d1), xi = (int)(i / ((size_t)d1 * d2));
-    if (xi < 1 || xi > d0 - 2 || yi < 1 || yi > d1 - 2 || zi < 1 || zi > d2 - 2) continue;
-    if (L.cls[i] == 0) continue;
-    float ap = get_ax(L, p, xi, yi, zi, i);
-    q[i] = ap;
-    v[0] += (double)p[i] * (double)ap;
-  }
+  const RowIter it = row_iter(d2);
+  const int nr = *nrows;
+  if (it.on)
+    for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
+      const int row = rows[k];
+      const int xi = row / d1, yi = row - xi * d1;
+      if (xi < 1 || xi > d0 - 2 || yi < 1 || yi > d1 - 2) continue;
+      const size_t base = (size_t)row * d2;
+      for (int zi = it.z0; zi < d2; zi += it.zs) {
+        if (zi < 1 || zi > d2 - 2) continue;
+        const size_t i = base + zi;
+        if (L.cls[i] == 0) continue;
+        float ap = get_ax(L, p, xi, yi, zi, i);
+        q[i] = ap;
+        v[0] += (double)p[i] * (double)ap;
+      }
+    }
   block_partials<1>(v, partial);
 }
 
 // x += alpha p; r -= alpha q; partial r.r over interior liquid cells (:889-902)
 __global__ void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
                                const float* __restrict__ q, const uint8_t* __restrict__ cls,
-                               const float* __restrict__ sc, int d0, int d1, int d2, double* __restrict__ partial) {
+                               const float* __restrict__ sc, int d0, int d1, int d2, const int* __restrict__ rows,
+                               const int* __restrict__ nrows, double* __restrict__ partial) {
   const float alpha = sc[2];
-  size_t total = (size_t)d0 * d1 * d2;
   double v[1] = {0.0};
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    int zi = (int)(i % d2), yi = (int)((i / d2) % d1), xi = (int)(i / ((size_t)d1 * d2));
-    if (xi < 1 || xi > d0 - 2 || yi < 1 || yi > d1 - 2 || zi < 1 || zi > d2 - 2) continue;
-    if (cls[i] == 0) continue;
-    x[i] += alpha * p[i];
-    float rn = r[i] - alpha * q[i];
-    r[i] = rn;
-    v[0] += (double)rn * (double)rn;
-  }
+  const RowIter it = row_iter(d2);
+  const int nr = *nrows;
+  if (it.on)
+    for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
+      const int row = rows[k];
+      const int xi = row / d1, yi = row - xi * d1;
+      if (xi < 1 || xi > d0 - 2 || yi < 1 || yi > d1 - 2) continue;
+      const size_t base = (size_t)row * d2;
+      for (int zi = it.z0; zi < d2; zi += it.zs) {
+        if (zi < 1 || zi > d2 - 2) continue;
+        const size_t i = base + zi;
+        if (cls[i] == 0) continue;
+        x[i] += alpha * p[i];
+        float rn = r[i] - alpha * q[i];
+        r[i] = rn;
+        v[0] += (double)rn * (double)rn;
+      }
+    }
   block_partials<1>(v, partial);
 }
 
-__global__ void k_cg_dot_rz(const float* __restrict__ r, const float* __restrict__ z, size_t n,
-                            double* __restrict__ partial) {
+// partial r.z (r is zero off the liquid cells)
+__global__ void k_cg_dot_rz(const float* __restrict__ r, const float* __restrict__ z, const int* __restrict__ rows,
+                            const int* __restrict__ nrows, int d2, double* __restrict__ partial) {
   double v[1] = {0.0};
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    v[0] += (double)r[i] * (double)z[i];
+  const RowIter it = row_iter(d2);
+  const int nr = *nrows;
+  if (it.on)
+    for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
+      const size_t base = (size_t)rows[k] * d2;
+      for (int zi = it.z0; zi < d2; zi += it.zs) v[0] += (double)r[base + zi] * (double)z[base + zi];
+    }
   block_partials<1>(v, partial);
 }
 
 // p = z + beta p on liquid cells (:913-921)
 __global__ void k_cg_update_p(float* __restrict__ p, const float* __restrict__ z, const uint8_t* __restrict__ cls,
-                              const float* __restrict__ sc, size_t n) {
+                              const float* __restrict__ sc, const int* __restrict__ rows,
+                              const int* __restrict__ nrows, int d2) {
   const float beta = sc[3];
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    if (cls[i] != 0) p[i] = z[i] + beta * p[i];
+  const RowIter it = row_iter(d2);
+  const int nr = *nrows;
+  if (!it.on) return;
+  for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
+    const size_t base = (size_t)rows[k] * d2;
+    for (int zi = it.z0; zi < d2; zi += it.zs) {
+      const size_t i = base + zi;
+      if (cls[i] != 0) p[i] = z[i] + beta * p[i];
+    }
+  }
 }
 
 __device__ double sum_partials_block(const double* __restrict__ part, int n) {
@@ -523,24 +637,37 @@ void launch_build_lhs(Ctx& c, float dt) {
     hipLaunchKernelGGL(k_build_lhs, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.adiag, L.t[0], L.t[1], L.t[2], L.cls,
                        F, (int)l, L.d[0], L.d[1], L.d[2], dt, c.cfg.density, f * c.g.dx[0], f * c.g.dx[1],
                        f * c.g.dx[2]);
+    const int nrow = L.d[0] * L.d[1];
+    hipLaunchKernelGGL(k_row_flags, dim3((unsigned)div_up(nrow, kBlock / 64)), dim3(kBlock), 0, c.s, L.cls, c.row_flag,
+                       nrow, L.d[2]);
+    hipLaunchKernelGGL(k_row_compact, dim3(1), dim3(1024), 0, c.s, c.row_flag, nrow, L.rows, L.nrows);
   }
 }
 
 namespace {
+
+// blocks for a colour pass over the active rows (enough for all rows, capped; the kernel strides)
+unsigned smooth_blocks(const Level& L) {
+  const int H = (L.d[2] + 1) / 2;
+  const int R = H >= kBlock ? 1 : kBlock / H;
+  const size_t need = div_up((size_t)L.d[0] * L.d[1], (size_t)R);
+  return (unsigned)std::max<size_t>(1, std::min<size_t>(need, 8192));
+}
 
 // one red-black sweep on level l: Smooth(l, first) then Smooth(l, 1 - first).
 // Large levels use the fused tile kernel; small ones (launch-bound) two colour passes.
 void sweep(Ctx& c, int l, int first, int mode) {
   Level& L = c.lv[l];
   if ((long long)L.n < c.cfg.fused_sweep_min_cells) {
+    const unsigned nb = smooth_blocks(L);
     for (int k = 0; k < 2; ++k) {
       const int phase = k == 0 ? first : 1 - first;
       if (l == 0) {
         ProfScope ps(c, LH_PROF_SMOOTH_L0);
-        hipLaunchKernelGGL(k_smooth<true>, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), phase,
+        hipLaunchKernelGGL(k_smooth<true>, dim3(nb), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), L.rows, L.nrows, phase,
                            c.cfg.omega, mode);
       } else {
-        hipLaunchKernelGGL(k_smooth<false>, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), phase,
+        hipLaunchKernelGGL(k_smooth<false>, dim3(nb), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), L.rows, L.nrows, phase,
                            c.cfg.omega, mode);
       }
     }
@@ -730,7 +857,8 @@ int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, 
   if (!S.warm_start_used)
     hipLaunchKernelGGL(k_cg_init, dim3(grid_for(n)), dim3(kBlock), 0, c.s, L0.r, c.x, c.p, L0.z, b, L0.cls, n);
   apply_precond(c);
-  hipLaunchKernelGGL(k_cg_first, dim3(nb), dim3(kBlock), 0, c.s, c.p, L0.z, L0.r, L0.cls, n, c.partial);
+  hipLaunchKernelGGL(k_cg_first, dim3(nb), dim3(kBlock), 0, c.s, c.p, L0.z, L0.r, L0.cls, L0.rows, L0.nrows, L0.d[2],
+                     c.partial);
   hipLaunchKernelGGL(k_sc_first, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dsc64);
   hipMemcpyAsync(c.h_pin, c.dsc, 8 * sizeof(float), hipMemcpyDeviceToHost, c.s);
   if (!hip_ok(c, hipStreamSynchronize(c.s), "solve init")) return LH_ERR_DEVICE;
@@ -752,16 +880,17 @@ int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, 
     }
     {
       ProfScope ps(c, LH_PROF_SPMV);
-      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.q, c.p, P0, c.partial);
+      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.q, c.p, P0, L0.rows, L0.nrows, c.partial);
     }
     hipLaunchKernelGGL(k_sc_alpha, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dflag + 2);
     hipLaunchKernelGGL(k_cg_update_xr, dim3(nb), dim3(kBlock), 0, c.s, c.x, L0.r, c.p, c.q, L0.cls, c.dsc, L0.d[0],
-                       L0.d[1], L0.d[2], c.partial);
+                       L0.d[1], L0.d[2], L0.rows, L0.nrows, c.partial);
     hipLaunchKernelGGL(k_sc_residual, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc);
     apply_precond(c);
-    hipLaunchKernelGGL(k_cg_dot_rz, dim3(nb), dim3(kBlock), 0, c.s, L0.r, L0.z, n, c.partial);
+    hipLaunchKernelGGL(k_cg_dot_rz, dim3(nb), dim3(kBlock), 0, c.s, L0.r, L0.z, L0.rows, L0.nrows, L0.d[2], c.partial);
     hipLaunchKernelGGL(k_sc_beta, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dflag + 2);
-    hipLaunchKernelGGL(k_cg_update_p, dim3(grid_for(n)), dim3(kBlock), 0, c.s, c.p, L0.z, L0.cls, c.dsc, n);
+    hipLaunchKernelGGL(k_cg_update_p, dim3(nb), dim3(kBlock), 0, c.s, c.p, L0.z, L0.cls, c.dsc, L0.rows, L0.nrows,
+                       L0.d[2]);
     hipMemcpyAsync(c.h_pin, c.dsc, 8 * sizeof(float), hipMemcpyDeviceToHost, c.s);
     hipMemcpyAsync(c.h_flag, c.dflag + 2, sizeof(int), hipMemcpyDeviceToHost, c.s);
     if (!hip_ok(c, hipStreamSynchronize(c.s), "solve iteration")) return LH_ERR_DEVICE;

@@ -213,8 +213,9 @@ int alloc_ctx(Ctx& c) {
     q.n = (size_t)q.d[0] * q.d[1] * q.d[2];
     ok = ok && dalloc(c, q.adiag, q.n) && dalloc(c, q.t[0], q.n) && dalloc(c, q.t[1], q.n) &&
          dalloc(c, q.t[2], q.n) && dalloc(c, q.cls, q.n) && dalloc(c, q.r, q.n) && dalloc(c, q.z, q.n) &&
-         dalloc(c, q.z2, q.n);
+         dalloc(c, q.z2, q.n) && dalloc(c, q.rows, (size_t)q.d[0] * q.d[1]) && dalloc(c, q.nrows, 1);
   }
+  ok = ok && dalloc(c, c.row_flag, (size_t)c.cfg.N[0] * c.cfg.N[1]);
   c.npartial = 1024;
   ok = ok && dalloc(c, c.p, c.ncell) && dalloc(c, c.x, c.ncell) && dalloc(c, c.q, c.ncell) &&
        dalloc(c, c.partial, 4 * (size_t)c.npartial) && dalloc(c, c.dsc, 16) && dalloc(c, c.dsc64, 16) &&
@@ -239,6 +240,7 @@ int alloc_ctx(Ctx& c) {
   launch_fill_f32(c, c.phi, c.ncell, 3 * g.dx[0]);
   launch_fill_f32(c, c.rhs, c.ncell, 0.f);
   for (auto& q : c.lv) {
+    hipMemsetAsync(q.nrows, 0, sizeof(int), c.s);
     launch_fill_f32(c, q.adiag, q.n, 0.f);
     for (int a = 0; a < 3; ++a) launch_fill_f32(c, q.t[a], q.n, 0.f);
     launch_fill_u8(c, q.cls, q.n, 0);
@@ -262,8 +264,9 @@ void free_ctx(Ctx& c) {
   F(c.bin_count); F(c.bin_start); F(c.scan_tmp); F(c.pkey); F(c.pslot);
   F(c.pbuf[0]); F(c.pbuf[1]);
   for (auto& q : c.lv) {
-    F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2);
+    F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2); F(q.rows); F(q.nrows);
   }
+  F(c.row_flag);
   F(c.p); F(c.x); F(c.q); F(c.partial); F(c.dsc); F(c.dsc64); F(c.dmaxv); F(c.dflag);
   F(c.x64); F(c.r64); F(c.rf); F(c.b_user); F(c.x0_user);
   F(c.xv_snap); F(c.xv_final); F(c.xcount); F(c.xlist[0]); F(c.xlist[1]); F(c.xlist[2]);

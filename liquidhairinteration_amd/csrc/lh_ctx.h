@@ -62,6 +62,9 @@ struct Level {
   float* r = nullptr;
   float* z = nullptr;
   float* z2 = nullptr;  // ping-pong partner of z for the fused red-black sweep
+  // z-rows (x, y) holding at least one liquid cell (cls != 0), ascending; rebuilt with the operator
+  int* rows = nullptr;   // capacity d[0] * d[1]
+  int* nrows = nullptr;  // device count
 };
 
 // attributes of one particle in the SoA particle buffer (kParticleAttrs x cap)
@@ -94,6 +97,7 @@ struct Ctx {
   int* pkey = nullptr;       // per particle bin key
   int* pslot = nullptr;      // per particle provisional slot
   int* scan_tmp = nullptr;   // scan block sums
+  int* row_flag = nullptr;   // per z-row flags while the active-row lists are built
   bool bins_valid = false;
   // solver (CGSolver<3>)
   std::vector<Level> lv;
