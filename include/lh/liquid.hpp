@@ -78,6 +78,17 @@ class CGSolver {
     if (rc != LH_OK && rc != LH_ERR_NOT_CONVERGED) detail::check(ctx_, rc);
     return st;
   }
+  // learned initial guess (no reference counterpart: the reference starts from x = 0)
+  void LoadWarmStart(const std::string& path) { detail::check(ctx_, lh_warmstart_load(ctx_, path.c_str())); }
+  void SaveWarmStart(const std::string& path) { detail::check(ctx_, lh_warmstart_save(ctx_, path.c_str())); }
+  void InitWarmStart(uint64_t seed) { detail::check(ctx_, lh_warmstart_init(ctx_, seed)); }
+  void EnableWarmStart(bool on) { detail::check(ctx_, lh_warmstart_enable(ctx_, on ? 1 : 0)); }
+  // one Adam step on the last built system and its solved pressure; returns the loss
+  float TrainWarmStart(float lr) {
+    float loss = 0.f;
+    detail::check(ctx_, lh_warmstart_train(ctx_, lr, nullptr, &loss));
+    return loss;
+  }
 
  private:
   lh_ctx* ctx_;
