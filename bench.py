@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned-vs-plain solve comparison")
+    ap.add_argument("--learned", action="store_true",
+                    help="start every pressure solve of the timed workload from the learned guess")
     ap.add_argument("--ws-grid", type=int, default=128)
     ap.add_argument("--ws-scenes", type=int, default=2)
     ap.add_argument("--ws-steps", type=int, default=8)
@@ -110,6 +112,11 @@ def main():
     scene = lh.Scene(cfg)
     model = scene.GetLiquidModel()
     model.Init()
+    if args.learned:
+        from liquidhairinteration_amd import warmstart as W
+
+        model.GetWarmStart().load(W.WEIGHTS)
+        model.GetWarmStart().enable(True)
     P = model.GetParticleSystem().NumParticles()
     n0 = 1
     for d in scene.GetGridDimension():
@@ -156,10 +163,12 @@ def main():
                        cfg["SolverParameter"]["tolerance"], cfg["SolverParameter"].get("toleranceMode", "absolute")),
                    "precision": cfg.get("precision", "fp32") + (
                        " (FP32 MGPCG inside an FP64 residual loop to %g)" % cfg.get("refinedTolerance", 1e-9)
-                       if cfg.get("precision") == "refined" else "")},
+                       if cfg.get("precision") == "refined" else ""),
+                   "initial_guess": "learned (network, energy-scaled, zero-guess fallback)" if args.learned else "zero"},
         "roofline": roof,
         "kernels": kernels,
         "last_step": {"pcg_iterations": last["solve"]["iterations"], "converged": last["solve"]["converged"],
+                      "warm_start_used": last["solve"]["warm_start_used"],
                       "solve_ms": round(last["solve"]["solve_ms"], 3), "step_ms": round(last["step_ms"], 3),
                       "divergence_linf": last["divergence_linf"], "sim_time": last["sim_time"]},
         "cpu_baseline": None,
