@@ -130,7 +130,7 @@ __global__ void k_bin_scatter(const int* __restrict__ key, int64_t P, const int*
 // place particle i at start[key] + (number of particles in its bin with a smaller id)
 __global__ void k_bin_permute(const float* __restrict__ src, float* __restrict__ dst, int64_t cap, int64_t P,
                               const int* __restrict__ key, const int* __restrict__ start,
-                              const int* __restrict__ slot_owner) {
+                              const int* __restrict__ slot_owner, float4* __restrict__ pack) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
     int k = key[i];
     int s0 = start[k], s1 = start[k + 1];
@@ -144,6 +144,15 @@ __global__ void k_bin_permute(const float* __restrict__ src, float* __restrict__
     size_t d = (size_t)(s0 + rank);
 #pragma unroll
     for (int a = 0; a < kParticleAttrs; ++a) dst[(size_t)a * cap + d] = src[(size_t)a * cap + i];
+    const float x = src[(size_t)PA_X * cap + i], y = src[(size_t)(PA_X + 1) * cap + i],
+                z = src[(size_t)(PA_X + 2) * cap + i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float4* r = pack + 2 * ((size_t)a * cap + d);
+      r[0] = make_float4(x, y, z, src[(size_t)(PA_VX + a) * cap + i]);
+      r[1] = make_float4(src[(size_t)(PA_C + 3 * a) * cap + i], src[(size_t)(PA_C + 3 * a + 1) * cap + i],
+                         src[(size_t)(PA_C + 3 * a + 2) * cap + i], 0.f);
+    }
   }
 }
 
@@ -187,7 +196,7 @@ __device__ __forceinline__ void tile_coords(int d0, int d1, int d2, int c[3], in
   c[2] = c0[2] + (threadIdx.x & 63);
 }
 
-__global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const float* __restrict__ pb, int64_t cap,
+__global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const float4* __restrict__ pack, int64_t cap,
                                               const int* __restrict__ start, P2GP q) {
   const int a = q.a;
   const int d0 = q.g.N[0] + (a == 0), d1 = q.g.N[1] + (a == 1), d2 = q.g.N[2] + (a == 2);
@@ -204,13 +213,7 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const flo
   if (c[0] >= d0 || c[1] >= d1 || c[2] >= d2) return;
   float ws = 0.f, vs = 0.f;
   if (any) {
-    const float* px = pb + (size_t)PA_X * cap;
-    const float* py = px + cap;
-    const float* pz = py + cap;
-    const float* pv = pb + (size_t)(PA_VX + a) * cap;
-    const float* cc0 = pb + (size_t)(PA_C + 3 * a) * cap;  // C.col(a)
-    const float* cc1 = cc0 + cap;
-    const float* cc2 = cc1 + cap;
+    const float4* rec = pack + 2 * (size_t)a * cap;  // x y z v_a | C.col(a) 0
     const float mn0 = q.g.mn[0], mn1 = q.g.mn[1], mn2 = q.g.mn[2];
     const float dx0 = q.g.dx[0], dx1 = q.g.dx[1], dx2 = q.g.dx[2];
     int lo[3], hi[3];
@@ -229,14 +232,15 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const flo
         size_t k0 = ((size_t)hx * q.H[1] + hy) * q.H[2];
         int s0 = start[k0 + lo[2]], s1 = start[k0 + hi[2] + 1];
         for (int s = s0; s < s1; ++s) {
-          float ppx = px[s], ppy = py[s], ppz = pz[s];
+          const float4 A = rec[2 * (size_t)s], B = rec[2 * (size_t)s + 1];
+          const float ppx = A.x, ppy = A.y, ppz = A.z;
           float gx = (ppx - mn0) / dx0, gy = (ppy - mn1) / dx1, gz = (ppz - mn2) / dx2;
           float wgt = quad_w(gx - ccx) * quad_w(gy - ccy) * quad_w(gz - ccz);
           if (wgt == 0.f) continue;
           float d0v = ppx - fx, d1v = ppy - fy, d2v = ppz - fz;
-          float dot = sum3(cc0[s] * d0v, cc1[s] * d1v, cc2[s] * d2v);
+          float dot = sum3(B.x * d0v, B.y * d1v, B.z * d2v);
           ws += wgt;
-          vs += wgt * (pv[s] + dot);
+          vs += wgt * (A.w + dot);
         }
       }
   }
@@ -263,7 +267,7 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const flo
 // and a row is skipped when its distance lower bound already exceeds the
 // current min by a margin far above the float error of the distance: skipped
 // particles cannot change the result.
-__global__ void __launch_bounds__(256) k_liquid_phi(float* __restrict__ phi, const float* __restrict__ pb,
+__global__ void __launch_bounds__(256) k_liquid_phi(float* __restrict__ phi, const float4* __restrict__ rec,
                                                      int64_t cap, const int* __restrict__ start, GridP g, int H0,
                                                      int H1, int H2, int dW0, int dW1, int dW2, float radius,
                                                      float init) {
@@ -284,9 +288,6 @@ __global__ void __launch_bounds__(256) k_liquid_phi(float* __restrict__ phi, con
   if (xi >= N0 || yi >= N1 || zi >= N2) return;
   float best = init;
   if (any) {
-    const float* px = pb + (size_t)PA_X * cap;
-    const float* py = px + cap;
-    const float* pz = py + cap;
     const float cpx = ((float)xi + 0.5f) * g.dx[0] + g.mn[0];
     const float cpy = ((float)yi + 0.5f) * g.dx[1] + g.mn[1];
     const float cpz = ((float)zi + 0.5f) * g.dx[2] + g.mn[2];
@@ -309,7 +310,8 @@ __global__ void __launch_bounds__(256) k_liquid_phi(float* __restrict__ phi, con
           size_t k0 = ((size_t)hx * H1 + hy) * H2;
           int s0 = start[k0 + hz0], s1 = start[k0 + hz1 + 1];
           for (int s = s0; s < s1; ++s) {
-            float ph = norm3(cpx - px[s], cpy - py[s], cpz - pz[s]) - radius;
+            const float4 A = rec[2 * (size_t)s];  // x y z of the bin-ordered particle record
+            float ph = norm3(cpx - A.x, cpy - A.y, cpz - A.z) - radius;
             best = (best < ph) ? best : ph;
           }
         }
@@ -512,7 +514,7 @@ int run_binning(Ctx& c) {
   scan_excl(c, c.bin_count, c.bin_start, c.nbins, c.scan_tmp);
   hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, c.s, c.pkey, c.P, c.bin_start, c.bin_count, c.pslot);
   hipLaunchKernelGGL(k_bin_permute, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.pbuf[1 - c.cur], c.cap, c.P,
-                     c.pkey, c.bin_start, c.pslot);
+                     c.pkey, c.bin_start, c.pslot, (float4*)c.ppack);
   c.cur = 1 - c.cur;
   c.bins_valid = true;
   return LH_OK;
@@ -522,7 +524,7 @@ void launch_liquid_phi(Ctx& c) {
   if (c.P == 0 || !c.bins_valid) return;
   ProfScope ps(c, LH_PROF_LIQUID_PHI);
   unsigned nt = (unsigned)(div_up(c.g.N[0], kTX) * div_up(c.g.N[1], kTY) * div_up(c.g.N[2], kTZ));
-  hipLaunchKernelGGL(k_liquid_phi, dim3(nt), dim3(kBlock), 0, c.s, c.phi, c.pbuf[c.cur], c.cap, c.bin_start, c.g,
+  hipLaunchKernelGGL(k_liquid_phi, dim3(nt), dim3(kBlock), 0, c.s, c.phi, (const float4*)c.ppack, c.cap, c.bin_start, c.g,
                      c.H[0], c.H[1], c.H[2], c.dW[0], c.dW[1], c.dW[2], c.radius, 3 * c.g.dx[0]);
 }
 
@@ -549,7 +551,7 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
     ProfScope ps(c, LH_PROF_P2G);
     unsigned nt = (unsigned)(div_up(c.g.N[0] + (a == 0), kTX) * div_up(c.g.N[1] + (a == 1), kTY) *
                              div_up(c.g.N[2] + (a == 2), kTZ));
-    hipLaunchKernelGGL(k_p2g, dim3(nt), dim3(kBlock), 0, c.s, c.v[a], c.pbuf[c.cur], c.cap, c.bin_start, qa);
+    hipLaunchKernelGGL(k_p2g, dim3(nt), dim3(kBlock), 0, c.s, c.v[a], (const float4*)c.ppack, c.cap, c.bin_start, qa);
   }
 }
 

@@ -262,7 +262,7 @@ void free_ctx(Ctx& c) {
   }
   F(c.pressure); F(c.phi); F(c.rhs); F(c.solid_phi); F(c.is_valid); F(c.xstate); F(c.d_sdf);
   F(c.bin_count); F(c.bin_start); F(c.scan_tmp); F(c.pkey); F(c.pslot);
-  F(c.pbuf[0]); F(c.pbuf[1]);
+  F(c.pbuf[0]); F(c.pbuf[1]); F(c.ppack);
   for (auto& q : c.lv) {
     F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2); F(q.rows); F(q.nrows);
   }
@@ -287,11 +287,13 @@ int ensure_particle_capacity(Ctx& c, int64_t n) {
   if (c.pbuf[1]) hipFree(c.pbuf[1]);
   if (c.pkey) hipFree(c.pkey);
   if (c.pslot) hipFree(c.pslot);
+  if (c.ppack) hipFree(c.ppack);
   c.pbuf[0] = c.pbuf[1] = nullptr;
   c.pkey = c.pslot = nullptr;
+  c.ppack = nullptr;
   int64_t cap = std::max<int64_t>(n, 1);
   bool ok = dalloc(c, c.pbuf[0], (size_t)kParticleAttrs * cap) && dalloc(c, c.pbuf[1], (size_t)kParticleAttrs * cap) &&
-            dalloc(c, c.pkey, cap) && dalloc(c, c.pslot, cap);
+            dalloc(c, c.pkey, cap) && dalloc(c, c.pslot, cap) && dalloc(c, c.ppack, (size_t)24 * cap);
   if (!ok) {
     c.cap = 0;
     return LH_ERR_OOM;
@@ -701,6 +703,7 @@ int lh_extrapolate(lh_ctx* ctx) {
 int lh_g2p(lh_ctx* ctx) {
   return guarded(ctx, [](Ctx& c) {
     launch_g2p_advect(c, true, false, 0.f);
+    c.bins_valid = false;  // the bin-ordered copies of v and C are stale: the next P2G re-bins
     return read_maxv(c);
   });
 }

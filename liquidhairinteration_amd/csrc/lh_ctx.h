@@ -89,6 +89,9 @@ struct Ctx {
   int64_t P = 0, cap = 0;
   float* pbuf[2] = {nullptr, nullptr};
   int cur = 0;
+  // per component a, in bin order, one 32-byte record per particle: x, y, z, v_a, C.col(a), 0
+  // (written with the bin permutation; the transfers read it with two 16-byte loads)
+  float* ppack = nullptr;
   // half-cell bins: 2N per axis
   int H[3] = {0, 0, 0};
   size_t nbins = 0;
