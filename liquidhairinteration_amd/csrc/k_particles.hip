@@ -196,6 +196,17 @@ __device__ __forceinline__ void tile_coords(int d0, int d1, int d2, int c[3], in
   c[2] = c0[2] + (threadIdx.x & 63);
 }
 
+__device__ __forceinline__ void p2g_acc4(float4 A, float4 B, float mn0, float mn1, float mn2, float dx0, float dx1,
+                                         float dx2, float ccx, float ccy, float ccz, float fx, float fy, float fz,
+                                         float& ws, float& vs) {
+  const float gx = (A.x - mn0) / dx0, gy = (A.y - mn1) / dx1, gz = (A.z - mn2) / dx2;
+  const float wgt = quad_w(gx - ccx) * quad_w(gy - ccy) * quad_w(gz - ccz);
+  if (wgt == 0.f) return;
+  const float d0v = A.x - fx, d1v = A.y - fy, d2v = A.z - fz;
+  const float dot = sum3(B.x * d0v, B.y * d1v, B.z * d2v);
+  ws += wgt;
+  vs += wgt * (A.w + dot);
+}
 __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const float4* __restrict__ pack, int64_t cap,
                                               const int* __restrict__ start, P2GP q) {
   const int a = q.a;
@@ -231,17 +242,14 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const flo
       for (int hy = lo[1]; hy <= hi[1]; ++hy) {
         size_t k0 = ((size_t)hx * q.H[1] + hy) * q.H[2];
         int s0 = start[k0 + lo[2]], s1 = start[k0 + hi[2] + 1];
-        for (int s = s0; s < s1; ++s) {
-          const float4 A = rec[2 * (size_t)s], B = rec[2 * (size_t)s + 1];
-          const float ppx = A.x, ppy = A.y, ppz = A.z;
-          float gx = (ppx - mn0) / dx0, gy = (ppy - mn1) / dx1, gz = (ppz - mn2) / dx2;
-          float wgt = quad_w(gx - ccx) * quad_w(gy - ccy) * quad_w(gz - ccz);
-          if (wgt == 0.f) continue;
-          float d0v = ppx - fx, d1v = ppy - fy, d2v = ppz - fz;
-          float dot = sum3(B.x * d0v, B.y * d1v, B.z * d2v);
-          ws += wgt;
-          vs += wgt * (A.w + dot);
+        int s = s0;
+        for (; s + 1 < s1; s += 2) {
+          const float4 A0 = rec[2 * (size_t)s], B0 = rec[2 * (size_t)s + 1];
+          const float4 A1 = rec[2 * (size_t)s + 2], B1 = rec[2 * (size_t)s + 3];
+          p2g_acc4(A0, B0, mn0, mn1, mn2, dx0, dx1, dx2, ccx, ccy, ccz, fx, fy, fz, ws, vs);
+          p2g_acc4(A1, B1, mn0, mn1, mn2, dx0, dx1, dx2, ccx, ccy, ccz, fx, fy, fz, ws, vs);
         }
+        if (s < s1) p2g_acc4(rec[2 * (size_t)s], rec[2 * (size_t)s + 1], mn0, mn1, mn2, dx0, dx1, dx2, ccx, ccy, ccz, fx, fy, fz, ws, vs);
       }
   }
   float v = !is_small(ws) ? vs / ws : 0.0f;
