@@ -25,13 +25,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
 
 
-def algorithmic_bytes(kernel, n0, P):
+def algorithmic_bytes(kernel, n0, P, active0, fused):
     """Bytes per launch counted once per array element the kernel must touch.
 
     smooth_l0: one colour pass = half of a red-black sweep, whose minimum is 28 B/cell
-               (Adiag + 3 face terms = 16 B, r 4 B, z read 4 B + write 4 B) -> 14 B/cell
-               (the fused one-launch sweep, when selected, counts 28 B/cell);
-    spmv:      p read, q write, 4 coefficient floats -> 24 B/cell;
+               (Adiag + 3 face terms = 16 B, r 4 B, z read 4 B + write 4 B) -> 14 B per
+               cell of the level-0 rows holding liquid (the pass visits only those); the
+               fused one-launch sweep, when selected, is dense: 28 B/cell;
+    spmv:      p read, q write, 4 coefficient floats -> 24 B per cell of those rows;
     p2g:       one component: 20 B/particle (position, velocity component,
                affine column) + 4 B per face written (~n0 faces);
     g2p:       72 B/particle (read position, write position, velocity, C) + 3
@@ -39,12 +40,23 @@ def algorithmic_bytes(kernel, n0, P):
     liquid_phi: 12 B/particle + 4 B/cell.
     """
     return {
-        "smooth_l0": 14.0 * n0,
-        "spmv": 24.0 * n0,
+        "smooth_l0": 28.0 * n0 if fused else 14.0 * active0,
+        "spmv": 24.0 * active0,
         "p2g": 20.0 * P + 4.0 * n0,
         "g2p": 72.0 * P + 12.0 * n0,
         "liquid_phi": 12.0 * P + 4.0 * n0,
     }[kernel]
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per dispatch of `kernel` on `workload` from the committed rocprofv3 PMC runs
+    (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a copy kernel of
+    known size), or None when that pair was not measured"""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path)).get(workload, {}).get(kernel)
+    return None if rec is None else rec["bytes_per_dispatch"]
 
 
 def cpu_baseline(cfg, steps=1):
@@ -130,17 +142,28 @@ def main():
     el = time.perf_counter() - t0
     prof = {k: model.profile_read(k) for k in ("smooth_l0", "p2g", "g2p", "liquid_phi", "spmv")}
     model.profile_enable(False)
+    # level-0 rows holding liquid in the last step's operator (the smoother and SpMV visit those)
+    (_, _, d2), rows0 = model.GetSolver().LevelInfo(0)
+    active0 = rows0 * d2
+    fused = cfg["SolverParameter"].get("fusedSweepMinCells", 1 << 62) <= n0
+    workload = os.path.basename(args.config).replace(".json", "")
+
+    def rate(k):
+        n, ms = prof[k]
+        return algorithmic_bytes(k, n0, P, active0, fused) / (ms / 1e3 / n) / 1e9 if n else None
 
     # dominant instrumented kernel by total device time in the timed region
     dom = max(prof, key=lambda k: prof[k][1])
     launches, tot_ms = prof[dom]
     avg_s = tot_ms / 1e3 / max(launches, 1)
-    ach = algorithmic_bytes(dom, n0, P) / avg_s / 1e9
+    ach = rate(dom)
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "launches": launches,
-            "avg_launch_us": round(avg_s * 1e6, 2)}
-    kernels = {k: {"launches": v[0], "ms": round(v[1], 3),
-                   "pct_of_timed": round(100.0 * v[1] / (el * 1e3), 1)} for k, v in prof.items()}
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, workload), "launches": launches,
+            "avg_launch_us": round(avg_s * 1e6, 2),
+            "algorithmic_bytes_per_launch": algorithmic_bytes(dom, n0, P, active0, fused)}
+    kernels = {k: {"launches": v[0], "ms": round(v[1], 3), "pct_of_timed": round(100.0 * v[1] / (el * 1e3), 1),
+                   "avg_us": round(1e3 * v[1] / v[0], 2) if v[0] else None,
+                   "achieved_gbs": round(rate(k), 1) if v[0] else None} for k, v in prof.items()}
 
     out = {
         "metric": "liquid steps/s at 256^3; ms per solve to 1e-6 residual, learned vs plain MGPCG",
@@ -156,7 +179,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded jittered particle blocks)",
         "config": {"workload": os.path.basename(args.config).replace(".json", ""),
-                   "grid": list(scene.GetGridDimension()), "particles": P,
+                   "grid": list(scene.GetGridDimension()), "particles": P, "level0_active_rows": rows0,
                    "solver": "MGPCG L=%d pre/post=%d/%d %s, tol %g %s" % (
                        cfg["SolverParameter"]["numLevel"], cfg["SolverParameter"]["numPreSmooth"],
                        cfg["SolverParameter"]["numPostSmooth"], cfg["SolverParameter"].get("smoother", "reference"),

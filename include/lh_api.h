@@ -136,6 +136,9 @@ int lh_build_system(lh_ctx* ctx, float dt);
 int lh_solve_rhs(lh_ctx* ctx, const float* b, const float* x0, lh_solve_stats* stats);
 /* the last solution in FP64 (refined mode; the FP32 pressure widened otherwise) */
 int lh_get_solution64(lh_ctx* ctx, double* host, size_t count);
+/* multigrid level l of the built operator: its dims and the number of z-rows (x, y)
+ * holding a liquid cell -- the rows the smoother and the PCG vector kernels visit */
+int lh_solver_level_info(lh_ctx* ctx, int level, int dims[3], int64_t* active_rows);
 /* initial guess used by the following pressure solves (NULL removes it) */
 int lh_set_warm_start(lh_ctx* ctx, const float* x0, size_t count);
 

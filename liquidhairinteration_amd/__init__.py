@@ -120,6 +120,7 @@ def load_library():
         "lh_solve_rhs": ([vp, fp, fp, C.POINTER(SolveStats)], C.c_int),
         "lh_get_solution64": ([vp, dp, C.c_size_t], C.c_int),
         "lh_set_warm_start": ([vp, fp, C.c_size_t], C.c_int),
+        "lh_solver_level_info": ([vp, C.c_int, ip, C.POINTER(C.c_int64)], C.c_int),
         "lh_write_positions": ([vp, C.c_char_p, C.c_int], C.c_int),
         "lh_load_positions_line": ([vp, C.c_char_p], C.c_int),
         "lh_warmstart_init": ([vp, C.c_uint64], C.c_int),
@@ -268,6 +269,13 @@ class CGSolver:
         x = np.empty(shape, np.float64)
         self._h.check(self._h.L.lh_get_solution64(self._h.h, x.ctypes.data_as(C.POINTER(C.c_double)), x.size))
         return x
+
+    def LevelInfo(self, level):
+        """(dims, active z-rows) of multigrid level `level` of the built operator"""
+        dims = (C.c_int * 3)()
+        rows = C.c_int64()
+        self._h.check(self._h.L.lh_solver_level_info(self._h.h, int(level), dims, C.byref(rows)))
+        return tuple(dims), int(rows.value)
 
     def SetWarmStart(self, x0):
         if x0 is None:

@@ -941,6 +941,25 @@ int lh_get_solution64(lh_ctx* ctx, double* host, size_t count) {
   });
 }
 
+int lh_solver_level_info(lh_ctx* ctx, int level, int dims[3], int64_t* active_rows) {
+  return guarded(ctx, [=](Ctx& c) {
+    if (level < 0 || level >= (int)c.lv.size()) {
+      c.err = "lh_solver_level_info: no such level";
+      return LH_ERR_ARG;
+    }
+    const Level& L = c.lv[level];
+    if (dims)
+      for (int a = 0; a < 3; ++a) dims[a] = L.d[a];
+    if (active_rows) {
+      int n = 0;
+      hipMemcpyAsync(&n, L.nrows, sizeof(int), hipMemcpyDeviceToHost, c.s);
+      if (!hip_ok(c, hipStreamSynchronize(c.s), "level info")) return LH_ERR_DEVICE;
+      *active_rows = n;
+    }
+    return LH_OK;
+  });
+}
+
 int lh_set_warm_start(lh_ctx* ctx, const float* x0, size_t count) {
   return guarded(ctx, [=](Ctx& c) {
     if (!x0) {
