@@ -185,15 +185,30 @@ __device__ bool tile_has_particles(const int* __restrict__ start, const int lo[3
   return __syncthreads_or(cnt != 0) != 0;
 }
 
-__device__ __forceinline__ void tile_coords(int d0, int d1, int d2, int c[3], int c0[3]) {
-  const int nty = (d1 + kTY - 1) / kTY, ntz = (d2 + kTZ - 1) / kTZ;
-  const int bz = blockIdx.x % ntz, by = (blockIdx.x / ntz) % nty, bx = blockIdx.x / (ntz * nty);
+// Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8), each XCD with
+// its own L2.  Tiles adjacent in x read most of the same particles, so tiles are numbered x
+// fastest and dealt to the XCDs in chunks of kXChunk consecutive x-tiles: chunk q runs on
+// XCD q % 8, which keeps a chunk's shared particles in one L2 while the chunks of a busy
+// region still spread over all XCDs.  The grid is a multiple of 8 kXChunk blocks (tile_grid);
+// blocks past the last tile return false.
+constexpr int kXcds = 8, kXChunk = 4;
+inline unsigned tile_grid(size_t ntiles) {
+  return (unsigned)(div_up(ntiles, (size_t)kXcds * kXChunk) * kXcds * kXChunk);
+}
+
+__device__ __forceinline__ bool tile_coords(int d0, int d1, int d2, int c[3], int c0[3]) {
+  const int ntx = (d0 + kTX - 1) / kTX, nty = (d1 + kTY - 1) / kTY, ntz = (d2 + kTZ - 1) / kTZ;
+  const int x = (int)(blockIdx.x % kXcds), j = (int)(blockIdx.x / kXcds);
+  const int t = ((j / kXChunk) * kXcds + x) * kXChunk + j % kXChunk;
+  if (t >= ntx * nty * ntz) return false;
+  const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
   c0[0] = bx * kTX;
   c0[1] = by * kTY;
   c0[2] = bz * kTZ;
   c[0] = c0[0];
   c[1] = c0[1] + (threadIdx.x >> 6);
   c[2] = c0[2] + (threadIdx.x & 63);
+  return true;
 }
 
 __device__ __forceinline__ void p2g_acc4(float4 A, float4 B, float mn0, float mn1, float mn2, float dx0, float dx1,
@@ -212,7 +227,7 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const flo
   const int a = q.a;
   const int d0 = q.g.N[0] + (a == 0), d1 = q.g.N[1] + (a == 1), d2 = q.g.N[2] + (a == 2);
   int c[3], c0[3];
-  tile_coords(d0, d1, d2, c, c0);
+  if (!tile_coords(d0, d1, d2, c, c0)) return;
   const int T[3] = {kTX, kTY, kTZ};
   int blo[3], bhi[3];
 #pragma unroll
@@ -281,7 +296,7 @@ __global__ void __launch_bounds__(256) k_liquid_phi(float* __restrict__ phi, con
                                                      float init) {
   const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
   int c[3], c0[3];
-  tile_coords(N0, N1, N2, c, c0);
+  if (!tile_coords(N0, N1, N2, c, c0)) return;
   const int T[3] = {kTX, kTY, kTZ};
   const int dW[3] = {dW0, dW1, dW2};
   const int H[3] = {H0, H1, H2};
@@ -531,7 +546,7 @@ int run_binning(Ctx& c) {
 void launch_liquid_phi(Ctx& c) {
   if (c.P == 0 || !c.bins_valid) return;
   ProfScope ps(c, LH_PROF_LIQUID_PHI);
-  unsigned nt = (unsigned)(div_up(c.g.N[0], kTX) * div_up(c.g.N[1], kTY) * div_up(c.g.N[2], kTZ));
+  unsigned nt = tile_grid(div_up(c.g.N[0], kTX) * div_up(c.g.N[1], kTY) * div_up(c.g.N[2], kTZ));
   hipLaunchKernelGGL(k_liquid_phi, dim3(nt), dim3(kBlock), 0, c.s, c.phi, (const float4*)c.ppack, c.cap, c.bin_start, c.g,
                      c.H[0], c.H[1], c.H[2], c.dW[0], c.dW[1], c.dW[2], c.radius, 3 * c.g.dx[0]);
 }
@@ -557,8 +572,8 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
     P2GP qa = q;
     if (!fuse_body_forces || a != 1) qa.gdt = 0.f;
     ProfScope ps(c, LH_PROF_P2G);
-    unsigned nt = (unsigned)(div_up(c.g.N[0] + (a == 0), kTX) * div_up(c.g.N[1] + (a == 1), kTY) *
-                             div_up(c.g.N[2] + (a == 2), kTZ));
+    unsigned nt = tile_grid(div_up(c.g.N[0] + (a == 0), kTX) * div_up(c.g.N[1] + (a == 1), kTY) *
+                            div_up(c.g.N[2] + (a == 2), kTZ));
     hipLaunchKernelGGL(k_p2g, dim3(nt), dim3(kBlock), 0, c.s, c.v[a], (const float4*)c.ppack, c.cap, c.bin_start, qa);
   }
 }
