@@ -436,12 +436,20 @@ __global__ void k_xcopy_range(float* __restrict__ t, const float* __restrict__ v
   }
 }
 
+// *differ |= 1 if the byte arrays differ: 16 bytes per thread and step, one atomic per block
 __global__ void k_cmp_u8(const uint8_t* __restrict__ a, const uint8_t* __restrict__ b, size_t n,
                          int* __restrict__ differ) {
   int d = 0;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+  const size_t n16 = n / 16;
+  const uint4* a4 = reinterpret_cast<const uint4*>(a);
+  const uint4* b4 = reinterpret_cast<const uint4*>(b);
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x) {
+    const uint4 p = a4[i], q = b4[i];
+    d |= ((p.x ^ q.x) | (p.y ^ q.y) | (p.z ^ q.z) | (p.w ^ q.w)) != 0u;
+  }
+  for (size_t i = 16 * n16 + blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     d |= (a[i] != b[i]) ? 1 : 0;
-  if (__any(d) && (threadIdx.x & 63) == 0) atomicOr(differ, 1);
+  if (__syncthreads_or(d) && threadIdx.x == 0) atomicOr(differ, 1);
 }
 
 __global__ void k_xcommit(uint8_t* __restrict__ valid, const uint8_t* __restrict__ st, XRange R) {
@@ -693,8 +701,8 @@ int run_extrapolate(Ctx& c) {
   if (!ensure_xcache(c)) return -1;
   if (c.xcache_ok) {
     hipMemsetAsync(c.dflag + 12, 0, sizeof(int), c.s);
-    hipLaunchKernelGGL(k_cmp_u8, dim3(grid_for(c.nnode)), dim3(kBlock), 0, c.s, c.is_valid, c.xv_snap, c.nnode,
-                       c.dflag + 12);
+    hipLaunchKernelGGL(k_cmp_u8, dim3((unsigned)std::min<size_t>(2048, div_up(c.nnode / 16 + 1, kBlock))), dim3(kBlock),
+                       0, c.s, c.is_valid, c.xv_snap, c.nnode, c.dflag + 12);
     hipMemcpyAsync(c.h_flag + 12, c.dflag + 12, sizeof(int), hipMemcpyDeviceToHost, c.s);
     if (!hip_ok(c, hipStreamSynchronize(c.s), "extrapolate")) return -1;
     if (c.h_flag[12] == 0) {

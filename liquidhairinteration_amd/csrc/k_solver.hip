@@ -131,6 +131,38 @@ __global__ __launch_bounds__(1024) void k_row_compact(const int* __restrict__ fl
   if (t == 1023) *nrows = s[1023];
 }
 
+// qflag[X, Y] of a coarse level: its own liquid row, or a child row (2X+i, 2Y+j) that is a
+// liquid or q row of the finer level (fqflag null on level 0, whose r is zero off liquid)
+__global__ void k_qrow_flags(const int* __restrict__ caflag, const int* __restrict__ faflag,
+                             const int* __restrict__ fqflag, int* __restrict__ cqflag, int c0, int c1, int f1) {
+  const int row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= c0 * c1) return;
+  const int X = row / c1, Y = row - X * c1;
+  int f = caflag[row];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int fr = (2 * X + (k >> 1)) * f1 + 2 * Y + (k & 1);
+    f |= faflag[fr] | (fqflag ? fqflag[fr] : 0);
+  }
+  cqflag[row] = f ? 1 : 0;
+}
+
+// zflag of a level: its own liquid row, or a parent row (x/2, y/2) in the coarser level's
+// z-support (czflag null on the coarsest level)
+__global__ void k_zrow_flags(const int* __restrict__ aflag, const int* __restrict__ czflag, int* __restrict__ zflag,
+                             int d0, int d1, int c0, int c1) {
+  const int row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= d0 * d1) return;
+  const int x = row / d1, y = row - x * d1;
+  int f = aflag[row];
+  if (czflag && (x >> 1) < c0 && (y >> 1) < c1) f |= czflag[(x >> 1) * c1 + (y >> 1)];
+  zflag[row] = f ? 1 : 0;
+}
+__global__ void k_or_flags(int* __restrict__ a, const int* __restrict__ b, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) a[i] |= b[i];
+}
+
 // ---------------------------------------------------------------- assembly
 struct FineP {
   int N[3];
@@ -381,35 +413,60 @@ __global__ void __launch_bounds__(256) k_rbgs_sweep(float* __restrict__ zout, co
 
 // Restrict, PressureSolver.cpp:796-819
 __global__ void k_restrict(float* __restrict__ rc, float* __restrict__ zc, const float* __restrict__ rf,
-                           const float* __restrict__ zf, LvP Lf, int c0, int c1, int c2) {
-  size_t total = (size_t)c0 * c1 * c2;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    int zi = (int)(i % c2), yi = (int)((i / c2) % c1), xi = (int)(i / ((size_t)c1 * c2));
-    float acc = 0.0f;
+                           const float* __restrict__ zf, LvP Lf, int c1, int c2, const int* __restrict__ rows,
+                           const int* __restrict__ nrows) {
+  const RowIter it = row_iter(c2);
+  const int nr = *nrows;
+  if (!it.on) return;
+  for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
+    const int row = rows[k];
+    const int xi = row / c1, yi = row - xi * c1;
+    for (int zi = it.z0; zi < c2; zi += it.zs) {
+      float acc = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      int x = 2 * xi + ((k >> 2) & 1), y = 2 * yi + ((k >> 1) & 1), w = 2 * zi + (k & 1);
-      size_t f = ((size_t)x * Lf.d[1] + y) * Lf.d[2] + w;
-      float d = rf[f] - get_ax(Lf, zf, x, y, w, f);
-      acc = (float)((double)acc + 0.125 * (double)d);
+      for (int q = 0; q < 8; ++q) {
+        int x = 2 * xi + ((q >> 2) & 1), y = 2 * yi + ((q >> 1) & 1), w = 2 * zi + (q & 1);
+        size_t f = ((size_t)x * Lf.d[1] + y) * Lf.d[2] + w;
+        float d = rf[f] - get_ax(Lf, zf, x, y, w, f);
+        acc = (float)((double)acc + 0.125 * (double)d);
+      }
+      const size_t i = (size_t)row * c2 + zi;
+      rc[i] = acc;
+      zc[i] = 0.0f;
     }
-    rc[i] = acc;
-    zc[i] = 0.0f;
   }
 }
 
-// Prolong, PressureSolver.cpp:774-794
-__global__ void k_prolong(float* __restrict__ zf, const float* __restrict__ zc, int f1, int f2, int c0, int c1,
-                          int c2) {
-  size_t total = (size_t)c0 * c1 * c2;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    int zi = (int)(i % c2), yi = (int)((i / c2) % c1), xi = (int)(i / ((size_t)c1 * c2));
-    float v = zc[i];
+// Prolong, PressureSolver.cpp:774-794, from the coarse level's z-support rows (its z is zero
+// on every other row: nothing but restriction, which clears it, writes there)
+__global__ void k_prolong(float* __restrict__ zf, const float* __restrict__ zc, int f1, int f2, int c1, int c2,
+                          const int* __restrict__ rows, const int* __restrict__ nrows) {
+  const RowIter it = row_iter(c2);
+  const int nr = *nrows;
+  if (!it.on) return;
+  for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
+    const int row = rows[k];
+    const int xi = row / c1, yi = row - xi * c1;
+    for (int zi = it.z0; zi < c2; zi += it.zs) {
+      const float v = zc[(size_t)row * c2 + zi];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      int x = 2 * xi + ((k >> 2) & 1), y = 2 * yi + ((k >> 1) & 1), w = 2 * zi + (k & 1);
-      zf[((size_t)x * f1 + y) * f2 + w] += v;
+      for (int q = 0; q < 8; ++q) {
+        int x = 2 * xi + ((q >> 2) & 1), y = 2 * yi + ((q >> 1) & 1), w = 2 * zi + (q & 1);
+        zf[((size_t)x * f1 + y) * f2 + w] += v;
+      }
     }
+  }
+}
+
+// z = 0 on the rows of a level's row list
+__global__ void k_zero_rows(float* __restrict__ z, const int* __restrict__ rows, const int* __restrict__ nrows,
+                            int d2) {
+  const RowIter it = row_iter(d2);
+  const int nr = *nrows;
+  if (!it.on) return;
+  for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
+    const size_t base = (size_t)rows[k] * d2;
+    for (int zi = it.z0; zi < d2; zi += it.zs) z[base + zi] = 0.0f;
   }
 }
 
@@ -638,9 +695,42 @@ void launch_build_lhs(Ctx& c, float dt) {
                        F, (int)l, L.d[0], L.d[1], L.d[2], dt, c.cfg.density, f * c.g.dx[0], f * c.g.dx[1],
                        f * c.g.dx[2]);
     const int nrow = L.d[0] * L.d[1];
-    hipLaunchKernelGGL(k_row_flags, dim3((unsigned)div_up(nrow, kBlock / 64)), dim3(kBlock), 0, c.s, L.cls, c.row_flag,
+    hipLaunchKernelGGL(k_row_flags, dim3((unsigned)div_up(nrow, kBlock / 64)), dim3(kBlock), 0, c.s, L.cls, L.aflag,
                        nrow, L.d[2]);
-    hipLaunchKernelGGL(k_row_compact, dim3(1), dim3(1024), 0, c.s, c.row_flag, nrow, L.rows, L.nrows);
+    hipLaunchKernelGGL(k_row_compact, dim3(1), dim3(1024), 0, c.s, L.aflag, nrow, L.rows, L.nrows);
+  }
+  // levels >= 1: rows where restriction makes r nonzero, finest first (each needs the finer
+  // level's set) ...
+  const int L = (int)c.lv.size();
+  for (int l = 1; l < L; ++l) {
+    Level& Lc = c.lv[l];
+    const Level& Lf = c.lv[l - 1];
+    const int nrow = Lc.d[0] * Lc.d[1];
+    hipLaunchKernelGGL(k_qrow_flags, dim3((unsigned)div_up(nrow, kBlock)), dim3(kBlock), 0, c.s, Lc.aflag, Lf.aflag,
+                       l >= 2 ? Lf.qflag : nullptr, Lc.qflag, Lc.d[0], Lc.d[1], Lf.d[1]);
+  }
+  // ... the z-supports, coarsest first ...
+  for (int l = L - 1; l >= 1; --l) {
+    Level& Lv = c.lv[l];
+    const int nrow = Lv.d[0] * Lv.d[1];
+    const bool coarser = l + 1 < L;
+    hipLaunchKernelGGL(k_zrow_flags, dim3((unsigned)div_up(nrow, kBlock)), dim3(kBlock), 0, c.s, Lv.aflag,
+                       coarser ? c.lv[l + 1].zflag : nullptr, Lv.zflag, Lv.d[0], Lv.d[1],
+                       coarser ? c.lv[l + 1].d[0] : 0, coarser ? c.lv[l + 1].d[1] : 0);
+  }
+  // ... then the restriction rows = r rows or z-support, and the prolongation rows = z-support.
+  // The q flags of the next level were built from the r rows alone, as they must be.
+  for (int l = 1; l < L; ++l) {
+    Level& Lv = c.lv[l];
+    const int nrow = Lv.d[0] * Lv.d[1];
+    hipLaunchKernelGGL(k_row_compact, dim3(1), dim3(1024), 0, c.s, Lv.zflag, nrow, Lv.zrows, Lv.nzrows);
+  }
+  for (int l = 1; l < L; ++l) {
+    Level& Lv = c.lv[l];
+    const int nrow = Lv.d[0] * Lv.d[1];
+    hipLaunchKernelGGL(k_or_flags, dim3((unsigned)div_up(nrow, kBlock)), dim3(kBlock), 0, c.s, Lv.qflag, Lv.zflag,
+                       nrow);
+    hipLaunchKernelGGL(k_row_compact, dim3(1), dim3(1024), 0, c.s, Lv.qflag, nrow, Lv.qrows, Lv.nqrows);
   }
 }
 
@@ -650,6 +740,13 @@ namespace {
 unsigned smooth_blocks(const Level& L) {
   const int H = (L.d[2] + 1) / 2;
   const int R = H >= kBlock ? 1 : kBlock / H;
+  const size_t need = div_up((size_t)L.d[0] * L.d[1], (size_t)R);
+  return (unsigned)std::max<size_t>(1, std::min<size_t>(need, 8192));
+}
+
+// blocks for a kernel over the cells of a level's row list (RowIter mapping; the kernel strides)
+unsigned row_blocks(const Level& L) {
+  const int R = L.d[2] >= kBlock ? 1 : kBlock / L.d[2];
   const size_t need = div_up((size_t)L.d[0] * L.d[1], (size_t)R);
   return (unsigned)std::max<size_t>(1, std::min<size_t>(need, 8192));
 }
@@ -694,13 +791,15 @@ void apply_precond(Ctx& c) {
   const int L = (int)c.lv.size();
   const int sym = (c.cfg.smoother == LH_SMOOTHER_SYMMETRIC);
   const int mode = sym ? 1 : 0;
-  if (sym) launch_fill_f32(c, L0.z, L0.n, 0.f);
+  // z[0] = 0 where it is read: on the liquid rows (elsewhere it only meets zero adiag and r)
+  if (sym)
+    hipLaunchKernelGGL(k_zero_rows, dim3(row_blocks(L0)), dim3(kBlock), 0, c.s, L0.z, L0.rows, L0.nrows, L0.d[2]);
   for (int l = 1; l < L; ++l) {
     for (int it = 0; it < c.cfg.num_pre; ++it) sweep(c, l - 1, 0, mode);
     Level& Lc = c.lv[l];
     Level& Lf = c.lv[l - 1];
-    hipLaunchKernelGGL(k_restrict, dim3(grid_for(Lc.n)), dim3(kBlock), 0, c.s, Lc.r, Lc.z, Lf.r, Lf.z, lvp(Lf),
-                       Lc.d[0], Lc.d[1], Lc.d[2]);
+    hipLaunchKernelGGL(k_restrict, dim3(row_blocks(Lc)), dim3(kBlock), 0, c.s, Lc.r, Lc.z, Lf.r, Lf.z, lvp(Lf),
+                       Lc.d[1], Lc.d[2], Lc.qrows, Lc.nqrows);
   }
   for (int it = 0; it < c.cfg.num_final; ++it) sweep(c, L - 1, 0, mode);
   if (sym)
@@ -708,8 +807,8 @@ void apply_precond(Ctx& c) {
   for (int l = L - 2; l >= 0; --l) {
     Level& Lc = c.lv[l + 1];
     Level& Lf = c.lv[l];
-    hipLaunchKernelGGL(k_prolong, dim3(grid_for(Lc.n)), dim3(kBlock), 0, c.s, Lf.z, Lc.z, Lf.d[1], Lf.d[2], Lc.d[0],
-                       Lc.d[1], Lc.d[2]);
+    hipLaunchKernelGGL(k_prolong, dim3(row_blocks(Lc)), dim3(kBlock), 0, c.s, Lf.z, Lc.z, Lf.d[1], Lf.d[2], Lc.d[1],
+                       Lc.d[2], Lc.zrows, Lc.nzrows);
     for (int it = 0; it < c.cfg.num_post; ++it) sweep(c, l, sym ? 1 : 0, mode);
   }
 }
@@ -778,35 +877,57 @@ __device__ __forceinline__ bool interior_liquid(const LvP& L, size_t i, int& xi,
   xi = (int)(i / ((size_t)d1 * d2));
   return xi >= 1 && xi <= L.d[0] - 2 && yi >= 1 && yi <= d1 - 2 && zi >= 1 && zi <= d2 - 2 && L.cls[i] != 0;
 }
-// r64 = b - A x64 on interior liquid cells (0 elsewhere); partials of |b|^2, |r64|^2 there
+// r64 = b - A x64 on the interior liquid cells of the level-0 liquid rows (0 on their other
+// cells); partials of |b|^2, |r64|^2 there.  Off those rows r64 is never read: the PCG of the
+// correction takes its right-hand side on liquid cells only.
 __global__ void k_ir_residual(double* __restrict__ r64, const double* __restrict__ x64, const float* __restrict__ b,
-                              LvP L, double* __restrict__ partial) {
-  size_t total = (size_t)L.d[0] * L.d[1] * L.d[2];
+                              LvP L, const int* __restrict__ rows, const int* __restrict__ nrows,
+                              double* __restrict__ partial) {
+  const int d1 = L.d[1], d2 = L.d[2];
   double v[2] = {0.0, 0.0};
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    int xi, yi, zi;
-    double rv = 0.0;
-    if (interior_liquid(L, i, xi, yi, zi)) {
-      rv = (double)b[i] - get_ax64(L, x64, xi, yi, zi, i);
-      v[0] += (double)b[i] * (double)b[i];
-      v[1] += rv * rv;
+  const RowIter it = row_iter(d2);
+  const int nr = *nrows;
+  if (it.on)
+    for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
+      const size_t base = (size_t)rows[k] * d2;
+      for (int zi = it.z0; zi < d2; zi += it.zs) {
+        const size_t i = base + zi;
+        int xi, yi, zz;
+        double rv = 0.0;
+        if (interior_liquid(L, i, xi, yi, zz)) {
+          rv = (double)b[i] - get_ax64(L, x64, xi, yi, zz, i);
+          v[0] += (double)b[i] * (double)b[i];
+          v[1] += rv * rv;
+        }
+        r64[i] = rv;
+      }
     }
-    r64[i] = rv;
-  }
+  (void)d1;
   block_partials<2>(v, partial);
 }
-// FP32 right-hand side of the correction equation, scaled to unit norm
+// FP32 right-hand side of the correction equation, scaled to unit norm (level-0 liquid rows)
 __global__ void k_ir_rhs(float* __restrict__ rf, const double* __restrict__ r64, const double* __restrict__ sc64,
-                         size_t n) {
+                         const int* __restrict__ rows, const int* __restrict__ nrows, int d2) {
   const double inv = (sc64[5] > 0.0) ? 1.0 / sqrt(sc64[5]) : 0.0;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    rf[i] = (float)(r64[i] * inv);
+  const RowIter it = row_iter(d2);
+  const int nr = *nrows;
+  if (!it.on) return;
+  for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
+    const size_t base = (size_t)rows[k] * d2;
+    for (int zi = it.z0; zi < d2; zi += it.zs) rf[base + zi] = (float)(r64[base + zi] * inv);
+  }
 }
+// x64 += ||r|| e over the level-0 liquid rows (e, the correction, is zero off the PCG's cells)
 __global__ void k_ir_update(double* __restrict__ x64, const float* __restrict__ e, const double* __restrict__ sc64,
-                            size_t n) {
+                            const int* __restrict__ rows, const int* __restrict__ nrows, int d2) {
   const double s = sqrt(sc64[5]);
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    x64[i] += s * (double)e[i];
+  const RowIter it = row_iter(d2);
+  const int nr = *nrows;
+  if (!it.on) return;
+  for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
+    const size_t base = (size_t)rows[k] * d2;
+    for (int zi = it.z0; zi < d2; zi += it.zs) x64[base + zi] += s * (double)e[base + zi];
+  }
 }
 __global__ void k_ir_to_f32(float* __restrict__ out, const double* __restrict__ x64, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -951,7 +1072,8 @@ int refine_solve(Ctx& c, const float* b, const float* x0, lh_solve_stats& S) {
   }
   int rc = LH_OK;
   for (int outer = 0;; ++outer) {
-    hipLaunchKernelGGL(k_ir_residual, dim3(nb), dim3(kBlock), 0, c.s, c.r64, c.x64, b, P0, c.partial);
+    hipLaunchKernelGGL(k_ir_residual, dim3(nb), dim3(kBlock), 0, c.s, c.r64, c.x64, b, P0, L0.rows, L0.nrows,
+                       c.partial);
     hipLaunchKernelGGL(k_sc_norms, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc64);
     hipMemcpyAsync(c.h_pin + 16, c.dsc64 + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, c.s);
     if (!hip_ok(c, hipStreamSynchronize(c.s), "refinement")) return LH_ERR_DEVICE;
@@ -970,7 +1092,8 @@ int refine_solve(Ctx& c, const float* b, const float* x0, lh_solve_stats& S) {
       rc = LH_ERR_NOT_CONVERGED;
       break;
     }
-    hipLaunchKernelGGL(k_ir_rhs, dim3(grid_for(n)), dim3(kBlock), 0, c.s, c.rf, c.r64, c.dsc64, n);
+    hipLaunchKernelGGL(k_ir_rhs, dim3(row_blocks(L0)), dim3(kBlock), 0, c.s, c.rf, c.r64, c.dsc64, L0.rows, L0.nrows,
+                       L0.d[2]);
     // inner tolerance: the correction only has to bring ||r|| under tol64 ||b||, so it is
     // solved to max(tol, 0.25 tol64 ||b|| / ||r||) relative (at most 0.1), never tighter than
     // the configured FP32 tolerance
@@ -978,7 +1101,8 @@ int refine_solve(Ctx& c, const float* b, const float* x0, lh_solve_stats& S) {
     inner = std::min(inner, 0.1);
     int irc = pcg_solve(c, c.rf, nullptr, (float)inner, LH_TOL_RELATIVE, c.cfg.max_iter, S);
     if (irc == LH_ERR_DEVICE || irc == LH_ERR_NAN) return irc;
-    hipLaunchKernelGGL(k_ir_update, dim3(grid_for(n)), dim3(kBlock), 0, c.s, c.x64, c.x, c.dsc64, n);
+    hipLaunchKernelGGL(k_ir_update, dim3(row_blocks(L0)), dim3(kBlock), 0, c.s, c.x64, c.x, c.dsc64, L0.rows,
+                       L0.nrows, L0.d[2]);
   }
   S.warm_start_used = x0 ? 1 : 0;
   S.converged = (rc == LH_OK) ? 1 : 0;

@@ -213,9 +213,12 @@ int alloc_ctx(Ctx& c) {
     q.n = (size_t)q.d[0] * q.d[1] * q.d[2];
     ok = ok && dalloc(c, q.adiag, q.n) && dalloc(c, q.t[0], q.n) && dalloc(c, q.t[1], q.n) &&
          dalloc(c, q.t[2], q.n) && dalloc(c, q.cls, q.n) && dalloc(c, q.r, q.n) && dalloc(c, q.z, q.n) &&
-         dalloc(c, q.z2, q.n) && dalloc(c, q.rows, (size_t)q.d[0] * q.d[1]) && dalloc(c, q.nrows, 1);
+         dalloc(c, q.z2, q.n) && dalloc(c, q.rows, (size_t)q.d[0] * q.d[1]) && dalloc(c, q.nrows, 1) &&
+         dalloc(c, q.aflag, (size_t)q.d[0] * q.d[1]) && dalloc(c, q.qrows, (size_t)q.d[0] * q.d[1]) &&
+         dalloc(c, q.nqrows, 1) && dalloc(c, q.qflag, (size_t)q.d[0] * q.d[1]) &&
+         dalloc(c, q.zrows, (size_t)q.d[0] * q.d[1]) && dalloc(c, q.nzrows, 1) &&
+         dalloc(c, q.zflag, (size_t)q.d[0] * q.d[1]);
   }
-  ok = ok && dalloc(c, c.row_flag, (size_t)c.cfg.N[0] * c.cfg.N[1]);
   c.npartial = 1024;
   ok = ok && dalloc(c, c.p, c.ncell) && dalloc(c, c.x, c.ncell) && dalloc(c, c.q, c.ncell) &&
        dalloc(c, c.partial, 4 * (size_t)c.npartial) && dalloc(c, c.dsc, 16) && dalloc(c, c.dsc64, 16) &&
@@ -241,6 +244,8 @@ int alloc_ctx(Ctx& c) {
   launch_fill_f32(c, c.rhs, c.ncell, 0.f);
   for (auto& q : c.lv) {
     hipMemsetAsync(q.nrows, 0, sizeof(int), c.s);
+    hipMemsetAsync(q.nqrows, 0, sizeof(int), c.s);
+    hipMemsetAsync(q.nzrows, 0, sizeof(int), c.s);
     launch_fill_f32(c, q.adiag, q.n, 0.f);
     for (int a = 0; a < 3; ++a) launch_fill_f32(c, q.t[a], q.n, 0.f);
     launch_fill_u8(c, q.cls, q.n, 0);
@@ -265,8 +270,8 @@ void free_ctx(Ctx& c) {
   F(c.pbuf[0]); F(c.pbuf[1]); F(c.ppack);
   for (auto& q : c.lv) {
     F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2); F(q.rows); F(q.nrows);
+    F(q.aflag); F(q.qrows); F(q.nqrows); F(q.qflag); F(q.zrows); F(q.nzrows); F(q.zflag);
   }
-  F(c.row_flag);
   F(c.p); F(c.x); F(c.q); F(c.partial); F(c.dsc); F(c.dsc64); F(c.dmaxv); F(c.dflag);
   F(c.x64); F(c.r64); F(c.rf); F(c.b_user); F(c.x0_user);
   F(c.xv_snap); F(c.xv_final); F(c.xcount); F(c.xlist[0]); F(c.xlist[1]); F(c.xlist[2]);

@@ -65,6 +65,17 @@ struct Level {
   // z-rows (x, y) holding at least one liquid cell (cls != 0), ascending; rebuilt with the operator
   int* rows = nullptr;   // capacity d[0] * d[1]
   int* nrows = nullptr;  // device count
+  int* aflag = nullptr;  // per-row flag of that set
+  // levels >= 1, rows of the restriction: where it can produce a nonzero r (parents of the
+  // finer level's liquid and restriction rows) or must clear z (the z-support below)
+  int* qrows = nullptr;
+  int* nqrows = nullptr;
+  int* qflag = nullptr;
+  // levels >= 1, z-support: rows where z can be nonzero -- this level's liquid rows and the
+  // children of the coarser level's z-support (prolongation adds there); prolongation rows
+  int* zrows = nullptr;
+  int* nzrows = nullptr;
+  int* zflag = nullptr;
 };
 
 // attributes of one particle in the SoA particle buffer (kParticleAttrs x cap)
@@ -100,7 +111,6 @@ struct Ctx {
   int* pkey = nullptr;       // per particle bin key
   int* pslot = nullptr;      // per particle provisional slot
   int* scan_tmp = nullptr;   // scan block sums
-  int* row_flag = nullptr;   // per z-row flags while the active-row lists are built
   bool bins_valid = false;
   // solver (CGSolver<3>)
   std::vector<Level> lv;
