@@ -141,6 +141,34 @@ def test_free_running_steps_bit_exact(case):
     scene.close()
 
 
+def test_dam_break_128_free_running_bit_exact():
+    """the 128^3 dam break (block x in [0, 0.4], y in [0, 0.5], z in [0, 1], 8 particles per
+    cell, ~3.3 M particles), 3-level symmetric MGPCG to 1e-6 relative: three free-running
+    steps bit-identical to the oracle (particles, iteration counts, CFL dt, liquid mask)"""
+    print("case dam128 symmetric", flush=True)
+    cfg = small_dam(128, "symmetric")
+    orc = Oracle(cfg, dot_mode=1, p2g_order=1)
+    orc.init()
+    scene = lh.Scene(cfg)
+    m = scene.GetLiquidModel()
+    m.Init()
+    dt = np.float32(cfg.get("firstDt", 0.01))
+    dtg = float(dt)
+    for k in range(3):
+        st, _, _ = orc.step(dt)
+        dt = np.float32(orc.get_cfl()[0])
+        dtg, sst = m.RunSteps(1, dtg)
+        print("step", k, "iterations", st.iterations, flush=True)
+        assert sst["solve"]["iterations"] == st.iterations
+        assert np.float32(dtg) == dt
+    same(m.get_field("liquidPhi") <= 0, orc.get(F["liquidPhi"]) <= 0, "liquid mask after 3 steps")
+    pg, vg, _ = m.GetParticleSystem().get()
+    po, vo, _ = orc.particles()
+    same(pg, po, "positions after 3 steps")
+    same(vg, vo, "velocities after 3 steps")
+    scene.close()
+
+
 def test_reference_run_known_answer():
     """LiquidData.csv frame 1 on the GPU: every particle falls by (0, -g dt^2, 0)."""
     import os
