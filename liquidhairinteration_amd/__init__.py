@@ -107,6 +107,7 @@ def load_library():
         "lh_g2p": ([vp], C.c_int),
         "lh_advect": ([vp, C.c_float], C.c_int),
         "lh_divergence": ([vp, dp, dp, ip], C.c_int),
+        "lh_correct_volume": ([vp, C.c_float], C.c_int),
         "lh_num_particles": ([vp, C.POINTER(C.c_int64)], C.c_int),
         "lh_set_particles": ([vp, fp, fp, fp, C.c_int64], C.c_int),
         "lh_get_particles": ([vp, fp, fp, fp, C.c_int64], C.c_int),
@@ -448,6 +449,10 @@ class LiquidModel:
 
     def AdvectParticle(self, dt):
         self._h.check(self._h.L.lh_advect(self._h.h, float(dt)))
+
+    def CorrectVolume(self, dt):
+        """LiquidModel<3>::CorrectVolume (LiquidModel.cpp:1085-1132)"""
+        self._h.check(self._h.L.lh_correct_volume(self._h.h, float(dt)))
 
     def ComputeDivergence(self):
         """Mean weighted divergence over liquid cells (LiquidModel.cpp:1065-1083); also returns L-inf and count."""

@@ -519,6 +519,175 @@ __global__ void k_g2p_advect(float* __restrict__ pb, int64_t cap, int64_t P, con
   if (threadIdx.x == 0) atomicMax(maxbits, __float_as_uint(sm[0]));
 }
 
+// ------------------------------------------------------------------ CorrectVolume
+// LiquidModel<3>::CorrectVolume, LiquidModel.cpp:1085-1132, for the particles with
+// id % correctCycle == correctStep.  The reference's hash search (NeighborSearch.h:22-57,
+// 131-136) files each particle under the hash slots of the 3x3x3 cells around its own cell
+// (clamped to the grid), 2 NumParticles slots, and scans the slots of the 3x3x3 cells
+// around the querying particle, so a neighbour closer than the radius is visited
+//   visits = #{(C1, C2) in R(pi) x R(npi) : slot(C1) == slot(C2)}
+// times: once per cell the two regions share, plus once per pair of distinct cells whose
+// slots collide.  Without collisions among the cells involved this is the overlap product
+//   prod_axes |[max(c1-1,0), min(c1+1,N-1)] & [max(c2-1,0), min(c2+1,N-1)]|.
+// A neighbour within the radius (2 dx) lies within 2 cells per axis, so C1 and C2 are at
+// most 4 cells apart: a per-cell flag (slot shared with another cell within 4) marks the
+// regions that need the exact count; all others take the product.  Neighbours come from the
+// half-cell bins (cells within 2, plus one half-cell of slack) in bin order, and the random
+// kick of coincident particles (the reference's random_device-seeded mt19937) comes from a
+// counter generator.
+struct CVP {
+  GridP g;
+  int H[3];
+  int cycle, step;
+  float dt, wf, radius;
+  unsigned long long seed, count;
+  float mx[3];
+  int nslot;
+};
+
+__device__ __forceinline__ uint64_t cv_splitmix(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+// uniform [0, 1) for component k of the v-th visit of neighbour npid by particle pid in
+// the count-th CorrectVolume call
+__device__ __forceinline__ float cv_rand(uint64_t seed, uint64_t count, int64_t pid, int64_t npid, int v, int k) {
+  uint64_t h = cv_splitmix(seed ^ 0x436F7272566F6Cull);
+  h = cv_splitmix(h ^ count);
+  h = cv_splitmix(h ^ (uint64_t)pid);
+  h = cv_splitmix(h ^ ((uint64_t)npid * 128u + (uint64_t)v * 4u + (uint64_t)k));
+  return (float)(h >> 40) * (1.0f / 16777216.0f);
+}
+__device__ __forceinline__ int cv_cell(float p, float mn, float dx) { return (int)((p - mn) / dx); }
+// hash slot of cell (x, y, z) among m, NeighborSearch.h:45 with the constants of :134-136: the
+// products wrap at 32 bits, % truncates toward zero, then the result is made >= 0
+__device__ __forceinline__ int ns_slot(int x, int y, int z, int m) {
+  const uint32_t h = (73856093u * (uint32_t)x) ^ (19349663u * (uint32_t)y) ^ (83492791u * (uint32_t)z);
+  const int hs = (int)h;
+  return ((hs % m) + m) % m;
+}
+
+// flag[cell] = 1 when another cell within 4 per axis (inside the grid) has its slot
+__global__ void __launch_bounds__(256) k_cv_flags(uint8_t* __restrict__ flag, int N0, int N1, int N2, int m) {
+  const int64_t n = (int64_t)N0 * N1 * N2;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int z = (int)(i % N2), y = (int)((i / N2) % N1), x = (int)(i / ((int64_t)N1 * N2));
+    const int s = ns_slot(x, y, z, m);
+    int hit = 0;
+    for (int x2 = max(x - 4, 0); x2 <= min(x + 4, N0 - 1) && !hit; ++x2)
+      for (int y2 = max(y - 4, 0); y2 <= min(y + 4, N1 - 1) && !hit; ++y2)
+        for (int z2 = max(z - 4, 0); z2 <= min(z + 4, N2 - 1); ++z2)
+          if (ns_slot(x2, y2, z2, m) == s && (x2 != x || y2 != y || z2 != z)) {
+            hit = 1;
+            break;
+          }
+    flag[i] = (uint8_t)hit;
+  }
+}
+
+// new positions of the selected particles into out (3 x cap), bin-ordered particles
+__global__ void __launch_bounds__(256) k_correct_volume(const float* __restrict__ pb, int64_t cap, int64_t P,
+                                                        const int* __restrict__ start,
+                                                        const float* __restrict__ solid,
+                                                        const uint8_t* __restrict__ flag, CVP q,
+                                                        float* __restrict__ out) {
+  const int* N = q.g.N;
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < P; s += (int64_t)gridDim.x * blockDim.x) {
+    const int pid = __float_as_int(pb[(size_t)PA_ID * cap + s]);
+    if (pid % q.cycle != q.step) continue;
+    const float p[3] = {pb[(size_t)PA_X * cap + s], pb[(size_t)(PA_X + 1) * cap + s], pb[(size_t)(PA_X + 2) * cap + s]};
+    const float r = q.radius, r2 = r * r;
+    int lo[3], hi[3], hlo[3], hhi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int cc = cv_cell(p[a], q.g.mn[a], q.g.dx[a]);
+      lo[a] = min(max(cc - 1, 0), N[a] - 1);
+      hi[a] = min(max(cc + 1, 0), N[a] - 1);
+      hlo[a] = max(0, 2 * (cc - 2) - 1);
+      hhi[a] = min(q.H[a] - 1, 2 * (cc + 2) + 2);
+    }
+    bool exact = false;  // a cell of the region shares its slot with a cell it can meet
+    for (int x1 = lo[0]; x1 <= hi[0]; ++x1)
+      for (int y1 = lo[1]; y1 <= hi[1]; ++y1)
+        for (int z1 = lo[2]; z1 <= hi[2]; ++z1) exact |= flag[((size_t)x1 * N[1] + y1) * N[2] + z1] != 0;
+    int reg[27], nreg = 0;  // slots of the region's cells (exact count only)
+    if (exact)
+      for (int x1 = lo[0]; x1 <= hi[0]; ++x1)
+        for (int y1 = lo[1]; y1 <= hi[1]; ++y1)
+          for (int z1 = lo[2]; z1 <= hi[2]; ++z1) reg[nreg++] = ns_slot(x1, y1, z1, q.nslot);
+    float rep[3] = {0.f, 0.f, 0.f};
+    for (int hx = hlo[0]; hx <= hhi[0]; ++hx)
+      for (int hy = hlo[1]; hy <= hhi[1]; ++hy) {
+        const size_t k0 = ((size_t)hx * q.H[1] + hy) * q.H[2];
+        const int s1 = start[k0 + hhi[2] + 1];
+        for (int t = start[k0 + hlo[2]]; t < s1; ++t) {
+          const int npid = __float_as_int(pb[(size_t)PA_ID * cap + t]);
+          if (npid == pid) continue;
+          const float qq[3] = {pb[(size_t)PA_X * cap + t], pb[(size_t)(PA_X + 1) * cap + t],
+                               pb[(size_t)(PA_X + 2) * cap + t]};
+          const float d0 = p[0] - qq[0], d1 = p[1] - qq[1], d2 = p[2] - qq[2];
+          const float sq = d0 * d0 + (d1 * d1 + d2 * d2);
+          if (sq >= r2) continue;
+          int l2[3], h2[3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const int c2 = cv_cell(qq[a], q.g.mn[a], q.g.dx[a]);
+            l2[a] = min(max(c2 - 1, 0), N[a] - 1);
+            h2[a] = min(max(c2 + 1, 0), N[a] - 1);
+          }
+          int mult = 0;
+          if (exact) {
+            for (int x2 = l2[0]; x2 <= h2[0]; ++x2)
+              for (int y2 = l2[1]; y2 <= h2[1]; ++y2)
+                for (int z2 = l2[2]; z2 <= h2[2]; ++z2) {
+                  const int s2 = ns_slot(x2, y2, z2, q.nslot);
+                  for (int k = 0; k < nreg; ++k) mult += reg[k] == s2 ? 1 : 0;
+                }
+          } else {
+            mult = 1;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) mult *= max(0, min(hi[a], h2[a]) - max(lo[a], l2[a]) + 1);
+          }
+          const float dist = sqrtf(sq);
+          const float w = q.wf * quad_w(dist / r);
+          for (int v = 0; v < mult; ++v) {
+            if (dist > 1e-4f * r) {
+              const float wr = w * r;
+              rep[0] += wr * (d0 / dist);
+              rep[1] += wr * (d1 / dist);
+              rep[2] += wr * (d2 / dist);
+            } else {
+#pragma unroll
+              for (int k = 0; k < 3; ++k) rep[k] += r / q.dt * cv_rand(q.seed, q.count, pid, npid, v, k);
+            }
+          }
+        }
+      }
+    float np[3] = {p[0] + rep[0] * q.dt, p[1] + rep[1] * q.dt, p[2] + rep[2] * q.dt};
+    const float sphi = solid_phi_any(solid, q.g, q.mx, np[0], np[1], np[2]);
+    if (sphi < 0.0f) {
+      float gr[3];
+      solid_grad_any(solid, q.g, q.mx, np[0], np[1], np[2], gr);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) np[k] -= sphi * gr[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) out[(size_t)k * cap + s] = np[k];
+  }
+}
+// positions of the selected particles <- out
+__global__ void k_correct_commit(float* __restrict__ pb, int64_t cap, int64_t P, const float* __restrict__ out,
+                                 int cycle, int step) {
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < P; s += (int64_t)gridDim.x * blockDim.x) {
+    const int pid = __float_as_int(pb[(size_t)PA_ID * cap + s]);
+    if (pid % cycle != step) continue;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) pb[(size_t)(PA_X + k) * cap + s] = out[(size_t)k * cap + s];
+  }
+}
+
 }  // namespace
 
 int run_binning(Ctx& c) {
@@ -576,6 +745,54 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
                             div_up(c.g.N[2] + (a == 2), kTZ));
     hipLaunchKernelGGL(k_p2g, dim3(nt), dim3(kBlock), 0, c.s, c.v[a], (const float4*)c.ppack, c.cap, c.bin_start, qa);
   }
+}
+
+int run_correct_volume(Ctx& c, float dt) {
+  if (c.P == 0) return LH_OK;
+  int rc = run_binning(c);  // neighbours of the advected positions
+  if (rc) return rc;
+  CVP q;
+  q.g = c.g;
+  for (int a = 0; a < 3; ++a) {
+    q.H[a] = c.H[a];
+    q.mx[a] = c.cfg.mx[a];
+  }
+  q.cycle = std::max(1, c.cfg.correct_cycle);
+  q.step = c.correct_step;
+  q.dt = dt;
+  q.wf = sqrtf(c.g.dx[0] * c.g.dx[1] * c.g.dx[2]);  // std::sqrt(dx.x() * dx.y() * dx.z())
+  q.radius = c.radius;
+  q.seed = c.cfg.seed;
+  q.count = c.correct_count;
+  if (c.P > (int64_t)(INT32_MAX / 2)) {
+    c.err = "CorrectVolume: the hash table of 2 x NumParticles slots exceeds a 32-bit index";
+    return LH_ERR_DOMAIN;
+  }
+  q.nslot = (int)(2 * c.P);  // HashNeighborSearch: scale factor 2 x NumParticles
+  const size_t ncell = (size_t)c.g.N[0] * c.g.N[1] * c.g.N[2];
+  if (!c.cv_flag) {
+    if (hipMalloc((void**)&c.cv_flag, ncell) != hipSuccess) {
+      c.cv_flag = nullptr;
+      c.err = "hipMalloc failed (CorrectVolume cell flags)";
+      return LH_ERR_DEVICE;
+    }
+    c.cv_nslot = 0;
+  }
+  if (c.cv_nslot != q.nslot) {
+    hipLaunchKernelGGL(k_cv_flags, dim3(grid_for(ncell)), dim3(kBlock), 0, c.s, c.cv_flag, c.g.N[0], c.g.N[1],
+                       c.g.N[2], q.nslot);
+    c.cv_nslot = q.nslot;
+  }
+  float* out = c.pbuf[1 - c.cur];  // free between permutations: 3 x cap floats of it
+  const unsigned gp = grid_for((size_t)c.P);
+  hipLaunchKernelGGL(k_correct_volume, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, c.bin_start,
+                     c.solid_phi, c.cv_flag, q, out);
+  hipLaunchKernelGGL(k_correct_commit, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, out, q.cycle,
+                     q.step);
+  c.bins_valid = false;
+  c.correct_step = (q.step + 1) % q.cycle;
+  c.correct_count++;
+  return LH_OK;
 }
 
 void launch_g2p_advect(Ctx& c, bool do_g2p, bool do_advect, float dt) {

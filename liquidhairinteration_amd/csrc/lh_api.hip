@@ -139,6 +139,7 @@ Config parse_config(const std::string& text) {
   else if (prec != "fp32") throw std::runtime_error("Unknown precision " + prec);
   c.refined_tol = j.get("refinedTolerance", 1e-9);
   c.first_dt = (float)j.get("firstDt", 0.01);
+  c.correct_volume = (j.has("correctVolume") && j.at("correctVolume").as_bool()) ? 1 : 0;
   c.steps = (int)j.get("steps", 0.0);
   for (int a = 0; a < 3; ++a)
     if (c.N[a] < 3) throw std::runtime_error("gridDimension must be >= 3 on every axis");
@@ -267,7 +268,7 @@ void free_ctx(Ctx& c) {
   }
   F(c.pressure); F(c.phi); F(c.rhs); F(c.solid_phi); F(c.is_valid); F(c.xstate); F(c.d_sdf);
   F(c.bin_count); F(c.bin_start); F(c.scan_tmp); F(c.pkey); F(c.pslot);
-  F(c.pbuf[0]); F(c.pbuf[1]); F(c.ppack);
+  F(c.pbuf[0]); F(c.pbuf[1]); F(c.ppack); F(c.cv_flag);
   for (auto& q : c.lv) {
     F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2); F(q.rows); F(q.nrows);
     F(q.aflag); F(q.qrows); F(q.nqrows); F(q.qflag); F(q.zrows); F(q.nzrows); F(q.zflag);
@@ -513,6 +514,10 @@ int do_step(Ctx& c, float dt, lh_solve_stats* st) {
   if (run_extrapolate(c) < 0) return LH_ERR_DEVICE;
   launch_g2p_advect(c, true, true, dt);  // Grid2Particle + AdvectParticle (+ max|v|)
   c.bins_valid = false;
+  if (c.cfg.correct_volume) {  // CorrectVolume(dt), LiquidModel.cpp:527 (commented out in the reference)
+    int crc = run_correct_volume(c, dt);
+    if (crc) return crc;
+  }
   c.time += (double)dt;
   c.last_solve = local;
   int rc = check_domain(c);
@@ -718,6 +723,14 @@ int lh_advect(lh_ctx* ctx, float dt) {
     launch_g2p_advect(c, false, true, dt);
     c.bins_valid = false;
     return read_maxv(c);
+  });
+}
+
+int lh_correct_volume(lh_ctx* ctx, float dt) {
+  return guarded(ctx, [dt](Ctx& c) {
+    int rc = run_correct_volume(c, dt);
+    if (rc) return rc;
+    return hip_ok(c, hipStreamSynchronize(c.s), "correct volume") ? LH_OK : LH_ERR_DEVICE;
   });
 }
 

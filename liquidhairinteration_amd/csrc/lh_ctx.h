@@ -46,6 +46,7 @@ struct Config {
   int tank_axis = -1;
   float first_dt = 0.01f;
   int steps = 0;
+  int correct_volume = 0;  // CorrectVolume after AdvectParticle (commented out of the reference's 3D step)
 };
 
 struct GridP {
@@ -134,6 +135,12 @@ struct Ctx {
   int* h_flag = nullptr;
   double time = 0.0;
   float last_maxv = 0.f;
+  int correct_step = 0;              // m_correctStep (LiquidModel.h:73)
+  unsigned long long correct_count = 0;  // CorrectVolume calls so far (random-stream counter)
+  // per cell: 1 when its hash slot (2 P slots) equals that of another cell within 4 cells,
+  // valid for cv_nslot slots (0: not built)
+  uint8_t* cv_flag = nullptr;
+  int cv_nslot = 0;
   bool solid_ready = false;
   // Extrapolate schedule cache: which faces each sweep validates, and from which
   // neighbours, is a function of isValid at the start of Extrapolate and of
@@ -192,6 +199,7 @@ int run_binning(Ctx& c);  // LH_OK or LH_ERR_DOMAIN
 void launch_liquid_phi(Ctx& c);
 void launch_p2g(Ctx& c, bool fuse_body_forces, float dt);
 void launch_g2p_advect(Ctx& c, bool do_g2p, bool do_advect, float dt);
+int run_correct_volume(Ctx& c, float dt);  // LiquidModel<3>::CorrectVolume; LH_OK or LH_ERR_DOMAIN
 // ---- launchers (k_solver.hip)
 int run_solve(Ctx& c, float dt, lh_solve_stats* st);  // CGSolver<3>::Solve
 void build_system(Ctx& c, float dt);                  // Init + BuildRHS + BuildLHS

@@ -62,6 +62,9 @@ struct orc_params {
   int tank_axis;  // -1: none
   int nsdf;       // number of SDF program nodes (12 floats each)
   const float* sdf;
+  int correct_volume;  // 1: CorrectVolume after AdvectParticle (commented out of the reference's 3D step)
+  int correct_cycle;   // LiquidParameter.correctCycle
+  unsigned long long seed;
 };
 struct orc_solve_stats {
   int iterations, converged;
@@ -201,6 +204,8 @@ struct Model {
   std::vector<Level> lv;
   int max_level = 1;
   double time = 0.0;
+  int correct_step = 0;      // m_correctStep (LiquidModel.h:73)
+  uint64_t correct_count = 0;  // CorrectVolume calls so far (random-stream counter)
   std::string err;
 };
 
@@ -1005,6 +1010,124 @@ void advect(Model& m, float dt) {
   }
 }
 
+inline uint64_t cv_splitmix(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+// uniform [0, 1) for the k-th component of the v-th visit of neighbour npi by particle pi
+// in the count-th CorrectVolume call (replaces the reference's random_device-seeded mt19937)
+inline float cv_rand(uint64_t seed, uint64_t count, int64_t pi, int64_t npi, int v, int k) {
+  uint64_t h = cv_splitmix(seed ^ 0x436F7272566F6Cull);
+  h = cv_splitmix(h ^ count);
+  h = cv_splitmix(h ^ (uint64_t)pi);
+  h = cv_splitmix(h ^ ((uint64_t)npi * 128u + (uint64_t)v * 4u + (uint64_t)k));
+  return (float)(h >> 40) * (1.0f / 16777216.0f);
+}
+inline int cv_cell(float p, float mn, float dx) { return (int)((p - mn) / dx); }  // cast<Integer>(): truncation
+// hash slot of cell (x, y, z), NeighborSearch.h:45 with the constants of :134-136: Integer is
+// a 32-bit int, so the products wrap; % truncates toward zero, then the result is made >= 0
+inline int ns_slot(int x, int y, int z, int m) {
+  const uint32_t h = (73856093u * (uint32_t)x) ^ (19349663u * (uint32_t)y) ^ (83492791u * (uint32_t)z);
+  const int hs = (int)h;
+  return ((hs % m) + m) % m;
+}
+inline int cv_clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// LiquidModel<3>::CorrectVolume, LiquidModel.cpp:1085-1132, on every particle with
+// id % correctCycle == correctStep.  The reference's hash search (NeighborSearch.h:22-131)
+// puts each particle into the slots of the 3x3x3 cells around its own cell (clamped to the
+// grid) and visits the slots of the 3x3x3 cells around the querying particle, so a
+// neighbour closer than the radius is visited once per cell the two clamped regions share:
+// mult = prod_axes |[max(c1-1,0), min(c1+1,N-1)] & [max(c2-1,0), min(c2+1,N-1)]|.
+// Two distinct cells whose slots collide add visits too, so the exact count is
+//   visits = #{(C1, C2) in R(pi) x R(npi) : slot(C1) == slot(C2)},  m = 2 * NumParticles slots.
+// Restated here with that count, the table built from the current positions (the recorded
+// run LiquidData.csv behaves so; the current source rebuilds it only at the end of
+// CorrectVolume), neighbours visited in half-cell bin order (the reference's order depends
+// on its parallel insertion).  Positions are updated after all particles are processed.
+bool correct_volume(Model& m, float dt) {
+  if (m.P == 0) return true;
+  Bins b;
+  if (!build_bins(m, b)) return false;
+  const int cyc = std::max(1, m.prm.correct_cycle), step = m.correct_step;
+  const int nslot = (int)(2 * m.P);  // HashNeighborSearch m_size = scaleFactor (2) * NumParticles
+  const float wf = std::sqrt(m.dx[0] * m.dx[1] * m.dx[2]);
+  std::vector<float> buf(3 * (size_t)m.P);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (i64 pi = 0; pi < m.P; ++pi) {
+    if (pi % cyc != step) continue;
+    const float* p = &m.pos[3 * pi];
+    const float r = m.radius[pi];
+    const float r2 = r * r;
+    int lo[3], hi[3], hlo[3], hhi[3];
+    for (int a = 0; a < 3; ++a) {
+      const int c = cv_cell(p[a], m.mn[a], m.dx[a]);
+      lo[a] = cv_clampi(c - 1, 0, m.N[a] - 1);
+      hi[a] = cv_clampi(c + 1, 0, m.N[a] - 1);
+      hlo[a] = std::max(0, 2 * (c - 2) - 1);
+      hhi[a] = std::min(b.H[a] - 1, 2 * (c + 2) + 2);
+    }
+    int reg[27], nreg = 0;  // slots of the cells around pi, with repeats
+    for (int x1 = lo[0]; x1 <= hi[0]; ++x1)
+      for (int y1 = lo[1]; y1 <= hi[1]; ++y1)
+        for (int z1 = lo[2]; z1 <= hi[2]; ++z1) reg[nreg++] = ns_slot(x1, y1, z1, nslot);
+    float rep[3] = {0.f, 0.f, 0.f};
+    for (int hx = hlo[0]; hx <= hhi[0]; ++hx)
+      for (int hy = hlo[1]; hy <= hhi[1]; ++hy) {
+        const size_t k0 = ((size_t)hx * b.H[1] + hy) * b.H[2];
+        for (i64 s = b.start[k0 + hlo[2]]; s < b.start[k0 + hhi[2] + 1]; ++s) {
+          const i64 npi = b.order[s];
+          if (npi == pi) continue;
+          const float* q = &m.pos[3 * npi];
+          const float d0 = p[0] - q[0], d1 = p[1] - q[1], d2 = p[2] - q[2];
+          const float sq = d0 * d0 + (d1 * d1 + d2 * d2);
+          if (sq >= r2) continue;
+          int nlo[3], nhi[3];
+          for (int a = 0; a < 3; ++a) {
+            const int c2 = cv_cell(q[a], m.mn[a], m.dx[a]);
+            nlo[a] = cv_clampi(c2 - 1, 0, m.N[a] - 1);
+            nhi[a] = cv_clampi(c2 + 1, 0, m.N[a] - 1);
+          }
+          int mult = 0;  // visits: slot matches between the two regions
+          for (int x2 = nlo[0]; x2 <= nhi[0]; ++x2)
+            for (int y2 = nlo[1]; y2 <= nhi[1]; ++y2)
+              for (int z2 = nlo[2]; z2 <= nhi[2]; ++z2) {
+                const int s2 = ns_slot(x2, y2, z2, nslot);
+                for (int k = 0; k < nreg; ++k) mult += (reg[k] == s2) ? 1 : 0;
+              }
+          const float dist = std::sqrt(sq);
+          const float w = wf * quad_w(dist / r);
+          for (int v = 0; v < mult; ++v) {
+            if (dist > 1e-4f * r) {
+              const float wr = w * r;
+              rep[0] += wr * (d0 / dist);
+              rep[1] += wr * (d1 / dist);
+              rep[2] += wr * (d2 / dist);
+            } else {
+              for (int k = 0; k < 3; ++k) rep[k] += r / dt * cv_rand(m.prm.seed, m.correct_count, pi, npi, v, k);
+            }
+          }
+        }
+      }
+    float np[3] = {p[0] + rep[0] * dt, p[1] + rep[1] * dt, p[2] + rep[2] * dt};
+    const float sp = solid_phi_at(m, np[0], np[1], np[2]);
+    if (sp < 0.0f) {
+      float g[3];
+      solid_gradient(m, np[0], np[1], np[2], g);
+      for (int k = 0; k < 3; ++k) np[k] -= sp * g[k];
+    }
+    for (int k = 0; k < 3; ++k) buf[3 * pi + k] = np[k];
+  }
+  for (i64 pi = 0; pi < m.P; ++pi)
+    if (pi % cyc == step)
+      for (int k = 0; k < 3; ++k) m.pos[3 * pi + k] = buf[3 * pi + k];
+  m.correct_step = (step + 1) % cyc;
+  m.correct_count++;
+  return true;
+}
+
 // LiquidModel::GetCFL, LiquidModel.h:49-55
 float get_cfl(const Model& m, float* maxv_out) {
   float maxVel = 0.0f;
@@ -1244,6 +1367,7 @@ int orc_velocity_update(void* h, float dt) {
 int orc_extrapolate(void* h) { extrapolate(*(Model*)h); return 0; }
 int orc_g2p(void* h) { g2p(*(Model*)h); return 0; }
 int orc_advect(void* h, float dt) { advect(*(Model*)h, dt); return 0; }
+int orc_correct_volume(void* h, float dt) { return correct_volume(*(Model*)h, dt) ? 0 : 7; }
 float orc_get_cfl(void* h, float* maxv) { return get_cfl(*(Model*)h, maxv); }
 void orc_divergence(void* h, double* mean, double* linf, int* count) { divergence(*(Model*)h, mean, linf, count); }
 float orc_solid_phi_at(void* h, float x, float y, float z) { return solid_phi_at(*(Model*)h, x, y, z); }
@@ -1264,6 +1388,7 @@ int orc_step(void* h, float dt, orc_solve_stats* st, double* div_mean, double* d
   extrapolate(m);
   g2p(m);
   advect(m, dt);
+  if (m.prm.correct_volume && !correct_volume(m, dt)) return 7;
   divergence(m, div_mean, div_linf, nullptr);
   m.time += (double)dt;
   return 0;

@@ -29,6 +29,7 @@ class Params(C.Structure):
         ("omega", C.c_float), ("dot_mode", C.c_int), ("smoother", C.c_int), ("p2g_order", C.c_int),
         ("tank_amp", C.c_float), ("tank_freq", C.c_float), ("tank_axis", C.c_int),
         ("nsdf", C.c_int), ("sdf", C.POINTER(C.c_float)),
+        ("correct_volume", C.c_int), ("correct_cycle", C.c_int), ("seed", C.c_ulonglong),
     ]
 
 
@@ -75,6 +76,7 @@ def lib():
         L.orc_solve_pressure.argtypes = [vp, C.c_float, C.POINTER(SolveStats)]
         L.orc_velocity_update.argtypes = [vp, C.c_float]
         L.orc_advect.argtypes = [vp, C.c_float]
+        L.orc_correct_volume.argtypes = [vp, C.c_float]
         L.orc_get_cfl.restype = C.c_float
         L.orc_get_cfl.argtypes = [vp, fp]
         L.orc_divergence.argtypes = [vp, dp, dp, ip]
@@ -171,6 +173,9 @@ def params_from_config(cfg, dot_mode=1, p2g_order=1, smoother=None):
         p.tank_axis = int(tank.get("axis", 0))
     else:
         p.tank_amp, p.tank_freq, p.tank_axis = 0.0, 0.0, -1
+    p.correct_volume = 1 if cfg.get("correctVolume", False) else 0
+    p.correct_cycle = int(lq["correctCycle"])
+    p.seed = int(cfg.get("seed", 0))
     return p
 
 
@@ -306,6 +311,11 @@ class Oracle:
 
     def advect(self, dt):
         self.L.orc_advect(self.h, dt)
+
+    def correct_volume(self, dt):
+        """LiquidModel<3>::CorrectVolume (LiquidModel.cpp:1085-1132) on the current particles"""
+        rc = self.L.orc_correct_volume(self.h, dt)
+        assert rc == 0, rc
 
     def get_cfl(self):
         mv = C.c_float()

@@ -38,6 +38,11 @@ def small_dam(N=24, smoother="reference", tank=False, fused=False):
     return cfg
 
 
+def cv_cycle(cfg, k):
+    cfg["LiquidParameter"]["correctCycle"] = k
+    return cfg
+
+
 def same(a, b, what):
     a, b = np.asarray(a), np.asarray(b)
     assert a.shape == b.shape, what
@@ -53,6 +58,11 @@ CASES = {
     "dam24_tank_sphere": lambda: small_dam(24, "symmetric", tank=True),
     "dam24_reference_fused_sweep": lambda: small_dam(24, "reference", fused=True),
     "dam40_symmetric_fused_sweep": lambda: small_dam(40, "symmetric", fused=True),
+    # CorrectVolume: at 24^3 (9072 particles, 18144 hash slots) most regions hold a cell whose
+    # slot collides with a nearby cell (exact visit count), at 40^3 (60112 particles) few do
+    # (overlap product); every second particle id moves per call at 40^3
+    "dam24_correct_volume": lambda: dict(small_dam(24, "symmetric"), correctVolume=True),
+    "dam40_correct_volume": lambda: cv_cycle(dict(small_dam(40, "symmetric", fused=True), correctVolume=True), 2),
 }
 
 
@@ -110,6 +120,10 @@ def test_stages_bit_exact(case):
         orc.advect(dt)
         m.AdvectParticle(dt)
         same(ps.get()[0], orc.particles()[0], "advected positions step %d" % step)
+        if cfg.get("correctVolume"):
+            orc.correct_volume(dt)
+            m.CorrectVolume(dt)
+            same(ps.get()[0], orc.particles()[0], "CorrectVolume positions step %d" % step)
         orc.set_time(orc.time + float(dt))
         dto = np.float32(orc.get_cfl()[0])
         assert np.float32(m.GetCFL()) == dto
@@ -117,7 +131,8 @@ def test_stages_bit_exact(case):
     scene.close()
 
 
-@pytest.mark.parametrize("case", ["dam24_reference_smoother", "dam24_tank_sphere", "dam40_symmetric_fused_sweep"])
+@pytest.mark.parametrize("case", ["dam24_reference_smoother", "dam24_tank_sphere", "dam40_symmetric_fused_sweep",
+                                  "dam24_correct_volume"])
 def test_free_running_steps_bit_exact(case):
     print("case", case, flush=True)
     cfg = CASES[case]()

@@ -1,6 +1,15 @@
 """The oracle against the reference's own recorded run (tests/golden/liquiddata_kat.npz,
 extracted from LiquidData.csv by tests/golden/make_liquiddata_kat.py).
 
+The recording was made with CorrectVolume on (LiquidModel.cpp:1085-1132), with the hash
+neighbour table holding the advected positions.  With "correctVolume": true the oracle
+reproduces it: every particle of frames 1 and 2 to the file's 6 significant digits (frame 2
+includes a pressure solve), and the centroid height of frames 1-18 to 2e-7.  That pins the
+CorrectVolume restatement, including the hash search's visit count (cells shared by the two
+3x3x3 regions plus colliding slots: NeighborSearch.h:22-57, 131-136).
+
+Without CorrectVolume (the current source's 3D step):
+
 * frame 0 is ParticleSystem<3>::SpawnParticles of Config/PureLiquid3D.json;
 * frame 1: every particle with index % 8 != 0 moved by exactly (0, -g dt^2, 0),
   dt = 0.01 (main.cpp:32) -- P2G from rest, gravity on all interior faces, zero
@@ -9,6 +18,7 @@ extracted from LiquidData.csv by tests/golden/make_liquiddata_kat.py).
 * frame 2: the centroid height, which CorrectVolume (a symmetric repulsion)
   leaves unchanged to first order.
 """
+import json
 import os
 
 import numpy as np
@@ -54,6 +64,33 @@ def test_frame1_free_fall_known_answer(kat, run3):
     moved_by_gravity_only = np.arange(5000) % 8 != 0
     np.testing.assert_allclose(frames[1][moved_by_gravity_only], kat["frame1"][moved_by_gravity_only],
                                rtol=0, atol=1e-6)
+
+
+@pytest.fixture(scope="module")
+def run_cv():
+    cfg = json.load(open(config_file("pure_liquid_3d.json")))
+    cfg["correctVolume"] = True
+    o = Oracle(cfg, dot_mode=0, p2g_order=0)
+    o.init()
+    frames = [o.particles()[0].copy()]
+    dt = np.float32(0.01)
+    for _ in range(18):
+        o.step(dt)
+        frames.append(o.particles()[0].copy())
+        dt = np.float32(o.get_cfl()[0])
+    return frames
+
+
+def test_correct_volume_reproduces_recorded_frames(kat, run_cv):
+    # 6 significant digits of coordinates in [0, 1): <= 5e-7 rounding of the recording
+    for k in (1, 2):
+        np.testing.assert_allclose(run_cv[k], kat["frame%d" % k], rtol=0, atol=6e-7)
+
+
+def test_correct_volume_recorded_centroid(kat, run_cv):
+    cy = kat["centroid_y"]
+    for k in range(1, 19):
+        assert abs(float(run_cv[k][:, 1].mean()) - float(cy[k])) < 2e-7, k
 
 
 def test_frame2_centroid(kat, run3):
