@@ -48,6 +48,66 @@ def algorithmic_bytes(kernel, n0, P, active0, fused):
     }[kernel]
 
 
+# instrumented kernel -> its symbol in the library (rocprofv3 kernel names)
+KERNEL_SYMBOL = {"smooth_l0": "k_smooth<true>", "spmv": "k_cg_spmv", "p2g": "k_p2g", "g2p": "k_g2p_advect",
+                 "liquid_phi": "k_liquid_phi"}
+COPY_BYTES = 256 * 1024 * 1024  # the calibration copy: bytes read and bytes written per launch
+
+
+def pmc_calibration_copy():
+    """three launches of a copy of COPY_BYTES (read once, written once) for scaling the
+    FETCH_SIZE / WRITE_SIZE counters; torch's elementwise multiply by 1 is the copy kernel"""
+    import torch
+
+    a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def pmc_reduce(fetch_csv, write_csv, workload):
+    """rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE counter_collection.csv files of one
+    `bench.py --pmc-calibrate` run each -> profiles/pmc_traffic.json[workload]: per kernel,
+    KiB per dispatch of each counter, the scale that maps the copy kernel's reading to its
+    known bytes, and the scaled HBM bytes per dispatch"""
+    import csv
+
+    def per_kernel(path, counter):
+        sums, counts, copy = {}, {}, []
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            name, v = r["Kernel_Name"], float(r["Counter_Value"])
+            if "MulFunctor" in name and int(r["Grid_Size"]) * 16 == COPY_BYTES:  # 4 floats per work-item
+                copy.append(v)
+            for k, sym in KERNEL_SYMBOL.items():
+                if "::" + sym + "(" in name:
+                    sums[k] = sums.get(k, 0.0) + v
+                    counts[k] = counts.get(k, 0) + 1
+        if not copy:
+            raise SystemExit("%s: no calibration copy dispatch (run bench.py with --pmc-calibrate)" % path)
+        scale = COPY_BYTES / 1024.0 / (sum(copy) / len(copy))
+        return {k: (sums[k] / counts[k], counts[k]) for k in sums}, scale
+
+    fetch, fs = per_kernel(fetch_csv, "FETCH_SIZE")
+    write, ws = per_kernel(write_csv, "WRITE_SIZE")
+    out = {}
+    for k in sorted(set(fetch) & set(write)):
+        f, n = fetch[k]
+        w, _ = write[k]
+        out[k] = {"kernel": KERNEL_SYMBOL[k], "dispatches": n, "fetch_kib": round(f, 1), "write_kib": round(w, 1),
+                  "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
+                  "bytes_per_dispatch": int(round((f * fs + w * ws) * 1024))}
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    allw = json.load(open(path)) if os.path.exists(path) else {}
+    allw[workload] = out
+    with open(path, "w") as fh:
+        json.dump(allw, fh, indent=1)
+    print(json.dumps({workload: out}, indent=1))
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per dispatch of `kernel` on `workload` from the committed rocprofv3 PMC runs
     (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a copy kernel of
@@ -112,7 +172,15 @@ def main():
     ap.add_argument("--ws-grid", type=int, default=128)
     ap.add_argument("--ws-scenes", type=int, default=2)
     ap.add_argument("--ws-steps", type=int, default=8)
+    ap.add_argument("--pmc-calibrate", action="store_true",
+                    help="launch a copy kernel of known size first (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--pmc-reduce", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="reduce two rocprofv3 --pmc runs of this workload into profiles/pmc_traffic.json and exit")
     args = ap.parse_args()
+    workload = os.path.basename(args.config).replace(".json", "")
+    if args.pmc_reduce:
+        pmc_reduce(args.pmc_reduce[0], args.pmc_reduce[1], workload)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
               file=sys.stderr)
@@ -120,6 +188,8 @@ def main():
 
     import liquidhairinteration_amd as lh
 
+    if args.pmc_calibrate:
+        pmc_calibration_copy()
     cfg = json.load(open(args.config))
     scene = lh.Scene(cfg)
     model = scene.GetLiquidModel()
@@ -146,7 +216,6 @@ def main():
     (_, _, d2), rows0 = model.GetSolver().LevelInfo(0)
     active0 = rows0 * d2
     fused = cfg["SolverParameter"].get("fusedSweepMinCells", 1 << 62) <= n0
-    workload = os.path.basename(args.config).replace(".json", "")
 
     def rate(k):
         n, ms = prof[k]

@@ -164,6 +164,7 @@ struct P2GP {
   float gdt;         // gravity * dt, added to interior vy faces
   int tank_axis;
   float adt;         // tank acceleration * dt on interior faces of tank_axis
+  float inv[3];      // 1 / dx per axis when every dx is a power of two (then x / dx == x * inv exactly)
 };
 
 // Tiles of 1 x 4 x 64 faces (or cells) per 256-thread workgroup: each
@@ -211,10 +212,15 @@ __device__ __forceinline__ bool tile_coords(int d0, int d1, int d2, int c[3], in
   return true;
 }
 
+// one particle record into a face's weight and momentum sums (LiquidModel.cpp:613-721):
+// grid coordinates (x - min) / dx, by an exact power-of-two multiply when kPow2
+template <bool kPow2>
 __device__ __forceinline__ void p2g_acc4(float4 A, float4 B, float mn0, float mn1, float mn2, float dx0, float dx1,
                                          float dx2, float ccx, float ccy, float ccz, float fx, float fy, float fz,
                                          float& ws, float& vs) {
-  const float gx = (A.x - mn0) / dx0, gy = (A.y - mn1) / dx1, gz = (A.z - mn2) / dx2;
+  const float gx = kPow2 ? (A.x - mn0) * dx0 : (A.x - mn0) / dx0;
+  const float gy = kPow2 ? (A.y - mn1) * dx1 : (A.y - mn1) / dx1;
+  const float gz = kPow2 ? (A.z - mn2) * dx2 : (A.z - mn2) / dx2;
   const float wgt = quad_w(gx - ccx) * quad_w(gy - ccy) * quad_w(gz - ccz);
   if (wgt == 0.f) return;
   const float d0v = A.x - fx, d1v = A.y - fy, d2v = A.z - fz;
@@ -222,6 +228,19 @@ __device__ __forceinline__ void p2g_acc4(float4 A, float4 B, float mn0, float mn
   ws += wgt;
   vs += wgt * (A.w + dot);
 }
+// record index of the t-th particle of a face's run of up to 6 bin columns: column j holds
+// the t in [cum[j-1], cum[j]), at records t + off[j]
+__device__ __forceinline__ int p2g_rec(int t, const int off[6], const int cum[6]) {
+  int o = off[5];
+#pragma unroll
+  for (int j = 4; j >= 0; --j) o = (t < cum[j]) ? off[j] : o;
+  return t + o;
+}
+// Gather form of Particle2Grid for one velocity component: one face per thread, summing
+// the particles of the half-cell bins around it in bin order (the order the oracle follows).
+// For each x column of bins the (up to 6) y columns are concatenated into one run, so a
+// wave's lanes wait for the longest run of 6 columns instead of the longest of each column.
+template <bool kPow2>
 __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const float4* __restrict__ pack, int64_t cap,
                                               const int* __restrict__ start, P2GP q) {
   const int a = q.a;
@@ -241,7 +260,8 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const flo
   if (any) {
     const float4* rec = pack + 2 * (size_t)a * cap;  // x y z v_a | C.col(a) 0
     const float mn0 = q.g.mn[0], mn1 = q.g.mn[1], mn2 = q.g.mn[2];
-    const float dx0 = q.g.dx[0], dx1 = q.g.dx[1], dx2 = q.g.dx[2];
+    const float dx0 = kPow2 ? q.inv[0] : q.g.dx[0], dx1 = kPow2 ? q.inv[1] : q.g.dx[1],
+                dx2 = kPow2 ? q.inv[2] : q.g.dx[2];
     int lo[3], hi[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -252,20 +272,36 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const flo
     const float ccx = (a == 0) ? (float)c[0] : (float)c[0] + 0.5f;
     const float ccy = (a == 1) ? (float)c[1] : (float)c[1] + 0.5f;
     const float ccz = (a == 2) ? (float)c[2] : (float)c[2] + 0.5f;
-    const float fx = ccx * dx0 + mn0, fy = ccy * dx1 + mn1, fz = ccz * dx2 + mn2;
-    for (int hx = lo[0]; hx <= hi[0]; ++hx)
-      for (int hy = lo[1]; hy <= hi[1]; ++hy) {
-        size_t k0 = ((size_t)hx * q.H[1] + hy) * q.H[2];
-        int s0 = start[k0 + lo[2]], s1 = start[k0 + hi[2] + 1];
-        int s = s0;
-        for (; s + 1 < s1; s += 2) {
-          const float4 A0 = rec[2 * (size_t)s], B0 = rec[2 * (size_t)s + 1];
-          const float4 A1 = rec[2 * (size_t)s + 2], B1 = rec[2 * (size_t)s + 3];
-          p2g_acc4(A0, B0, mn0, mn1, mn2, dx0, dx1, dx2, ccx, ccy, ccz, fx, fy, fz, ws, vs);
-          p2g_acc4(A1, B1, mn0, mn1, mn2, dx0, dx1, dx2, ccx, ccy, ccz, fx, fy, fz, ws, vs);
+    const float fx = ccx * q.g.dx[0] + mn0, fy = ccy * q.g.dx[1] + mn1, fz = ccz * q.g.dx[2] + mn2;
+    for (int hx = lo[0]; hx <= hi[0]; ++hx) {
+      int off[6], cum[6], total = 0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const int hy = lo[1] + j;
+        int s0 = 0, n = 0;
+        if (hy <= hi[1]) {
+          const size_t k0 = ((size_t)hx * q.H[1] + hy) * q.H[2];
+          s0 = start[k0 + lo[2]];
+          n = start[k0 + hi[2] + 1] - s0;
         }
-        if (s < s1) p2g_acc4(rec[2 * (size_t)s], rec[2 * (size_t)s + 1], mn0, mn1, mn2, dx0, dx1, dx2, ccx, ccy, ccz, fx, fy, fz, ws, vs);
+        off[j] = s0 - total;
+        total += n;
+        cum[j] = total;
       }
+      int t = 0;
+      for (; t + 1 < total; t += 2) {
+        const int i0 = p2g_rec(t, off, cum), i1 = p2g_rec(t + 1, off, cum);
+        const float4 A0 = rec[2 * (size_t)i0], B0 = rec[2 * (size_t)i0 + 1];
+        const float4 A1 = rec[2 * (size_t)i1], B1 = rec[2 * (size_t)i1 + 1];
+        p2g_acc4<kPow2>(A0, B0, mn0, mn1, mn2, dx0, dx1, dx2, ccx, ccy, ccz, fx, fy, fz, ws, vs);
+        p2g_acc4<kPow2>(A1, B1, mn0, mn1, mn2, dx0, dx1, dx2, ccx, ccy, ccz, fx, fy, fz, ws, vs);
+      }
+      if (t < total) {
+        const int i0 = p2g_rec(t, off, cum);
+        p2g_acc4<kPow2>(rec[2 * (size_t)i0], rec[2 * (size_t)i0 + 1], mn0, mn1, mn2, dx0, dx1, dx2, ccx, ccy, ccz, fx,
+                        fy, fz, ws, vs);
+      }
+    }
   }
   float v = !is_small(ws) ? vs / ws : 0.0f;
   // interior ranges of ApplyGravity (a == 1) and of the tank acceleration
@@ -736,6 +772,14 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
       q.tank_axis = c.cfg.tank_axis;
     }
   }
+  // dx = 2^-k on every axis: (x - min) / dx == (x - min) * 2^k exactly
+  bool pow2 = true;
+  for (int k = 0; k < 3; ++k) {
+    int e = 0;
+    const float m = std::frexp(c.g.dx[k], &e);
+    q.inv[k] = std::ldexp(1.0f, 1 - e);
+    pow2 = pow2 && m == 0.5f && std::isnormal(q.inv[k]) && c.g.dx[k] * q.inv[k] == 1.0f;
+  }
   for (int a = 0; a < 3; ++a) {
     q.a = a;
     P2GP qa = q;
@@ -743,7 +787,12 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
     ProfScope ps(c, LH_PROF_P2G);
     unsigned nt = tile_grid(div_up(c.g.N[0] + (a == 0), kTX) * div_up(c.g.N[1] + (a == 1), kTY) *
                             div_up(c.g.N[2] + (a == 2), kTZ));
-    hipLaunchKernelGGL(k_p2g, dim3(nt), dim3(kBlock), 0, c.s, c.v[a], (const float4*)c.ppack, c.cap, c.bin_start, qa);
+    if (pow2)
+      hipLaunchKernelGGL(k_p2g<true>, dim3(nt), dim3(kBlock), 0, c.s, c.v[a], (const float4*)c.ppack, c.cap,
+                         c.bin_start, qa);
+    else
+      hipLaunchKernelGGL(k_p2g<false>, dim3(nt), dim3(kBlock), 0, c.s, c.v[a], (const float4*)c.ppack, c.cap,
+                         c.bin_start, qa);
   }
 }
 

@@ -59,7 +59,54 @@ LvP lvp(const Level& L) {
   return p;
 }
 
+// (A z)[c], GetAx (PressureSolver.cpp:745-772): off-diagonal terms only towards neighbours of
+// class 2, added in the order +x, -x, +y, -y, +z, -z.  Every load is issued before any test
+// (out-of-range neighbours read cell c instead) and the terms are selected rather than
+// branched around, so a launch waits for one round trip to memory, not one per test; the
+// value is the same as adding only the present terms.
+struct AxAt {
+  float ax;   // (A z)[c]
+  float ad;   // adiag[c]
+  float zc;   // z[c]
+  float v;    // v[c] of the extra array handed in (r, b, ...)
+  int cls;    // cls[c]
+};
+__device__ __forceinline__ AxAt get_ax_full(const LvP& L, const float* __restrict__ z, const float* __restrict__ v,
+                                            int x, int y, int w, size_t c) {
+  const size_t sx = (size_t)L.d[1] * L.d[2], sy = (size_t)L.d[2];
+  const bool hxp = x + 1 < L.d[0], hxm = x >= 1, hyp = y + 1 < L.d[1], hym = y >= 1, hzp = w + 1 < L.d[2],
+             hzm = w >= 1;
+  const size_t cxp = hxp ? c + sx : c, cxm = hxm ? c - sx : c, cyp = hyp ? c + sy : c, cym = hym ? c - sy : c,
+               czp = hzp ? c + 1 : c, czm = hzm ? c - 1 : c;
+  float zc = z[c], ad = L.adiag[c], vc = v[c];
+  int k = L.cls[c], kxp = L.cls[cxp], kxm = L.cls[cxm], kyp = L.cls[cyp], kym = L.cls[cym], kzp = L.cls[czp],
+      kzm = L.cls[czm];
+  float zxp = z[cxp], zxm = z[cxm], zyp = z[cyp], zym = z[cym], zzp = z[czp], zzm = z[czm];
+  float txp = L.t0[c], txm = L.t0[cxm], typ = L.t1[c], tym = L.t1[cym], tzp = L.t2[c], tzm = L.t2[czm];
+  // an empty statement that takes every loaded value: the loads complete here, together,
+  // and the compiler cannot move any of them behind the selects or the caller's tests
+  asm volatile("" : "+v"(zc), "+v"(ad), "+v"(vc), "+v"(k), "+v"(kxp), "+v"(kxm), "+v"(kyp), "+v"(kym), "+v"(kzp),
+               "+v"(kzm));
+  asm volatile("" : "+v"(zxp), "+v"(zxm), "+v"(zyp), "+v"(zym), "+v"(zzp), "+v"(zzm), "+v"(txp), "+v"(txm),
+               "+v"(typ), "+v"(tym), "+v"(tzp), "+v"(tzm));
+  const float res0 = zc * ad;
+  float res = res0;
+  res = (hxp && kxp == 2) ? res + zxp * (-txp) : res;
+  res = (hxm && kxm == 2) ? res + zxm * (-txm) : res;
+  res = (hyp && kyp == 2) ? res + zyp * (-typ) : res;
+  res = (hym && kym == 2) ? res + zym * (-tym) : res;
+  res = (hzp && kzp == 2) ? res + zzp * (-tzp) : res;
+  res = (hzm && kzm == 2) ? res + zzm * (-tzm) : res;
+  return {k == 0 ? res0 : res, ad, zc, vc, k};
+}
 __device__ __forceinline__ float get_ax(const LvP& L, const float* __restrict__ z, int x, int y, int w, size_t c) {
+  return get_ax_full(L, z, z, x, y, w, c).ax;
+}
+// The same value, reading a neighbour's z and face term only when its class is 2 and
+// nothing beyond z, adiag and cls for a cell of class 0: fewer bytes, more round trips.
+// For the bandwidth-bound finest level, where most cells of the visited rows are air.
+__device__ __forceinline__ float get_ax_lazy(const LvP& L, const float* __restrict__ z, int x, int y, int w,
+                                             size_t c) {
   float res = z[c] * L.adiag[c];
   if (L.cls[c] == 0) return res;
   const size_t sx = (size_t)L.d[1] * L.d[2], sy = (size_t)L.d[2];
@@ -296,23 +343,31 @@ __global__ __launch_bounds__(kBlock) void k_smooth(float* __restrict__ z, const 
   const int sub = H >= kBlock ? 0 : (int)threadIdx.x / H;
   if (sub >= R) return;
   const int h0 = (int)threadIdx.x - sub * (H >= kBlock ? 0 : H), hs = H >= kBlock ? kBlock : H;
+  // the row id is fetched together with the row count (rows holds d0 * d1 entries)
+  const int kcap = L.d[0] * d1 - 1, kstep = gridDim.x * R;
+  int k = blockIdx.x * R + sub;
+  int row_next = rows[min(k, kcap)];
   const int nr = *nrows;
-  for (int k = blockIdx.x * R + sub; k < nr; k += gridDim.x * R) {
-    const int row = rows[k];
+  for (; k < nr; k += kstep) {
+    const int row = row_next;
+    row_next = rows[min(k + kstep, kcap)];
     const int xi = row / d1, yi = row - xi * d1;
     const int par = (xi + yi + phase) & 1;
     for (int h = h0; h < H; h += hs) {
       const int zi = 2 * h + par;
       if (zi >= d2) continue;
       const size_t i = (size_t)row * d2 + zi;
-      float ad = L.adiag[i];
+      // finest level: most cells of a visited row are air (adiag 0), skipped after one load
+      if (kLevel0 && is_small(L.adiag[i])) continue;
+      const AxAt g = get_ax_full(L, z, r, xi, yi, zi, i);
+      const float ad = g.ad, rv = g.v, zv = g.zc;
       if (is_small(ad)) continue;
-      float term = (r[i] - get_ax(L, z, xi, yi, zi, i)) / ad;
+      float term = (rv - g.ax) / ad;
       if (mode == 0) {
         float wt = w * term;
-        z[i] = (float)((1.0 - (double)w) * (double)z[i] + (double)wt);
+        z[i] = (float)((1.0 - (double)w) * (double)zv + (double)wt);
       } else {
-        z[i] = z[i] + w * term;
+        z[i] = zv + w * term;
       }
     }
   }
@@ -411,15 +466,21 @@ __global__ void __launch_bounds__(256) k_rbgs_sweep(float* __restrict__ zout, co
   }
 }
 
-// Restrict, PressureSolver.cpp:796-819
+// Restrict, PressureSolver.cpp:796-819 (kLazy: out of the finest level, get_ax_lazy)
+template <bool kLazy>
 __global__ void k_restrict(float* __restrict__ rc, float* __restrict__ zc, const float* __restrict__ rf,
                            const float* __restrict__ zf, LvP Lf, int c1, int c2, const int* __restrict__ rows,
-                           const int* __restrict__ nrows) {
+                           const int* __restrict__ nrows, int nrow_cap) {
   const RowIter it = row_iter(c2);
-  const int nr = *nrows;
   if (!it.on) return;
-  for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
-    const int row = rows[k];
+  // the row id is fetched together with the row count (rows holds nrow_cap entries)
+  const int kstep = gridDim.x * it.R;
+  int k = blockIdx.x * it.R + it.sub;
+  int row_next = rows[min(k, nrow_cap - 1)];
+  const int nr = *nrows;
+  for (; k < nr; k += kstep) {
+    const int row = row_next;
+    row_next = rows[min(k + kstep, nrow_cap - 1)];
     const int xi = row / c1, yi = row - xi * c1;
     for (int zi = it.z0; zi < c2; zi += it.zs) {
       float acc = 0.0f;
@@ -427,7 +488,13 @@ __global__ void k_restrict(float* __restrict__ rc, float* __restrict__ zc, const
       for (int q = 0; q < 8; ++q) {
         int x = 2 * xi + ((q >> 2) & 1), y = 2 * yi + ((q >> 1) & 1), w = 2 * zi + (q & 1);
         size_t f = ((size_t)x * Lf.d[1] + y) * Lf.d[2] + w;
-        float d = rf[f] - get_ax(Lf, zf, x, y, w, f);
+        float d;
+        if (kLazy) {
+          d = rf[f] - get_ax_lazy(Lf, zf, x, y, w, f);
+        } else {
+          const AxAt g = get_ax_full(Lf, zf, rf, x, y, w, f);
+          d = g.v - g.ax;
+        }
         acc = (float)((double)acc + 0.125 * (double)d);
       }
       const size_t i = (size_t)row * c2 + zi;
@@ -543,20 +610,25 @@ __global__ void k_cg_spmv(float* __restrict__ q, const float* __restrict__ p, Lv
   const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
   double v[1] = {0.0};
   const RowIter it = row_iter(d2);
+  // the row id is fetched together with the row count (rows holds d0 * d1 entries)
+  const int kcap = d0 * d1 - 1, kstep = gridDim.x * it.R;
+  int k = blockIdx.x * it.R + it.sub;
+  int row_next = rows[min(k, kcap)];
   const int nr = *nrows;
   if (it.on)
-    for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
-      const int row = rows[k];
+    for (; k < nr; k += kstep) {
+      const int row = row_next;
+      row_next = rows[min(k + kstep, kcap)];
       const int xi = row / d1, yi = row - xi * d1;
       if (xi < 1 || xi > d0 - 2 || yi < 1 || yi > d1 - 2) continue;
       const size_t base = (size_t)row * d2;
       for (int zi = it.z0; zi < d2; zi += it.zs) {
         if (zi < 1 || zi > d2 - 2) continue;
         const size_t i = base + zi;
-        if (L.cls[i] == 0) continue;
-        float ap = get_ax(L, p, xi, yi, zi, i);
-        q[i] = ap;
-        v[0] += (double)p[i] * (double)ap;
+        const AxAt g = get_ax_full(L, p, p, xi, yi, zi, i);
+        if (g.cls == 0) continue;
+        q[i] = g.ax;
+        v[0] += (double)g.zc * (double)g.ax;
       }
     }
   block_partials<1>(v, partial);
@@ -798,8 +870,12 @@ void apply_precond(Ctx& c) {
     for (int it = 0; it < c.cfg.num_pre; ++it) sweep(c, l - 1, 0, mode);
     Level& Lc = c.lv[l];
     Level& Lf = c.lv[l - 1];
-    hipLaunchKernelGGL(k_restrict, dim3(row_blocks(Lc)), dim3(kBlock), 0, c.s, Lc.r, Lc.z, Lf.r, Lf.z, lvp(Lf),
-                       Lc.d[1], Lc.d[2], Lc.qrows, Lc.nqrows);
+    if (l == 1)
+      hipLaunchKernelGGL(k_restrict<true>, dim3(row_blocks(Lc)), dim3(kBlock), 0, c.s, Lc.r, Lc.z, Lf.r, Lf.z,
+                         lvp(Lf), Lc.d[1], Lc.d[2], Lc.qrows, Lc.nqrows, Lc.d[0] * Lc.d[1]);
+    else
+      hipLaunchKernelGGL(k_restrict<false>, dim3(row_blocks(Lc)), dim3(kBlock), 0, c.s, Lc.r, Lc.z, Lf.r, Lf.z,
+                         lvp(Lf), Lc.d[1], Lc.d[2], Lc.qrows, Lc.nqrows, Lc.d[0] * Lc.d[1]);
   }
   for (int it = 0; it < c.cfg.num_final; ++it) sweep(c, L - 1, 0, mode);
   if (sym)
