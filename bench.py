@@ -3,15 +3,18 @@
 Workload (default): two-block collision on a 256^3 MAC grid (configs/collision_256.json):
 two 0.3^3 blocks at opposite corners of the unit cube, 8 jittered particles per
 cell, v = +-(2,0,0) + N(0, 0.05^2), solid walls 3 cells thick, MGPCG (3 levels,
-2/2 red-black Gauss-Seidel sweeps, symmetric V-cycle) to 1e-6 relative residual.
+2/2 red-black Gauss-Seidel sweeps, 12 + 12 on the coarsest, symmetric V-cycle) to 1e-6
+relative residual inside an FP64 residual loop to 1e-9.
 One step = PreStep + Step + PostStep + GetCFL (main.cpp:50-53) run by the library's
 native loop; particles and grid stay resident in HBM for the whole timed region.
 
-Prints one JSON line: steps/s, the live roofline of the dominant kernel (HIP
-events around its launches inside the timed region), the CPU baseline (the
-oracle, timed on this host for one step of the same workload) and, under
-"warm_start", ms and iterations per solve to 1e-6 relative residual of the learned
-initial guess versus plain MGPCG on identical systems of held-out 128^3 scenes.
+Prints one JSON line: steps/s over K timed steps with nothing else in the timed region; per
+kernel, the average launch time from HIP events around every launch over the next K steps (the
+events stay out of the timed region: each pair costs microseconds of queue time) and, for the
+dominant one by total time, its roofline; the CPU baseline (the oracle, timed on this host for
+one step of the same workload) and, under "warm_start", ms and iterations per solve to 1e-6
+relative residual of the learned initial guess versus plain MGPCG on identical systems of
+held-out 128^3 scenes.
 """
 import argparse
 import json
@@ -25,23 +28,43 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
 
 
-def algorithmic_bytes(kernel, n0, P, active0, fused):
+def level_dims(cfg):
+    dims = [list(cfg["gridDimension"])]
+    for _ in range(1, cfg["SolverParameter"]["numLevel"]):
+        dims.append([(d + 1) // 2 for d in dims[-1]])
+    return dims
+
+
+def row_sweep_levels(cfg):
+    """levels swept by one k_sweep_rows launch per sweep (both colours) instead of two passes"""
+    if cfg["SolverParameter"].get("coarseSweep", "rows") != "rows":
+        return set()
+    return {l for l, d in enumerate(level_dims(cfg)) if l >= 1 and d[2] <= 64}
+
+
+def algorithmic_bytes(kernel, n0, P, active, fused, rows_sweep=frozenset()):
     """Bytes per launch counted once per array element the kernel must touch.
 
     smooth_l0: one colour pass = half of a red-black sweep, whose minimum is 28 B/cell
                (Adiag + 3 face terms = 16 B, r 4 B, z read 4 B + write 4 B) -> 14 B per
                cell of the level-0 rows holding liquid (the pass visits only those); the
                fused one-launch sweep, when selected, is dense: 28 B/cell;
+    smooth_l1, smooth_l2: the same 14 B per cell of the rows holding liquid on level 1, and
+               on level 2 (the coarsest of the three-level hierarchy); 28 B when the level
+               is swept by one launch per sweep (both colours: k_sweep_rows);
     spmv:      p read, q write, 4 coefficient floats -> 24 B per cell of those rows;
     p2g:       one component: 20 B/particle (position, velocity component,
                affine column) + 4 B per face written (~n0 faces);
     g2p:       72 B/particle (read position, write position, velocity, C) + 3
                velocity components read around the particles (~12 B/cell);
     liquid_phi: 12 B/particle + 4 B/cell.
+    active[l] = cells of the level-l rows that hold liquid.
     """
     return {
-        "smooth_l0": 28.0 * n0 if fused else 14.0 * active0,
-        "spmv": 24.0 * active0,
+        "smooth_l0": 28.0 * n0 if fused else 14.0 * active[0],
+        "smooth_l1": (28.0 if 1 in rows_sweep else 14.0) * active.get(1, 0),
+        "smooth_l2": (28.0 if 2 in rows_sweep else 14.0) * active.get(2, 0),
+        "spmv": 24.0 * active[0],
         "p2g": 20.0 * P + 4.0 * n0,
         "g2p": 72.0 * P + 12.0 * n0,
         "liquid_phi": 12.0 * P + 4.0 * n0,
@@ -49,8 +72,10 @@ def algorithmic_bytes(kernel, n0, P, active0, fused):
 
 
 # instrumented kernel -> its symbol in the library (rocprofv3 kernel names)
-KERNEL_SYMBOL = {"smooth_l0": "k_smooth<true>", "spmv": "k_cg_spmv", "p2g": "k_p2g", "g2p": "k_g2p_advect",
-                 "liquid_phi": "k_liquid_phi"}
+# (on the 256^3 workloads: level 1 has 128-cell rows and runs two colour passes of k_smooth<false>
+# per sweep, level 2 one k_sweep_rows launch per sweep)
+KERNEL_SYMBOL = {"smooth_l0": "k_smooth<true>", "smooth_l1": "k_smooth<false>", "smooth_l2": "k_sweep_rows<1>",
+                 "spmv": "k_cg_spmv", "p2g": "k_p2g", "g2p": "k_g2p_advect", "liquid_phi": "k_liquid_phi"}
 COPY_BYTES = 256 * 1024 * 1024  # the calibration copy: bytes read and bytes written per launch
 
 
@@ -67,7 +92,7 @@ def pmc_calibration_copy():
     del a, b
 
 
-def pmc_reduce(fetch_csv, write_csv, workload):
+def pmc_reduce(fetch_csv, write_csv, workload, cfg):
     """rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE counter_collection.csv files of one
     `bench.py --pmc-calibrate` run each -> profiles/pmc_traffic.json[workload]: per kernel,
     KiB per dispatch of each counter, the scale that maps the copy kernel's reading to its
@@ -179,7 +204,7 @@ def main():
     args = ap.parse_args()
     workload = os.path.basename(args.config).replace(".json", "")
     if args.pmc_reduce:
-        pmc_reduce(args.pmc_reduce[0], args.pmc_reduce[1], workload)
+        pmc_reduce(args.pmc_reduce[0], args.pmc_reduce[1], workload, json.load(open(args.config)))
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
@@ -206,33 +231,52 @@ def main():
     dt = float(cfg.get("firstDt", 0.01))
     if args.warmup > 0:
         dt, _ = model.RunSteps(args.warmup, dt)
-    model.profile_enable(True)
+    # the timed region: K steps with nothing but the step in it
     t0 = time.perf_counter()
     dt, last = model.RunSteps(args.steps, dt)
     el = time.perf_counter() - t0
-    prof = {k: model.profile_read(k) for k in ("smooth_l0", "p2g", "g2p", "liquid_phi", "spmv")}
+    # kernel timings: the next K steps with HIP events around every launch of the listed kernels
+    # (each event pair costs several microseconds of queue time, so they stay out of the timed region)
+    kern = ("smooth_l0", "smooth_l1", "smooth_l2", "p2g", "g2p", "liquid_phi", "spmv")
+    model.profile_enable(True)
+    t1 = time.perf_counter()
+    dt, _ = model.RunSteps(args.steps, dt)
+    el_prof = time.perf_counter() - t1
+    prof = {k: model.profile_read(k) for k in kern}
     model.profile_enable(False)
-    # level-0 rows holding liquid in the last step's operator (the smoother and SpMV visit those)
-    (_, _, d2), rows0 = model.GetSolver().LevelInfo(0)
-    active0 = rows0 * d2
+    prof = {k: v for k, v in prof.items() if v[0]}
+    # cells of the rows holding liquid per level in the last step's operator (the smoother and
+    # SpMV visit those)
+    active, rows = {}, {}
+    for lvl in range(cfg["SolverParameter"]["numLevel"]):
+        (_, _, d2), nr = model.GetSolver().LevelInfo(lvl)
+        active[lvl], rows[lvl] = nr * d2, nr
     fused = cfg["SolverParameter"].get("fusedSweepMinCells", 1 << 62) <= n0
+    rsw = row_sweep_levels(cfg)
+
+    def abytes(k):
+        return algorithmic_bytes(k, n0, P, active, fused, rsw)
 
     def rate(k):
         n, ms = prof[k]
-        return algorithmic_bytes(k, n0, P, active0, fused) / (ms / 1e3 / n) / 1e9 if n else None
+        return abytes(k) / (ms / 1e3 / n) / 1e9 if n else None
 
-    # dominant instrumented kernel by total device time in the timed region
+    # dominant instrumented kernel by total device time
     dom = max(prof, key=lambda k: prof[k][1])
     launches, tot_ms = prof[dom]
     avg_s = tot_ms / 1e3 / max(launches, 1)
     ach = rate(dom)
-    roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, workload), "launches": launches,
+    roof = {"kernel": dom, "symbol": KERNEL_SYMBOL.get(dom), "bound": "hbm", "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(dom, workload), "launches": launches,
             "avg_launch_us": round(avg_s * 1e6, 2),
-            "algorithmic_bytes_per_launch": algorithmic_bytes(dom, n0, P, active0, fused)}
-    kernels = {k: {"launches": v[0], "ms": round(v[1], 3), "pct_of_timed": round(100.0 * v[1] / (el * 1e3), 1),
-                   "avg_us": round(1e3 * v[1] / v[0], 2) if v[0] else None,
-                   "achieved_gbs": round(rate(k), 1) if v[0] else None} for k, v in prof.items()}
+            "algorithmic_bytes_per_launch": abytes(dom)}
+    kernels = {k: {"launches": v[0], "ms": round(v[1], 3), "pct_of_pass": round(100.0 * v[1] / (el_prof * 1e3), 1),
+                   "avg_us": round(1e3 * v[1] / v[0], 2), "achieved_gbs": round(rate(k), 1)}
+               for k, v in prof.items()}
+    kernels["_timing"] = ("HIP events on the library's stream around every launch of these kernels, over the %d "
+                          "steps that follow the timed region (%.2f ms/step with the events)"
+                          % (args.steps, el_prof * 1e3 / args.steps))
 
     out = {
         "metric": "liquid steps/s at 256^3; ms per solve to 1e-6 residual, learned vs plain MGPCG",
@@ -248,10 +292,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded jittered particle blocks)",
         "config": {"workload": os.path.basename(args.config).replace(".json", ""),
-                   "grid": list(scene.GetGridDimension()), "particles": P, "level0_active_rows": rows0,
-                   "solver": "MGPCG L=%d pre/post=%d/%d %s, tol %g %s" % (
+                   "grid": list(scene.GetGridDimension()), "particles": P, "active_rows_per_level": rows,
+                   "solver": "MGPCG L=%d pre/post=%d/%d coarsest=%d %s, tol %g %s" % (
                        cfg["SolverParameter"]["numLevel"], cfg["SolverParameter"]["numPreSmooth"],
-                       cfg["SolverParameter"]["numPostSmooth"], cfg["SolverParameter"].get("smoother", "reference"),
+                       cfg["SolverParameter"]["numPostSmooth"], cfg["SolverParameter"]["numFinalSmooth"],
+                       cfg["SolverParameter"].get("smoother", "reference"),
                        cfg["SolverParameter"]["tolerance"], cfg["SolverParameter"].get("toleranceMode", "absolute")),
                    "precision": cfg.get("precision", "fp32") + (
                        " (FP32 MGPCG inside an FP64 residual loop to %g)" % cfg.get("refinedTolerance", 1e-9)

@@ -207,7 +207,9 @@ enum lh_prof_kernel {
   LH_PROF_NN_FWD = 5,      /* warm-start network forward (all layers) */
   LH_PROF_NN_BWD = 6,      /* warm-start network backward (all layers, gradients only) */
   LH_PROF_CONV_D0 = 7,     /* the network's largest convolution: 48 -> 16 channels at full resolution */
-  LH_PROF_COUNT = 8
+  LH_PROF_SMOOTH_L1 = 8,   /* red-black smoother on multigrid level 1: one colour pass */
+  LH_PROF_SMOOTH_L2 = 9,   /* red-black smoother on levels 2 and coarser: one colour pass */
+  LH_PROF_COUNT = 10
 };
 int lh_profile_enable(lh_ctx* ctx, int on);
 int lh_profile_read(lh_ctx* ctx, int kernel, int* launches, double* total_ms);

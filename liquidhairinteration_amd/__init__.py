@@ -488,7 +488,7 @@ class LiquidModel:
 
     # live kernel timing (HIP events on the library's stream)
     PROF_KERNELS = {"smooth_l0": 0, "p2g": 1, "g2p": 2, "liquid_phi": 3, "spmv": 4, "nn_fwd": 5, "nn_bwd": 6,
-                    "conv_d0": 7}
+                    "conv_d0": 7, "smooth_l1": 8, "smooth_l2": 9}
 
     def profile_enable(self, on=True):
         self._h.check(self._h.L.lh_profile_enable(self._h.h, 1 if on else 0))

@@ -411,6 +411,139 @@ __device__ __forceinline__ float get_ax7(const LvP& L, size_t c, int x, int y, i
   return res;
 }
 
+// The operands of one cell's update, loaded without any test (out-of-range neighbours read
+// the cell itself): z and class of the cell and its six neighbours, its diagonal, its three
+// +face terms and the three -face terms of its minus neighbours, and r.
+struct Sten {
+  float zc, ad, rv, zxp, zxm, zyp, zym, zzp, zzm, txp, txm, typ, tym, tzp, tzm;
+  int k, kxp, kxm, kyp, kym, kzp, kzm;
+  bool hxp, hxm, hyp, hym, hzp, hzm;
+};
+__device__ __forceinline__ Sten sten_load(const LvP& L, const float* __restrict__ z, const float* __restrict__ r,
+                                          int x, int y, int w, size_t c) {
+  Sten s;
+  const size_t sx = (size_t)L.d[1] * L.d[2], sy = (size_t)L.d[2];
+  s.hxp = x + 1 < L.d[0];
+  s.hxm = x >= 1;
+  s.hyp = y + 1 < L.d[1];
+  s.hym = y >= 1;
+  s.hzp = w + 1 < L.d[2];
+  s.hzm = w >= 1;
+  const size_t cxp = s.hxp ? c + sx : c, cxm = s.hxm ? c - sx : c, cyp = s.hyp ? c + sy : c,
+               cym = s.hym ? c - sy : c, czp = s.hzp ? c + 1 : c, czm = s.hzm ? c - 1 : c;
+  s.zc = z[c];
+  s.ad = L.adiag[c];
+  s.rv = r[c];
+  s.k = L.cls[c];
+  s.kxp = L.cls[cxp];
+  s.kxm = L.cls[cxm];
+  s.kyp = L.cls[cyp];
+  s.kym = L.cls[cym];
+  s.kzp = L.cls[czp];
+  s.kzm = L.cls[czm];
+  s.zxp = z[cxp];
+  s.zxm = z[cxm];
+  s.zyp = z[cyp];
+  s.zym = z[cym];
+  s.zzp = z[czp];
+  s.zzm = z[czm];
+  s.txp = L.t0[c];
+  s.txm = L.t0[cxm];
+  s.typ = L.t1[c];
+  s.tym = L.t1[cym];
+  s.tzp = L.t2[c];
+  s.tzm = L.t2[czm];
+  return s;
+}
+// the loaded values are complete here (see get_ax_full)
+__device__ __forceinline__ void sten_pin(Sten& s) {
+  asm volatile("" : "+v"(s.zc), "+v"(s.ad), "+v"(s.rv), "+v"(s.k), "+v"(s.kxp), "+v"(s.kxm), "+v"(s.kyp),
+               "+v"(s.kym), "+v"(s.kzp), "+v"(s.kzm));
+  asm volatile("" : "+v"(s.zxp), "+v"(s.zxm), "+v"(s.zyp), "+v"(s.zym), "+v"(s.zzp), "+v"(s.zzm), "+v"(s.txp),
+               "+v"(s.txm), "+v"(s.typ), "+v"(s.tym), "+v"(s.tzp), "+v"(s.tzm));
+}
+// GetAx of the cell with the given neighbour z values (order and conditions of get_ax)
+__device__ __forceinline__ float sten_ax(const Sten& s, float zc, float zxp, float zxm, float zyp, float zym,
+                                         float zzp, float zzm) {
+  const float res0 = zc * s.ad;
+  float res = res0;
+  res = (s.hxp && s.kxp == 2) ? res + zxp * (-s.txp) : res;
+  res = (s.hxm && s.kxm == 2) ? res + zxm * (-s.txm) : res;
+  res = (s.hyp && s.kyp == 2) ? res + zyp * (-s.typ) : res;
+  res = (s.hym && s.kym == 2) ? res + zym * (-s.tym) : res;
+  res = (s.hzp && s.kzp == 2) ? res + zzp * (-s.tzp) : res;
+  res = (s.hzm && s.kzm == 2) ? res + zzm * (-s.tzm) : res;
+  return s.k == 0 ? res0 : res;
+}
+template <int MODE>
+__device__ __forceinline__ float sten_relax(const Sten& s, float ax, float zc, float w) {
+  if (is_small(s.ad)) return zc;
+  const float term = (s.rv - ax) / s.ad;
+  if (MODE == 0) {
+    const float wt = w * term;
+    return (float)((1.0 - (double)w) * (double)zc + (double)wt);
+  }
+  return zc + w * term;
+}
+
+// One red-black sweep (Smooth(l, first) then Smooth(l, 1 - first)) on the listed rows of a
+// level whose rows hold at most 64 cells, in one launch: a wave owns a row, lane = z.  A cell
+// of the second colour needs the first-colour results of its six neighbours: the two in its
+// row come from the neighbouring lanes, the four in the rows x +- 1, y +- 1 are recomputed
+// by the lane itself from zin (they depend on zin only).  zin is read, zout written (every
+// cell of the row), so no wave sees another's writes; the values are those of two colour
+// passes, operation for operation.  The rows are the level's z-support (liquid rows and the
+// rows a coarser correction was prolonged into, whose cells are copied): every value read
+// later from z lies there; other rows are only ever multiplied by a zero diagonal.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_sweep_rows(float* __restrict__ zout, const float* __restrict__ zin,
+                                                       const float* __restrict__ r, LvP L,
+                                                       const int* __restrict__ rows, const int* __restrict__ nrows,
+                                                       int first, float w) {
+  const int d1 = L.d[1], d2 = L.d[2];
+  const int lane = (int)threadIdx.x & 63;
+  const int kcap = L.d[0] * d1 - 1, kstep = gridDim.x * (kBlock / 64);
+  int k = blockIdx.x * (kBlock / 64) + ((int)threadIdx.x >> 6);
+  int row_next = rows[min(k, kcap)];
+  const int nr = *nrows;
+  const size_t sx = (size_t)d1 * d2, sy = (size_t)d2;
+  for (; k < nr; k += kstep) {
+    const int row = row_next;
+    row_next = rows[min(k + kstep, kcap)];
+    const int x = row / d1, y = row - x * d1;
+    const int wz = min(lane, d2 - 1);
+    const size_t c = (size_t)row * d2 + wz;
+    const bool first_colour = ((x + y + wz) & 1) == first;
+    // the four x/y neighbours (second-colour lanes) or the cell itself (first-colour lanes)
+    const bool nxp = !first_colour && x + 1 < L.d[0], nxm = !first_colour && x >= 1;
+    const bool nyp = !first_colour && y + 1 < d1, nym = !first_colour && y >= 1;
+    Sten own = sten_load(L, zin, r, x, y, wz, c);
+    Sten s0 = sten_load(L, zin, r, nxp ? x + 1 : x, y, wz, nxp ? c + sx : c);
+    Sten s1 = sten_load(L, zin, r, nxm ? x - 1 : x, y, wz, nxm ? c - sx : c);
+    Sten s2 = sten_load(L, zin, r, x, nyp ? y + 1 : y, wz, nyp ? c + sy : c);
+    Sten s3 = sten_load(L, zin, r, x, nym ? y - 1 : y, wz, nym ? c - sy : c);
+    sten_pin(own);
+    sten_pin(s0);
+    sten_pin(s1);
+    sten_pin(s2);
+    sten_pin(s3);
+    // first colour: own update; second colour: the x/y neighbours' updates
+    const float v_own = sten_relax<MODE>(own, sten_ax(own, own.zc, own.zxp, own.zxm, own.zyp, own.zym, own.zzp,
+                                                      own.zzm), own.zc, w);
+    const float v0 = sten_relax<MODE>(s0, sten_ax(s0, s0.zc, s0.zxp, s0.zxm, s0.zyp, s0.zym, s0.zzp, s0.zzm), s0.zc, w);
+    const float v1 = sten_relax<MODE>(s1, sten_ax(s1, s1.zc, s1.zxp, s1.zxm, s1.zyp, s1.zym, s1.zzp, s1.zzm), s1.zc, w);
+    const float v2 = sten_relax<MODE>(s2, sten_ax(s2, s2.zc, s2.zxp, s2.zxm, s2.zyp, s2.zym, s2.zzp, s2.zzm), s2.zc, w);
+    const float v3 = sten_relax<MODE>(s3, sten_ax(s3, s3.zc, s3.zxp, s3.zxm, s3.zyp, s3.zym, s3.zzp, s3.zzm), s3.zc, w);
+    // the row after the first colour: first-colour lanes hold their update, the others zin
+    const float after1 = first_colour ? v_own : own.zc;
+    const float zl = __shfl(after1, lane - 1 < 0 ? 0 : lane - 1);
+    const float zr = __shfl(after1, lane + 1 > 63 ? 63 : lane + 1);
+    float out = after1;
+    if (!first_colour) out = sten_relax<MODE>(own, sten_ax(own, own.zc, v0, v1, v2, v3, zr, zl), own.zc, w);
+    if (lane < d2) zout[c] = out;
+  }
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(256) k_rbgs_sweep(float* __restrict__ zout, const float* __restrict__ zin,
                                                      const float* __restrict__ r, LvP L, int first, float w) {
@@ -827,28 +960,51 @@ unsigned row_blocks(const Level& L) {
 // Large levels use the fused tile kernel; small ones (launch-bound) two colour passes.
 void sweep(Ctx& c, int l, int first, int mode) {
   Level& L = c.lv[l];
+  if (l > 0 && L.d[2] <= 64 && c.cfg.coarse_sweep_rows) {
+    const unsigned nb = (unsigned)std::max<size_t>(
+        1, std::min<size_t>(div_up((size_t)L.d[0] * L.d[1], (size_t)(kBlock / 64)), 8192));
+    // over the z-support rows (liquid rows plus the rows the coarser correction reaches): the
+    // output buffer must hold every value a later prolongation reads
+    if (mode == 0)
+      hipLaunchKernelGGL(k_sweep_rows<0>, dim3(nb), dim3(kBlock), 0, c.s, L.z2, L.z, L.r, lvp(L), L.zrows, L.nzrows,
+                         first, c.cfg.omega);
+    else
+      hipLaunchKernelGGL(k_sweep_rows<1>, dim3(nb), dim3(kBlock), 0, c.s, L.z2, L.z, L.r, lvp(L), L.zrows, L.nzrows,
+                         first, c.cfg.omega);
+    std::swap(L.z, L.z2);
+    return;
+  }
   if ((long long)L.n < c.cfg.fused_sweep_min_cells) {
     const unsigned nb = smooth_blocks(L);
     for (int k = 0; k < 2; ++k) {
       const int phase = k == 0 ? first : 1 - first;
-      if (l == 0) {
-        ProfScope ps(c, LH_PROF_SMOOTH_L0);
+      if (l == 0)
         hipLaunchKernelGGL(k_smooth<true>, dim3(nb), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), L.rows, L.nrows, phase,
                            c.cfg.omega, mode);
-      } else {
+      else
         hipLaunchKernelGGL(k_smooth<false>, dim3(nb), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), L.rows, L.nrows, phase,
                            c.cfg.omega, mode);
-      }
     }
     return;
   }
-  ProfScope ps(c, l == 0 ? LH_PROF_SMOOTH_L0 : LH_PROF_COUNT);
   unsigned nb = (unsigned)(div_up(L.d[0], kSwX) * div_up(L.d[1], kSwY) * div_up(L.d[2], kSwZ));
   if (mode == 0)
     hipLaunchKernelGGL(k_rbgs_sweep<0>, dim3(nb), dim3(256), 0, c.s, L.z2, L.z, L.r, lvp(L), first, c.cfg.omega);
   else
     hipLaunchKernelGGL(k_rbgs_sweep<1>, dim3(nb), dim3(256), 0, c.s, L.z2, L.z, L.r, lvp(L), first, c.cfg.omega);
   std::swap(L.z, L.z2);
+}
+
+// `fwd` sweeps starting with colour `first`, then `rev` starting with the other colour, on
+// level l: consecutive launches, timed as one run when profiling
+void sweeps(Ctx& c, int l, int first, int fwd, int rev, int mode) {
+  if (fwd + rev <= 0) return;
+  const bool fused = (long long)c.lv[l].n >= c.cfg.fused_sweep_min_cells ||
+                     (l > 0 && c.lv[l].d[2] <= 64 && c.cfg.coarse_sweep_rows);
+  ProfScope ps(c, l == 0 ? LH_PROF_SMOOTH_L0 : (l == 1 ? LH_PROF_SMOOTH_L1 : LH_PROF_SMOOTH_L2),
+               (fwd + rev) * (fused ? 1 : 2));
+  for (int it = 0; it < fwd; ++it) sweep(c, l, first, mode);
+  for (int it = 0; it < rev; ++it) sweep(c, l, 1 - first, mode);
 }
 
 // ApplyPreconditioner, PressureSolver.cpp:719-772 (symmetric mode: z[0] = 0 first,
@@ -867,7 +1023,7 @@ void apply_precond(Ctx& c) {
   if (sym)
     hipLaunchKernelGGL(k_zero_rows, dim3(row_blocks(L0)), dim3(kBlock), 0, c.s, L0.z, L0.rows, L0.nrows, L0.d[2]);
   for (int l = 1; l < L; ++l) {
-    for (int it = 0; it < c.cfg.num_pre; ++it) sweep(c, l - 1, 0, mode);
+    sweeps(c, l - 1, 0, c.cfg.num_pre, 0, mode);
     Level& Lc = c.lv[l];
     Level& Lf = c.lv[l - 1];
     if (l == 1)
@@ -877,15 +1033,13 @@ void apply_precond(Ctx& c) {
       hipLaunchKernelGGL(k_restrict<false>, dim3(row_blocks(Lc)), dim3(kBlock), 0, c.s, Lc.r, Lc.z, Lf.r, Lf.z,
                          lvp(Lf), Lc.d[1], Lc.d[2], Lc.qrows, Lc.nqrows, Lc.d[0] * Lc.d[1]);
   }
-  for (int it = 0; it < c.cfg.num_final; ++it) sweep(c, L - 1, 0, mode);
-  if (sym)
-    for (int it = 0; it < c.cfg.num_final; ++it) sweep(c, L - 1, 1, mode);
+  sweeps(c, L - 1, 0, c.cfg.num_final, sym ? c.cfg.num_final : 0, mode);
   for (int l = L - 2; l >= 0; --l) {
     Level& Lc = c.lv[l + 1];
     Level& Lf = c.lv[l];
     hipLaunchKernelGGL(k_prolong, dim3(row_blocks(Lc)), dim3(kBlock), 0, c.s, Lf.z, Lc.z, Lf.d[1], Lf.d[2], Lc.d[1],
                        Lc.d[2], Lc.zrows, Lc.nzrows);
-    for (int it = 0; it < c.cfg.num_post; ++it) sweep(c, l, sym ? 1 : 0, mode);
+    sweeps(c, l, sym ? 1 : 0, c.cfg.num_post, 0, mode);
   }
 }
 

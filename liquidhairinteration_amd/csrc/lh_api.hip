@@ -85,6 +85,10 @@ Config parse_config(const std::string& text) {
   else if (sm == "symmetric") c.smoother = LH_SMOOTHER_SYMMETRIC;
   else throw std::runtime_error("Unknown smoother " + sm);
   c.fused_sweep_min_cells = (long long)s.get("fusedSweepMinCells", (double)(1LL << 62));
+  std::string cs = s.get("coarseSweep", "rows");
+  if (cs == "rows") c.coarse_sweep_rows = 1;
+  else if (cs == "passes") c.coarse_sweep_rows = 0;
+  else throw std::runtime_error("Unknown coarseSweep " + cs);
   // LiquidParameter (SceneParameter.h:55-63)
   const Json& lq = j.at("LiquidParameter");
   c.density = lq.at("density").as_float();
@@ -252,6 +256,7 @@ int alloc_ctx(Ctx& c) {
     launch_fill_u8(c, q.cls, q.n, 0);
     launch_fill_f32(c, q.r, q.n, 0.f);
     launch_fill_f32(c, q.z, q.n, 0.f);
+    launch_fill_f32(c, q.z2, q.n, 0.f);
   }
   if (!hip_ok(c, hipStreamSynchronize(c.s), "init")) return LH_ERR_DEVICE;
   return LH_OK;
@@ -879,7 +884,7 @@ int lh_profile_enable(lh_ctx* ctx, int on) {
   Ctx& c = ctx->c;
   hipStreamSynchronize(c.s);
   c.prof = (on != 0);
-  for (int k = 0; k < LH_PROF_COUNT; ++k) c.prof_used[k] = 0;
+  for (int k = 0; k < LH_PROF_COUNT; ++k) c.prof_used[k] = c.prof_launch[k] = 0;
   return LH_OK;
 }
 
@@ -892,7 +897,7 @@ int lh_profile_read(lh_ctx* ctx, int kernel, int* launches, double* total_ms) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c.prof_ev[kernel][2 * u], c.prof_ev[kernel][2 * u + 1]) == hipSuccess) tot += ms;
   }
-  if (launches) *launches = c.prof_used[kernel];
+  if (launches) *launches = c.prof_launch[kernel];
   if (total_ms) *total_ms = tot;
   return LH_OK;
 }

@@ -31,6 +31,9 @@ struct Config {
   int smoother = LH_SMOOTHER_REFERENCE;
   // levels with at least this many cells use the fused one-kernel red-black sweep
   long long fused_sweep_min_cells = 1LL << 62;  // off: two colour passes measured faster at 256^3
+  // coarse levels (>= 1) with rows of at most 64 cells: one launch per red-black sweep over the
+  // liquid rows (k_sweep_rows) instead of two colour passes; same values
+  int coarse_sweep_rows = 1;
   // mixed-precision refinement (FP32 inner MGPCG under an FP64 residual loop)
   int refined = 0;
   double refined_tol = 1e-9;
@@ -156,17 +159,20 @@ struct Ctx {
   // live kernel timing: event pairs per instrumented kernel
   bool prof = false;
   std::vector<hipEvent_t> prof_ev[LH_PROF_COUNT];
-  int prof_used[LH_PROF_COUNT] = {};
+  int prof_used[LH_PROF_COUNT] = {};    // event pairs recorded
+  int prof_launch[LH_PROF_COUNT] = {};  // kernel launches inside them
   std::string err;
 };
 
-// brackets one kernel launch with hipEventRecord pairs when c.prof is set
+// brackets n consecutive launches of one kernel (nothing else in between) with one
+// hipEventRecord pair when c.prof is set: an event pair costs microseconds of queue time,
+// so runs of short launches are timed as a whole and divided by their count
 struct ProfScope {
   Ctx& c;
-  int k;
+  int k, n;
   hipEvent_t e1 = nullptr;
-  ProfScope(Ctx& cc, int kk) : c(cc), k(kk) {
-    if (!c.prof || k < 0 || k >= LH_PROF_COUNT) return;
+  ProfScope(Ctx& cc, int kk, int nn = 1) : c(cc), k(kk), n(nn) {
+    if (!c.prof || k < 0 || k >= LH_PROF_COUNT || n <= 0) return;
     auto& v = c.prof_ev[k];
     int u = c.prof_used[k];
     while ((int)v.size() < 2 * (u + 1)) {
@@ -181,6 +187,7 @@ struct ProfScope {
     if (!e1) return;
     hipEventRecord(e1, c.s);
     c.prof_used[k]++;
+    c.prof_launch[k] += n;
   }
 };
 

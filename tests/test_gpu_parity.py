@@ -38,6 +38,11 @@ def small_dam(N=24, smoother="reference", tank=False, fused=False):
     return cfg
 
 
+def coarse_passes(cfg):
+    cfg["SolverParameter"]["coarseSweep"] = "passes"
+    return cfg
+
+
 def cv_cycle(cfg, k):
     cfg["LiquidParameter"]["correctCycle"] = k
     return cfg
@@ -55,6 +60,9 @@ CASES = {
     "reference_config_64": lambda: json.load(open(config_file("pure_liquid_3d.json"))),
     "dam24_reference_smoother": lambda: small_dam(24, "reference"),
     "dam24_symmetric_smoother": lambda: small_dam(24, "symmetric"),
+    # coarse levels as two colour passes per sweep (default: one row-sweep launch)
+    "dam24_symmetric_colour_passes": lambda: coarse_passes(small_dam(24, "symmetric")),
+    "dam24_reference_colour_passes": lambda: coarse_passes(small_dam(24, "reference")),
     "dam24_tank_sphere": lambda: small_dam(24, "symmetric", tank=True),
     "dam24_reference_fused_sweep": lambda: small_dam(24, "reference", fused=True),
     "dam40_symmetric_fused_sweep": lambda: small_dam(40, "symmetric", fused=True),
