@@ -37,9 +37,11 @@ def level_dims(cfg):
 
 def row_sweep_levels(cfg):
     """levels swept by one k_sweep_rows launch per sweep (both colours) instead of two passes"""
-    if cfg["SolverParameter"].get("coarseSweep", "rows") != "rows":
+    sp = cfg["SolverParameter"]
+    if sp.get("coarseSweep", "rows") != "rows":
         return set()
-    return {l for l, d in enumerate(level_dims(cfg)) if l >= 1 and d[2] <= 64}
+    lim = sp.get("rowSweepMaxCells", 1 << 18)
+    return {l for l, d in enumerate(level_dims(cfg)) if l >= 1 and d[0] * d[1] * d[2] <= lim}
 
 
 def algorithmic_bytes(kernel, n0, P, active, fused, rows_sweep=frozenset()):
@@ -49,9 +51,9 @@ def algorithmic_bytes(kernel, n0, P, active, fused, rows_sweep=frozenset()):
                (Adiag + 3 face terms = 16 B, r 4 B, z read 4 B + write 4 B) -> 14 B per
                cell of the level-0 rows holding liquid (the pass visits only those); the
                fused one-launch sweep, when selected, is dense: 28 B/cell;
-    smooth_l1, smooth_l2: the same 14 B per cell of the rows holding liquid on level 1, and
-               on level 2 (the coarsest of the three-level hierarchy); 28 B when the level
-               is swept by one launch per sweep (both colours: k_sweep_rows);
+    smooth_l1, smooth_l2: the same per cell of the rows holding liquid on level 1, and on
+               level 2 (the coarsest of the three-level hierarchy): 14 B per colour pass, 28 B
+               per k_sweep_rows launch (a whole sweep, on levels of <= rowSweepMaxCells);
     spmv:      p read, q write, 4 coefficient floats -> 24 B per cell of those rows;
     p2g:       one component: 20 B/particle (position, velocity component,
                affine column) + 4 B per face written (~n0 faces);
@@ -71,11 +73,34 @@ def algorithmic_bytes(kernel, n0, P, active, fused, rows_sweep=frozenset()):
     }[kernel]
 
 
-# instrumented kernel -> its symbol in the library (rocprofv3 kernel names)
-# (on the 256^3 workloads: level 1 has 128-cell rows and runs two colour passes of k_smooth<false>
-# per sweep, level 2 one k_sweep_rows launch per sweep)
-KERNEL_SYMBOL = {"smooth_l0": "k_smooth<true>", "smooth_l1": "k_smooth<false>", "smooth_l2": "k_sweep_rows<1>",
-                 "spmv": "k_cg_spmv", "p2g": "k_p2g", "g2p": "k_g2p_advect", "liquid_phi": "k_liquid_phi"}
+# instrumented kernel -> its symbol in the library (rocprofv3 kernel names); the coarse levels'
+# symbols depend on the configuration (level_symbol)
+KERNEL_SYMBOL = {"smooth_l0": "k_smooth<true>", "spmv": "k_cg_spmv", "p2g": "k_p2g", "g2p": "k_g2p_advect",
+                 "liquid_phi": "k_liquid_phi"}
+
+
+def smooth_grid(d):
+    """work-items of one k_smooth launch on a level of dims d (the library's smooth_blocks)"""
+    H = (d[2] + 1) // 2
+    R = 1 if H >= 256 else 256 // H
+    return max(1, min(-(-d[0] * d[1] // R), 8192)) * 256
+
+
+def row_sweep_grid(d):
+    """work-items of one k_sweep_rows launch on a level of dims d (a wave per 64-cell row segment)"""
+    nseg = -(-d[2] // 64)
+    return max(1, min(-(-d[0] * d[1] * nseg // 4), 8192)) * 256
+
+
+def level_symbol(cfg, l):
+    """(kernel symbol, grid size) of the smoother launches on coarse level l"""
+    d = level_dims(cfg)[l]
+    mode = 1 if cfg["SolverParameter"].get("smoother", "reference") == "symmetric" else 0
+    if l in row_sweep_levels(cfg):
+        return "k_sweep_rows<%d>" % mode, row_sweep_grid(d)
+    return "k_smooth<false>", smooth_grid(d)
+
+
 COPY_BYTES = 256 * 1024 * 1024  # the calibration copy: bytes read and bytes written per launch
 
 
@@ -99,6 +124,12 @@ def pmc_reduce(fetch_csv, write_csv, workload, cfg):
     known bytes, and the scaled HBM bytes per dispatch"""
     import csv
 
+    symbols = dict(KERNEL_SYMBOL)
+    level_grid = {}
+    for l in (1, 2):
+        if l < cfg["SolverParameter"]["numLevel"]:
+            symbols["smooth_l%d" % l], level_grid["smooth_l%d" % l] = level_symbol(cfg, l)
+
     def per_kernel(path, counter):
         sums, counts, copy = {}, {}, []
         for r in csv.DictReader(open(path)):
@@ -107,8 +138,11 @@ def pmc_reduce(fetch_csv, write_csv, workload, cfg):
             name, v = r["Kernel_Name"], float(r["Counter_Value"])
             if "MulFunctor" in name and int(r["Grid_Size"]) * 16 == COPY_BYTES:  # 4 floats per work-item
                 copy.append(v)
-            for k, sym in KERNEL_SYMBOL.items():
-                if "::" + sym + "(" in name:
+            for k, sym in symbols.items():
+                if k in level_grid and int(r["Grid_Size"]) != level_grid[k]:
+                    continue
+                # the symbol, alone or with template arguments (k_p2g<true>)
+                if "::" + sym + "(" in name or ("<" not in sym and "::" + sym + "<" in name):
                     sums[k] = sums.get(k, 0.0) + v
                     counts[k] = counts.get(k, 0) + 1
         if not copy:
@@ -122,7 +156,7 @@ def pmc_reduce(fetch_csv, write_csv, workload, cfg):
     for k in sorted(set(fetch) & set(write)):
         f, n = fetch[k]
         w, _ = write[k]
-        out[k] = {"kernel": KERNEL_SYMBOL[k], "dispatches": n, "fetch_kib": round(f, 1), "write_kib": round(w, 1),
+        out[k] = {"kernel": symbols[k], "dispatches": n, "fetch_kib": round(f, 1), "write_kib": round(w, 1),
                   "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
                   "bytes_per_dispatch": int(round((f * fs + w * ws) * 1024))}
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -266,7 +300,8 @@ def main():
     launches, tot_ms = prof[dom]
     avg_s = tot_ms / 1e3 / max(launches, 1)
     ach = rate(dom)
-    roof = {"kernel": dom, "symbol": KERNEL_SYMBOL.get(dom), "bound": "hbm", "achieved": round(ach, 1),
+    symbol = level_symbol(cfg, int(dom[-1]))[0] if dom in ("smooth_l1", "smooth_l2") else KERNEL_SYMBOL[dom]
+    roof = {"kernel": dom, "symbol": symbol, "bound": "hbm", "achieved": round(ach, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(dom, workload), "launches": launches,
             "avg_launch_us": round(avg_s * 1e6, 2),

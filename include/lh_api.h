@@ -125,8 +125,11 @@ int lh_g2p(lh_ctx* ctx);                             /* LiquidModel.cpp:723-823 
 int lh_advect(lh_ctx* ctx, float dt);                /* LiquidModel.cpp:912-935 */
 int lh_divergence(lh_ctx* ctx, double* mean, double* linf, int* count); /* LiquidModel.cpp:1065-1083 */
 /* LiquidModel.cpp:1085-1132: particles with id % correctCycle == step pushed apart by their
- * neighbours (the reference's hash-search visit multiplicity kept; coincident particles get a
- * seeded counter-RNG kick); step advances.  Runs inside lh_step when "correctVolume": true. */
+ * neighbours closer than 2 dx, each counted as often as the reference's hash search
+ * (NeighborSearch.h:22-57, 131-136; 2 NumParticles slots) visits it: cells the two 3x3x3 regions
+ * share plus slot collisions; coincident particles get a seeded counter-RNG kick; positions are
+ * written after all are computed; step advances.  Runs inside lh_step after lh_advect when
+ * "correctVolume": true. */
 int lh_correct_volume(lh_ctx* ctx, float dt);
 
 /* ---- solver-level access (CGSolver<3>::BuildRHS / BuildLHS / Solve split,

@@ -487,60 +487,74 @@ __device__ __forceinline__ float sten_relax(const Sten& s, float ax, float zc, f
 }
 
 // One red-black sweep (Smooth(l, first) then Smooth(l, 1 - first)) on the listed rows of a
-// level whose rows hold at most 64 cells, in one launch: a wave owns a row, lane = z.  A cell
-// of the second colour needs the first-colour results of its six neighbours: the two in its
-// row come from the neighbouring lanes, the four in the rows x +- 1, y +- 1 are recomputed
-// by the lane itself from zin (they depend on zin only).  zin is read, zout written (every
-// cell of the row), so no wave sees another's writes; the values are those of two colour
-// passes, operation for operation.  The rows are the level's z-support (liquid rows and the
-// rows a coarser correction was prolonged into, whose cells are copied): every value read
-// later from z lies there; other rows are only ever multiplied by a zero diagonal.
+// coarse level in one launch.  A wave owns a 64-cell segment of a row, lane = z.  A cell of
+// the second colour needs the first-colour results of its six neighbours: the two in its row
+// come from the neighbouring lanes (or, at a segment end, are recomputed by the lane from zin),
+// the four in the rows x +- 1, y +- 1 are recomputed by the lane itself from zin (a first-colour
+// update depends on zin only).  zin is read, zout written (every cell of the segment), so no
+// wave sees another's writes; the values are those of two colour passes, operation for
+// operation.  The rows are the level's z-support (liquid rows and the rows a coarser
+// correction was prolonged into, whose cells are copied): every value read later from z lies
+// there; other rows are only ever multiplied by a zero diagonal.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_sweep_rows(float* __restrict__ zout, const float* __restrict__ zin,
                                                        const float* __restrict__ r, LvP L,
                                                        const int* __restrict__ rows, const int* __restrict__ nrows,
                                                        int first, float w) {
   const int d1 = L.d[1], d2 = L.d[2];
+  const int nseg = (d2 + 63) >> 6;
   const int lane = (int)threadIdx.x & 63;
   const int kcap = L.d[0] * d1 - 1, kstep = gridDim.x * (kBlock / 64);
-  int k = blockIdx.x * (kBlock / 64) + ((int)threadIdx.x >> 6);
-  int row_next = rows[min(k, kcap)];
+  int k = blockIdx.x * (kBlock / 64) + ((int)threadIdx.x >> 6);  // (list index, segment) = (k / nseg, k % nseg)
+  int row_next = rows[min(k / nseg, kcap)];
   const int nr = *nrows;
   const size_t sx = (size_t)d1 * d2, sy = (size_t)d2;
-  for (; k < nr; k += kstep) {
+  for (; k < nr * nseg; k += kstep) {
     const int row = row_next;
-    row_next = rows[min(k + kstep, kcap)];
+    row_next = rows[min((k + kstep) / nseg, kcap)];
+    const int seg = k - (k / nseg) * nseg;
     const int x = row / d1, y = row - x * d1;
-    const int wz = min(lane, d2 - 1);
+    const int zr0 = seg * 64 + lane;
+    const int wz = min(zr0, d2 - 1);
     const size_t c = (size_t)row * d2 + wz;
     const bool first_colour = ((x + y + wz) & 1) == first;
     // the four x/y neighbours (second-colour lanes) or the cell itself (first-colour lanes)
     const bool nxp = !first_colour && x + 1 < L.d[0], nxm = !first_colour && x >= 1;
     const bool nyp = !first_colour && y + 1 < d1, nym = !first_colour && y >= 1;
+    // the z neighbour in the next / previous segment (second-colour lanes at a segment end)
+    const bool nzp = !first_colour && lane == 63 && wz + 1 < d2;
+    const bool nzm = !first_colour && lane == 0 && wz >= 1;
     Sten own = sten_load(L, zin, r, x, y, wz, c);
     Sten s0 = sten_load(L, zin, r, nxp ? x + 1 : x, y, wz, nxp ? c + sx : c);
     Sten s1 = sten_load(L, zin, r, nxm ? x - 1 : x, y, wz, nxm ? c - sx : c);
     Sten s2 = sten_load(L, zin, r, x, nyp ? y + 1 : y, wz, nyp ? c + sy : c);
     Sten s3 = sten_load(L, zin, r, x, nym ? y - 1 : y, wz, nym ? c - sy : c);
+    Sten sb = sten_load(L, zin, r, x, y, nzp ? wz + 1 : (nzm ? wz - 1 : wz), nzp ? c + 1 : (nzm ? c - 1 : c));
     sten_pin(own);
     sten_pin(s0);
     sten_pin(s1);
     sten_pin(s2);
     sten_pin(s3);
-    // first colour: own update; second colour: the x/y neighbours' updates
+    sten_pin(sb);
+    // first colour: own update; second colour: the neighbours' updates
     const float v_own = sten_relax<MODE>(own, sten_ax(own, own.zc, own.zxp, own.zxm, own.zyp, own.zym, own.zzp,
                                                       own.zzm), own.zc, w);
     const float v0 = sten_relax<MODE>(s0, sten_ax(s0, s0.zc, s0.zxp, s0.zxm, s0.zyp, s0.zym, s0.zzp, s0.zzm), s0.zc, w);
     const float v1 = sten_relax<MODE>(s1, sten_ax(s1, s1.zc, s1.zxp, s1.zxm, s1.zyp, s1.zym, s1.zzp, s1.zzm), s1.zc, w);
     const float v2 = sten_relax<MODE>(s2, sten_ax(s2, s2.zc, s2.zxp, s2.zxm, s2.zyp, s2.zym, s2.zzp, s2.zzm), s2.zc, w);
     const float v3 = sten_relax<MODE>(s3, sten_ax(s3, s3.zc, s3.zxp, s3.zxm, s3.zyp, s3.zym, s3.zzp, s3.zzm), s3.zc, w);
-    // the row after the first colour: first-colour lanes hold their update, the others zin
+    const float vb = sten_relax<MODE>(sb, sten_ax(sb, sb.zc, sb.zxp, sb.zxm, sb.zyp, sb.zym, sb.zzp, sb.zzm), sb.zc, w);
+    // the segment after the first colour: first-colour lanes hold their update, the others zin
     const float after1 = first_colour ? v_own : own.zc;
-    const float zl = __shfl(after1, lane - 1 < 0 ? 0 : lane - 1);
-    const float zr = __shfl(after1, lane + 1 > 63 ? 63 : lane + 1);
+    // the shuffles run with every lane of the wave active (a lane outside a shuffle supplies
+    // no value to the others); segment-end lanes then take their recomputed neighbour
+    const float sl = __shfl(after1, lane - 1 < 0 ? 0 : lane - 1);
+    const float sr = __shfl(after1, lane + 1 > 63 ? 63 : lane + 1);
+    const float zl = lane == 0 ? vb : sl;
+    const float zr = lane == 63 ? vb : sr;
     float out = after1;
     if (!first_colour) out = sten_relax<MODE>(own, sten_ax(own, own.zc, v0, v1, v2, v3, zr, zl), own.zc, w);
-    if (lane < d2) zout[c] = out;
+    if (zr0 < d2) zout[c] = out;
   }
 }
 
@@ -960,9 +974,10 @@ unsigned row_blocks(const Level& L) {
 // Large levels use the fused tile kernel; small ones (launch-bound) two colour passes.
 void sweep(Ctx& c, int l, int first, int mode) {
   Level& L = c.lv[l];
-  if (l > 0 && L.d[2] <= 64 && c.cfg.coarse_sweep_rows) {
+  if (l > 0 && c.cfg.coarse_sweep_rows && (long long)L.n <= c.cfg.row_sweep_max_cells) {
+    const size_t nseg = div_up((size_t)L.d[2], (size_t)64);
     const unsigned nb = (unsigned)std::max<size_t>(
-        1, std::min<size_t>(div_up((size_t)L.d[0] * L.d[1], (size_t)(kBlock / 64)), 8192));
+        1, std::min<size_t>(div_up((size_t)L.d[0] * L.d[1] * nseg, (size_t)(kBlock / 64)), 8192));
     // over the z-support rows (liquid rows plus the rows the coarser correction reaches): the
     // output buffer must hold every value a later prolongation reads
     if (mode == 0)
@@ -1000,7 +1015,7 @@ void sweep(Ctx& c, int l, int first, int mode) {
 void sweeps(Ctx& c, int l, int first, int fwd, int rev, int mode) {
   if (fwd + rev <= 0) return;
   const bool fused = (long long)c.lv[l].n >= c.cfg.fused_sweep_min_cells ||
-                     (l > 0 && c.lv[l].d[2] <= 64 && c.cfg.coarse_sweep_rows);
+                     (l > 0 && c.cfg.coarse_sweep_rows && (long long)c.lv[l].n <= c.cfg.row_sweep_max_cells);
   ProfScope ps(c, l == 0 ? LH_PROF_SMOOTH_L0 : (l == 1 ? LH_PROF_SMOOTH_L1 : LH_PROF_SMOOTH_L2),
                (fwd + rev) * (fused ? 1 : 2));
   for (int it = 0; it < fwd; ++it) sweep(c, l, first, mode);

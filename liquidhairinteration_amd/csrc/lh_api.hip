@@ -89,6 +89,7 @@ Config parse_config(const std::string& text) {
   if (cs == "rows") c.coarse_sweep_rows = 1;
   else if (cs == "passes") c.coarse_sweep_rows = 0;
   else throw std::runtime_error("Unknown coarseSweep " + cs);
+  c.row_sweep_max_cells = (long long)s.get("rowSweepMaxCells", (double)(1LL << 18));
   // LiquidParameter (SceneParameter.h:55-63)
   const Json& lq = j.at("LiquidParameter");
   c.density = lq.at("density").as_float();

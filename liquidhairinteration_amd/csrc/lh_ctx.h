@@ -31,9 +31,13 @@ struct Config {
   int smoother = LH_SMOOTHER_REFERENCE;
   // levels with at least this many cells use the fused one-kernel red-black sweep
   long long fused_sweep_min_cells = 1LL << 62;  // off: two colour passes measured faster at 256^3
-  // coarse levels (>= 1) with rows of at most 64 cells: one launch per red-black sweep over the
-  // liquid rows (k_sweep_rows) instead of two colour passes; same values
+  // coarse levels (>= 1) of at most row_sweep_max_cells cells: one launch per red-black sweep
+  // over the z-support rows (k_sweep_rows) instead of two colour passes; same values.  It pays
+  // where the passes are launch-bound (64^3: 5.8 us per sweep against 2 x 3.9 us) and loses
+  // where they are not (128^3 at 256^3 scale: 30 us against 2 x 5.6 us: four neighbour
+  // updates recomputed per cell)
   int coarse_sweep_rows = 1;
+  long long row_sweep_max_cells = 1LL << 18;
   // mixed-precision refinement (FP32 inner MGPCG under an FP64 residual loop)
   int refined = 0;
   double refined_tol = 1e-9;

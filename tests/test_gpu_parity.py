@@ -38,6 +38,20 @@ def small_dam(N=24, smoother="reference", tank=False, fused=False):
     return cfg
 
 
+def tall_dam():
+    """16 x 24 x 160 cells of 1/160 over 0.1 x 0.15 x 1: level 1 has 80-cell z-rows, swept in two
+    64-cell segments (the row sweep's segment-end recomputation)"""
+    cfg = small_dam(24, "symmetric")
+    h = 1.0 / 160
+    cfg["gridDimension"] = [16, 24, 160]
+    cfg["maxCoord"] = [0.1, 0.15, 1.0]
+    cfg["Boundary"]["child"]["center"] = [0.05, 0.075, 0.5]
+    cfg["Boundary"]["child"]["halfExtent"] = [0.05 - 3 * h, 0.075 - 3 * h, 0.5 - 3 * h]
+    cfg["FluidBlocks"] = [{"min": [0.0, 0.0, 0.0], "max": [0.1, 0.08, 0.55], "velocity": [0, 0, 1.0],
+                           "velocityNoise": 0.0}]
+    return cfg
+
+
 def coarse_passes(cfg):
     cfg["SolverParameter"]["coarseSweep"] = "passes"
     return cfg
@@ -62,6 +76,7 @@ CASES = {
     "dam24_symmetric_smoother": lambda: small_dam(24, "symmetric"),
     # coarse levels as two colour passes per sweep (default: one row-sweep launch)
     "dam24_symmetric_colour_passes": lambda: coarse_passes(small_dam(24, "symmetric")),
+    "tall_16x24x160_row_segments": tall_dam,
     "dam24_reference_colour_passes": lambda: coarse_passes(small_dam(24, "reference")),
     "dam24_tank_sphere": lambda: small_dam(24, "symmetric", tank=True),
     "dam24_reference_fused_sweep": lambda: small_dam(24, "reference", fused=True),
@@ -140,7 +155,7 @@ def test_stages_bit_exact(case):
 
 
 @pytest.mark.parametrize("case", ["dam24_reference_smoother", "dam24_tank_sphere", "dam40_symmetric_fused_sweep",
-                                  "dam24_correct_volume"])
+                                  "dam24_correct_volume", "tall_16x24x160_row_segments"])
 def test_free_running_steps_bit_exact(case):
     print("case", case, flush=True)
     cfg = CASES[case]()
