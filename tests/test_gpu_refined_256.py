@@ -1,0 +1,83 @@
+"""Mixed-precision refinement at full size: the 256^3 two-block collision (6.5 M particles), FP32
+MGPCG inside the FP64 residual loop to 1e-9 relative, against the oracle's FP64 PCG to 1e-13 on
+the same systems.
+
+Snapshots are taken along one GPU run (every SNAP_EVERY steps): the GPU and the oracle build the
+system from the same particles (the right-hand sides must be bit-identical), then the GPU's
+refined solution is compared with the FP64 oracle's on the unknowns the system determines.
+LH_SNAPSHOTS sets the number of snapshots (default 2; the measurement uses 20) and LH_REPORT a
+JSON file for the per-snapshot figures.
+"""
+import json
+import os
+import time
+
+import numpy as np
+import pytest
+
+from oracle import Oracle
+
+lh = pytest.importorskip("liquidhairinteration_amd")
+pytestmark = pytest.mark.gpu
+
+SNAP_EVERY = 3
+
+
+def determined(orc):
+    """interior liquid cells with a nonzero diagonal (cells with every face closed have a zero
+    row; their x is not determined by the system)"""
+    ad, _ = orc.level(0)
+    phi = orc.get(10)
+    m = np.zeros(phi.shape, bool)
+    m[1:-1, 1:-1, 1:-1] = True
+    return m & (phi <= 0) & (ad != 0)
+
+
+def test_refined_256_against_fp64_oracle():
+    nsnap = int(os.environ.get("LH_SNAPSHOTS", "2"))
+    cfg = json.load(open(lh.config_path("collision_256")))
+    assert cfg["precision"] == "refined" and cfg["refinedTolerance"] == 1e-9
+    print("collision 256^3, %d snapshots" % nsnap, flush=True)
+    orc = Oracle(cfg, dot_mode=1, p2g_order=1)
+    orc.init()
+    scene = lh.Scene(cfg)
+    m = scene.GetLiquidModel()
+    m.Init()
+    dt = float(cfg["firstDt"])
+    rows = []
+    for k in range(nsnap):
+        if k:
+            dt, _ = m.RunSteps(SNAP_EVERY, dt)
+        t0 = time.perf_counter()
+        pos, vel, Cm = m.GetParticleSystem().get()
+        m.PreStep(dt)
+        m.Particle2Grid()
+        m.ApplyGravity(dt)
+        m.GetSolver().BuildSystem(dt)
+        orc.set_particles(pos, vel, Cm)
+        orc.set_time(m.time)
+        orc.clear_grid()
+        orc.compute_liquid_phi()
+        orc.p2g()
+        orc.apply_gravity(np.float32(dt))
+        orc.build_system(np.float32(dt))
+        b = orc.get(13)
+        assert np.array_equal(m.get_field("rhs"), b), "snapshot %d: right-hand sides differ" % k
+        x64, st64 = orc.solve_fp64(b.astype(np.float64), rel_tol=1e-13, max_iter=1000)
+        st = m.GetSolver().SolveRHS()
+        xg = m.GetSolver().GetSolution64(b.shape)
+        d = determined(orc)
+        err = float(np.linalg.norm(xg[d] - x64[d]) / np.linalg.norm(x64[d]))
+        row = {"snapshot": k, "sim_time": m.time, "dt": dt, "liquid_cells": int(d.sum()),
+               "outer_iterations": st["outer_iterations"], "pcg_iterations": st["iterations"],
+               "true_residual": st["true_residual"], "solve_ms": st["solve_ms"],
+               "oracle_iterations": st64.iterations, "rel_error_vs_fp64": err,
+               "host_s": round(time.perf_counter() - t0, 1)}
+        print(json.dumps(row), flush=True)
+        rows.append(row)
+        assert st["converged"] == 1 and st["true_residual"] <= 1e-9 * (1 + 1e-6)
+        assert err < 1e-7
+    scene.close()
+    if os.environ.get("LH_REPORT"):
+        with open(os.environ["LH_REPORT"], "w") as fh:
+            json.dump(rows, fh, indent=1)
