@@ -113,6 +113,9 @@ class LiquidModel {
     detail::check(ctx_, lh_get_cfl(ctx_, &dt));
     return dt;
   }
+  // snapshots (no reference counterpart): what the next step depends on
+  void SaveState(const std::string& path) const { detail::check(ctx_, lh_save(ctx_, path.c_str())); }
+  void LoadState(const std::string& path) { detail::check(ctx_, lh_load(ctx_, path.c_str())); }
   // stage methods (LiquidModel.h:25-47)
   void ComputeLiquidPhi() { detail::check(ctx_, lh_compute_liquid_phi(ctx_)); }
   void ComputeSolidPhi() { detail::check(ctx_, lh_compute_solid_phi(ctx_)); }

@@ -227,6 +227,17 @@ int lh_load_positions_line(lh_ctx* ctx, const char* line);
 /* simulated time and the tank-acceleration clock */
 int lh_set_time(lh_ctx* ctx, double t);
 int lh_get_time(const lh_ctx* ctx, double* t);
+/* ---- snapshots -------------------------------------------------------------
+ * No reference counterpart (the reference can only replay positions from its
+ * trajectory file, Scene.cpp:26-45).  lh_save writes the state the next step
+ * depends on: the particles (positions, velocities, affine matrices, in id
+ * order), the persistent isValid field (LiquidModel.cpp:948-951 keeps it across
+ * steps), the time, the CorrectVolume cycle counters and the last max |v| (so
+ * lh_get_cfl gives the same dt).  lh_load restores it into a context created from
+ * the same configuration (same grid; LH_ERR_ARG otherwise): the following steps
+ * are bit-identical to the saved run's.  Little-endian binary, magic "LHSNAP01". */
+int lh_save(lh_ctx* ctx, const char* path);
+int lh_load(lh_ctx* ctx, const char* path);
 
 #ifdef __cplusplus
 }

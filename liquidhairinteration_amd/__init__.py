@@ -123,6 +123,8 @@ def load_library():
         "lh_set_warm_start": ([vp, fp, C.c_size_t], C.c_int),
         "lh_solver_level_info": ([vp, C.c_int, ip, C.POINTER(C.c_int64)], C.c_int),
         "lh_write_positions": ([vp, C.c_char_p, C.c_int], C.c_int),
+        "lh_save": ([vp, C.c_char_p], C.c_int),
+        "lh_load": ([vp, C.c_char_p], C.c_int),
         "lh_load_positions_line": ([vp, C.c_char_p], C.c_int),
         "lh_warmstart_init": ([vp, C.c_uint64], C.c_int),
         "lh_warmstart_load": ([vp, C.c_char_p], C.c_int),
@@ -406,6 +408,15 @@ class LiquidModel:
         v = C.c_float()
         self._h.check(self._h.L.lh_get_cfl(self._h.h, C.byref(v)))
         return float(v.value)
+
+    def SaveState(self, path):
+        """snapshot of what the next step depends on (particles, isValid, time, CorrectVolume
+        counters, last max |v|); no reference counterpart"""
+        self._h.check(self._h.L.lh_save(self._h.h, os.fspath(path).encode()))
+
+    def LoadState(self, path):
+        """restore a SaveState snapshot into a model built from the same configuration"""
+        self._h.check(self._h.L.lh_load(self._h.h, os.fspath(path).encode()))
 
     def RunSteps(self, nsteps, dt):
         """nsteps x {PreStep, Step, PostStep, dt = GetCFL()} in one native call; returns (next dt, stats)."""

@@ -173,6 +173,8 @@ struct P2GP {
 // that no particle can reach writes its closed-form value and leaves; the test
 // reads the two end offsets of every (hx, hy) bin row of the tile's reach.
 constexpr int kTX = 1, kTY = 4, kTZ = 64;
+// P2G tiles: 1 x kPTY x kPTZ faces, one per thread (kPTY * kPTZ == kBlock)
+constexpr int kPTY = 4, kPTZ = 64;
 
 __device__ bool tile_has_particles(const int* __restrict__ start, const int lo[3], const int hi[3], int H1, int H2) {
   const int ny = hi[1] - lo[1] + 1;
@@ -197,18 +199,20 @@ inline unsigned tile_grid(size_t ntiles) {
   return (unsigned)(div_up(ntiles, (size_t)kXcds * kXChunk) * kXcds * kXChunk);
 }
 
+template <int TY = kTY, int TZ = kTZ>
 __device__ __forceinline__ bool tile_coords(int d0, int d1, int d2, int c[3], int c0[3]) {
-  const int ntx = (d0 + kTX - 1) / kTX, nty = (d1 + kTY - 1) / kTY, ntz = (d2 + kTZ - 1) / kTZ;
+  static_assert(TY * TZ == kBlock, "one cell or face per thread");
+  const int ntx = (d0 + kTX - 1) / kTX, nty = (d1 + TY - 1) / TY, ntz = (d2 + TZ - 1) / TZ;
   const int x = (int)(blockIdx.x % kXcds), j = (int)(blockIdx.x / kXcds);
   const int t = ((j / kXChunk) * kXcds + x) * kXChunk + j % kXChunk;
   if (t >= ntx * nty * ntz) return false;
   const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
   c0[0] = bx * kTX;
-  c0[1] = by * kTY;
-  c0[2] = bz * kTZ;
+  c0[1] = by * TY;
+  c0[2] = bz * TZ;
   c[0] = c0[0];
-  c[1] = c0[1] + (threadIdx.x >> 6);
-  c[2] = c0[2] + (threadIdx.x & 63);
+  c[1] = c0[1] + (int)threadIdx.x / TZ;
+  c[2] = c0[2] + (int)threadIdx.x % TZ;
   return true;
 }
 
@@ -246,8 +250,8 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const flo
   const int a = q.a;
   const int d0 = q.g.N[0] + (a == 0), d1 = q.g.N[1] + (a == 1), d2 = q.g.N[2] + (a == 2);
   int c[3], c0[3];
-  if (!tile_coords(d0, d1, d2, c, c0)) return;
-  const int T[3] = {kTX, kTY, kTZ};
+  if (!tile_coords<kPTY, kPTZ>(d0, d1, d2, c, c0)) return;
+  const int T[3] = {kTX, kPTY, kPTZ};
   int blo[3], bhi[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -785,8 +789,8 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
     P2GP qa = q;
     if (!fuse_body_forces || a != 1) qa.gdt = 0.f;
     ProfScope ps(c, LH_PROF_P2G);
-    unsigned nt = tile_grid(div_up(c.g.N[0] + (a == 0), kTX) * div_up(c.g.N[1] + (a == 1), kTY) *
-                            div_up(c.g.N[2] + (a == 2), kTZ));
+    unsigned nt = tile_grid(div_up(c.g.N[0] + (a == 0), kTX) * div_up(c.g.N[1] + (a == 1), kPTY) *
+                            div_up(c.g.N[2] + (a == 2), kPTZ));
     if (pow2)
       hipLaunchKernelGGL(k_p2g<true>, dim3(nt), dim3(kBlock), 0, c.s, c.v[a], (const float4*)c.ppack, c.cap,
                          c.bin_start, qa);

@@ -1119,6 +1119,87 @@ int lh_load_positions_line(lh_ctx* ctx, const char* line) {
   return lh_set_particles(ctx, pos.data(), vel.data(), C.data(), n);
 }
 
+namespace {
+struct SnapHeader {
+  char magic[8];
+  int32_t n[3];
+  int32_t correct_step;
+  int64_t particles;
+  double time;
+  uint64_t correct_count;
+  float last_maxv;
+  int32_t reserved;
+};
+size_t valid_count(const Ctx& c) { return (size_t)(c.g.N[0] + 1) * (c.g.N[1] + 1) * (c.g.N[2] + 1); }
+}  // namespace
+
+int lh_save(lh_ctx* ctx, const char* path) {
+  if (!ctx || !path) return LH_ERR_ARG;
+  Ctx& c = ctx->c;
+  const int64_t P = c.P;
+  std::vector<float> pos(3 * (size_t)P), vel(3 * (size_t)P), C(9 * (size_t)P);
+  std::vector<uint8_t> valid(valid_count(c));
+  int rc = lh_get_particles(ctx, pos.data(), vel.data(), C.data(), P);
+  if (rc) return rc;
+  rc = lh_get_field(ctx, LH_FIELD_IS_VALID, valid.data(), valid.size());
+  if (rc) return rc;
+  SnapHeader h;
+  std::memset(&h, 0, sizeof(h));
+  std::memcpy(h.magic, "LHSNAP01", 8);
+  for (int a = 0; a < 3; ++a) h.n[a] = c.g.N[a];
+  h.correct_step = c.correct_step;
+  h.particles = P;
+  h.time = c.time;
+  h.correct_count = c.correct_count;
+  h.last_maxv = c.last_maxv;
+  std::ofstream f(path, std::ios::binary);
+  f.write((const char*)&h, sizeof(h));
+  f.write((const char*)pos.data(), pos.size() * sizeof(float));
+  f.write((const char*)vel.data(), vel.size() * sizeof(float));
+  f.write((const char*)C.data(), C.size() * sizeof(float));
+  f.write((const char*)valid.data(), valid.size());
+  if (!f) {
+    c.err = std::string("lh_save: cannot write ") + path;
+    return LH_ERR_ARG;
+  }
+  return LH_OK;
+}
+
+int lh_load(lh_ctx* ctx, const char* path) {
+  if (!ctx || !path) return LH_ERR_ARG;
+  Ctx& c = ctx->c;
+  std::ifstream f(path, std::ios::binary);
+  SnapHeader h;
+  if (!f || !f.read((char*)&h, sizeof(h)) || std::memcmp(h.magic, "LHSNAP01", 8) != 0) {
+    c.err = std::string("lh_load: not a snapshot file: ") + path;
+    return LH_ERR_ARG;
+  }
+  if (h.n[0] != c.g.N[0] || h.n[1] != c.g.N[1] || h.n[2] != c.g.N[2] || h.particles < 0) {
+    c.err = "lh_load: the snapshot's grid does not match this context";
+    return LH_ERR_ARG;
+  }
+  const int64_t P = h.particles;
+  std::vector<float> pos(3 * (size_t)P), vel(3 * (size_t)P), C(9 * (size_t)P);
+  std::vector<uint8_t> valid(valid_count(c));
+  f.read((char*)pos.data(), pos.size() * sizeof(float));
+  f.read((char*)vel.data(), vel.size() * sizeof(float));
+  f.read((char*)C.data(), C.size() * sizeof(float));
+  f.read((char*)valid.data(), valid.size());
+  if (!f) {
+    c.err = std::string("lh_load: truncated snapshot ") + path;
+    return LH_ERR_ARG;
+  }
+  int rc = lh_set_particles(ctx, pos.data(), vel.data(), C.data(), P);
+  if (rc) return rc;
+  rc = lh_set_field(ctx, LH_FIELD_IS_VALID, valid.data(), valid.size());
+  if (rc) return rc;
+  c.time = h.time;
+  c.correct_step = h.correct_step;
+  c.correct_count = h.correct_count;
+  c.last_maxv = h.last_maxv;
+  return LH_OK;
+}
+
 int lh_set_time(lh_ctx* ctx, double t) {
   if (!ctx) return LH_ERR_ARG;
   ctx->c.time = t;

@@ -265,3 +265,33 @@ def test_extrapolation_in_place_validity_chain():
     same(g, orc.get(F["vx"]), "extrapolated vx")
     same(m.get_field("isValid"), orc.get(F["isValid"]), "validity")
     scene.close()
+
+
+def test_snapshot_resume_bit_exact(tmp_path):
+    """lh_save after 3 steps, 3 more steps; a fresh context that loads the snapshot and runs
+    3 steps ends bit-identical (particles, isValid, time, next dt)"""
+    print("case snapshot resume, dam24 CorrectVolume", flush=True)
+    cfg = CASES["dam24_correct_volume"]()
+    snap = str(tmp_path / "state.lhsnap")
+    sc = lh.Scene(cfg)
+    m = sc.GetLiquidModel()
+    m.Init()
+    dt, _ = m.RunSteps(3, float(cfg.get("firstDt", 0.01)))
+    m.SaveState(snap)
+    dt_a, _ = m.RunSteps(3, dt)
+    pa, va, ca = m.GetParticleSystem().get()
+    valid_a, time_a = m.get_field("isValid"), m.time
+    sc.close()
+    sc = lh.Scene(cfg)
+    m = sc.GetLiquidModel()
+    m.Init()
+    m.LoadState(snap)
+    assert m.GetCFL() == dt
+    dt_b, _ = m.RunSteps(3, dt)
+    pb, vb, cb = m.GetParticleSystem().get()
+    same(pb, pa, "positions after resume")
+    same(vb, va, "velocities after resume")
+    same(cb, ca, "affine matrices after resume")
+    same(m.get_field("isValid"), valid_a, "isValid after resume")
+    assert m.time == time_a and dt_b == dt_a
+    sc.close()
