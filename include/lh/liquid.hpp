@@ -113,6 +113,9 @@ class LiquidModel {
     detail::check(ctx_, lh_get_cfl(ctx_, &dt));
     return dt;
   }
+  // LiquidModel.h:57, with the boundary tree as the JSON text of a config "Boundary" node;
+  // ComputeSolidPhi() afterwards (each step for a moving solid)
+  void SetBoundary(const std::string& boundaryJson) { detail::check(ctx_, lh_set_boundary(ctx_, boundaryJson.c_str())); }
   // snapshots (no reference counterpart): what the next step depends on
   void SaveState(const std::string& path) const { detail::check(ctx_, lh_save(ctx_, path.c_str())); }
   void LoadState(const std::string& path) { detail::check(ctx_, lh_load(ctx_, path.c_str())); }

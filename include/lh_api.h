@@ -114,6 +114,12 @@ int lh_run_steps(lh_ctx* ctx, int nsteps, float* dt, lh_step_stats* last);
 
 /* ---- stage entry points (public stage methods of LiquidModel.h:25-47) ---- */
 int lh_compute_solid_phi(lh_ctx* ctx);               /* LiquidModel.cpp:562-611 */
+/* LiquidModel<3>::SetBoundary (LiquidModel.h:57): replace the solid by a boundary
+ * tree given as the JSON text of a config "Boundary" node (Rect / Sphere leaves
+ * under Union / Intersection / Subtract / None, Boundary.h:27-94; at most 32
+ * nodes).  Like the reference it does not recompute solidPhi: call
+ * lh_compute_solid_phi, before each step for a moving solid. */
+int lh_set_boundary(lh_ctx* ctx, const char* json);
 int lh_clear_grid(lh_ctx* ctx);                      /* Grid.cpp:157-169 */
 int lh_compute_liquid_phi(lh_ctx* ctx);              /* LiquidModel.cpp:539-560 */
 int lh_p2g(lh_ctx* ctx);                             /* LiquidModel.cpp:613-721 */

@@ -17,6 +17,7 @@ fallback: importing works without a GPU, but creating a Scene needs liblh.so
 and a visible MI355X.
 """
 import ctypes as C
+import json
 import os
 
 import numpy as np
@@ -124,6 +125,7 @@ def load_library():
         "lh_solver_level_info": ([vp, C.c_int, ip, C.POINTER(C.c_int64)], C.c_int),
         "lh_write_positions": ([vp, C.c_char_p, C.c_int], C.c_int),
         "lh_save": ([vp, C.c_char_p], C.c_int),
+        "lh_set_boundary": ([vp, C.c_char_p], C.c_int),
         "lh_load": ([vp, C.c_char_p], C.c_int),
         "lh_load_positions_line": ([vp, C.c_char_p], C.c_int),
         "lh_warmstart_init": ([vp, C.c_uint64], C.c_int),
@@ -408,6 +410,12 @@ class LiquidModel:
         v = C.c_float()
         self._h.check(self._h.L.lh_get_cfl(self._h.h, C.byref(v)))
         return float(v.value)
+
+    def SetBoundary(self, node):
+        """LiquidModel<3>::SetBoundary (LiquidModel.h:57) with a config "Boundary" node (dict or
+        JSON text); call ComputeSolidPhi afterwards (each step for a moving solid)"""
+        text = node if isinstance(node, str) else json.dumps(node)
+        self._h.check(self._h.L.lh_set_boundary(self._h.h, text.encode()))
 
     def SaveState(self, path):
         """snapshot of what the next step depends on (particles, isValid, time, CorrectVolume

@@ -196,7 +196,7 @@ int alloc_ctx(Ctx& c) {
   ok = ok && dalloc(c, c.pressure, c.ncell) && dalloc(c, c.phi, c.ncell) && dalloc(c, c.rhs, c.ncell) &&
        dalloc(c, c.solid_phi, c.nnode) && dalloc(c, c.is_valid, c.nnode) && dalloc(c, c.xstate, fmax);
   c.nsdf = (int)(c.cfg.sdf.size() / 12);
-  ok = ok && dalloc(c, c.d_sdf, c.cfg.sdf.size());
+  ok = ok && dalloc(c, c.d_sdf, (size_t)12 * kMaxSdfNodes);  // room for any tree lh_set_boundary accepts
   for (int a = 0; a < 3; ++a) c.H[a] = 2 * c.cfg.N[a];
   c.nbins = (size_t)c.H[0] * c.H[1] * c.H[2];
   if (c.nbins >= (size_t)0x7fffffff) {
@@ -623,6 +623,22 @@ int lh_grid_info(const lh_ctx* ctx, int dims[3], float min_coord[3], float dx[3]
 int lh_field_shape(const lh_ctx* ctx, int field, int shape[3]) {
   if (!ctx) return LH_ERR_ARG;
   return field_count(ctx->c, field, shape) ? LH_OK : LH_ERR_ARG;
+}
+
+int lh_set_boundary(lh_ctx* ctx, const char* json) {
+  return guarded(ctx, [json](Ctx& c) {
+    if (!json) return LH_ERR_ARG;
+    std::vector<float> prog;
+    emit_sdf(Json::parse(json), prog);
+    if (prog.empty() || prog.size() / 12 > (size_t)kMaxSdfNodes) {
+      c.err = "Boundary tree too large";
+      return LH_ERR_CONFIG;
+    }
+    c.cfg.sdf = prog;
+    c.nsdf = (int)(prog.size() / 12);
+    hipMemcpyAsync(c.d_sdf, c.cfg.sdf.data(), c.cfg.sdf.size() * sizeof(float), hipMemcpyHostToDevice, c.s);
+    return hip_ok(c, hipStreamSynchronize(c.s), "boundary") ? LH_OK : LH_ERR_DEVICE;
+  });
 }
 
 int lh_compute_solid_phi(lh_ctx* ctx) {

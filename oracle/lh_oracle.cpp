@@ -1227,6 +1227,12 @@ void* orc_create(const orc_params* prm) {
   return m;
 }
 void orc_destroy(void* h) { delete (Model*)h; }
+// LiquidModel<3>::SetBoundary (LiquidModel.h:57): a new SDF program; solidPhi is recomputed by
+// orc_compute_solid_phi, as the reference recomputes it in ComputeSolidPhi
+void orc_set_sdf(void* h, const float* prog, int nnodes) {
+  Model* m = (Model*)h;
+  m->sdf.assign(prog, prog + 12 * (size_t)nnodes);
+}
 void orc_set_threads(int n) {
 #ifdef _OPENMP
   omp_set_num_threads(n);

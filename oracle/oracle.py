@@ -54,6 +54,7 @@ def lib():
         L.orc_create.restype = vp
         L.orc_create.argtypes = [C.POINTER(Params)]
         L.orc_destroy.argtypes = [vp]
+        L.orc_set_sdf.argtypes = [vp, fp, C.c_int]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_max_threads.restype = C.c_int
         L.orc_grid_info.argtypes = [vp, ip, fp, fp]
@@ -279,6 +280,11 @@ class Oracle:
         return float(self.L.orc_get_time(self.h))
 
     # --- stages
+    def set_boundary(self, node):
+        """LiquidModel<3>::SetBoundary with a config "Boundary" node"""
+        prog = sdf_program(node)
+        self.L.orc_set_sdf(self.h, _fptr(prog), len(prog) // 12)
+
     def compute_solid_phi(self):
         self.L.orc_compute_solid_phi(self.h)
 

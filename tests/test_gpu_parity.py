@@ -295,3 +295,38 @@ def test_snapshot_resume_bit_exact(tmp_path):
     same(m.get_field("isValid"), valid_a, "isValid after resume")
     assert m.time == time_a and dt_b == dt_a
     sc.close()
+
+
+def test_moving_sphere_bit_exact():
+    """SetBoundary + ComputeSolidPhi before every step (a moving obstacle): the sphere of the
+    tank scene moves 0.02 along x per step; solidPhi, liquidW and the step stay bit-identical"""
+    import copy
+
+    print("case moving sphere, dam24 tank", flush=True)
+    cfg = small_dam(24, "symmetric", tank=True)
+    orc = Oracle(cfg, dot_mode=1, p2g_order=1)
+    orc.init()
+    scene = lh.Scene(cfg)
+    m = scene.GetLiquidModel()
+    m.Init()
+    dt = np.float32(cfg.get("firstDt", 0.01))
+    dtg = float(dt)
+    for k in range(4):
+        b = copy.deepcopy(cfg["Boundary"])
+        b["right"]["center"] = [0.5 + 0.02 * (k + 1), 0.5, 0.5]
+        m.SetBoundary(b)
+        m.ComputeSolidPhi()
+        orc.set_boundary(b)
+        orc.compute_solid_phi()
+        for f in ("solidPhi", "liquidWx", "liquidWy", "liquidWz"):
+            same(m.get_field(f), orc.get(F[f]), "%s, step %d" % (f, k))
+        st, _, _ = orc.step(dt)
+        dt = np.float32(orc.get_cfl()[0])
+        dtg, sst = m.RunSteps(1, dtg)
+        assert sst["solve"]["iterations"] == st.iterations
+        assert np.float32(dtg) == dt
+        pg, vg, _ = m.GetParticleSystem().get()
+        po, vo, _ = orc.particles()
+        same(pg, po, "positions, step %d" % k)
+        same(vg, vo, "velocities, step %d" % k)
+    scene.close()
