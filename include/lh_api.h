@@ -245,6 +245,47 @@ int lh_get_time(const lh_ctx* ctx, double* t);
 int lh_save(lh_ctx* ctx, const char* path);
 int lh_load(lh_ctx* ctx, const char* path);
 
+/* ---- the 2D model: LiquidModel<2> / CGSolver<2> --------------------------
+ * The reference's other instantiation of the same classes (LiquidModel.cpp:10-474,
+ * PressureSolver.cpp:9-418), selected there by DIMENSION (Definition.h:3).  A 2D
+ * context is created from a config whose gridDimension / minCoord / maxCoord have
+ * two entries; Boundary trees use 2D centers and extents (Sphere = circle); without
+ * FluidBlocks, Init spawns the reference's lattice (spacing dx / 2 in the box shrunk
+ * by 30 % per side, at most numParticles).  Particles: pos and vel 2 floats, C 4
+ * floats (2x2 column-major) per particle, in spawn order.  Fields use the lh_field
+ * ids that exist in 2D (v, tempV, liquidW x/y, pressure, liquidPhi, solidPhi,
+ * isValid, rhs) with Array2 layout (x-major, y fastest). */
+typedef struct lh2_ctx lh2_ctx;
+int lh2_create(const char* json_path, lh2_ctx** out);
+int lh2_create_from_string(const char* json_text, lh2_ctx** out);
+void lh2_destroy(lh2_ctx* ctx);
+const char* lh2_last_error(const lh2_ctx* ctx);
+int lh2_init(lh2_ctx* ctx);                              /* LiquidModel.cpp:44-49 */
+int lh2_pre_step(lh2_ctx* ctx, float dt);                /* :51-55 */
+int lh2_step(lh2_ctx* ctx, float dt, lh_solve_stats* stats); /* :57-66 */
+int lh2_get_cfl(lh2_ctx* ctx, float* dt);                /* LiquidModel.h:49-55 */
+int lh2_run_steps(lh2_ctx* ctx, int n, float* dt, lh_solve_stats* last);
+int lh2_clear_grid(lh2_ctx* ctx);                        /* Grid.cpp:52-61 */
+int lh2_compute_liquid_phi(lh2_ctx* ctx);                /* LiquidModel.cpp:78-98 */
+int lh2_compute_solid_phi(lh2_ctx* ctx);                 /* :100-125 */
+int lh2_set_boundary(lh2_ctx* ctx, const char* json);    /* LiquidModel.h:57 */
+int lh2_p2g(lh2_ctx* ctx);                               /* :127-186 */
+int lh2_apply_gravity(lh2_ctx* ctx, float dt);           /* :306-313 */
+int lh2_solve_pressure(lh2_ctx* ctx, float dt, lh_solve_stats* stats); /* PressureSolver.cpp:329-416 */
+int lh2_velocity_update(lh2_ctx* ctx, float dt);         /* LiquidModel.cpp:238-291 */
+int lh2_extrapolate(lh2_ctx* ctx);                       /* :341-409 */
+int lh2_g2p(lh2_ctx* ctx);                               /* :188-236 */
+int lh2_advect(lh2_ctx* ctx, float dt);                  /* :315-339 */
+int lh2_correct_volume(lh2_ctx* ctx, float dt);          /* :428-468 */
+int lh2_divergence(lh2_ctx* ctx, double* mean, double* linf, int* count); /* :411-426 */
+int lh2_num_particles(const lh2_ctx* ctx, int64_t* n);
+int lh2_set_particles(lh2_ctx* ctx, const float* pos, const float* vel, const float* C, int64_t n);
+int lh2_get_particles(lh2_ctx* ctx, float* pos, float* vel, float* C, int64_t n);
+int lh2_get_field(lh2_ctx* ctx, int field, void* host, size_t count);
+int lh2_set_field(lh2_ctx* ctx, int field, const void* host, size_t count);
+int lh2_set_time(lh2_ctx* ctx, double t);
+int lh2_get_time(const lh2_ctx* ctx, double* t);
+
 #ifdef __cplusplus
 }
 #endif

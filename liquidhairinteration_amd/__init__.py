@@ -22,7 +22,7 @@ import os
 
 import numpy as np
 
-__all__ = ["Scene", "LiquidModel", "CGSolver", "LHError", "library_path", "load_library", "FIELDS",
+__all__ = ["Scene", "LiquidModel", "CGSolver", "Scene2D", "LiquidModel2D", "LHError", "library_path", "load_library", "FIELDS",
            "SolveStats", "StepStats", "config_path"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -554,3 +554,6 @@ class Scene:
 
     def close(self):
         self._h.close()
+
+
+from .model2d import LiquidModel2D, Scene2D  # noqa: E402  (the 2D instantiation, LiquidModel<2>)

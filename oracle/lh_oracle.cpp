@@ -65,6 +65,7 @@ struct orc_params {
   int correct_volume;  // 1: CorrectVolume after AdvectParticle (commented out of the reference's 3D step)
   int correct_cycle;   // LiquidParameter.correctCycle
   unsigned long long seed;
+  long long num_particles;  // LiquidParameter.numParticles (the 2D lattice spawn's cap)
 };
 struct orc_solve_stats {
   int iterations, converged;
@@ -1463,3 +1464,931 @@ float orc_sdf_eval(const float* prog, int nnodes, float x, float y, float z) {
 }
 void orc_barycentric(float x, int lo, int hi, int* idx, float* frac) { barycentric(x, lo, hi, *idx, *frac); }
 }
+
+// ======================================================================== 2D
+// LiquidModel<2> (LiquidModel.cpp:10-474) and CGSolver<2> (PressureSolver.cpp:9-418):
+// the same restatement rules as the 3D model above (serial where the reference races,
+// dot products in FP64 rounded to float with dot_mode 1).  Array2 is x-major, y fastest
+// (Array.h:5-60).  Differences from 3D that the reference has: particle radius = dx (so P2G,
+// G2P and ComputeLiquidPhi visit the 3 x 3 cells around a particle's cell, which cuts the
+// kernel's support on the face-normal axis), lattice spawn with spacing dx / 2 inside the box
+// shrunk by 30 % per side, Restrict weight 0.25, the PCG residual divided by the liquid cell
+// count, CorrectVolume always in the step.
+namespace {
+
+struct A2 {
+  int d[2] = {1, 1};
+  std::vector<float> v;
+  void resize(int x, int y) { d[0] = x; d[1] = y; v.assign((size_t)x * y, 0.f); }
+  inline size_t id(int x, int y) const { return (size_t)x * d[1] + y; }
+  inline float& operator()(int x, int y) { return v[id(x, y)]; }
+  inline float operator()(int x, int y) const { return v[id(x, y)]; }
+  void fill(float a) { std::fill(v.begin(), v.end(), a); }
+};
+inline float norm2(float a, float b) { return std::sqrt(a * a + b * b); }
+inline float max2(float a, float b) { return (a < b) ? b : a; }
+
+// Boundary.h:27-94 on 2D points (the same program format; z ignored; Sphere is a circle)
+float sdf_eval2(const std::vector<float>& prog, float px, float py) {
+  float st[32];
+  int sp = 0;
+  const size_t n = prog.size() / 12;
+  for (size_t k = 0; k < n; ++k) {
+    const float* q = &prog[12 * k];
+    const int kind = (int)q[0];
+    if (kind == 0) {
+      float ax = std::fabs(px - q[1]) - q[4];
+      float ay = std::fabs(py - q[2]) - q[5];
+      float outside = norm2(ax > 0.f ? ax : 0.f, ay > 0.f ? ay : 0.f);
+      float mc = max2(ax, ay);
+      float inside = (mc < 0.f) ? mc : 0.f;
+      st[sp++] = (q[8] != 0.f) ? -outside - inside : outside + inside;
+    } else if (kind == 1) {
+      float d = norm2(px - q[1], py - q[2]) - q[7];
+      st[sp++] = (q[8] != 0.f) ? -d : d;
+    } else if (kind == 13) {
+    } else {
+      float b = st[--sp];
+      float a = st[--sp];
+      float r;
+      if (kind == 10) r = (a < b) ? a : b;
+      else if (kind == 11) r = (a > b) ? a : b;
+      else { float nb = -b; r = (a > nb) ? a : nb; }
+      st[sp++] = r;
+    }
+  }
+  return sp > 0 ? st[sp - 1] : 1.0f;
+}
+
+struct Level2 {
+  int d[2];
+  A2 adiag;
+  std::vector<float> acoef;  // 4 per cell: right, left, top, bottom
+};
+
+struct Model2 {
+  orc_params prm;
+  std::vector<float> sdf;
+  int N[2];
+  float mn[2], mx[2], dx[2];
+  A2 vx, vy, tvx, tvy, wx, wy, lwx, lwy, pressure, liquidPhi, solidPhi, rhs;
+  std::vector<uint8_t> isValid;  // (N0+1) x (N1+1)
+  std::vector<float> pos, vel, C, radius;  // 2, 2, 4 (2x2 column-major) per particle
+  i64 P = 0;
+  std::vector<Level2> lv;
+  int max_level = 1;
+  double time = 0.0;
+  int correct_step = 0;
+  uint64_t correct_count = 0;
+  int last_iterations = 0;
+};
+
+inline size_t vid2(const Model2& m, int x, int y) { return (size_t)x * (m.N[1] + 1) + y; }
+
+// Grid<2> ctor (Grid.cpp:9-17), CGSolver<2> ctor (PressureSolver.cpp:11-35)
+void model2_alloc(Model2& m) {
+  const int* N = m.N;
+  m.vx.resize(N[0] + 1, N[1]); m.vy.resize(N[0], N[1] + 1);
+  m.tvx.resize(N[0] + 1, N[1]); m.tvy.resize(N[0], N[1] + 1);
+  m.wx.resize(N[0] + 1, N[1]); m.wy.resize(N[0], N[1] + 1);
+  m.lwx.resize(N[0] + 1, N[1]); m.lwy.resize(N[0], N[1] + 1);
+  m.pressure.resize(N[0], N[1]); m.liquidPhi.resize(N[0], N[1]); m.rhs.resize(N[0], N[1]);
+  m.solidPhi.resize(N[0] + 1, N[1] + 1);
+  m.isValid.assign((size_t)(N[0] + 1) * (N[1] + 1), 0);
+  for (int a = 0; a < 2; ++a) m.dx[a] = (float)((m.mx[a] - m.mn[a]) / (float)N[a]);
+  int L = std::max(1, m.prm.num_level);
+  if (m.prm.precond != 2) L = 1;
+  m.max_level = L;
+  m.lv.resize(L);
+  for (int l = 0; l < L; ++l) {
+    Level2& q = m.lv[l];
+    q.d[0] = N[0] / (1 << l);
+    q.d[1] = N[1] / (1 << l);
+    q.adiag.resize(q.d[0], q.d[1]);
+    q.acoef.assign((size_t)q.d[0] * q.d[1] * 4, 0.f);
+  }
+}
+
+// Grid<2>::GetSolidPhi / GetSolidGradient, Grid.cpp:33-50, 63-80
+inline bool in_grid2(const Model2& m, float x, float y) {
+  return m.mn[0] <= x && m.mn[1] <= y && x <= m.mx[0] && y <= m.mx[1];
+}
+void solid_gradient2(const Model2& m, float x, float y, float g[2]);
+float solid_phi2(const Model2& m, float x, float y) {
+  if (!in_grid2(m, x, y)) {
+    float vx = clampf(x, m.mn[0] + m.dx[0], m.mx[0] - m.dx[0]);
+    float vy = clampf(y, m.mn[1] + m.dx[1], m.mx[1] - m.dx[1]);
+    float g[2];
+    solid_gradient2(m, vx, vy, g);
+    return solid_phi2(m, vx, vy) + g[0] * (x - vx) + g[1] * (y - vy);
+  }
+  float lx = (x - m.mn[0]) / m.dx[0];
+  float ly = (y - m.mn[1]) / m.dx[1];
+  int xi, yi;
+  float fx, fy;
+  barycentric(lx, 0, m.N[0] + 1, xi, fx);
+  barycentric(ly, 0, m.N[1] + 1, yi, fy);
+  const A2& s = m.solidPhi;
+  return bilerp(s(xi, yi), s(xi + 1, yi), s(xi, yi + 1), s(xi + 1, yi + 1), fx, fy);
+}
+void solid_gradient2(const Model2& m, float x, float y, float g[2]) {
+  if (!in_grid2(m, x, y)) {
+    float vx = clampf(x, m.mn[0] + m.dx[0], m.mx[0] - m.dx[0]);
+    float vy = clampf(y, m.mn[1] + m.dx[1], m.mx[1] - m.dx[1]);
+    solid_gradient2(m, vx, vy, g);
+    return;
+  }
+  const float eps = 1e-4f;
+  float left = solid_phi2(m, x - eps, y), right = solid_phi2(m, x + eps, y);
+  float bottom = solid_phi2(m, x, y - eps), top = solid_phi2(m, x, y + eps);
+  const float two_eps = (float)(2.0 * (double)eps);
+  float a = (right - left) / two_eps, b = (top - bottom) / two_eps;
+  float n2 = a * a + b * b;
+  if (n2 > 0.f) {
+    float n = std::sqrt(n2);
+    a = a / n; b = b / n;
+  }
+  g[0] = a; g[1] = b;
+}
+
+// LiquidModel<2>::ComputeSolidPhi, LiquidModel.cpp:100-125
+void compute_solid_phi2(Model2& m) {
+  const int* N = m.N;
+  for (int xi = 0; xi < N[0] + 1; ++xi)
+    for (int yi = 0; yi < N[1] + 1; ++yi)
+      m.solidPhi(xi, yi) = sdf_eval2(m.sdf, (float)xi * m.dx[0] + m.mn[0], (float)yi * m.dx[1] + m.mn[1]);
+  for (int xi = 0; xi < N[0] + 1; ++xi)
+    for (int yi = 0; yi < N[1]; ++yi)
+      m.lwx(xi, yi) = clampf(1.0f - fraction_in_region(m.solidPhi(xi, yi + 1), m.solidPhi(xi, yi)), 0.f, 1.f);
+  for (int xi = 0; xi < N[0]; ++xi)
+    for (int yi = 0; yi < N[1] + 1; ++yi)
+      m.lwy(xi, yi) = clampf(1.0f - fraction_in_region(m.solidPhi(xi + 1, yi), m.solidPhi(xi, yi)), 0.f, 1.f);
+}
+
+// Grid<2>::Clear, Grid.cpp:52-61
+void clear_grid2(Model2& m) {
+  m.pressure.fill(0.f);
+  m.liquidPhi.fill(3 * m.dx[0]);
+  m.vx.fill(0.f); m.wx.fill(0.f); m.tvx.fill(0.f);
+  m.vy.fill(0.f); m.wy.fill(0.f); m.tvy.fill(0.f);
+}
+
+// LiquidModel<2>::ComputeLiquidPhi, LiquidModel.cpp:78-98 (min: order-free)
+void compute_liquid_phi2(Model2& m) {
+  const int* N = m.N;
+  for (i64 pi = 0; pi < m.P; ++pi) {
+    const float* p = &m.pos[2 * pi];
+    const float r = m.radius[pi];
+    const int cx = (int)((p[0] - m.mn[0]) / m.dx[0]), cy = (int)((p[1] - m.mn[1]) / m.dx[1]);
+    const int dX = (int)std::ceil(r / m.dx[0]), dY = (int)std::ceil(r / m.dx[1]);
+    for (int xi = std::max(0, cx - dX); xi <= std::min(N[0] - 1, cx + dX); ++xi)
+      for (int yi = std::max(0, cy - dY); yi <= std::min(N[1] - 1, cy + dY); ++yi) {
+        float px = ((float)xi + 0.5f) * m.dx[0] + m.mn[0], py = ((float)yi + 0.5f) * m.dx[1] + m.mn[1];
+        float phi = norm2(px - p[0], py - p[1]) - r;
+        float& d = m.liquidPhi(xi, yi);
+        d = (d < phi) ? d : phi;  // min(a, b) macro
+      }
+  }
+}
+
+// particles grouped by cell (x-major), by id inside a cell: the order the GPU gathers in
+struct Bins2 {
+  std::vector<i64> start, order;
+};
+void build_bins2(const Model2& m, Bins2& b) {
+  const size_t nc = (size_t)m.N[0] * m.N[1];
+  std::vector<i64> key(m.P);
+  b.start.assign(nc + 1, 0);
+  for (i64 pi = 0; pi < m.P; ++pi) {
+    int cx = (int)((m.pos[2 * pi] - m.mn[0]) / m.dx[0]), cy = (int)((m.pos[2 * pi + 1] - m.mn[1]) / m.dx[1]);
+    cx = std::min(std::max(cx, 0), m.N[0] - 1);
+    cy = std::min(std::max(cy, 0), m.N[1] - 1);
+    key[pi] = (i64)cx * m.N[1] + cy;
+    b.start[key[pi] + 1]++;
+  }
+  for (size_t k = 0; k < nc; ++k) b.start[k + 1] += b.start[k];
+  b.order.assign(m.P, 0);
+  std::vector<i64> fill(b.start.begin(), b.start.end() - 1);
+  for (i64 pi = 0; pi < m.P; ++pi) b.order[fill[key[pi]]++] = pi;
+}
+
+// LiquidModel<2>::Particle2Grid, LiquidModel.cpp:127-186.  A particle reaches the faces
+// (c + d) for d in [-1, 1]^2 around its cell c (radius dx), so face f receives exactly the
+// particles of the 3 x 3 cells around it; summed per face in cell order, id order inside a cell
+// (order 1), or in particle order (order 0).
+void p2g2(Model2& m, int order) {
+  const int* N = m.N;
+  A2* V[2] = {&m.vx, &m.vy};
+  A2* W[2] = {&m.wx, &m.wy};
+  auto term = [&](i64 pi, int a, int xi, int yi, float& w, float& vw) {
+    const float* p = &m.pos[2 * pi];
+    float gx = (p[0] - m.mn[0]) / m.dx[0], gy = (p[1] - m.mn[1]) / m.dx[1];
+    float ccx = (a == 0) ? (float)xi : (float)xi + 0.5f;
+    float ccy = (a == 1) ? (float)yi : (float)yi + 0.5f;
+    w = quad_w(gx - ccx) * quad_w(gy - ccy);
+    float d0 = p[0] - (ccx * m.dx[0] + m.mn[0]), d1 = p[1] - (ccy * m.dx[1] + m.mn[1]);
+    const float* col = &m.C[4 * pi + 2 * a];
+    vw = w * (m.vel[2 * pi + a] + (col[0] * d0 + col[1] * d1));
+  };
+  if (order == 0) {
+    for (i64 pi = 0; pi < m.P; ++pi) {
+      const float* p = &m.pos[2 * pi];
+      const int cx = (int)((p[0] - m.mn[0]) / m.dx[0]), cy = (int)((p[1] - m.mn[1]) / m.dx[1]);
+      const int dX = (int)std::ceil(m.radius[pi] / m.dx[0]), dY = (int)std::ceil(m.radius[pi] / m.dx[1]);
+      for (int a = 0; a < 2; ++a)
+        for (int ddx = -dX; ddx <= dX; ++ddx)
+          for (int ddy = -dY; ddy <= dY; ++ddy) {
+            int xi = cx + ddx, yi = cy + ddy;
+            if (xi < 0 || yi < 0 || xi > N[0] - (a == 0 ? 0 : 1) || yi > N[1] - (a == 1 ? 0 : 1)) continue;
+            float w, vw;
+            term(pi, a, xi, yi, w, vw);
+            (*W[a])(xi, yi) += w;
+            (*V[a])(xi, yi) += vw;
+          }
+    }
+  } else {
+    Bins2 b;
+    build_bins2(m, b);
+    for (int a = 0; a < 2; ++a) {
+      const int d0 = N[0] + (a == 0), d1 = N[1] + (a == 1);
+      for (int xi = 0; xi < d0; ++xi)
+        for (int yi = 0; yi < d1; ++yi) {
+          float ws = 0.f, vs = 0.f;
+          for (int cx = std::max(0, xi - 1); cx <= std::min(N[0] - 1, xi + 1); ++cx)
+            for (int cy = std::max(0, yi - 1); cy <= std::min(N[1] - 1, yi + 1); ++cy) {
+              const i64 k = (i64)cx * N[1] + cy;
+              for (i64 s = b.start[k]; s < b.start[k + 1]; ++s) {
+                float w, vw;
+                term(b.order[s], a, xi, yi, w, vw);
+                ws += w;
+                vs += vw;
+              }
+            }
+          (*W[a])(xi, yi) = ws;
+          (*V[a])(xi, yi) = vs;
+        }
+    }
+  }
+  for (int a = 0; a < 2; ++a)
+    for (size_t i = 0; i < V[a]->v.size(); ++i) {
+      float w = W[a]->v[i];
+      V[a]->v[i] = !is_small(w) ? V[a]->v[i] / w : 0.0f;
+    }
+}
+
+// LiquidModel<2>::ApplyGravity, LiquidModel.cpp:306-313
+void apply_gravity2(Model2& m, float dt) {
+  for (int xi = 1; xi < m.N[0] - 1; ++xi)
+    for (int yi = 1; yi < m.N[1]; ++yi) m.vy(xi, yi) += m.prm.gravity * dt;
+}
+
+// CGSolver<2>::GetApprox*, PressureSolver.cpp:69-94
+float approx_phi2(const Model2& m, int xi, int yi, int l) {
+  if (l == 0) return m.liquidPhi(xi, yi);
+  const int f = 1 << l;
+  const int bx = f * xi + (1 << (l - 1)) - 1, by = f * yi + (1 << (l - 1)) - 1;
+  const A2& p = m.liquidPhi;
+  return bilerp(p(bx, by), p(bx + 1, by), p(bx, by + 1), p(bx + 1, by + 1), 0.5f, 0.5f);
+}
+float approx_w2(const Model2& m, int a, int xi, int yi, int l) {
+  if (l == 0) return a == 0 ? m.lwx(xi, yi) : m.lwy(xi, yi);
+  const int f = 1 << l;
+  if (a == 0) {
+    const int bx = f * xi, by = f * yi + (1 << (l - 1)) - 1;
+    return (float)(0.5 * (double)(m.lwx(bx, by) + m.lwx(bx, by + 1)));
+  }
+  const int bx = f * xi + (1 << (l - 1)) - 1, by = f * yi;
+  return (float)(0.5 * (double)(m.lwy(bx, by) + m.lwy(bx + 1, by)));
+}
+
+// CGSolver<2>::BuildLHS, PressureSolver.cpp:96-175 (faces right, left, top, bottom)
+void build_lhs2(Model2& m, float rho, float dt) {
+  for (int l = 0; l < m.max_level; ++l) {
+    Level2& q = m.lv[l];
+    q.adiag.fill(0.f);
+    std::fill(q.acoef.begin(), q.acoef.end(), 0.f);
+    const float f = (float)(1 << l);
+    const float dxl[2] = {f * m.dx[0], f * m.dx[1]};
+    for (int xi = 0; xi < q.d[0]; ++xi)
+      for (int yi = 0; yi < q.d[1]; ++yi) {
+        const float cphi = approx_phi2(m, xi, yi, l);
+        if (cphi > 0.0f) continue;
+        float& diag = q.adiag(xi, yi);
+        float* co = &q.acoef[q.adiag.id(xi, yi) * 4];
+        const int ax[4] = {0, 0, 1, 1}, sg[4] = {1, -1, 1, -1};
+        for (int k = 0; k < 4; ++k) {
+          const int a = ax[k], s = sg[k];
+          const int c[2] = {xi, yi};
+          if (s > 0 ? !(c[a] + 1 < q.d[a]) : !(c[a] - 1 >= 0)) continue;
+          int fc[2] = {xi, yi};
+          if (s > 0) fc[a] += 1;
+          const float wv = approx_w2(m, a, fc[0], fc[1], l);
+          if (!(wv > 0.0f)) continue;
+          const float term = wv * dt / (rho * dxl[a] * dxl[a]);
+          int nb[2] = {xi, yi};
+          nb[a] += s;
+          const float phi = approx_phi2(m, nb[0], nb[1], l);
+          if (phi < 0.0f) {
+            co[k] -= term;
+            diag += term;
+          } else {
+            diag += term / max_tenth(fraction_in_region(cphi, phi));
+          }
+        }
+      }
+  }
+}
+
+// CGSolver<2>::BuildRHS, PressureSolver.cpp:194-206
+void build_rhs2(Model2& m) {
+  for (int xi = 0; xi < m.N[0]; ++xi)
+    for (int yi = 0; yi < m.N[1]; ++yi) {
+      float& b = m.rhs(xi, yi);
+      b = 0.0f;
+      if (m.liquidPhi(xi, yi) > 0.0f) continue;
+      b -= m.lwx(xi + 1, yi) * m.vx(xi + 1, yi) / m.dx[0];
+      b += m.lwx(xi, yi) * m.vx(xi, yi) / m.dx[0];
+      b -= m.lwy(xi, yi + 1) * m.vy(xi, yi + 1) / m.dx[1];
+      b += m.lwy(xi, yi) * m.vy(xi, yi) / m.dx[1];
+    }
+}
+
+// CGSolver<2>::GetAx, PressureSolver.cpp:208-223 (order: diag, right, left, top, bottom)
+inline float get_ax2(const Level2& q, const std::vector<float>& g, int xi, int yi) {
+  const size_t c = (size_t)xi * q.d[1] + yi;
+  const float* co = &q.acoef[c * 4];
+  float res = g[c] * q.adiag.v[c];
+  if (xi + 1 < q.d[0]) res += g[c + q.d[1]] * co[0];
+  if (xi - 1 >= 0) res += g[c - q.d[1]] * co[1];
+  if (yi + 1 < q.d[1]) res += g[c + 1] * co[2];
+  if (yi - 1 >= 0) res += g[c - 1] * co[3];
+  return res;
+}
+
+struct Vecs2 {
+  std::vector<std::vector<float>> r, z;
+  std::vector<float> p, x;
+};
+
+// CGSolver<2>::Smooth, PressureSolver.cpp:312-327 (symmetric: z + w term, as the 3D build option)
+void smooth2(const Model2& m, Vecs2& v, int l, int phase, int mode) {
+  const Level2& q = m.lv[l];
+  const double w = (double)m.prm.omega;
+  std::vector<float>& z = v.z[l];
+  const std::vector<float>& r = v.r[l];
+  for (int xi = 0; xi < q.d[0]; ++xi)
+    for (int yi = 0; yi < q.d[1]; ++yi) {
+      if (((xi + yi) & 1) != phase) continue;
+      const size_t c = (size_t)xi * q.d[1] + yi;
+      const float ad = q.adiag.v[c];
+      if (is_small(ad)) continue;
+      const float term = (r[c] - get_ax2(q, z, xi, yi)) / ad;
+      if (mode == 0) {
+        const float wt = m.prm.omega * term;
+        z[c] = (float)((1.0 - w) * (double)z[c] + (double)wt);
+      } else {
+        z[c] = z[c] + m.prm.omega * term;
+      }
+    }
+}
+
+// CGSolver<2>::Restrict, PressureSolver.cpp:296-310 (children (0,0), (1,0), (0,1), (1,1))
+void restrict2(const Model2& m, Vecs2& v, int to) {
+  const Level2& qc = m.lv[to];
+  const Level2& qf = m.lv[to - 1];
+  for (int xi = 0; xi < qc.d[0]; ++xi)
+    for (int yi = 0; yi < qc.d[1]; ++yi) {
+      const size_t c = (size_t)xi * qc.d[1] + yi;
+      float acc = 0.f;
+      const int ord[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+      for (int k = 0; k < 4; ++k) {
+        const int x = 2 * xi + ord[k][0], y = 2 * yi + ord[k][1];
+        const size_t f = (size_t)x * qf.d[1] + y;
+        const float d = v.r[to - 1][f] - get_ax2(qf, v.z[to - 1], x, y);
+        acc = (float)((double)acc + 0.25 * (double)d);
+      }
+      v.r[to][c] = acc;
+      v.z[to][c] = 0.f;
+    }
+}
+
+// CGSolver<2>::Prolong, PressureSolver.cpp:280-294
+void prolong2(const Model2& m, Vecs2& v, int to) {
+  const Level2& qc = m.lv[to + 1];
+  const Level2& qf = m.lv[to];
+  for (int xi = 0; xi < qc.d[0]; ++xi)
+    for (int yi = 0; yi < qc.d[1]; ++yi) {
+      const float val = v.z[to + 1][(size_t)xi * qc.d[1] + yi];
+      const int ord[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+      for (int k = 0; k < 4; ++k)
+        v.z[to][(size_t)(2 * xi + ord[k][0]) * qf.d[1] + (2 * yi + ord[k][1])] += val;
+    }
+}
+
+// CGSolver<2>::ApplyPreconditioner, PressureSolver.cpp:225-278
+void apply_precond2(const Model2& m, Vecs2& v, int smoother) {
+  const A2& phi = m.liquidPhi;
+  if (m.prm.precond == 0) {
+    for (size_t c = 0; c < phi.v.size(); ++c)
+      if (!(phi.v[c] > 0.0f)) v.z[0][c] = v.r[0][c];
+    return;
+  }
+  if (m.prm.precond == 1) {
+    for (size_t c = 0; c < phi.v.size(); ++c) {
+      const float dn = m.lv[0].adiag.v[c];
+      if (!is_small(dn)) v.z[0][c] = v.r[0][c] / dn;
+    }
+    return;
+  }
+  const int L = m.max_level;
+  if (smoother == 1) std::fill(v.z[0].begin(), v.z[0].end(), 0.f);
+  for (int l = 1; l < L; ++l) {
+    for (int it = 0; it < m.prm.num_pre; ++it) {
+      smooth2(m, v, l - 1, 0, smoother);
+      smooth2(m, v, l - 1, 1, smoother);
+    }
+    restrict2(m, v, l);
+  }
+  for (int it = 0; it < m.prm.num_final; ++it) {
+    smooth2(m, v, L - 1, 0, smoother);
+    smooth2(m, v, L - 1, 1, smoother);
+  }
+  if (smoother == 1)
+    for (int it = 0; it < m.prm.num_final; ++it) {
+      smooth2(m, v, L - 1, 1, smoother);
+      smooth2(m, v, L - 1, 0, smoother);
+    }
+  for (int l = L - 2; l >= 0; --l) {
+    prolong2(m, v, l);
+    for (int it = 0; it < m.prm.num_post; ++it) {
+      smooth2(m, v, l, smoother == 1 ? 1 : 0, smoother);
+      smooth2(m, v, l, smoother == 1 ? 0 : 1, smoother);
+    }
+  }
+}
+
+float reduce2(const std::vector<float>& a, const std::vector<float>& b, int dot_mode) {
+  if (dot_mode == 0) {
+    float s = 0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+  }
+  double s = 0;
+  for (size_t i = 0; i < a.size(); ++i) s += (double)a[i] * (double)b[i];
+  return (float)s;
+}
+
+// CGSolver<2>::Solve, PressureSolver.cpp:329-416: the residual is ||r|| divided by the number of
+// liquid cells (all of them for the initial residual, the interior ones counted in the loop)
+int solve2(Model2& m, float dt, orc_solve_stats* st) {
+  const int* N = m.N;
+  const A2& phi = m.liquidPhi;
+  build_rhs2(m);
+  build_lhs2(m, m.prm.density, dt);
+  const int L = m.max_level;
+  Vecs2 v;
+  v.r.assign(L, {});
+  v.z.assign(L, {});
+  for (int l = 0; l < L; ++l) {
+    const size_t n = (size_t)m.lv[l].d[0] * m.lv[l].d[1];
+    v.r[l].assign(n, 0.f);
+    v.z[l].assign(n, 0.f);
+  }
+  const size_t n0 = phi.v.size();
+  v.p.assign(n0, 0.f);
+  v.x.assign(n0, 0.f);
+  int nliquid = 0;
+  for (size_t c = 0; c < n0; ++c)
+    if (!(phi.v[c] > 0.0f)) {
+      v.r[0][c] = m.rhs.v[c];
+      ++nliquid;
+    }
+  apply_precond2(m, v, m.prm.smoother);
+  for (size_t c = 0; c < n0; ++c)
+    if (!(phi.v[c] > 0.0f)) v.p[c] = v.z[0][c];
+  const int dm = m.prm.dot_mode;
+  float oldRZ = reduce2(v.r[0], v.z[0], dm);
+  float residual = std::sqrt(reduce2(v.r[0], v.r[0], dm)) / (float)nliquid;
+  const float res0 = residual;
+  auto done = [&](float res) {
+    if (m.prm.tol_mode == 0) return res < m.prm.tol;
+    return (double)res <= (double)m.prm.tol * (double)res0;
+  };
+  const Level2& q0 = m.lv[0];
+  int it = 0;
+  bool conv = false;
+  for (it = 0; it < m.prm.max_iter; ++it) {
+    if (done(residual)) { conv = true; break; }
+    double dd = 0;
+    float df = 0;
+    for (int xi = 1; xi < N[0] - 1; ++xi)
+      for (int yi = 1; yi < N[1] - 1; ++yi) {
+        if (phi(xi, yi) > 0.0f) continue;
+        const float ap = get_ax2(q0, v.p, xi, yi);
+        const size_t c = phi.id(xi, yi);
+        if (dm == 0) df += v.p[c] * ap;
+        else dd += (double)v.p[c] * (double)ap;
+      }
+    const float denom = (dm == 0) ? df : (float)dd;
+    const float alpha = oldRZ / denom;
+    if (std::isnan(alpha) || std::isinf(alpha)) break;
+    double rd = 0;
+    float rf = 0;
+    int cnt = 0;
+    for (int xi = 1; xi < N[0] - 1; ++xi)
+      for (int yi = 1; yi < N[1] - 1; ++yi) {
+        if (phi(xi, yi) > 0.0f) continue;
+        const size_t c = phi.id(xi, yi);
+        const float ap = get_ax2(q0, v.p, xi, yi);
+        v.x[c] += alpha * v.p[c];
+        v.r[0][c] -= alpha * ap;
+        if (dm == 0) rf += v.r[0][c] * v.r[0][c];
+        else rd += (double)v.r[0][c] * (double)v.r[0][c];
+        ++cnt;
+      }
+    residual = std::sqrt(dm == 0 ? rf : (float)rd) / (float)cnt;
+    apply_precond2(m, v, m.prm.smoother);
+    const float newRZ = reduce2(v.r[0], v.z[0], dm);
+    const float beta = newRZ / oldRZ;
+    for (size_t c = 0; c < n0; ++c)
+      if (!(phi.v[c] > 0.0f)) v.p[c] = v.z[0][c] + beta * v.p[c];
+    oldRZ = newRZ;
+  }
+  if (!conv && it >= m.prm.max_iter) conv = done(residual);
+  m.pressure.v = v.x;
+  m.last_iterations = it;
+  if (st) {
+    st->iterations = it;
+    st->converged = conv ? 1 : 0;
+    st->initial_residual = res0;
+    st->final_residual = residual;
+    st->rhs_norm = res0;
+  }
+  return 0;
+}
+
+// LiquidModel<2>::SolvePressure velocity update, LiquidModel.cpp:238-291
+void velocity_update2(Model2& m, float dt) {
+  const float rho = m.prm.density;
+  for (int xi = 1; xi < m.N[0] - 1; ++xi)
+    for (int yi = 1; yi < m.N[1] - 1; ++yi) {
+      const float cphi = m.liquidPhi(xi, yi);
+      for (int a = 0; a < 2; ++a) {
+        A2& v = a == 0 ? m.vx : m.vy;
+        const float w = a == 0 ? m.lwx(xi, yi) : m.lwy(xi, yi);
+        if (w > 0.0f) {
+          const int nx = a == 0 ? xi - 1 : xi, ny = a == 0 ? yi : yi - 1;
+          const float phi = m.liquidPhi(nx, ny);
+          if (cphi < 0.0f || phi < 0.0f) {
+            float frac = 1.0f;
+            if (cphi >= 0.0f || phi >= 0.0f) frac = max_tenth(fraction_in_region(cphi, phi));
+            const float gp = (m.pressure(xi, yi) - m.pressure(nx, ny)) / (m.dx[a] * frac);
+            v(xi, yi) -= gp * dt / rho;
+          }
+        } else {
+          v(xi, yi) = 0.0f;
+        }
+      }
+    }
+}
+
+// LiquidModel<2>::Extrapolate, LiquidModel.cpp:341-409 (in-place validity, x then y)
+void extrapolate2(Model2& m) {
+  if (m.P == 0) return;
+  A2* V[2] = {&m.vx, &m.vy};
+  A2* T[2] = {&m.tvx, &m.tvy};
+  const A2* W[2] = {&m.lwx, &m.lwy};
+  for (int a = 0; a < 2; ++a) {
+    const int xhi = a == 0 ? m.N[0] - 1 : m.N[0] - 2, yhi = a == 0 ? m.N[1] - 2 : m.N[1] - 1;
+    for (int xi = 1; xi <= xhi; ++xi)
+      for (int yi = 1; yi <= yhi; ++yi) m.isValid[vid2(m, xi, yi)] = ((*W[a])(xi, yi) > 0.0f) ? 1 : 0;
+    for (int iter = 0; iter < 5; ++iter) {
+      A2& v = *V[a];
+      A2& t = *T[a];
+      for (int xi = 1; xi <= xhi; ++xi)
+        for (int yi = 1; yi <= yhi; ++yi) {
+          t(xi, yi) = v(xi, yi);
+          if (m.isValid[vid2(m, xi, yi)]) continue;
+          float sum = 0.0f;
+          int count = 0;
+          for (int ddx = -1; ddx <= 1; ++ddx)
+            for (int ddy = -1; ddy <= 1; ++ddy)
+              if (m.isValid[vid2(m, xi + ddx, yi + ddy)]) {
+                sum += v(xi + ddx, yi + ddy);
+                ++count;
+              }
+          if (count > 0) {
+            t(xi, yi) = sum / (float)count;
+            m.isValid[vid2(m, xi, yi)] = 1;
+          }
+        }
+      std::swap(V[a]->v, T[a]->v);
+    }
+  }
+}
+
+// LiquidModel<2>::Grid2Particle, LiquidModel.cpp:188-236 (not normalised; unsigned gradient)
+void g2p2(Model2& m) {
+  const int* N = m.N;
+  for (i64 pi = 0; pi < m.P; ++pi) {
+    const float* p = &m.pos[2 * pi];
+    const float gx = (p[0] - m.mn[0]) / m.dx[0], gy = (p[1] - m.mn[1]) / m.dx[1];
+    const int cx = (int)gx, cy = (int)gy;
+    const int dX = (int)std::ceil(m.radius[pi] / m.dx[0]), dY = (int)std::ceil(m.radius[pi] / m.dx[1]);
+    float vel[2] = {0.f, 0.f};
+    float Cm[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int a = 0; a < 2; ++a) {
+      const A2& V = a == 0 ? m.vx : m.vy;
+      for (int ddx = -dX; ddx <= dX; ++ddx)
+        for (int ddy = -dY; ddy <= dY; ++ddy) {
+          const int xi = cx + ddx, yi = cy + ddy;
+          if (xi < 0 || yi < 0 || xi > N[0] - (a == 0 ? 0 : 1) || yi > N[1] - (a == 1 ? 0 : 1)) continue;
+          const float ccx = (a == 0) ? (float)xi : (float)xi + 0.5f;
+          const float ccy = (a == 1) ? (float)yi : (float)yi + 0.5f;
+          const float wX = quad_w(gx - ccx), wY = quad_w(gy - ccy);
+          const float gX = quad_gw(gx - ccx), gY = quad_gw(gy - ccy);
+          const float weight = wX * wY;
+          const float fv = V(xi, yi);
+          vel[a] += fv * weight;
+          Cm[2 * a + 0] += fv * (wY * gX);
+          Cm[2 * a + 1] += fv * (wX * gY);
+        }
+    }
+    m.vel[2 * pi] = vel[0];
+    m.vel[2 * pi + 1] = vel[1];
+    for (int k = 0; k < 4; ++k) m.C[4 * pi + k] = Cm[k];
+  }
+}
+
+// LiquidModel<2>::AdvectParticle, LiquidModel.cpp:315-339 (velocity reflection dead, Q7)
+void advect2(Model2& m, float dt) {
+  for (i64 pi = 0; pi < m.P; ++pi) {
+    float* pp = &m.pos[2 * pi];
+    const float* v = &m.vel[2 * pi];
+    float p0 = pp[0] + v[0] * dt, p1 = pp[1] + v[1] * dt;
+    const float sp = solid_phi2(m, p0, p1);
+    if (sp < 0.0f) {
+      float g[2];
+      solid_gradient2(m, p0, p1, g);
+      const float s = sp + kEps;
+      p0 -= s * g[0];
+      p1 -= s * g[1];
+    }
+    pp[0] = p0;
+    pp[1] = p1;
+  }
+}
+
+// hash slot of a 2D cell (NeighborSearch.h:28-31 with the constants of :134-135)
+inline int ns_slot2(int x, int y, int mm) {
+  const uint32_t h = (73856093u * (uint32_t)x) ^ (19349663u * (uint32_t)y);
+  const int hs = (int)h;
+  return ((hs % mm) + mm) % mm;
+}
+
+// LiquidModel<2>::CorrectVolume, LiquidModel.cpp:430-468, on the hash table of the advected
+// positions (AdvectParticle rebuilds it, :338): neighbours closer than the radius (dx), each
+// visited once per pair of region cells (3 x 3 around each particle) whose slots match; the
+// coincident-particle kick from the counter generator; positions written after all particles.
+void correct_volume2(Model2& m, float dt) {
+  const int cyc = std::max(1, m.prm.correct_cycle), step = m.correct_step;
+  const int nslot = (int)(2 * m.P);
+  const float wf = std::sqrt(m.dx[0] * m.dx[1]);
+  Bins2 b;
+  build_bins2(m, b);
+  std::vector<float> buf(2 * (size_t)m.P);
+  for (i64 pi = 0; pi < m.P; ++pi) {
+    if (pi % cyc != step) continue;
+    const float* p = &m.pos[2 * pi];
+    const float r = m.radius[pi];
+    const float r2 = r * r;
+    int lo[2], hi[2];
+    for (int a = 0; a < 2; ++a) {
+      const int c = cv_cell(p[a], m.mn[a], m.dx[a]);
+      lo[a] = cv_clampi(c - 1, 0, m.N[a] - 1);
+      hi[a] = cv_clampi(c + 1, 0, m.N[a] - 1);
+    }
+    int reg[9], nreg = 0;
+    for (int x1 = lo[0]; x1 <= hi[0]; ++x1)
+      for (int y1 = lo[1]; y1 <= hi[1]; ++y1) reg[nreg++] = ns_slot2(x1, y1, nslot);
+    float rep[2] = {0.f, 0.f};
+    // candidates: the particles of the cells within 2 of p's cell (every neighbour within dx)
+    const int cx = cv_cell(p[0], m.mn[0], m.dx[0]), cy = cv_cell(p[1], m.mn[1], m.dx[1]);
+    for (int bx = std::max(0, cx - 2); bx <= std::min(m.N[0] - 1, cx + 2); ++bx)
+      for (int by = std::max(0, cy - 2); by <= std::min(m.N[1] - 1, cy + 2); ++by) {
+        const i64 k = (i64)bx * m.N[1] + by;
+        for (i64 s = b.start[k]; s < b.start[k + 1]; ++s) {
+          const i64 npi = b.order[s];
+          if (npi == pi) continue;
+          const float* q = &m.pos[2 * npi];
+          const float d0 = p[0] - q[0], d1 = p[1] - q[1];
+          const float sq = d0 * d0 + d1 * d1;
+          if (sq >= r2) continue;
+          int nlo[2], nhi[2];
+          for (int a = 0; a < 2; ++a) {
+            const int c2 = cv_cell(q[a], m.mn[a], m.dx[a]);
+            nlo[a] = cv_clampi(c2 - 1, 0, m.N[a] - 1);
+            nhi[a] = cv_clampi(c2 + 1, 0, m.N[a] - 1);
+          }
+          int mult = 0;
+          for (int x2 = nlo[0]; x2 <= nhi[0]; ++x2)
+            for (int y2 = nlo[1]; y2 <= nhi[1]; ++y2) {
+              const int s2 = ns_slot2(x2, y2, nslot);
+              for (int kk = 0; kk < nreg; ++kk) mult += (reg[kk] == s2) ? 1 : 0;
+            }
+          const float dist = std::sqrt(sq);
+          const float w = wf * quad_w(dist / r);
+          for (int vv = 0; vv < mult; ++vv) {
+            if (dist > 1e-4f * r) {
+              const float wr = w * r;
+              rep[0] += wr * (d0 / dist);
+              rep[1] += wr * (d1 / dist);
+            } else {
+              for (int kk = 0; kk < 2; ++kk) rep[kk] += r / dt * cv_rand(m.prm.seed, m.correct_count, pi, npi, vv, kk);
+            }
+          }
+        }
+      }
+    float np[2] = {p[0] + rep[0] * dt, p[1] + rep[1] * dt};
+    const float sp = solid_phi2(m, np[0], np[1]);
+    if (sp < 0.0f) {
+      float g[2];
+      solid_gradient2(m, np[0], np[1], g);
+      np[0] -= sp * g[0];
+      np[1] -= sp * g[1];
+    }
+    buf[2 * pi] = np[0];
+    buf[2 * pi + 1] = np[1];
+  }
+  for (i64 pi = 0; pi < m.P; ++pi)
+    if (pi % cyc == step) {
+      m.pos[2 * pi] = buf[2 * pi];
+      m.pos[2 * pi + 1] = buf[2 * pi + 1];
+    }
+  m.correct_step = (step + 1) % cyc;
+  m.correct_count++;
+}
+
+// LiquidModel::GetCFL, LiquidModel.h:49-55
+float get_cfl2(const Model2& m, float* maxv_out) {
+  float maxVel = 0.0f;
+  for (i64 pi = 0; pi < m.P; ++pi) {
+    const float* v = &m.vel[2 * pi];
+    const float c = max2(std::fabs(v[0]), std::fabs(v[1]));
+    maxVel = (maxVel > c) ? maxVel : c;
+  }
+  if (maxv_out) *maxv_out = maxVel;
+  return clampf(m.dx[0] / maxVel, 0.001f, 0.1f);
+}
+
+// LiquidModel<2>::ComputeDivergence, LiquidModel.cpp:411-426 (+ L-inf)
+void divergence2(const Model2& m, double* mean, double* linf, int* count) {
+  float dv = 0.0f, mx = 0.0f;
+  int cnt = 0;
+  for (int xi = 0; xi < m.N[0]; ++xi)
+    for (int yi = 0; yi < m.N[1]; ++yi) {
+      if (m.liquidPhi(xi, yi) > 0.0f) continue;
+      ++cnt;
+      float d = m.lwx(xi + 1, yi) * m.vx(xi + 1, yi) - m.lwx(xi, yi) * m.vx(xi, yi);
+      d += m.lwy(xi, yi + 1) * m.vy(xi, yi + 1) - m.lwy(xi, yi) * m.vy(xi, yi);
+      d /= m.dx[0];
+      dv += d;
+      mx = std::max(mx, std::fabs(d));
+    }
+  if (mean) *mean = cnt ? (double)(dv / (float)cnt) : 0.0;
+  if (linf) *linf = mx;
+  if (count) *count = cnt;
+}
+
+// ParticleSystem<2>::SpawnParticles (ParticleSystem.cpp:23-42) at LiquidModel<2>::Init's box
+// (:44-49): spacing radius / 2 from min + 0.3 (max - min); at most numParticles (the break
+// leaves only the y loop)
+void spawn2(Model2& m) {
+  const float rad = m.dx[0];
+  float lo[2], hi[2];
+  for (int a = 0; a < 2; ++a) {
+    const float off = 0.3f * (m.mx[a] - m.mn[a]);
+    lo[a] = m.mn[a] + off;
+    hi[a] = m.mx[a] - off;
+  }
+  const float interval = (float)(0.5 * (double)rad);
+  const int nx = (int)((hi[0] - lo[0]) / interval), ny = (int)((hi[1] - lo[1]) / interval);
+  const i64 cap = m.prm.num_particles;
+  const i64 n = std::min((i64)nx * ny, cap);
+  m.P = n;
+  m.pos.assign(2 * n, 0.f);
+  m.vel.assign(2 * n, 0.f);
+  m.C.assign(4 * n, 0.f);
+  m.radius.assign(n, rad);
+  for (int xi = 0; xi < nx; ++xi)
+    for (int yi = 0; yi < ny; ++yi) {
+      const i64 index = (i64)xi * ny + yi;
+      if (index >= n) break;
+      m.pos[2 * index] = (float)xi * interval + lo[0];
+      m.pos[2 * index + 1] = (float)yi * interval + lo[1];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+void* orc2_create(const orc_params* prm) {
+  Model2* m = new Model2();
+  m->prm = *prm;
+  if (prm->nsdf > 0 && prm->sdf) m->sdf.assign(prm->sdf, prm->sdf + 12 * prm->nsdf);
+  m->prm.sdf = nullptr;
+  for (int a = 0; a < 2; ++a) { m->N[a] = prm->N[a]; m->mn[a] = prm->mn[a]; m->mx[a] = prm->mx[a]; }
+  model2_alloc(*m);
+  return m;
+}
+void orc2_destroy(void* h) { delete (Model2*)h; }
+void orc2_init(void* h) {
+  Model2* m = (Model2*)h;
+  compute_solid_phi2(*m);
+  spawn2(*m);
+  m->time = 0.0;
+}
+void orc2_set_sdf(void* h, const float* prog, int nnodes) {
+  ((Model2*)h)->sdf.assign(prog, prog + 12 * (size_t)nnodes);
+}
+void orc2_compute_solid_phi(void* h) { compute_solid_phi2(*(Model2*)h); }
+i64 orc2_num_particles(void* h) { return ((Model2*)h)->P; }
+// particles: pos (2P), vel (2P), C (4P, column-major 2x2); dir 0 get, 1 set (resizes)
+void orc2_particles(void* h, float* pos, float* vel, float* C, i64 n, int dir) {
+  Model2* m = (Model2*)h;
+  if (dir == 1) {
+    m->P = n;
+    m->pos.assign(pos, pos + 2 * n);
+    m->vel.assign(2 * n, 0.f);
+    m->C.assign(4 * n, 0.f);
+    if (vel) m->vel.assign(vel, vel + 2 * n);
+    if (C) m->C.assign(C, C + 4 * n);
+    m->radius.assign(n, m->dx[0]);
+    return;
+  }
+  std::copy(m->pos.begin(), m->pos.end(), pos);
+  if (vel) std::copy(m->vel.begin(), m->vel.end(), vel);
+  if (C) std::copy(m->C.begin(), m->C.end(), C);
+}
+// fields: 0 vx, 1 vy, 3 tempVx, 4 tempVy, 6 liquidWx, 7 liquidWy, 9 pressure, 10 liquidPhi,
+// 11 solidPhi, 12 isValid (uint8), 13 rhs; dir 0 get, 1 set
+int orc2_field(void* h, int f, void* data, size_t n, int dir) {
+  Model2* m = (Model2*)h;
+  if (f == 12) {
+    if (n != m->isValid.size()) return -1;
+    if (dir) std::memcpy(m->isValid.data(), data, n);
+    else std::memcpy(data, m->isValid.data(), n);
+    return 0;
+  }
+  A2* a = nullptr;
+  switch (f) {
+    case 0: a = &m->vx; break;
+    case 1: a = &m->vy; break;
+    case 3: a = &m->tvx; break;
+    case 4: a = &m->tvy; break;
+    case 6: a = &m->lwx; break;
+    case 7: a = &m->lwy; break;
+    case 9: a = &m->pressure; break;
+    case 10: a = &m->liquidPhi; break;
+    case 11: a = &m->solidPhi; break;
+    case 13: a = &m->rhs; break;
+    default: return -1;
+  }
+  if (n != a->v.size()) return -1;
+  if (dir) std::memcpy(a->v.data(), data, n * sizeof(float));
+  else std::memcpy(data, a->v.data(), n * sizeof(float));
+  return 0;
+}
+void orc2_clear_grid(void* h) { clear_grid2(*(Model2*)h); }
+void orc2_compute_liquid_phi(void* h) { compute_liquid_phi2(*(Model2*)h); }
+void orc2_p2g(void* h, int order) { p2g2(*(Model2*)h, order); }
+void orc2_apply_gravity(void* h, float dt) { apply_gravity2(*(Model2*)h, dt); }
+int orc2_solve_pressure(void* h, float dt, orc_solve_stats* st) { return solve2(*(Model2*)h, dt, st); }
+void orc2_velocity_update(void* h, float dt) { velocity_update2(*(Model2*)h, dt); }
+void orc2_extrapolate(void* h) { extrapolate2(*(Model2*)h); }
+void orc2_g2p(void* h) { g2p2(*(Model2*)h); }
+void orc2_advect(void* h, float dt) { advect2(*(Model2*)h, dt); }
+void orc2_correct_volume(void* h, float dt) { correct_volume2(*(Model2*)h, dt); }
+float orc2_get_cfl(void* h, float* maxv) { return get_cfl2(*(Model2*)h, maxv); }
+void orc2_divergence(void* h, double* mean, double* linf, int* count) { divergence2(*(Model2*)h, mean, linf, count); }
+void orc2_set_time(void* h, double t) { ((Model2*)h)->time = t; }
+double orc2_time(void* h) { return ((Model2*)h)->time; }
+// LiquidModel<2>: PreStep (:51-55), Step (:57-66), no PostStep work without _DEBUG
+int orc2_step(void* h, float dt, orc_solve_stats* st) {
+  Model2* m = (Model2*)h;
+  clear_grid2(*m);
+  compute_liquid_phi2(*m);
+  p2g2(*m, m->prm.p2g_order);
+  apply_gravity2(*m, dt);
+  if (m->P > 0) {
+    solve2(*m, dt, st);
+    velocity_update2(*m, dt);
+  }
+  extrapolate2(*m);
+  g2p2(*m);
+  advect2(*m, dt);
+  correct_volume2(*m, dt);
+  m->time += (double)dt;
+  return 0;
+}
+}  // extern "C"

@@ -30,6 +30,7 @@ class Params(C.Structure):
         ("tank_amp", C.c_float), ("tank_freq", C.c_float), ("tank_axis", C.c_int),
         ("nsdf", C.c_int), ("sdf", C.POINTER(C.c_float)),
         ("correct_volume", C.c_int), ("correct_cycle", C.c_int), ("seed", C.c_ulonglong),
+        ("num_particles", C.c_longlong),
     ]
 
 
@@ -119,8 +120,8 @@ def sdf_program(node):
     def emit(n):
         t = n["type"]
         if t in ("Rect", "Sphere"):
-            c = n.get("center", [0, 0, 0])
-            h = n.get("halfExtent", [0, 0, 0])
+            c = list(n.get("center", [0, 0, 0])) + [0.0] * 3  # 2D trees: z ignored
+            h = list(n.get("halfExtent", [0, 0, 0])) + [0.0] * 3
             out.append([0 if t == "Rect" else 1, c[0], c[1], c[2], h[0], h[1], h[2],
                         n.get("radius", 0.0), 1.0 if n.get("useInside", False) else 0.0, 0, 0, 0])
             return
@@ -147,6 +148,8 @@ def params_from_config(cfg, dot_mode=1, p2g_order=1, smoother=None):
     lq = cfg["LiquidParameter"]
     p = Params()
     for a in range(3):
+        p.N[a], p.mn[a], p.mx[a] = 1, 0.0, 1.0
+    for a in range(len(cfg["gridDimension"])):
         p.N[a] = int(cfg["gridDimension"][a])
         p.mn[a] = float(cfg["minCoord"][a])
         p.mx[a] = float(cfg["maxCoord"][a])
@@ -177,6 +180,7 @@ def params_from_config(cfg, dot_mode=1, p2g_order=1, smoother=None):
     p.correct_volume = 1 if cfg.get("correctVolume", False) else 0
     p.correct_cycle = int(lq["correctCycle"])
     p.seed = int(cfg.get("seed", 0))
+    p.num_particles = int(lq["numParticles"])
     return p
 
 
@@ -397,3 +401,141 @@ def scalar_lib():
     L.orc_sdf_eval.argtypes = [fp, C.c_int, C.c_float, C.c_float, C.c_float]
     L.orc_barycentric.argtypes = [C.c_float, C.c_int, C.c_int, C.POINTER(C.c_int), fp]
     return L
+
+
+class Oracle2D:
+    """Stage-level handle on the CPU restatement of LiquidModel<2> / CGSolver<2>
+    (LiquidModel.cpp:10-474, PressureSolver.cpp:9-418)."""
+
+    def __init__(self, cfg, dot_mode=1, p2g_order=1):
+        self.cfg = load_config(cfg)
+        assert len(self.cfg["gridDimension"]) == 2
+        self.L = lib()
+        L = self.L
+        vp, fp, dp, ip = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_int)
+        L.orc2_create.restype = vp
+        L.orc2_create.argtypes = [C.POINTER(Params)]
+        for name, args in (("orc2_destroy", [vp]), ("orc2_init", [vp]), ("orc2_clear_grid", [vp]),
+                           ("orc2_compute_liquid_phi", [vp]), ("orc2_compute_solid_phi", [vp]),
+                           ("orc2_p2g", [vp, C.c_int]), ("orc2_apply_gravity", [vp, C.c_float]),
+                           ("orc2_velocity_update", [vp, C.c_float]), ("orc2_extrapolate", [vp]),
+                           ("orc2_g2p", [vp]), ("orc2_advect", [vp, C.c_float]),
+                           ("orc2_correct_volume", [vp, C.c_float]), ("orc2_set_time", [vp, C.c_double]),
+                           ("orc2_set_sdf", [vp, fp, C.c_int]), ("orc2_divergence", [vp, dp, dp, ip]),
+                           ("orc2_particles", [vp, fp, fp, fp, C.c_longlong, C.c_int])):
+            getattr(L, name).argtypes = args
+            getattr(L, name).restype = None
+        L.orc2_solve_pressure.argtypes = [vp, C.c_float, C.POINTER(SolveStats)]
+        L.orc2_step.argtypes = [vp, C.c_float, C.POINTER(SolveStats)]
+        L.orc2_field.argtypes = [vp, C.c_int, vp, C.c_size_t, C.c_int]
+        L.orc2_get_cfl.restype = C.c_float
+        L.orc2_get_cfl.argtypes = [vp, fp]
+        L.orc2_num_particles.restype = C.c_longlong
+        L.orc2_num_particles.argtypes = [vp]
+        L.orc2_time.restype = C.c_double
+        L.orc2_time.argtypes = [vp]
+        prm = params_from_config(self.cfg, dot_mode=dot_mode, p2g_order=p2g_order)
+        self._sdf = sdf_program(self.cfg.get("Boundary", DEFAULT_BOUNDARY))
+        prm.nsdf = len(self._sdf) // 12
+        prm.sdf = _fptr(self._sdf)
+        self.N = [int(v) for v in self.cfg["gridDimension"]]
+        self.h = L.orc2_create(C.byref(prm))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc2_destroy(self.h)
+            self.h = None
+
+    def field_shape(self, f):
+        N = self.N
+        return {0: (N[0] + 1, N[1]), 1: (N[0], N[1] + 1), 3: (N[0] + 1, N[1]), 4: (N[0], N[1] + 1),
+                6: (N[0] + 1, N[1]), 7: (N[0], N[1] + 1), 9: tuple(N), 10: tuple(N), 11: (N[0] + 1, N[1] + 1),
+                12: (N[0] + 1, N[1] + 1), 13: tuple(N)}[f]
+
+    def get(self, f):
+        a = np.empty(self.field_shape(f), np.uint8 if f == 12 else np.float32)
+        assert self.L.orc2_field(self.h, f, a.ctypes.data, a.size, 0) == 0
+        return a
+
+    def set(self, f, a):
+        a = np.ascontiguousarray(a, np.uint8 if f == 12 else np.float32).reshape(self.field_shape(f))
+        assert self.L.orc2_field(self.h, f, a.ctypes.data, a.size, 1) == 0
+
+    def init(self):
+        self.L.orc2_init(self.h)
+
+    def particles(self):
+        n = int(self.L.orc2_num_particles(self.h))
+        pos = np.empty((n, 2), np.float32)
+        vel = np.empty((n, 2), np.float32)
+        Cm = np.empty((n, 2, 2), np.float32)  # [particle, column, row]
+        self.L.orc2_particles(self.h, _fptr(pos), _fptr(vel), _fptr(Cm), n, 0)
+        return pos, vel, Cm
+
+    def set_particles(self, pos, vel=None, Cm=None):
+        pos = np.ascontiguousarray(pos, np.float32)
+        v = None if vel is None else np.ascontiguousarray(vel, np.float32)
+        c = None if Cm is None else np.ascontiguousarray(Cm, np.float32)
+        self.L.orc2_particles(self.h, _fptr(pos), None if v is None else _fptr(v), None if c is None else _fptr(c),
+                              len(pos), 1)
+        self._keep = (pos, v, c)
+
+    def set_boundary(self, node):
+        prog = sdf_program(node)
+        self.L.orc2_set_sdf(self.h, _fptr(prog), len(prog) // 12)
+
+    def compute_solid_phi(self):
+        self.L.orc2_compute_solid_phi(self.h)
+
+    def clear_grid(self):
+        self.L.orc2_clear_grid(self.h)
+
+    def compute_liquid_phi(self):
+        self.L.orc2_compute_liquid_phi(self.h)
+
+    def p2g(self, order=1):
+        self.L.orc2_p2g(self.h, order)
+
+    def apply_gravity(self, dt):
+        self.L.orc2_apply_gravity(self.h, float(dt))
+
+    def solve_pressure(self, dt):
+        st = SolveStats()
+        self.L.orc2_solve_pressure(self.h, float(dt), C.byref(st))
+        return st
+
+    def velocity_update(self, dt):
+        self.L.orc2_velocity_update(self.h, float(dt))
+
+    def extrapolate(self):
+        self.L.orc2_extrapolate(self.h)
+
+    def g2p(self):
+        self.L.orc2_g2p(self.h)
+
+    def advect(self, dt):
+        self.L.orc2_advect(self.h, float(dt))
+
+    def correct_volume(self, dt):
+        self.L.orc2_correct_volume(self.h, float(dt))
+
+    def get_cfl(self):
+        mv = C.c_float()
+        return float(self.L.orc2_get_cfl(self.h, C.byref(mv))), float(mv.value)
+
+    def divergence(self):
+        mean, linf, cnt = C.c_double(), C.c_double(), C.c_int()
+        self.L.orc2_divergence(self.h, C.byref(mean), C.byref(linf), C.byref(cnt))
+        return mean.value, linf.value, cnt.value
+
+    @property
+    def time(self):
+        return float(self.L.orc2_time(self.h))
+
+    def set_time(self, t):
+        self.L.orc2_set_time(self.h, float(t))
+
+    def step(self, dt):
+        st = SolveStats()
+        self.L.orc2_step(self.h, float(dt), C.byref(st))
+        return st
