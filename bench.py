@@ -217,6 +217,59 @@ def warm_start_eval(grid, scenes, steps):
     return out
 
 
+def bench_2d(args, cfg):
+    """the 2D model (LiquidModel<2>): steps/s of the library's native loop, and the oracle's 2D
+    restatement (single-threaded) on a bounded sample of the same workload"""
+    import numpy as np
+
+    import liquidhairinteration_amd as lh
+
+    sc = lh.Scene2D(cfg)
+    m = sc.GetLiquidModel()
+    m.Init()
+    P = m.GetParticleSystem().NumParticles()
+    dt = float(cfg.get("firstDt", 0.01))
+    if args.warmup > 0:
+        dt, _ = m.RunSteps(args.warmup, dt)
+    t0 = time.perf_counter()
+    dt, last = m.RunSteps(args.steps, dt)
+    el = time.perf_counter() - t0
+    sc.close()
+    sp = cfg["SolverParameter"]
+    out = {
+        "metric": "liquid steps/s, 2D model (LiquidModel<2>)", "value": round(args.steps / el, 3),
+        "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(el * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (the reference's lattice spawn)",
+        "config": {"workload": os.path.basename(args.config).replace(".json", ""), "grid": cfg["gridDimension"],
+                   "particles": P, "solver": "MGPCG L=%d pre/post=%d/%d coarsest=%d %s, tol %g %s" % (
+                       sp["numLevel"], sp["numPreSmooth"], sp["numPostSmooth"], sp["numFinalSmooth"],
+                       sp.get("smoother", "reference"), sp["tolerance"], sp.get("toleranceMode", "absolute"))},
+        "last_step": {"pcg_iterations": last["iterations"], "converged": last["converged"]},
+        "roofline": None, "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle2D
+
+        orc = Oracle2D(cfg, dot_mode=1, p2g_order=1)
+        orc.init()
+        d = np.float32(cfg.get("firstDt", 0.01))
+        for _ in range(args.warmup):  # the same warm-up steps as the GPU run, untimed
+            orc.step(d)
+            d = np.float32(orc.get_cfl()[0])
+        n, t0 = 0, time.perf_counter()
+        while n < args.cpu_steps or (time.perf_counter() - t0 < 10.0 and n < 10 * args.cpu_steps):
+            orc.step(d)
+            d = np.float32(orc.get_cfl()[0])
+            n += 1
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n / el, 4), "unit": "steps/s", "cores": 1, "kind": "port",
+                               "sample": "%d step(s) after the warm-up steps, FP32 oracle restatement of the 2D "
+                                         "model, one thread (%.1f s)" % (n, el)}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -250,6 +303,9 @@ def main():
     if args.pmc_calibrate:
         pmc_calibration_copy()
     cfg = json.load(open(args.config))
+    if len(cfg["gridDimension"]) == 2:
+        bench_2d(args, cfg)
+        return
     scene = lh.Scene(cfg)
     model = scene.GetLiquidModel()
     model.Init()

@@ -475,13 +475,22 @@ __device__ void block_sum(double (&v)[NV], double* __restrict__ part) {
   if (threadIdx.x == 0)
     for (int k = 0; k < NV; ++k) part[(size_t)k * gridDim.x + blockIdx.x] = s[k][0];
 }
-// sums of the partials in block order (one thread: fixed order)
-__global__ void k_sum_partials(const double* __restrict__ part, int nb, int nv, double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// sums of the partials: one block, thread t adds partials t, t + 256, ... in order, then a
+// fixed tree (deterministic)
+__global__ void __launch_bounds__(kB) k_sum_partials(const double* __restrict__ part, int nb, int nv,
+                                                     double* __restrict__ out) {
+  __shared__ double s[kB];
   for (int k = 0; k < nv; ++k) {
-    double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += part[(size_t)k * nb + b];
-    out[k] = s;
+    double a = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kB) a += part[(size_t)k * nb + b];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int st = kB / 2; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = s[0];
+    __syncthreads();
   }
 }
 
@@ -583,14 +592,84 @@ __global__ void k_velocity_update(float* __restrict__ vx, float* __restrict__ vy
 
 // LiquidModel<2>::Extrapolate for one component, LiquidModel.cpp:341-409: one workgroup walks
 // the anti-diagonals t = 2x + y of the range x in [1, xhi], y in [1, yhi] five times; sweep k
-// reads src (v, tv, v, ...) and writes the other buffer, as std::swap(temp, v) does
+// reads src (v, tv, v, ...) and writes the other buffer, as std::swap(temp, v) does.  With
+// kLds the validity of every node lives in LDS as bits (row x: words [x * wpr, (x + 1) * wpr)),
+// loaded at the start and written back at the end; a step touches one face per row, so no two
+// threads of a step write the same word
+template <bool kLds>
 __global__ void __launch_bounds__(1024) k_extrapolate(float* __restrict__ v, float* __restrict__ tv,
                                                       uint8_t* __restrict__ valid, const float* __restrict__ lw,
-                                                      int d1, int vw, int xhi, int yhi) {
-  // d1: the component's face array y-size; vw: isValid row length (N1 + 1)
-  for (int x = 1 + (int)threadIdx.x; x <= xhi; x += blockDim.x)
-    for (int y = 1; y <= yhi; ++y) valid[(size_t)x * vw + y] = lw[(size_t)x * d1 + y] > 0.0f ? 1 : 0;
+                                                      int d1, int vw, int vrows, int xhi, int yhi) {
+  // d1: the component's face array y-size; vw: isValid row length (N1 + 1); vrows: N0 + 1
+  extern __shared__ uint32_t bits[];
+  const int wpr = (vw + 31) >> 5;
+  auto get = [&](int x, int y) -> bool {
+    if (kLds) return (bits[x * wpr + (y >> 5)] >> (y & 31)) & 1u;
+    return valid[(size_t)x * vw + y] != 0;
+  };
+  auto set = [&](int x, int y) {
+    if (kLds) bits[x * wpr + (y >> 5)] |= 1u << (y & 31);
+    else valid[(size_t)x * vw + y] = 1;
+  };
+  if (kLds) {
+    for (int w = threadIdx.x; w < vrows * wpr; w += blockDim.x) {
+      const int x = w / wpr, y0 = (w - x * wpr) * 32;
+      uint32_t b = 0;
+      for (int k = 0; k < 32 && y0 + k < vw; ++k) {
+        const int y = y0 + k;
+        bool val = valid[(size_t)x * vw + y] != 0;
+        if (x >= 1 && x <= xhi && y >= 1 && y <= yhi) val = lw[(size_t)x * d1 + y] > 0.0f;
+        b |= (val ? 1u : 0u) << k;
+      }
+      bits[w] = b;
+    }
+  } else {
+    for (int x = 1 + (int)threadIdx.x; x <= xhi; x += blockDim.x)
+      for (int y = 1; y <= yhi; ++y) valid[(size_t)x * vw + y] = lw[(size_t)x * d1 + y] > 0.0f ? 1 : 0;
+  }
   __syncthreads();
+  if (kLds && xhi <= (int)blockDim.x) {
+    // thread x - 1 owns row x: at step t it handles y = t - 2x, so its faces come in y order,
+    // and the ten source values of its next face (the face and its 3 x 3 block; the source is
+    // read-only during a sweep) are loaded before the barrier and used after it
+    const int x = 1 + (int)threadIdx.x;
+    const bool row = x <= xhi;
+    for (int it = 0; it < 5; ++it) {
+      const float* __restrict__ src = (it & 1) ? tv : v;
+      float* __restrict__ dst = (it & 1) ? v : tv;
+      float pf[3][3];
+      int ynext = 1;
+      if (row)
+        for (int dx = -1; dx <= 1; ++dx)
+          for (int dy = -1; dy <= 1; ++dy) pf[dx + 1][dy + 1] = src[(size_t)(x + dx) * d1 + (ynext + dy)];
+      for (int t = 3; t <= 2 * xhi + yhi; ++t) {
+        const int y = t - 2 * x;
+        if (row && y == ynext && y <= yhi) {
+          float out = pf[1][1];
+          if (!get(x, y)) {
+            float sum = 0.0f;
+            int cnt = 0;
+            for (int dx = -1; dx <= 1; ++dx)
+              for (int dy = -1; dy <= 1; ++dy)
+                if (get(x + dx, y + dy)) {
+                  sum += pf[dx + 1][dy + 1];
+                  ++cnt;
+                }
+            if (cnt > 0) {
+              out = sum / (float)cnt;
+              set(x, y);
+            }
+          }
+          dst[(size_t)x * d1 + y] = out;
+          ynext = y + 1;
+          if (ynext <= yhi)
+            for (int dx = -1; dx <= 1; ++dx)
+              for (int dy = -1; dy <= 1; ++dy) pf[dx + 1][dy + 1] = src[(size_t)(x + dx) * d1 + (ynext + dy)];
+        }
+        __syncthreads();
+      }
+    }
+  } else
   for (int it = 0; it < 5; ++it) {
     const float* __restrict__ src = (it & 1) ? tv : v;
     float* __restrict__ dst = (it & 1) ? v : tv;
@@ -600,18 +679,18 @@ __global__ void __launch_bounds__(1024) k_extrapolate(float* __restrict__ v, flo
         const int y = t - 2 * x;
         const size_t f = (size_t)x * d1 + y;
         float out = src[f];
-        if (!valid[(size_t)x * vw + y]) {
+        if (!get(x, y)) {
           float sum = 0.0f;
           int cnt = 0;
           for (int dx = -1; dx <= 1; ++dx)
             for (int dy = -1; dy <= 1; ++dy)
-              if (valid[(size_t)(x + dx) * vw + (y + dy)]) {
+              if (get(x + dx, y + dy)) {
                 sum += src[(size_t)(x + dx) * d1 + (y + dy)];
                 ++cnt;
               }
           if (cnt > 0) {
             out = sum / (float)cnt;
-            valid[(size_t)x * vw + y] = 1;
+            set(x, y);
           }
         }
         dst[f] = out;
@@ -619,6 +698,12 @@ __global__ void __launch_bounds__(1024) k_extrapolate(float* __restrict__ v, flo
       __syncthreads();
     }
   }
+  if (kLds)
+    for (int w = threadIdx.x; w < vrows * wpr; w += blockDim.x) {
+      const int x = w / wpr, y0 = (w - x * wpr) * 32;
+      const uint32_t b = bits[w];
+      for (int k = 0; k < 32 && y0 + k < vw; ++k) valid[(size_t)x * vw + y0 + k] = (uint8_t)((b >> k) & 1u);
+    }
 }
 
 // LiquidModel<2>::Grid2Particle, LiquidModel.cpp:188-236, and AdvectParticle (:315-339), and the
@@ -977,7 +1062,7 @@ int bin(Ctx2& c) {
 }
 
 double reduce(Ctx2& c, int nv, int nb, bool& good) {
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, c.s, c.partial, nb, nv, c.dres);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kB), 0, c.s, c.partial, nb, nv, c.dres);
   hipMemcpyAsync(c.h_pin, c.dres, nv * sizeof(double), hipMemcpyDeviceToHost, c.s);
   good = ok(c, hipStreamSynchronize(c.s), "reduce");
   return c.h_pin[0];
@@ -1107,14 +1192,25 @@ int solve(Ctx2& c, float dt, lh_solve_stats* st) {
 
 void extrapolate(Ctx2& c) {
   if (c.P == 0) return;
-  const int vw = c.g.N1 + 1;
+  const int vw = c.g.N1 + 1, vrows = c.g.N0 + 1;
+  const size_t lds = (size_t)vrows * ((vw + 31) / 32) * sizeof(uint32_t);
+  const bool in_lds = lds <= 64 * 1024;  // validity bits in LDS up to 64 KiB (grids to ~1400^2)
   // x component: x in [1, N0 - 1], y in [1, N1 - 2]; y component: x in [1, N0 - 2], y in [1, N1 - 1]
-  hipLaunchKernelGGL(k_extrapolate, dim3(1), dim3(1024), 0, c.s, c.vx, c.tvx, c.valid, c.lwx, c.g.N1, vw, c.g.N0 - 1,
-                     c.g.N1 - 2);
-  std::swap(c.vx, c.tvx);  // five sweeps: the result is in the other buffer
-  hipLaunchKernelGGL(k_extrapolate, dim3(1), dim3(1024), 0, c.s, c.vy, c.tvy, c.valid, c.lwy, c.g.N1 + 1, vw,
-                     c.g.N0 - 2, c.g.N1 - 1);
-  std::swap(c.vy, c.tvy);
+  for (int a = 0; a < 2; ++a) {
+    float* v = a == 0 ? c.vx : c.vy;
+    float* tv = a == 0 ? c.tvx : c.tvy;
+    const float* lw = a == 0 ? c.lwx : c.lwy;
+    const int d1 = a == 0 ? c.g.N1 : c.g.N1 + 1;
+    const int xhi = a == 0 ? c.g.N0 - 1 : c.g.N0 - 2, yhi = a == 0 ? c.g.N1 - 2 : c.g.N1 - 1;
+    if (in_lds)
+      hipLaunchKernelGGL(k_extrapolate<true>, dim3(1), dim3(1024), lds, c.s, v, tv, c.valid, lw, d1, vw, vrows, xhi,
+                         yhi);
+    else
+      hipLaunchKernelGGL(k_extrapolate<false>, dim3(1), dim3(1024), 0, c.s, v, tv, c.valid, lw, d1, vw, vrows, xhi,
+                         yhi);
+    if (a == 0) std::swap(c.vx, c.tvx);  // five sweeps: the result is in the other buffer
+    else std::swap(c.vy, c.tvy);
+  }
 }
 
 void clear(Ctx2& c) {  // Grid<2>::Clear, Grid.cpp:52-61
