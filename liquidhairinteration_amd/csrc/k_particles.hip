@@ -173,7 +173,7 @@ struct P2GP {
 // that no particle can reach writes its closed-form value and leaves; the test
 // reads the two end offsets of every (hx, hy) bin row of the tile's reach.
 constexpr int kTX = 1, kTY = 4, kTZ = 64;
-// P2G tiles: 1 x kPTY x kPTZ faces, one per thread (kPTY * kPTZ == kBlock)
+// P2G tiles: kMX x kPTY x kPTZ faces, one (y, z) row of kMX x-faces per thread
 constexpr int kPTY = 4, kPTZ = 64;
 
 __device__ bool tile_has_particles(const int* __restrict__ start, const int lo[3], const int hi[3], int H1, int H2) {
@@ -190,24 +190,28 @@ __device__ bool tile_has_particles(const int* __restrict__ start, const int lo[3
 
 // Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8), each XCD with
 // its own L2.  Tiles adjacent in x read most of the same particles, so tiles are numbered x
-// fastest and dealt to the XCDs in chunks of kXChunk consecutive x-tiles: chunk q runs on
-// XCD q % 8, which keeps a chunk's shared particles in one L2 while the chunks of a busy
-// region still spread over all XCDs.  The grid is a multiple of 8 kXChunk blocks (tile_grid);
-// blocks past the last tile return false.
+// fastest and dealt to the XCDs in chunks of kXChunk consecutive x-tiles, which keeps a
+// chunk's shared particles in one L2.  Each group of 8 consecutive chunks is spread over the
+// 8 XCDs with a rotation hashed from the group: a fixed chunk-to-XCD map repeats with the
+// rows whenever a row is a multiple of 8 chunks long, and then the XCDs whose x stripes
+// hold the liquid get most of the work.  The grid is a multiple of 8 kXChunk blocks
+// (tile_grid); blocks past the last tile return false.
 constexpr int kXcds = 8, kXChunk = 4;
 inline unsigned tile_grid(size_t ntiles) {
   return (unsigned)(div_up(ntiles, (size_t)kXcds * kXChunk) * kXcds * kXChunk);
 }
 
-template <int TY = kTY, int TZ = kTZ>
+template <int TX = kTX, int TY = kTY, int TZ = kTZ>
 __device__ __forceinline__ bool tile_coords(int d0, int d1, int d2, int c[3], int c0[3]) {
-  static_assert(TY * TZ == kBlock, "one cell or face per thread");
-  const int ntx = (d0 + kTX - 1) / kTX, nty = (d1 + TY - 1) / TY, ntz = (d2 + TZ - 1) / TZ;
+  static_assert(TY * TZ == kBlock, "one (y, z) per thread");
+  const int ntx = (d0 + TX - 1) / TX, nty = (d1 + TY - 1) / TY, ntz = (d2 + TZ - 1) / TZ;
   const int x = (int)(blockIdx.x % kXcds), j = (int)(blockIdx.x / kXcds);
-  const int t = ((j / kXChunk) * kXcds + x) * kXChunk + j % kXChunk;
+  const unsigned grp = (unsigned)(j / kXChunk);
+  const int rot = (int)((grp * 2654435761u) >> 29);  // 0..7
+  const int t = ((int)grp * kXcds + ((x + rot) & (kXcds - 1))) * kXChunk + j % kXChunk;
   if (t >= ntx * nty * ntz) return false;
   const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
-  c0[0] = bx * kTX;
+  c0[0] = bx * TX;
   c0[1] = by * TY;
   c0[2] = bz * TZ;
   c[0] = c0[0];
@@ -216,22 +220,18 @@ __device__ __forceinline__ bool tile_coords(int d0, int d1, int d2, int c[3], in
   return true;
 }
 
-// one particle record into a face's weight and momentum sums (LiquidModel.cpp:613-721):
-// grid coordinates (x - min) / dx, by an exact power-of-two multiply when kPow2
-template <bool kPow2>
-__device__ __forceinline__ void p2g_acc4(float4 A, float4 B, float mn0, float mn1, float mn2, float dx0, float dx1,
-                                         float dx2, float ccx, float ccy, float ccz, float fx, float fy, float fz,
-                                         float& ws, float& vs) {
-  const float gx = kPow2 ? (A.x - mn0) * dx0 : (A.x - mn0) / dx0;
-  const float gy = kPow2 ? (A.y - mn1) * dx1 : (A.y - mn1) / dx1;
-  const float gz = kPow2 ? (A.z - mn2) * dx2 : (A.z - mn2) / dx2;
-  const float wgt = quad_w(gx - ccx) * quad_w(gy - ccy) * quad_w(gz - ccz);
-  if (wgt == 0.f) return;
-  const float d0v = A.x - fx, d1v = A.y - fy, d2v = A.z - fz;
-  const float dot = sum3(B.x * d0v, B.y * d1v, B.z * d2v);
-  ws += wgt;
-  vs += wgt * (A.w + dot);
-}
+// ------------------------------------------------------------------ x-marching P2G
+// A 256-thread workgroup owns kMX x-faces x 4 y x 64 z: each thread one (y, z) row of kMX
+// faces.  It marches over the x half-slabs of the tile's reach in ascending order and adds
+// each slab's particles to the (at most 3) consecutive x-faces whose reach holds the slab,
+// kept in a sliding window of registers: one read of a particle record serves those faces,
+// which share its y and z weights and the y and z part of the affine term.  Each face still
+// sums its slabs in ascending x and, within a slab, its y columns and z bins in order: the
+// bin order the oracle sums in.  At 256^3 (6.5 M particles) 4 x-faces per tile balance the
+// shared work against the number of busy waves: with 8 or 16 too few tiles hold particles
+// to fill the CUs.
+constexpr int kMX = 4;
+
 // record index of the t-th particle of a face's run of up to 6 bin columns: column j holds
 // the t in [cum[j-1], cum[j]), at records t + off[j]
 __device__ __forceinline__ int p2g_rec(int t, const int off[6], const int cum[6]) {
@@ -240,18 +240,76 @@ __device__ __forceinline__ int p2g_rec(int t, const int off[6], const int cum[6]
   for (int j = 4; j >= 0; --j) o = (t < cum[j]) ? off[j] : o;
   return t + o;
 }
-// Gather form of Particle2Grid for one velocity component: one face per thread, summing
-// the particles of the half-cell bins around it in bin order (the order the oracle follows).
-// For each x column of bins the (up to 6) y columns are concatenated into one run, so a
-// wave's lanes wait for the longest run of 6 columns instead of the longest of each column.
+
+// one particle record (x y z v_a | C.col(a)) into the weight and momentum sums of the first
+// NF faces of the window (LiquidModel.cpp:613-721): weight quad(gx - ccx) quad(gy - ccy)
+// quad(gz - ccz) with grid coordinates (x - min) / dx (an exact power-of-two multiply when
+// kPow2), momentum weight * (v + C.col(a) . (x - face)); zero weights add nothing
+template <bool kPow2, int NF>
+__device__ __forceinline__ void p2g_acc(float4 A, float4 B, float mn0, float mn1, float mn2, float dx0, float dx1,
+                                        float dx2, float ccy, float ccz, float fy, float fz, const float ccx[3],
+                                        const float fx[3], float ws[3], float vs[3]) {
+  const float gx = kPow2 ? (A.x - mn0) * dx0 : (A.x - mn0) / dx0;
+  const float gy = kPow2 ? (A.y - mn1) * dx1 : (A.y - mn1) / dx1;
+  const float gz = kPow2 ? (A.z - mn2) * dx2 : (A.z - mn2) / dx2;
+  const float wy = quad_w(gy - ccy), wz = quad_w(gz - ccz);
+  const float yz = B.y * (A.y - fy) + B.z * (A.z - fz);  // sum3(a, b, c) == a + (b + c)
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    const float wgt = quad_w(gx - ccx[k]) * wy * wz;
+    const float dot = B.x * (A.x - fx[k]) + yz;
+    const bool add = wgt != 0.f;
+    ws[k] = add ? ws[k] + wgt : ws[k];
+    vs[k] = add ? vs[k] + wgt * (A.w + dot) : vs[k];
+  }
+}
+
+// the records t in [0, total) of a lane's run of bin columns in one slab
+template <bool kPow2, int NF>
+__device__ __forceinline__ void p2g_slab(const float4* __restrict__ rec, const int off[6], const int cum[6], int total,
+                                         float mn0, float mn1, float mn2, float dx0, float dx1, float dx2, float ccy,
+                                         float ccz, float fy, float fz, const float ccx[3], const float fx[3],
+                                         float ws[3], float vs[3]) {
+  int t = 0;
+  // four records in flight, then pairs
+  for (; t + 3 < total; t += 4) {
+    const int i0 = p2g_rec(t, off, cum), i1 = p2g_rec(t + 1, off, cum);
+    const int i2 = p2g_rec(t + 2, off, cum), i3 = p2g_rec(t + 3, off, cum);
+    const float4 A0 = rec[2 * (size_t)i0], B0 = rec[2 * (size_t)i0 + 1];
+    const float4 A1 = rec[2 * (size_t)i1], B1 = rec[2 * (size_t)i1 + 1];
+    const float4 A2 = rec[2 * (size_t)i2], B2 = rec[2 * (size_t)i2 + 1];
+    const float4 A3 = rec[2 * (size_t)i3], B3 = rec[2 * (size_t)i3 + 1];
+    p2g_acc<kPow2, NF>(A0, B0, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
+    p2g_acc<kPow2, NF>(A1, B1, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
+    p2g_acc<kPow2, NF>(A2, B2, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
+    p2g_acc<kPow2, NF>(A3, B3, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
+  }
+  for (; t + 1 < total; t += 2) {
+    const int i0 = p2g_rec(t, off, cum), i1 = p2g_rec(t + 1, off, cum);
+    const float4 A0 = rec[2 * (size_t)i0], B0 = rec[2 * (size_t)i0 + 1];
+    const float4 A1 = rec[2 * (size_t)i1], B1 = rec[2 * (size_t)i1 + 1];
+    p2g_acc<kPow2, NF>(A0, B0, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
+    p2g_acc<kPow2, NF>(A1, B1, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
+  }
+  if (t < total) {
+    const int i0 = p2g_rec(t, off, cum);
+    p2g_acc<kPow2, NF>(rec[2 * (size_t)i0], rec[2 * (size_t)i0 + 1], mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz,
+                       ccx, fx, ws, vs);
+  }
+}
+
+// Gather form of Particle2Grid for one velocity component, each face summing the particles
+// of the half-cell bins around it in bin order.  Within a slab the (up to 6) y columns of a
+// face are concatenated into one run, so a wave's lanes wait for the longest run of 6
+// columns instead of the longest of each column.
 template <bool kPow2>
 __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const float4* __restrict__ pack, int64_t cap,
                                               const int* __restrict__ start, P2GP q) {
   const int a = q.a;
   const int d0 = q.g.N[0] + (a == 0), d1 = q.g.N[1] + (a == 1), d2 = q.g.N[2] + (a == 2);
   int c[3], c0[3];
-  if (!tile_coords<kPTY, kPTZ>(d0, d1, d2, c, c0)) return;
-  const int T[3] = {kTX, kPTY, kPTZ};
+  if (!tile_coords<kMX, kPTY, kPTZ>(d0, d1, d2, c, c0)) return;
+  const int T[3] = {kMX, kPTY, kPTZ};
   int blo[3], bhi[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -259,26 +317,63 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const flo
     bhi[k] = min(2 * (c0[k] + T[k] - 1) + ((k == a) ? 2 : 3), q.H[k] - 1);
   }
   const bool any = tile_has_particles(start, blo, bhi, q.H[1], q.H[2]);
-  if (c[0] >= d0 || c[1] >= d1 || c[2] >= d2) return;
-  float ws = 0.f, vs = 0.f;
-  if (any) {
-    const float4* rec = pack + 2 * (size_t)a * cap;  // x y z v_a | C.col(a) 0
-    const float mn0 = q.g.mn[0], mn1 = q.g.mn[1], mn2 = q.g.mn[2];
-    const float dx0 = kPow2 ? q.inv[0] : q.g.dx[0], dx1 = kPow2 ? q.inv[1] : q.g.dx[1],
-                dx2 = kPow2 ? q.inv[2] : q.g.dx[2];
-    int lo[3], hi[3];
+  if (c[1] >= d1 || c[2] >= d2) return;
+  const int xend = min(c0[0] + kMX, d0);
+  // bins [xlo, xhi] of face x along axis 0
+  auto xlo = [&](int x) { return max((a == 0) ? 2 * x - 3 : 2 * x - 2, 0); };
+  auto xhi = [&](int x) { return min(((a == 0) ? 2 * x - 3 : 2 * x - 2) + 5, q.H[0] - 1); };
+  auto finish = [&](int x, float ws, float vs) {
+    float v = !is_small(ws) ? vs / ws : 0.0f;
+    // interior ranges of ApplyGravity (a == 1) and of the tank acceleration
+    if (a == 1 || a == q.tank_axis) {
+      const int cc[3] = {x, c[1], c[2]};
+      bool in = true;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        int h = (k == a) ? q.g.N[k] - 1 : q.g.N[k] - 2;
+        if (cc[k] < 1 || cc[k] > h) in = false;
+      }
+      if (in) {
+        if (a == 1) v += q.gdt;
+        if (a == q.tank_axis) v += q.adt;
+      }
+    }
+    vout[((size_t)x * d1 + c[1]) * d2 + c[2]] = v;
+  };
+  if (!any) {
+    for (int x = c0[0]; x < xend; ++x) finish(x, 0.f, 0.f);
+    return;
+  }
+  const float4* rec = pack + 2 * (size_t)a * cap;  // x y z v_a | C.col(a) 0
+  const float mn0 = q.g.mn[0], mn1 = q.g.mn[1], mn2 = q.g.mn[2];
+  const float dx0 = kPow2 ? q.inv[0] : q.g.dx[0], dx1 = kPow2 ? q.inv[1] : q.g.dx[1],
+              dx2 = kPow2 ? q.inv[2] : q.g.dx[2];
+  int lo[3], hi[3];
+#pragma unroll
+  for (int k = 1; k < 3; ++k) {
+    int l = (k == a) ? 2 * c[k] - 3 : 2 * c[k] - 2;
+    lo[k] = max(l, 0);
+    hi[k] = min(l + 5, q.H[k] - 1);
+  }
+  const float ccy = (a == 1) ? (float)c[1] : (float)c[1] + 0.5f;
+  const float ccz = (a == 2) ? (float)c[2] : (float)c[2] + 0.5f;
+  const float fy = ccy * q.g.dx[1] + mn1, fz = ccz * q.g.dx[2] + mn2;
+  float ws[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f};  // faces cw, cw + 1, cw + 2
+  int cw = c0[0];
+  for (int hx = blo[0]; hx <= bhi[0]; ++hx) {
+    // the faces whose reach holds slab hx: a prefix of the window (the reaches of
+    // consecutive faces are ordered and at most 3 overlap), the same for the whole tile
+    float ccx[3], fx[3];
+    int nf = 0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      int l = (k == a) ? 2 * c[k] - 3 : 2 * c[k] - 2;
-      lo[k] = max(l, 0);
-      hi[k] = min(l + 5, q.H[k] - 1);
+      const int x = cw + k;
+      nf += (x < xend && xlo(x) <= hx && hx <= xhi(x)) ? 1 : 0;
+      ccx[k] = (a == 0) ? (float)x : (float)x + 0.5f;
+      fx[k] = ccx[k] * q.g.dx[0] + mn0;
     }
-    const float ccx = (a == 0) ? (float)c[0] : (float)c[0] + 0.5f;
-    const float ccy = (a == 1) ? (float)c[1] : (float)c[1] + 0.5f;
-    const float ccz = (a == 2) ? (float)c[2] : (float)c[2] + 0.5f;
-    const float fx = ccx * q.g.dx[0] + mn0, fy = ccy * q.g.dx[1] + mn1, fz = ccz * q.g.dx[2] + mn2;
-    for (int hx = lo[0]; hx <= hi[0]; ++hx) {
-      int off[6], cum[6], total = 0;
+    if (nf > 0) {
+      int off[6], cum[6], run = 0;
 #pragma unroll
       for (int j = 0; j < 6; ++j) {
         const int hy = lo[1] + j;
@@ -288,40 +383,29 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const flo
           s0 = start[k0 + lo[2]];
           n = start[k0 + hi[2] + 1] - s0;
         }
-        off[j] = s0 - total;
-        total += n;
-        cum[j] = total;
+        off[j] = s0 - run;
+        run += n;
+        cum[j] = run;
       }
-      int t = 0;
-      for (; t + 1 < total; t += 2) {
-        const int i0 = p2g_rec(t, off, cum), i1 = p2g_rec(t + 1, off, cum);
-        const float4 A0 = rec[2 * (size_t)i0], B0 = rec[2 * (size_t)i0 + 1];
-        const float4 A1 = rec[2 * (size_t)i1], B1 = rec[2 * (size_t)i1 + 1];
-        p2g_acc4<kPow2>(A0, B0, mn0, mn1, mn2, dx0, dx1, dx2, ccx, ccy, ccz, fx, fy, fz, ws, vs);
-        p2g_acc4<kPow2>(A1, B1, mn0, mn1, mn2, dx0, dx1, dx2, ccx, ccy, ccz, fx, fy, fz, ws, vs);
-      }
-      if (t < total) {
-        const int i0 = p2g_rec(t, off, cum);
-        p2g_acc4<kPow2>(rec[2 * (size_t)i0], rec[2 * (size_t)i0 + 1], mn0, mn1, mn2, dx0, dx1, dx2, ccx, ccy, ccz, fx,
-                        fy, fz, ws, vs);
-      }
+      if (nf == 3)
+        p2g_slab<kPow2, 3>(rec, off, cum, run, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
+      else if (nf == 2)
+        p2g_slab<kPow2, 2>(rec, off, cum, run, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
+      else
+        p2g_slab<kPow2, 1>(rec, off, cum, run, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
+    }
+    // faces whose last slab this was
+    while (cw < xend && xhi(cw) <= hx) {
+      finish(cw, ws[0], vs[0]);
+      ws[0] = ws[1];
+      vs[0] = vs[1];
+      ws[1] = ws[2];
+      vs[1] = vs[2];
+      ws[2] = 0.f;
+      vs[2] = 0.f;
+      ++cw;
     }
   }
-  float v = !is_small(ws) ? vs / ws : 0.0f;
-  // interior ranges of ApplyGravity (a == 1) and of the tank acceleration
-  if (a == 1 || a == q.tank_axis) {
-    bool in = true;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      int h = (k == a) ? q.g.N[k] - 1 : q.g.N[k] - 2;
-      if (c[k] < 1 || c[k] > h) in = false;
-    }
-    if (in) {
-      if (a == 1) v += q.gdt;
-      if (a == q.tank_axis) v += q.adt;
-    }
-  }
-  vout[((size_t)c[0] * d1 + c[1]) * d2 + c[2]] = v;
 }
 
 // ------------------------------------------------------------------ liquid phi
@@ -789,7 +873,7 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
     P2GP qa = q;
     if (!fuse_body_forces || a != 1) qa.gdt = 0.f;
     ProfScope ps(c, LH_PROF_P2G);
-    unsigned nt = tile_grid(div_up(c.g.N[0] + (a == 0), kTX) * div_up(c.g.N[1] + (a == 1), kPTY) *
+    unsigned nt = tile_grid(div_up(c.g.N[0] + (a == 0), kMX) * div_up(c.g.N[1] + (a == 1), kPTY) *
                             div_up(c.g.N[2] + (a == 2), kPTZ));
     if (pow2)
       hipLaunchKernelGGL(k_p2g<true>, dim3(nt), dim3(kBlock), 0, c.s, c.v[a], (const float4*)c.ppack, c.cap,
