@@ -17,6 +17,7 @@
 //  * Grid2Particle + AdvectParticle + GetCFL's max|v| (LiquidModel.cpp:723-935,
 //    LiquidModel.h:49-55) fused into one per-particle kernel.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -30,65 +31,11 @@ namespace lh {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kScanItems = 4;
-constexpr int kScanTile = kBlock * kScanItems;
 
 inline unsigned grid_for(size_t n) {
   size_t b = div_up(n, kBlock);
   if (b > 65536 * 8) b = 65536 * 8;
   return (unsigned)(b ? b : 1);
-}
-
-// ------------------------------------------------------------------ scan
-// Exclusive scan of int counts, one 1024-element tile per block.
-__global__ void k_scan_tile(const int* __restrict__ in, int* __restrict__ out, size_t n, int* __restrict__ sums) {
-  __shared__ int s[kBlock];
-  size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
-  int v[kScanItems];
-  int t = 0;
-  for (int k = 0; k < kScanItems; ++k) {
-    size_t i = base + k;
-    v[k] = (i < n) ? in[i] : 0;
-    t += v[k];
-  }
-  s[threadIdx.x] = t;
-  __syncthreads();
-  for (int off = 1; off < kBlock; off <<= 1) {
-    int a = ((int)threadIdx.x >= off) ? s[threadIdx.x - off] : 0;
-    __syncthreads();
-    s[threadIdx.x] += a;
-    __syncthreads();
-  }
-  int run = s[threadIdx.x] - t;  // exclusive prefix of this thread
-  for (int k = 0; k < kScanItems; ++k) {
-    size_t i = base + k;
-    if (i < n) out[i] = run;
-    run += v[k];
-  }
-  if (threadIdx.x == kBlock - 1) sums[blockIdx.x] = s[kBlock - 1];
-}
-__global__ void k_scan_add(int* __restrict__ out, size_t n, const int* __restrict__ offs) {
-  size_t i = (size_t)blockIdx.x * kScanTile + threadIdx.x;
-  int o = offs[blockIdx.x];
-  for (int k = 0; k < kScanItems; ++k, i += kBlock)
-    if (i < n) out[i] += o;
-}
-__global__ void k_set_total(const int* __restrict__ out, const int* __restrict__ in, size_t n, int* __restrict__ tot) {
-  *tot = (n ? out[n - 1] + in[n - 1] : 0);
-}
-
-// exclusive scan of in[0..n) into out[0..n], out[n] = total. tmp must hold the
-// block sums of every level.
-void scan_excl(Ctx& c, const int* in, int* out, size_t n, int* tmp) {
-  size_t nb = div_up(n, kScanTile);
-  int* sums = tmp;
-  int* sums_scan = tmp + nb;
-  hipLaunchKernelGGL(k_scan_tile, dim3((unsigned)nb), dim3(kBlock), 0, c.s, in, out, n, sums);
-  if (nb > 1) {
-    scan_excl(c, sums, sums_scan, nb, tmp + 2 * nb + 1);
-    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(kBlock), 0, c.s, out, n, sums_scan);
-  }
-  hipLaunchKernelGGL(k_set_total, dim3(1), dim3(1), 0, c.s, out, in, n, out + n);
 }
 
 // ------------------------------------------------------------------ binning
@@ -456,9 +403,19 @@ __global__ void __launch_bounds__(256) k_liquid_phi(float* __restrict__ phi, con
           }
           size_t k0 = ((size_t)hx * H1 + hy) * H2;
           int s0 = start[k0 + hz0], s1 = start[k0 + hz1 + 1];
-          for (int s = s0; s < s1; ++s) {
+          auto phi_of = [&](int s) {
             const float4 A = rec[2 * (size_t)s];  // x y z of the bin-ordered particle record
-            float ph = norm3(cpx - A.x, cpy - A.y, cpz - A.z) - radius;
+            return norm3(cpx - A.x, cpy - A.y, cpz - A.z) - radius;
+          };
+          int s = s0;
+          for (; s + 3 < s1; s += 4) {  // four loads in flight; the min is exact in any order
+            const float p0 = phi_of(s), p1 = phi_of(s + 1), p2 = phi_of(s + 2), p3 = phi_of(s + 3);
+            const float m01 = (p0 < p1) ? p0 : p1, m23 = (p2 < p3) ? p2 : p3;
+            const float m = (m01 < m23) ? m01 : m23;
+            best = (best < m) ? best : m;
+          }
+          for (; s < s1; ++s) {
+            const float ph = phi_of(s);
             best = (best < ph) ? best : ph;
           }
         }
@@ -814,6 +771,23 @@ __global__ void k_correct_commit(float* __restrict__ pb, int64_t cap, int64_t P,
 
 }  // namespace
 
+// ------------------------------------------------------------------ bin scan
+// bin_start = exclusive sum of bin_count over nbins + 1 entries (bin_count[nbins] stays 0,
+// so bin_start[nbins] is the particle count): one decoupled look-back pass, which reads the
+// 512^3 counts and writes the starts once each.
+size_t bin_scan_bytes(size_t nbins) {
+  size_t bytes = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int*)nullptr, (int*)nullptr, (int)(nbins + 1));
+  return bytes;
+}
+int bin_scan(Ctx& c) {
+  size_t bytes = c.scan_bytes;
+  return hip_ok(c, hipcub::DeviceScan::ExclusiveSum(c.scan_tmp, bytes, c.bin_count, c.bin_start, (int)(c.nbins + 1), c.s),
+                "bin scan")
+             ? LH_OK
+             : LH_ERR_DEVICE;
+}
+
 int run_binning(Ctx& c) {
   c.bins_valid = false;
   if (c.P == 0) return LH_OK;
@@ -827,7 +801,8 @@ int run_binning(Ctx& c) {
   unsigned gp = grid_for((size_t)c.P);
   hipLaunchKernelGGL(k_bin_key, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, b, c.pkey, c.bin_count,
                      c.dflag + 1);
-  scan_excl(c, c.bin_count, c.bin_start, c.nbins, c.scan_tmp);
+  int rc = bin_scan(c);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, c.s, c.pkey, c.P, c.bin_start, c.bin_count, c.pslot);
   hipLaunchKernelGGL(k_bin_permute, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.pbuf[1 - c.cur], c.cap, c.P,
                      c.pkey, c.bin_start, c.pslot, (float4*)c.ppack);

@@ -203,8 +203,9 @@ int alloc_ctx(Ctx& c) {
     c.err = "grid too large for 32-bit bin offsets";
     return LH_ERR_CONFIG;
   }
-  ok = ok && dalloc(c, c.bin_count, c.nbins) && dalloc(c, c.bin_start, c.nbins + 1) &&
-       dalloc(c, c.scan_tmp, 3 * div_up(c.nbins, 1024) + 64);
+  c.scan_bytes = bin_scan_bytes(c.nbins);
+  ok = ok && dalloc(c, c.bin_count, c.nbins + 1) && dalloc(c, c.bin_start, c.nbins + 1) &&
+       dalloc(c, c.scan_tmp, c.scan_bytes);
   // multigrid levels
   int L = std::max(1, c.cfg.num_level);
   if (c.cfg.precond != LH_PRECOND_MULTIGRID) L = 1;
@@ -236,7 +237,7 @@ int alloc_ctx(Ctx& c) {
     return LH_ERR_OOM;
   }
   hipMemcpyAsync(c.d_sdf, c.cfg.sdf.data(), c.cfg.sdf.size() * sizeof(float), hipMemcpyHostToDevice, c.s);
-  hipMemsetAsync(c.bin_count, 0, c.nbins * sizeof(int), c.s);
+  hipMemsetAsync(c.bin_count, 0, (c.nbins + 1) * sizeof(int), c.s);  // [nbins] stays 0 (the scan total)
   hipMemsetAsync(c.is_valid, 0, c.nnode, c.s);
   hipMemsetAsync(c.dflag, 0, 16 * sizeof(int), c.s);
   launch_fill_f32(c, c.solid_phi, c.nnode, 0.f);

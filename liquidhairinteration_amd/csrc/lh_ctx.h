@@ -118,7 +118,8 @@ struct Ctx {
   int* bin_start = nullptr;  // nbins + 1
   int* pkey = nullptr;       // per particle bin key
   int* pslot = nullptr;      // per particle provisional slot
-  int* scan_tmp = nullptr;   // scan block sums
+  uint8_t* scan_tmp = nullptr;  // temporary storage of the bin scan
+  size_t scan_bytes = 0;
   bool bins_valid = false;
   // solver (CGSolver<3>)
   std::vector<Level> lv;
@@ -206,6 +207,7 @@ int run_extrapolate(Ctx& c);
 void launch_divergence(Ctx& c, double* h_mean, double* h_linf, int* h_count);
 void launch_gravity(Ctx& c, float dt);
 // ---- launchers (k_particles.hip)
+size_t bin_scan_bytes(size_t nbins);  // temporary storage of the bin scan
 int run_binning(Ctx& c);  // LH_OK or LH_ERR_DOMAIN
 void launch_liquid_phi(Ctx& c);
 void launch_p2g(Ctx& c, bool fuse_body_forces, float dt);
