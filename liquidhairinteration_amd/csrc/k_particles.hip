@@ -498,7 +498,7 @@ __device__ void solid_grad_any(const float* sp, const GridP& g, const float mx[3
                     solid_phi_any(sp, g, mx, x, y, z + eps), solid_phi_any(sp, g, mx, x, y, z - eps), out);
 }
 
-__global__ void k_g2p_advect(float* __restrict__ pb, int64_t cap, int64_t P, const float* __restrict__ vx,
+__global__ void __launch_bounds__(256) k_g2p_advect(float* __restrict__ pb, int64_t cap, int64_t P, const float* __restrict__ vx,
                              const float* __restrict__ vy, const float* __restrict__ vz,
                              const float* __restrict__ sp, GP q, float mx0, float mx1, float mx2, int do_g2p,
                              int do_advect, unsigned* __restrict__ maxbits) {
@@ -523,40 +523,63 @@ __global__ void k_g2p_advect(float* __restrict__ pb, int64_t cap, int64_t P, con
       for (int a = 0; a < 3; ++a) {
         int dims[3] = {q.g.N[0], q.g.N[1], q.g.N[2]};
         dims[a] += 1;
-        // per axis: up to 3 taps with nonzero weight, in ascending index order
-        int idx[3][3];
+        // per axis: up to 3 taps with nonzero weight, in ascending index order; the taps with
+        // nonzero weight inside the grid are consecutive indices j0, j0 + 1, ...
+        int j0[3];
         float w[3][3], gw[3][3];
         int nt[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           nt[k] = 0;
+          j0[k] = 0;
           for (int t = -q.dW[k]; t <= q.dW[k]; ++t) {
             int j = ci[k] + t;
             if (j < 0 || j > dims[k] - 1) continue;
             float cc = (k == a) ? (float)j : (float)j + 0.5f;
-            float d = g[k] - cc;
-            float wv = quad_w(d);
-            if (wv == 0.f || nt[k] >= 3) continue;
-            idx[k][nt[k]] = j;
-            w[k][nt[k]] = wv;
-            gw[k][nt[k]] = quad_gw(d);
+            if (quad_w(g[k] - cc) == 0.f || nt[k] >= 3) continue;
+            if (nt[k] == 0) j0[k] = j;
             nt[k]++;
           }
+          // the taps' weights (the same operations as in the scan)
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            const int j = j0[k] + m;
+            const float d = g[k] - ((k == a) ? (float)j : (float)j + 0.5f);
+            w[k][m] = quad_w(d);
+            gw[k][m] = quad_gw(d);
+          }
         }
-        float acc = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
+        // all 27 loads in flight (taps past nt read the first tap and are not summed), then
+        // the sums in tap order
         const float* Va = V[a];
-        for (int ix = 0; ix < nt[0]; ++ix)
-          for (int iy = 0; iy < nt[1]; ++iy)
-            for (int iz = 0; iz < nt[2]; ++iz) {
+        float fv[3][3][3];
+#pragma unroll
+        for (int ix = 0; ix < 3; ++ix)
+#pragma unroll
+          for (int iy = 0; iy < 3; ++iy)
+#pragma unroll
+            for (int iz = 0; iz < 3; ++iz) {
+              const int jx = j0[0] + (ix < nt[0] ? ix : 0), jy = j0[1] + (iy < nt[1] ? iy : 0),
+                        jz = j0[2] + (iz < nt[2] ? iz : 0);
+              fv[ix][iy][iz] = Va[((size_t)jx * dims[1] + jy) * dims[2] + jz];
+            }
+        float acc = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
+#pragma unroll
+        for (int ix = 0; ix < 3; ++ix)
+#pragma unroll
+          for (int iy = 0; iy < 3; ++iy)
+#pragma unroll
+            for (int iz = 0; iz < 3; ++iz) {
+              const bool on = ix < nt[0] && iy < nt[1] && iz < nt[2];
               float wX = w[0][ix], wY = w[1][iy], wZ = w[2][iz];
               float gX = gw[0][ix], gY = gw[1][iy], gZ = gw[2][iz];
               float weight = wX * wY * wZ;
               float g0 = gX * wY * wZ, g1 = wX * gY * wZ, g2 = wX * wY * gZ;
-              float fv = Va[((size_t)idx[0][ix] * dims[1] + idx[1][iy]) * dims[2] + idx[2][iz]];
-              acc += fv * weight;
-              c0 += fv * g0;
-              c1 += fv * g1;
-              c2 += fv * g2;
+              const float f = fv[ix][iy][iz];
+              acc = on ? acc + f * weight : acc;
+              c0 = on ? c0 + f * g0 : c0;
+              c1 = on ? c1 + f * g1 : c1;
+              c2 = on ? c2 + f * g2 : c2;
             }
         vel[a] = acc;
         Cm[3 * a + 0] = c0;
