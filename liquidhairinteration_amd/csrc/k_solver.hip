@@ -867,7 +867,10 @@ __global__ void k_sc_residual(const double* __restrict__ part, int nb, float* __
   double rr = sum_partials_block(part, nb);
   if (threadIdx.x == 0) sc[4] = sqrtf((float)rr);
 }
-__global__ void k_sc_beta(const double* __restrict__ part, int nb, float* __restrict__ sc, int* __restrict__ flag) {
+// the last scalar kernel of an iteration also stores sc[0..7] and the NaN flags into pinned host
+// memory (host_sc, host_flag), which the host reads after its stream synchronisation
+__global__ void k_sc_beta(const double* __restrict__ part, int nb, float* __restrict__ sc, int* __restrict__ flag,
+                          float* __restrict__ host_sc, int* __restrict__ host_flag) {
   double rz = sum_partials_block(part, nb);
   if (threadIdx.x == 0) {
     float newRZ = (float)rz;
@@ -875,7 +878,12 @@ __global__ void k_sc_beta(const double* __restrict__ part, int nb, float* __rest
     sc[1] = newRZ;
     sc[3] = beta;
     sc[0] = newRZ;  // m_oldRZ = m_newRZ (:922)
-    if (isnan(beta) || isinf(beta)) atomicOr(flag, 2);
+    int f = *flag;
+    if (isnan(beta) || isinf(beta)) f |= 2;
+    *flag = f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) host_sc[k] = sc[k];
+    *host_flag = f;
   }
 }
 
@@ -1238,11 +1246,10 @@ int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, 
     hipLaunchKernelGGL(k_sc_residual, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc);
     apply_precond(c);
     hipLaunchKernelGGL(k_cg_dot_rz, dim3(nb), dim3(kBlock), 0, c.s, L0.r, L0.z, L0.rows, L0.nrows, L0.d[2], c.partial);
-    hipLaunchKernelGGL(k_sc_beta, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dflag + 2);
+    hipLaunchKernelGGL(k_sc_beta, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dflag + 2, c.h_pin_dev,
+                       c.h_flag_dev);
     hipLaunchKernelGGL(k_cg_update_p, dim3(nb), dim3(kBlock), 0, c.s, c.p, L0.z, L0.cls, c.dsc, L0.rows, L0.nrows,
                        L0.d[2]);
-    hipMemcpyAsync(c.h_pin, c.dsc, 8 * sizeof(float), hipMemcpyDeviceToHost, c.s);
-    hipMemcpyAsync(c.h_flag, c.dflag + 2, sizeof(int), hipMemcpyDeviceToHost, c.s);
     if (!hip_ok(c, hipStreamSynchronize(c.s), "solve iteration")) return LH_ERR_DEVICE;
     std::memcpy(hsc, c.h_pin, sizeof(hsc));
     residual = hsc[4];

@@ -236,6 +236,11 @@ int alloc_ctx(Ctx& c) {
     c.err = "hipHostMalloc failed";
     return LH_ERR_OOM;
   }
+  if (hipHostGetDevicePointer((void**)&c.h_pin_dev, c.h_pin, 0) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&c.h_flag_dev, c.h_flag, 0) != hipSuccess) {
+    c.err = "hipHostGetDevicePointer failed";
+    return LH_ERR_DEVICE;
+  }
   hipMemcpyAsync(c.d_sdf, c.cfg.sdf.data(), c.cfg.sdf.size() * sizeof(float), hipMemcpyHostToDevice, c.s);
   hipMemsetAsync(c.bin_count, 0, (c.nbins + 1) * sizeof(int), c.s);  // [nbins] stays 0 (the scan total)
   hipMemsetAsync(c.is_valid, 0, c.nnode, c.s);

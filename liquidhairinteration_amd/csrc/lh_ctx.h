@@ -141,6 +141,10 @@ struct Ctx {
   int* dflag = nullptr;       // misc device flags (extrapolation progress, domain errors)
   double* h_pin = nullptr;    // pinned host scalars
   int* h_flag = nullptr;
+  // device-side addresses of h_pin and h_flag: the PCG's last scalar kernel of an iteration
+  // stores the scalars and the NaN flag there itself instead of two copies per iteration
+  float* h_pin_dev = nullptr;
+  int* h_flag_dev = nullptr;
   double time = 0.0;
   float last_maxv = 0.f;
   int correct_step = 0;              // m_correctStep (LiquidModel.h:73)
