@@ -768,10 +768,13 @@ __global__ void k_cg_spmv(float* __restrict__ q, const float* __restrict__ p, Lv
 }
 
 // x += alpha p; r -= alpha q; partial r.r over interior liquid cells (:889-902)
+// x += alpha p, r -= alpha q (and r.r) on the updated cells; with zero_z also z = 0 on every
+// cell of the rows, the symmetric preconditioner's first step (z is read nowhere else)
 __global__ void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
                                const float* __restrict__ q, const uint8_t* __restrict__ cls,
                                const float* __restrict__ sc, int d0, int d1, int d2, const int* __restrict__ rows,
-                               const int* __restrict__ nrows, double* __restrict__ partial) {
+                               const int* __restrict__ nrows, double* __restrict__ partial, float* __restrict__ z,
+                               int zero_z) {
   const float alpha = sc[2];
   double v[1] = {0.0};
   const RowIter it = row_iter(d2);
@@ -780,11 +783,13 @@ __global__ void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r, con
     for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
       const int row = rows[k];
       const int xi = row / d1, yi = row - xi * d1;
-      if (xi < 1 || xi > d0 - 2 || yi < 1 || yi > d1 - 2) continue;
+      const bool inner = xi >= 1 && xi <= d0 - 2 && yi >= 1 && yi <= d1 - 2;
+      if (!inner && !zero_z) continue;
       const size_t base = (size_t)row * d2;
       for (int zi = it.z0; zi < d2; zi += it.zs) {
-        if (zi < 1 || zi > d2 - 2) continue;
         const size_t i = base + zi;
+        if (zero_z) z[i] = 0.0f;
+        if (!inner || zi < 1 || zi > d2 - 2) continue;
         if (cls[i] == 0) continue;
         x[i] += alpha * p[i];
         float rn = r[i] - alpha * q[i];
@@ -1018,7 +1023,7 @@ void sweeps(Ctx& c, int l, int first, int fwd, int rev, int mode) {
 
 // ApplyPreconditioner, PressureSolver.cpp:719-772 (symmetric mode: z[0] = 0 first,
 // post-smoothing and the second half of the coarsest sweeps in reverse colour order)
-void apply_precond(Ctx& c) {
+void apply_precond(Ctx& c, bool z_zeroed) {
   Level& L0 = c.lv[0];
   if (c.cfg.precond != LH_PRECOND_MULTIGRID) {
     hipLaunchKernelGGL(k_precond_simple, dim3(grid_for(L0.n)), dim3(kBlock), 0, c.s, L0.z, L0.r, L0.adiag, L0.cls,
@@ -1028,8 +1033,9 @@ void apply_precond(Ctx& c) {
   const int L = (int)c.lv.size();
   const int sym = (c.cfg.smoother == LH_SMOOTHER_SYMMETRIC);
   const int mode = sym ? 1 : 0;
-  // z[0] = 0 where it is read: on the liquid rows (elsewhere it only meets zero adiag and r)
-  if (sym)
+  // z[0] = 0 where it is read: on the liquid rows (elsewhere it only meets zero adiag and r),
+  // unless the caller has cleared it
+  if (sym && !z_zeroed)
     hipLaunchKernelGGL(k_zero_rows, dim3(row_blocks(L0)), dim3(kBlock), 0, c.s, L0.z, L0.rows, L0.nrows, L0.d[2]);
   for (int l = 1; l < L; ++l) {
     sweeps(c, l - 1, 0, c.cfg.num_pre, 0, mode);
@@ -1214,7 +1220,7 @@ int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, 
   }
   if (!S.warm_start_used)
     hipLaunchKernelGGL(k_cg_init, dim3(grid_for(n)), dim3(kBlock), 0, c.s, L0.r, c.x, c.p, L0.z, b, L0.cls, n);
-  apply_precond(c);
+  apply_precond(c, true);  // k_cg_init and k_cg_set_x0 clear z
   hipLaunchKernelGGL(k_cg_first, dim3(nb), dim3(kBlock), 0, c.s, c.p, L0.z, L0.r, L0.cls, L0.rows, L0.nrows, L0.d[2],
                      c.partial);
   hipLaunchKernelGGL(k_sc_first, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dsc64);
@@ -1241,10 +1247,11 @@ int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, 
       hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.q, c.p, P0, L0.rows, L0.nrows, c.partial);
     }
     hipLaunchKernelGGL(k_sc_alpha, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dflag + 2);
+    const int sym = c.cfg.precond == LH_PRECOND_MULTIGRID && c.cfg.smoother == LH_SMOOTHER_SYMMETRIC;
     hipLaunchKernelGGL(k_cg_update_xr, dim3(nb), dim3(kBlock), 0, c.s, c.x, L0.r, c.p, c.q, L0.cls, c.dsc, L0.d[0],
-                       L0.d[1], L0.d[2], L0.rows, L0.nrows, c.partial);
+                       L0.d[1], L0.d[2], L0.rows, L0.nrows, c.partial, L0.z, sym);
     hipLaunchKernelGGL(k_sc_residual, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc);
-    apply_precond(c);
+    apply_precond(c, sym != 0);
     hipLaunchKernelGGL(k_cg_dot_rz, dim3(nb), dim3(kBlock), 0, c.s, L0.r, L0.z, L0.rows, L0.nrows, L0.d[2], c.partial);
     hipLaunchKernelGGL(k_sc_beta, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dflag + 2, c.h_pin_dev,
                        c.h_flag_dev);
