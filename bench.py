@@ -35,13 +35,21 @@ def level_dims(cfg):
     return dims
 
 
-def row_sweep_levels(cfg):
-    """levels swept by one k_sweep_rows launch per sweep (both colours) instead of two passes"""
+def coarse_form(cfg, l):
+    """how coarse level l >= 1 is swept: "rows" (one k_sweep_rows launch per sweep, on levels of
+    <= rowSweepMaxCells with coarseSweep "rows"), "tiles" (one k_tile_sweep launch per sweep,
+    coarseSweep "tiles") or "passes" (two k_smooth colour passes)"""
     sp = cfg["SolverParameter"]
-    if sp.get("coarseSweep", "rows") != "rows":
-        return set()
-    lim = sp.get("rowSweepMaxCells", 1 << 18)
-    return {l for l, d in enumerate(level_dims(cfg)) if l >= 1 and d[0] * d[1] * d[2] <= lim}
+    mode = sp.get("coarseSweep", "rows")
+    if mode in ("passes", "tiles"):
+        return mode
+    d = level_dims(cfg)[l]
+    return "rows" if d[0] * d[1] * d[2] <= sp.get("rowSweepMaxCells", 1 << 18) else "passes"
+
+
+def row_sweep_levels(cfg):
+    """coarse levels swept by one launch per sweep (both colours) instead of two passes"""
+    return {l for l in range(1, cfg["SolverParameter"]["numLevel"]) if coarse_form(cfg, l) != "passes"}
 
 
 def algorithmic_bytes(kernel, n0, P, active, fused, rows_sweep=frozenset()):
@@ -53,7 +61,7 @@ def algorithmic_bytes(kernel, n0, P, active, fused, rows_sweep=frozenset()):
                fused one-launch sweep, when selected, is dense: 28 B/cell;
     smooth_l1, smooth_l2: the same per cell of the rows holding liquid on level 1, and on
                level 2 (the coarsest of the three-level hierarchy): 14 B per colour pass, 28 B
-               per k_sweep_rows launch (a whole sweep, on levels of <= rowSweepMaxCells);
+               per k_sweep_rows or k_tile_sweep launch (a whole sweep);
     spmv:      p read, q write, 4 coefficient floats -> 24 B per cell of those rows;
     p2g:       one component: 20 B/particle (position, velocity component,
                affine column) + 4 B per face written (~n0 faces);
@@ -92,12 +100,20 @@ def row_sweep_grid(d):
     return max(1, min(-(-d[0] * d[1] * nseg // 4), 8192)) * 256
 
 
+def tile_sweep_grid(d):
+    """work-items of one k_tile_sweep launch on a level of dims d (4 x 4 x 64 tiles, capped)"""
+    return max(1, min(-(-d[0] // 4) * -(-d[1] // 4) * -(-d[2] // 64), 8192)) * 256
+
+
 def level_symbol(cfg, l):
     """(kernel symbol, grid size) of the smoother launches on coarse level l"""
     d = level_dims(cfg)[l]
     mode = 1 if cfg["SolverParameter"].get("smoother", "reference") == "symmetric" else 0
-    if l in row_sweep_levels(cfg):
+    form = coarse_form(cfg, l)
+    if form == "rows":
         return "k_sweep_rows<%d>" % mode, row_sweep_grid(d)
+    if form == "tiles":
+        return "k_tile_sweep<%d>" % mode, tile_sweep_grid(d)
     return "k_smooth<false>", smooth_grid(d)
 
 

@@ -86,8 +86,9 @@ Config parse_config(const std::string& text) {
   else throw std::runtime_error("Unknown smoother " + sm);
   c.fused_sweep_min_cells = (long long)s.get("fusedSweepMinCells", (double)(1LL << 62));
   std::string cs = s.get("coarseSweep", "rows");
-  if (cs == "rows") c.coarse_sweep_rows = 1;
-  else if (cs == "passes") c.coarse_sweep_rows = 0;
+  if (cs == "rows") c.coarse_sweep = kCoarseRows;
+  else if (cs == "tiles") c.coarse_sweep = kCoarseTiles;
+  else if (cs == "passes") c.coarse_sweep = kCoarsePasses;
   else throw std::runtime_error("Unknown coarseSweep " + cs);
   c.row_sweep_max_cells = (long long)s.get("rowSweepMaxCells", (double)(1LL << 18));
   // LiquidParameter (SceneParameter.h:55-63)
@@ -224,7 +225,8 @@ int alloc_ctx(Ctx& c) {
          dalloc(c, q.aflag, (size_t)q.d[0] * q.d[1]) && dalloc(c, q.qrows, (size_t)q.d[0] * q.d[1]) &&
          dalloc(c, q.nqrows, 1) && dalloc(c, q.qflag, (size_t)q.d[0] * q.d[1]) &&
          dalloc(c, q.zrows, (size_t)q.d[0] * q.d[1]) && dalloc(c, q.nzrows, 1) &&
-         dalloc(c, q.zflag, (size_t)q.d[0] * q.d[1]);
+         dalloc(c, q.zflag, (size_t)q.d[0] * q.d[1]) && dalloc(c, q.tflag, div_up(q.d[0], 4) * div_up(q.d[1], 4)) &&
+         dalloc(c, q.tiles, div_up(q.d[0], 4) * div_up(q.d[1], 4)) && dalloc(c, q.ntiles, 1);
   }
   c.npartial = 1024;
   ok = ok && dalloc(c, c.p, c.ncell) && dalloc(c, c.x, c.ncell) && dalloc(c, c.q, c.ncell) &&
@@ -283,7 +285,7 @@ void free_ctx(Ctx& c) {
   F(c.pbuf[0]); F(c.pbuf[1]); F(c.ppack); F(c.cv_flag);
   for (auto& q : c.lv) {
     F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2); F(q.rows); F(q.nrows);
-    F(q.aflag); F(q.qrows); F(q.nqrows); F(q.qflag); F(q.zrows); F(q.nzrows); F(q.zflag);
+    F(q.aflag); F(q.qrows); F(q.nqrows); F(q.qflag); F(q.zrows); F(q.nzrows); F(q.zflag); F(q.tflag); F(q.tiles); F(q.ntiles);
   }
   F(c.p); F(c.x); F(c.q); F(c.partial); F(c.dsc); F(c.dsc64); F(c.dmaxv); F(c.dflag);
   F(c.x64); F(c.r64); F(c.rf); F(c.b_user); F(c.x0_user);

@@ -31,12 +31,15 @@ struct Config {
   int smoother = LH_SMOOTHER_REFERENCE;
   // levels with at least this many cells use the fused one-kernel red-black sweep
   long long fused_sweep_min_cells = 1LL << 62;  // off: two colour passes measured faster at 256^3
-  // coarse levels (>= 1) of at most row_sweep_max_cells cells: one launch per red-black sweep
-  // over the z-support rows (k_sweep_rows) instead of two colour passes; same values.  It pays
-  // where the passes are launch-bound (64^3: 5.8 us per sweep against 2 x 3.9 us) and loses
-  // where they are not (128^3 at 256^3 scale: 30 us against 2 x 5.6 us: four neighbour
-  // updates recomputed per cell)
-  int coarse_sweep_rows = 1;
+
+  // red-black sweeps on the coarse levels (>= 1), all with the values of two colour passes:
+  //   kCoarseRows (default): levels of at most row_sweep_max_cells cells in one launch over the
+  //     z-support rows (k_sweep_rows; 64^3: 5.8 us per sweep against 2 x 3.9 us), larger ones
+  //     as two colour passes (the row sweep's four recomputed neighbour updates per cell make it
+  //     slow there: 30 us against 2 x 5.6 us at 128^3);
+  //   kCoarseTiles: every coarse level in one launch over 4 x 4 x 64 tiles (k_tile_sweep);
+  //   kCoarsePasses: two colour passes (k_smooth) on every coarse level
+  int coarse_sweep = 1;
   long long row_sweep_max_cells = 1LL << 18;
   // mixed-precision refinement (FP32 inner MGPCG under an FP64 residual loop)
   int refined = 0;
@@ -61,6 +64,8 @@ struct GridP {
   float mn[3], dx[3];
 };
 
+enum { kCoarsePasses = 0, kCoarseRows = 1, kCoarseTiles = 2 };
+
 struct Level {
   int d[3] = {0, 0, 0};
   size_t n = 0;
@@ -84,6 +89,10 @@ struct Level {
   int* zrows = nullptr;
   int* nzrows = nullptr;
   int* zflag = nullptr;
+  // levels >= 1: the 4 x 4-row tiles holding a restriction row, for the tiled red-black sweep
+  int* tflag = nullptr;
+  int* tiles = nullptr;
+  int* ntiles = nullptr;
 };
 
 // attributes of one particle in the SoA particle buffer (kParticleAttrs x cap)

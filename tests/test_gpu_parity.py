@@ -57,6 +57,13 @@ def coarse_passes(cfg):
     return cfg
 
 
+def coarse_tiles(cfg):
+    """every coarse level swept by the tiled one-launch sweep (an option: by default coarse levels
+    use the row sweep or two colour passes)"""
+    cfg["SolverParameter"]["coarseSweep"] = "tiles"
+    return cfg
+
+
 def cv_cycle(cfg, k):
     cfg["LiquidParameter"]["correctCycle"] = k
     return cfg
@@ -81,6 +88,11 @@ CASES = {
     "dam24_tank_sphere": lambda: small_dam(24, "symmetric", tank=True),
     "dam24_reference_fused_sweep": lambda: small_dam(24, "reference", fused=True),
     "dam40_symmetric_fused_sweep": lambda: small_dam(40, "symmetric", fused=True),
+    # coarse levels in 4 x 4 x 64 tiles (partial tiles in x, y and z: 20^3 and 10^3 levels; the
+    # tall grid's 80- and 40-cell rows)
+    "dam40_symmetric_tile_sweep": lambda: coarse_tiles(small_dam(40, "symmetric")),
+    "dam24_reference_tile_sweep": lambda: coarse_tiles(small_dam(24, "reference")),
+    "tall_16x24x160_tile_sweep": lambda: coarse_tiles(tall_dam()),
     # CorrectVolume: at 24^3 (9072 particles, 18144 hash slots) most regions hold a cell whose
     # slot collides with a nearby cell (exact visit count), at 40^3 (60112 particles) few do
     # (overlap product); every second particle id moves per call at 40^3
@@ -155,7 +167,8 @@ def test_stages_bit_exact(case):
 
 
 @pytest.mark.parametrize("case", ["dam24_reference_smoother", "dam24_tank_sphere", "dam40_symmetric_fused_sweep",
-                                  "dam24_correct_volume", "tall_16x24x160_row_segments"])
+                                  "dam24_correct_volume", "tall_16x24x160_row_segments",
+                                  "dam40_symmetric_tile_sweep", "tall_16x24x160_tile_sweep"])
 def test_free_running_steps_bit_exact(case):
     print("case", case, flush=True)
     cfg = CASES[case]()
