@@ -194,12 +194,12 @@ def measured_traffic(kernel, workload):
     return None if rec is None else rec["bytes_per_dispatch"]
 
 
-def cpu_baseline(cfg, steps=1):
+def cpu_baseline(cfg, steps=1, threads=0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     from oracle import Oracle
 
-    threads = os.cpu_count() or 1
+    threads = threads or os.cpu_count() or 1
     orc = Oracle(cfg, dot_mode=1, p2g_order=1, threads=threads)
     orc.init()
     dt = np.float32(cfg.get("firstDt", 0.01))
@@ -210,7 +210,7 @@ def cpu_baseline(cfg, steps=1):
     el = time.perf_counter() - t0
     return {"value": steps / el, "unit": "steps/s", "cores": int(orc.L.orc_max_threads()), "kind": "port",
             "sample": "%d step(s) of the same workload from its initial state, FP32 oracle restatement, "
-                      "OpenMP over the host cores (%.1f s)" % (steps, el)}
+                      "OpenMP over %d host thread(s) (%.1f s)" % (steps, int(orc.L.orc_max_threads()), el)}
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA peak
@@ -294,6 +294,9 @@ def main():
     ap.add_argument("--config", default=os.path.join(ROOT, "liquidhairinteration_amd", "configs", "collision_256.json"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads for cpu_baseline (0: every core)")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="time only the CPU baseline of the workload (no GPU) and print it as one JSON line")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned-vs-plain solve comparison")
     ap.add_argument("--learned", action="store_true",
                     help="start every pressure solve of the timed workload from the learned guess")
@@ -308,6 +311,10 @@ def main():
     workload = os.path.basename(args.config).replace(".json", "")
     if args.pmc_reduce:
         pmc_reduce(args.pmc_reduce[0], args.pmc_reduce[1], workload, json.load(open(args.config)))
+        return
+    if args.cpu_only:
+        cfg = json.load(open(args.config))
+        print(json.dumps({"workload": workload, "cpu_baseline": cpu_baseline(cfg, args.cpu_steps, args.cpu_threads)}))
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
@@ -421,7 +428,7 @@ def main():
     if not args.no_warm_start:
         out["warm_start"] = warm_start_eval(args.ws_grid, args.ws_scenes, args.ws_steps)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps, args.cpu_threads)
     print(json.dumps(out))
 
 
