@@ -1387,7 +1387,20 @@ int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, 
                      c.partial);
   hipLaunchKernelGGL(k_sc_first, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dsc64);
   hipMemcpyAsync(c.h_pin, c.dsc, 8 * sizeof(float), hipMemcpyDeviceToHost, c.s);
-  if (!hip_ok(c, hipStreamSynchronize(c.s), "solve init")) return LH_ERR_DEVICE;
+  // An iteration's q = A p and alpha are queued before the host has read the previous residual,
+  // so the GPU runs them while the host waits: when that residual shows convergence they are
+  // discarded (they write only q, the partial sums, sc[2], sc[5] and the NaN flag, which is
+  // published by k_sc_beta of the next iteration and cleared per solve; x, r, p are untouched).
+  auto spmv_alpha = [&]() {
+    {
+      ProfScope ps(c, LH_PROF_SPMV);
+      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.q, c.p, P0, L0.rows, L0.nrows, c.partial);
+    }
+    hipLaunchKernelGGL(k_sc_alpha, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dflag + 2);
+  };
+  hipEventRecord(c.ev_iter, c.s);
+  spmv_alpha();
+  if (!hip_ok(c, hipEventSynchronize(c.ev_iter), "solve init")) return LH_ERR_DEVICE;
   float hsc[8];
   std::memcpy(hsc, c.h_pin, sizeof(hsc));
   float residual = hsc[4];
@@ -1404,11 +1417,7 @@ int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, 
       conv = true;
       break;
     }
-    {
-      ProfScope ps(c, LH_PROF_SPMV);
-      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.q, c.p, P0, L0.rows, L0.nrows, c.partial);
-    }
-    hipLaunchKernelGGL(k_sc_alpha, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dflag + 2);
+    // q = A p and alpha of this iteration are already queued
     const int sym = c.cfg.precond == LH_PRECOND_MULTIGRID && c.cfg.smoother == LH_SMOOTHER_SYMMETRIC;
     hipLaunchKernelGGL(k_cg_update_xr, dim3(nb), dim3(kBlock), 0, c.s, c.x, L0.r, c.p, c.q, L0.cls, c.dsc, L0.d[0],
                        L0.d[1], L0.d[2], L0.rows, L0.nrows, c.partial, L0.z, sym);
@@ -1419,7 +1428,9 @@ int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, 
                        c.h_flag_dev);
     hipLaunchKernelGGL(k_cg_update_p, dim3(nb), dim3(kBlock), 0, c.s, c.p, L0.z, L0.cls, c.dsc, L0.rows, L0.nrows,
                        L0.d[2]);
-    if (!hip_ok(c, hipStreamSynchronize(c.s), "solve iteration")) return LH_ERR_DEVICE;
+    hipEventRecord(c.ev_iter, c.s);
+    spmv_alpha();  // the next iteration's, running while the host waits for the event and decides
+    if (!hip_ok(c, hipEventSynchronize(c.ev_iter), "solve iteration")) return LH_ERR_DEVICE;
     std::memcpy(hsc, c.h_pin, sizeof(hsc));
     residual = hsc[4];
     if (*c.h_flag) {

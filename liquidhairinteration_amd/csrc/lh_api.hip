@@ -296,6 +296,7 @@ void free_ctx(Ctx& c) {
   if (c.h_flag) hipHostFree(c.h_flag);
   if (c.ev0) hipEventDestroy(c.ev0);
   if (c.ev1) hipEventDestroy(c.ev1);
+  if (c.ev_iter) hipEventDestroy(c.ev_iter);
   for (int k = 0; k < LH_PROF_COUNT; ++k)
     for (auto e : c.prof_ev[k]) hipEventDestroy(e);
   if (c.s) hipStreamDestroy(c.s);
@@ -573,7 +574,8 @@ int create_common(const std::string& text, lh_ctx** out) {
     return LH_ERR_CONFIG;
   }
   if (!hip_ok(c, hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking), "stream") ||
-      !hip_ok(c, hipEventCreate(&c.ev0), "event") || !hip_ok(c, hipEventCreate(&c.ev1), "event")) {
+      !hip_ok(c, hipEventCreate(&c.ev0), "event") || !hip_ok(c, hipEventCreate(&c.ev1), "event") ||
+      !hip_ok(c, hipEventCreateWithFlags(&c.ev_iter, hipEventDisableTiming), "event")) {
     *out = ctx;
     return LH_ERR_DEVICE;
   }

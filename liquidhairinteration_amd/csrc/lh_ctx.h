@@ -105,6 +105,7 @@ struct Ctx {
   float radius = 0.f;
   hipStream_t s = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_iter = nullptr;  // end of a PCG iteration's last kernel the host waits for
   // grid (Grid<3>, Grid.h:137-170)
   float *v[3] = {}, *tv[3] = {}, *lw[3] = {};
   float *pressure = nullptr, *phi = nullptr, *solid_phi = nullptr, *rhs = nullptr;
