@@ -97,58 +97,59 @@ __global__ void k_velocity_update(float* __restrict__ vx, float* __restrict__ vy
   const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
   const int m0 = N0 - 2, m1 = N1 - 2, m2 = N2 - 2;
   if (m0 <= 0 || m1 <= 0 || m2 <= 0) return;
-  size_t total = (size_t)m0 * m1 * m2;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    int zi = 1 + (int)(i % m2);
-    int yi = 1 + (int)((i / m2) % m1);
-    int xi = 1 + (int)(i / ((size_t)m1 * m2));
-    size_t c = ((size_t)xi * N1 + yi) * N2 + zi;
-    float cphi = phi[c];
-    float pc = pr[c];
-    // x face (x,y,z) of vx with dims (N0+1,N1,N2): same linear index pattern
-    {
-      size_t f = ((size_t)xi * N1 + yi) * N2 + zi;
-      size_t nb = ((size_t)(xi - 1) * N1 + yi) * N2 + zi;
-      if (lwx[f] > 0.0f) {
-        float ph = phi[nb];
-        if (cphi < 0.0f || ph < 0.0f) {
-          float frac = 1.0f;
-          if (cphi >= 0.0f || ph >= 0.0f) frac = max_tenth(fraction_in_region(cphi, ph));
-          float gp = (pc - pr[nb]) / (g.dx[0] * frac);
-          vx[f] -= gp * dt / rho;
+  // blocks over the interior (x, y) rows, threads over z: no per-cell division
+  const int nrow = m0 * m1;
+  for (int row = blockIdx.x; row < nrow; row += gridDim.x) {
+    const int xi = 1 + row / m1, yi = 1 + row % m1;
+    for (int zi = 1 + (int)threadIdx.x; zi <= m2; zi += (int)blockDim.x) {
+      size_t c = ((size_t)xi * N1 + yi) * N2 + zi;
+      float cphi = phi[c];
+      float pc = pr[c];
+      // x face (x,y,z) of vx with dims (N0+1,N1,N2): same linear index pattern
+      {
+        size_t f = ((size_t)xi * N1 + yi) * N2 + zi;
+        size_t nb = ((size_t)(xi - 1) * N1 + yi) * N2 + zi;
+        if (lwx[f] > 0.0f) {
+          float ph = phi[nb];
+          if (cphi < 0.0f || ph < 0.0f) {
+            float frac = 1.0f;
+            if (cphi >= 0.0f || ph >= 0.0f) frac = max_tenth(fraction_in_region(cphi, ph));
+            float gp = (pc - pr[nb]) / (g.dx[0] * frac);
+            vx[f] -= gp * dt / rho;
+          }
+        } else {
+          vx[f] = 0.0f;
         }
-      } else {
-        vx[f] = 0.0f;
       }
-    }
-    {
-      size_t f = ((size_t)xi * (N1 + 1) + yi) * N2 + zi;
-      size_t nb = ((size_t)xi * N1 + (yi - 1)) * N2 + zi;
-      if (lwy[f] > 0.0f) {
-        float ph = phi[nb];
-        if (cphi < 0.0f || ph < 0.0f) {
-          float frac = 1.0f;
-          if (cphi >= 0.0f || ph >= 0.0f) frac = max_tenth(fraction_in_region(cphi, ph));
-          float gp = (pc - pr[nb]) / (g.dx[1] * frac);
-          vy[f] -= gp * dt / rho;
+      {
+        size_t f = ((size_t)xi * (N1 + 1) + yi) * N2 + zi;
+        size_t nb = ((size_t)xi * N1 + (yi - 1)) * N2 + zi;
+        if (lwy[f] > 0.0f) {
+          float ph = phi[nb];
+          if (cphi < 0.0f || ph < 0.0f) {
+            float frac = 1.0f;
+            if (cphi >= 0.0f || ph >= 0.0f) frac = max_tenth(fraction_in_region(cphi, ph));
+            float gp = (pc - pr[nb]) / (g.dx[1] * frac);
+            vy[f] -= gp * dt / rho;
+          }
+        } else {
+          vy[f] = 0.0f;
         }
-      } else {
-        vy[f] = 0.0f;
       }
-    }
-    {
-      size_t f = ((size_t)xi * N1 + yi) * (N2 + 1) + zi;
-      size_t nb = c - 1;
-      if (lwz[f] > 0.0f) {
-        float ph = phi[nb];
-        if (cphi < 0.0f || ph < 0.0f) {
-          float frac = 1.0f;
-          if (cphi >= 0.0f || ph >= 0.0f) frac = max_tenth(fraction_in_region(cphi, ph));
-          float gp = (pc - pr[nb]) / (g.dx[2] * frac);
-          vz[f] -= gp * dt / rho;
+      {
+        size_t f = ((size_t)xi * N1 + yi) * (N2 + 1) + zi;
+        size_t nb = c - 1;
+        if (lwz[f] > 0.0f) {
+          float ph = phi[nb];
+          if (cphi < 0.0f || ph < 0.0f) {
+            float frac = 1.0f;
+            if (cphi >= 0.0f || ph >= 0.0f) frac = max_tenth(fraction_in_region(cphi, ph));
+            float gp = (pc - pr[nb]) / (g.dx[2] * frac);
+            vz[f] -= gp * dt / rho;
+          }
+        } else {
+          vz[f] = 0.0f;
         }
-      } else {
-        vz[f] = 0.0f;
       }
     }
   }
@@ -426,13 +427,14 @@ __global__ void k_xcopy_list(float* __restrict__ t, const float* __restrict__ v,
   }
 }
 
+// t = v on the range: blocks over its (x, y) rows, threads over z (no per-element division)
 __global__ void k_xcopy_range(float* __restrict__ t, const float* __restrict__ v, XRange R) {
-  const int m0 = R.hi[0] - R.lo[0] + 1, m1 = R.hi[1] - R.lo[1] + 1, m2 = R.hi[2] - R.lo[2] + 1;
-  size_t total = (size_t)m0 * m1 * m2;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    int z = R.lo[2] + (int)(i % m2), y = R.lo[1] + (int)((i / m2) % m1), x = R.lo[0] + (int)(i / ((size_t)m1 * m2));
-    size_t f = ((size_t)x * R.d[1] + y) * R.d[2] + z;
-    t[f] = v[f];
+  const int m1 = R.hi[1] - R.lo[1] + 1;
+  const int nrow = (R.hi[0] - R.lo[0] + 1) * m1;
+  for (int row = blockIdx.x; row < nrow; row += gridDim.x) {
+    const int x = R.lo[0] + row / m1, y = R.lo[1] + row % m1;
+    const size_t base = ((size_t)x * R.d[1] + y) * R.d[2];
+    for (int z = R.lo[2] + (int)threadIdx.x; z <= R.hi[2]; z += (int)blockDim.x) t[base + z] = v[base + z];
   }
 }
 
@@ -566,7 +568,9 @@ void launch_gravity(Ctx& c, float dt) {
 void launch_velocity_update(Ctx& c, float dt) {
   size_t n = (size_t)std::max(c.g.N[0] - 2, 0) * std::max(c.g.N[1] - 2, 0) * std::max(c.g.N[2] - 2, 0);
   if (!n) return;
-  hipLaunchKernelGGL(k_velocity_update, dim3(grid_for(n)), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], c.lw[0],
+  hipLaunchKernelGGL(k_velocity_update,
+                     dim3((unsigned)std::max(1, std::min((c.g.N[0] - 2) * (c.g.N[1] - 2), 65535))), dim3(kBlock),
+                     0, c.s, c.v[0], c.v[1], c.v[2], c.lw[0],
                      c.lw[1], c.lw[2], c.phi, c.pressure, c.g, dt, c.cfg.density);
 }
 
@@ -677,7 +681,8 @@ void extrapolate_replay(Ctx& c) {
     for (int iter = 0; iter < 5; ++iter) {
       int n = c.xcnt[a * 5 + iter];
       if (iter == 0) {
-        hipLaunchKernelGGL(k_xcopy_range, dim3(grid_for(m)), dim3(kBlock), 0, c.s, c.tv[a], c.v[a], R);
+        hipLaunchKernelGGL(k_xcopy_range, dim3((unsigned)std::min(m / (size_t)(R.hi[2] - R.lo[2] + 1), (size_t)65535)),
+                           dim3(kBlock), 0, c.s, c.tv[a], c.v[a], R);
       } else if (prev_n > 0) {
         hipLaunchKernelGGL(k_xcopy_list, dim3(grid_for(prev_n)), dim3(kBlock), 0, c.s, c.tv[a], c.v[a],
                            c.xlist[a] + 2 * prev_off, prev_n);

@@ -535,11 +535,17 @@ int do_step(Ctx& c, float dt, lh_solve_stats* st) {
   }
   c.time += (double)dt;
   c.last_solve = local;
-  int rc = check_domain(c);
-  if (rc) return rc;
-  rc = read_maxv(c);
+  // the domain flag and max |v| with one wait
+  hipMemcpyAsync(c.h_flag + 1, c.dflag + 1, sizeof(int), hipMemcpyDeviceToHost, c.s);
+  hipMemcpyAsync(c.h_flag + 4, c.dmaxv, sizeof(unsigned), hipMemcpyDeviceToHost, c.s);
+  if (!hip_ok(c, hipStreamSynchronize(c.s), "step end")) return LH_ERR_DEVICE;
   if (st) *st = local;
-  return rc;
+  if (c.h_flag[1]) {
+    c.err = "a particle left the grid box [minCoord, maxCoord)";
+    return LH_ERR_DOMAIN;
+  }
+  std::memcpy(&c.last_maxv, c.h_flag + 4, sizeof(float));
+  return LH_OK;
 }
 
 float cfl_from(const Ctx& c) {
