@@ -545,11 +545,15 @@ void launch_solid_phi(Ctx& c) {
     hipLaunchKernelGGL(k_face_open, dim3(grid_for(c.nface[a])), dim3(kBlock), 0, c.s, c.lw[a], c.solid_phi, c.g, a);
 }
 
-void launch_clear(Ctx& c) {
-  launch_fill_f32(c, c.pressure, c.ncell, 0.f);
-  launch_fill_f32(c, c.phi, c.ncell, 3 * c.g.dx[0]);
+void launch_clear(Ctx& c, bool in_step) {
+  // liquid phi writes every cell, P2G every face, the solve's result every pressure cell
+  const bool full = !in_step || c.P == 0;
+  if (full) {
+    launch_fill_f32(c, c.pressure, c.ncell, 0.f);
+    launch_fill_f32(c, c.phi, c.ncell, 3 * c.g.dx[0]);
+  }
   for (int a = 0; a < 3; ++a) {
-    launch_fill_f32(c, c.v[a], c.nface[a], 0.f);
+    if (full) launch_fill_f32(c, c.v[a], c.nface[a], 0.f);
     launch_fill_f32(c, c.tv[a], c.nface[a], 0.f);
   }
 }

@@ -500,8 +500,8 @@ int read_maxv(Ctx& c) {
 }
 
 // ---- step pieces
-int do_pre_step(Ctx& c) {
-  launch_clear(c);  // Grid<3>::Clear
+int do_pre_step(Ctx& c, bool in_step = false) {
+  launch_clear(c, in_step);  // Grid<3>::Clear
   int rc = run_binning(c);
   if (rc) return rc;
   launch_liquid_phi(c);  // ComputeLiquidPhi
@@ -815,7 +815,7 @@ int lh_run_steps(lh_ctx* ctx, int nsteps, float* dt, lh_step_stats* last) {
       hipEventCreate(&e0);
       hipEventCreate(&e1);
       hipEventRecord(e0, c.s);
-      int rc = do_pre_step(c);
+      int rc = do_pre_step(c, true);
       if (rc == LH_OK) rc = do_step(c, *dt, &st.solve);
       hipEventRecord(e1, c.s);
       hipEventSynchronize(e1);

@@ -214,7 +214,9 @@ struct ProfScope {
 void launch_fill_f32(Ctx& c, float* a, size_t n, float v);
 void launch_fill_u8(Ctx& c, uint8_t* a, size_t n, uint8_t v);
 void launch_solid_phi(Ctx& c);
-void launch_clear(Ctx& c);
+// Grid<3>::Clear; within a step that computes liquid phi, P2G and the solve on particles
+// (in_step and P > 0) the fields those overwrite entirely (liquidPhi, v, pressure) are skipped
+void launch_clear(Ctx& c, bool in_step = false);
 void launch_velocity_update(Ctx& c, float dt);
 // returns number of propagation rounds used; <0 on device error
 int run_extrapolate(Ctx& c);
