@@ -823,6 +823,22 @@ __global__ void k_precond_simple(float* __restrict__ z, const float* __restrict_
 }
 
 // ---------------------------------------------------------------- PCG vectors
+// the sum of n per-block FP64 partials, in a fixed order (the same in every block); every
+// thread of the block calls it
+__device__ double sum_partials_block(const double* __restrict__ part, int n) {
+  __shared__ double s[kBlock];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) a += part[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int k = kBlock / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+    __syncthreads();
+  }
+  double t = s[0];
+  __syncthreads();
+  return t;
+}
 template <int NV>
 __device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partial) {
   __shared__ double s[NV][kBlock];
@@ -905,14 +921,22 @@ __global__ void k_cg_spmv(float* __restrict__ q, const float* __restrict__ p, Lv
 }
 
 // x += alpha p; r -= alpha q; partial r.r over interior liquid cells (:889-902)
-// x += alpha p, r -= alpha q (and r.r) on the updated cells; with zero_z also z = 0 on every
-// cell of the rows, the symmetric preconditioner's first step (z is read nowhere else)
+// alpha = rho / (p.q) (CGSolver<3>::Solve, :879-883) from the SpMV's partials in every block, then
+// x += alpha p, r -= alpha q (and r.r) on the updated cells; with zero_z also z = 0 on every cell
+// of the rows, the symmetric preconditioner's first step (z is read nowhere else).  rho is
+// sc[6 + slot]; block 0 records sc[5] = p.q, sc[2] = alpha and the NaN flag bit 1.
 __global__ void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
-                               const float* __restrict__ q, const uint8_t* __restrict__ cls,
-                               const float* __restrict__ sc, int d0, int d1, int d2, const int* __restrict__ rows,
-                               const int* __restrict__ nrows, double* __restrict__ partial, float* __restrict__ z,
-                               int zero_z) {
-  const float alpha = sc[2];
+                               const float* __restrict__ q, const uint8_t* __restrict__ cls, float* __restrict__ sc,
+                               int slot, int* __restrict__ flag, const double* __restrict__ part_pq, int npq, int d0,
+                               int d1, int d2, const int* __restrict__ rows, const int* __restrict__ nrows,
+                               double* __restrict__ partial, float* __restrict__ z, int zero_z) {
+  const float denom = (float)sum_partials_block(part_pq, npq);
+  const float alpha = sc[6 + slot] / denom;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[5] = denom;
+    sc[2] = alpha;
+    if (isnan(alpha) || isinf(alpha)) atomicOr(flag, 1);
+  }
   double v[1] = {0.0};
   const RowIter it = row_iter(d2);
   const int nr = *nrows;
@@ -952,10 +976,33 @@ __global__ void k_cg_dot_rz(const float* __restrict__ r, const float* __restrict
 }
 
 // p = z + beta p on liquid cells (:913-921)
+// beta = rho_new / rho (:913-922) from the r.z partials in every block, then p = z + beta p on the
+// liquid cells.  Block 0 also forms the residual norm from the r.r partials and stores rho_new in
+// the other slot (no block reads it here), sc[0] = sc[1] = rho_new, sc[3] = beta, sc[4] = ||r||,
+// the NaN flag bit 2, and copies sc[0..7] and the flag into pinned host memory for the host.
 __global__ void k_cg_update_p(float* __restrict__ p, const float* __restrict__ z, const uint8_t* __restrict__ cls,
-                              const float* __restrict__ sc, const int* __restrict__ rows,
+                              float* __restrict__ sc, int slot, int* __restrict__ flag,
+                              const double* __restrict__ part_rz, const double* __restrict__ part_rr, int npart,
+                              float* __restrict__ host_sc, int* __restrict__ host_flag, const int* __restrict__ rows,
                               const int* __restrict__ nrows, int d2) {
-  const float beta = sc[3];
+  const float newRZ = (float)sum_partials_block(part_rz, npart);
+  const float beta = newRZ / sc[6 + slot];
+  if (blockIdx.x == 0) {
+    const double rr = sum_partials_block(part_rr, npart);
+    if (threadIdx.x == 0) {
+      sc[6 + (slot ^ 1)] = newRZ;
+      sc[1] = newRZ;
+      sc[3] = beta;
+      sc[0] = newRZ;  // m_oldRZ = m_newRZ (:922)
+      sc[4] = sqrtf((float)rr);
+      int f = *flag;
+      if (isnan(beta) || isinf(beta)) f |= 2;
+      *flag = f;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) host_sc[k] = sc[k];
+      *host_flag = f;
+    }
+  }
   const RowIter it = row_iter(d2);
   const int nr = *nrows;
   if (!it.on) return;
@@ -968,64 +1015,18 @@ __global__ void k_cg_update_p(float* __restrict__ p, const float* __restrict__ z
   }
 }
 
-__device__ double sum_partials_block(const double* __restrict__ part, int n) {
-  __shared__ double s[kBlock];
-  double a = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) a += part[i];
-  s[threadIdx.x] = a;
-  __syncthreads();
-  for (int k = kBlock / 2; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
-    __syncthreads();
-  }
-  double t = s[0];
-  __syncthreads();
-  return t;
-}
 
-// scalar kernels (one block): sc = [oldRZ, newRZ, alpha, beta, residual, denom]
+// sc = [oldRZ, newRZ, alpha, beta, residual, denom, rho slot 0, rho slot 1]; k_sc_first seeds it
 // sc64 = [rz64, rr64, denom64, res64]
 __global__ void k_sc_first(const double* __restrict__ part, int nb, float* __restrict__ sc, double* __restrict__ sc64) {
   double rz = sum_partials_block(part, nb);
   double rr = sum_partials_block(part + nb, nb);
   if (threadIdx.x == 0) {
     sc[0] = (float)rz;
+    sc[6] = (float)rz;  // rho of the first iteration (slot 0)
     sc[4] = sqrtf((float)rr);
     sc64[0] = rz;
     sc64[1] = rr;
-  }
-}
-__global__ void k_sc_alpha(const double* __restrict__ part, int nb, float* __restrict__ sc, int* __restrict__ flag) {
-  double dn = sum_partials_block(part, nb);
-  if (threadIdx.x == 0) {
-    float denom = (float)dn;
-    float alpha = sc[0] / denom;
-    sc[5] = denom;
-    sc[2] = alpha;
-    if (isnan(alpha) || isinf(alpha)) atomicOr(flag, 1);
-  }
-}
-__global__ void k_sc_residual(const double* __restrict__ part, int nb, float* __restrict__ sc) {
-  double rr = sum_partials_block(part, nb);
-  if (threadIdx.x == 0) sc[4] = sqrtf((float)rr);
-}
-// the last scalar kernel of an iteration also stores sc[0..7] and the NaN flags into pinned host
-// memory (host_sc, host_flag), which the host reads after its stream synchronisation
-__global__ void k_sc_beta(const double* __restrict__ part, int nb, float* __restrict__ sc, int* __restrict__ flag,
-                          float* __restrict__ host_sc, int* __restrict__ host_flag) {
-  double rz = sum_partials_block(part, nb);
-  if (threadIdx.x == 0) {
-    float newRZ = (float)rz;
-    float beta = newRZ / sc[0];
-    sc[1] = newRZ;
-    sc[3] = beta;
-    sc[0] = newRZ;  // m_oldRZ = m_newRZ (:922)
-    int f = *flag;
-    if (isnan(beta) || isinf(beta)) f |= 2;
-    *flag = f;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) host_sc[k] = sc[k];
-    *host_flag = f;
   }
 }
 
@@ -1387,19 +1388,15 @@ int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, 
                      c.partial);
   hipLaunchKernelGGL(k_sc_first, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dsc64);
   hipMemcpyAsync(c.h_pin, c.dsc, 8 * sizeof(float), hipMemcpyDeviceToHost, c.s);
-  // An iteration's q = A p and alpha are queued before the host has read the previous residual,
-  // so the GPU runs them while the host waits: when that residual shows convergence they are
-  // discarded (they write only q, the partial sums, sc[2], sc[5] and the NaN flag, which is
-  // published by k_sc_beta of the next iteration and cleared per solve; x, r, p are untouched).
-  auto spmv_alpha = [&]() {
-    {
-      ProfScope ps(c, LH_PROF_SPMV);
-      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.q, c.p, P0, L0.rows, L0.nrows, c.partial);
-    }
-    hipLaunchKernelGGL(k_sc_alpha, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dflag + 2);
+  // An iteration's q = A p is queued before the host has read the previous residual, so the GPU
+  // runs it while the host waits: when that residual shows convergence it is discarded (it
+  // writes only q and the partial sums; x, r, p are untouched).
+  auto spmv_next = [&]() {
+    ProfScope ps(c, LH_PROF_SPMV);
+    hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.q, c.p, P0, L0.rows, L0.nrows, c.partial);
   };
   hipEventRecord(c.ev_iter, c.s);
-  spmv_alpha();
+  spmv_next();
   if (!hip_ok(c, hipEventSynchronize(c.ev_iter), "solve init")) return LH_ERR_DEVICE;
   float hsc[8];
   std::memcpy(hsc, c.h_pin, sizeof(hsc));
@@ -1417,19 +1414,19 @@ int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, 
       conv = true;
       break;
     }
-    // q = A p and alpha of this iteration are already queued
+    // q = A p of this iteration is already queued
     const int sym = c.cfg.precond == LH_PRECOND_MULTIGRID && c.cfg.smoother == LH_SMOOTHER_SYMMETRIC;
-    hipLaunchKernelGGL(k_cg_update_xr, dim3(nb), dim3(kBlock), 0, c.s, c.x, L0.r, c.p, c.q, L0.cls, c.dsc, L0.d[0],
-                       L0.d[1], L0.d[2], L0.rows, L0.nrows, c.partial, L0.z, sym);
-    hipLaunchKernelGGL(k_sc_residual, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc);
+    // partials: p.q and then r.z in [0, nb), r.r in [2 nb, 3 nb); rho alternates between sc[6], sc[7]
+    const int slot = it & 1;
+    double* part_rr = c.partial + 2 * (size_t)nb;
+    hipLaunchKernelGGL(k_cg_update_xr, dim3(nb), dim3(kBlock), 0, c.s, c.x, L0.r, c.p, c.q, L0.cls, c.dsc, slot,
+                       c.dflag + 2, c.partial, nb, L0.d[0], L0.d[1], L0.d[2], L0.rows, L0.nrows, part_rr, L0.z, sym);
     apply_precond(c, sym != 0);
     hipLaunchKernelGGL(k_cg_dot_rz, dim3(nb), dim3(kBlock), 0, c.s, L0.r, L0.z, L0.rows, L0.nrows, L0.d[2], c.partial);
-    hipLaunchKernelGGL(k_sc_beta, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dflag + 2, c.h_pin_dev,
-                       c.h_flag_dev);
-    hipLaunchKernelGGL(k_cg_update_p, dim3(nb), dim3(kBlock), 0, c.s, c.p, L0.z, L0.cls, c.dsc, L0.rows, L0.nrows,
-                       L0.d[2]);
+    hipLaunchKernelGGL(k_cg_update_p, dim3(nb), dim3(kBlock), 0, c.s, c.p, L0.z, L0.cls, c.dsc, slot, c.dflag + 2,
+                       c.partial, part_rr, nb, c.h_pin_dev, c.h_flag_dev, L0.rows, L0.nrows, L0.d[2]);
     hipEventRecord(c.ev_iter, c.s);
-    spmv_alpha();  // the next iteration's, running while the host waits for the event and decides
+    spmv_next();  // the next iteration's, running while the host waits for the event and decides
     if (!hip_ok(c, hipEventSynchronize(c.ev_iter), "solve iteration")) return LH_ERR_DEVICE;
     std::memcpy(hsc, c.h_pin, sizeof(hsc));
     residual = hsc[4];

@@ -145,7 +145,7 @@ struct Ctx {
   lh_solve_stats last_solve = {};          // statistics of the last step's pressure solve
   double* partial = nullptr;  // per-block FP64 partial sums
   int npartial = 0;
-  float* dsc = nullptr;       // device scalars: [0] oldRZ [1] newRZ [2] alpha [3] beta [4] residual ...
+  float* dsc = nullptr;       // device scalars: [0] oldRZ [1] newRZ [2] alpha [3] beta [4] residual [5] p.q [6..7] rho
   double* dsc64 = nullptr;    // device FP64 scalars
   unsigned* dmaxv = nullptr;  // max |v| bits
   int* dflag = nullptr;       // misc device flags (extrapolation progress, domain errors)
