@@ -45,7 +45,7 @@ struct BinP {
 };
 
 __global__ void k_bin_key(const float* __restrict__ pb, int64_t cap, int64_t P, BinP b, int* __restrict__ key,
-                          int* __restrict__ count, int* __restrict__ err) {
+                          int* __restrict__ count, int* __restrict__ bflag, int* __restrict__ err) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
     int h[3];
     bool bad = false;
@@ -62,6 +62,7 @@ __global__ void k_bin_key(const float* __restrict__ pb, int64_t cap, int64_t P, 
     int k = (h[0] * b.H[1] + h[1]) * b.H[2] + h[2];
     key[i] = k;
     atomicAdd(&count[k], 1);
+    bflag[k / kScanBins] = 1;
   }
 }
 
@@ -101,6 +102,92 @@ __global__ void k_bin_permute(const float* __restrict__ src, float* __restrict__
                          src[(size_t)(PA_C + 3 * a + 2) * cap + i], 0.f);
     }
   }
+}
+
+// ------------------------------------------------------------------ bin scan
+// bin_start = exclusive sum of bin_count over nbins + 1 entries.  The counts are zero outside
+// the bins that hold particles, so the scan works on blocks of kScanBins bins (256 threads x 8)
+// and reads the counts of the blocks k_bin_key flagged only: (1) k_scan_bsum sums each flagged
+// block (an unflagged block sums to 0), (2) an exclusive scan of the block sums gives the block
+// offsets, (3) k_scan_write writes every start: a flagged block's from its counts, an unflagged
+// block's all equal to its offset; it clears the flags for the next binning.
+static_assert(kScanBins == 256 * 8, "bin scan block layout");
+
+__device__ __forceinline__ void scan_load8(const int* __restrict__ count, size_t base, size_t n, int v[8]) {
+  if (base + 8 <= n) {
+    const int4 a = *reinterpret_cast<const int4*>(count + base);
+    const int4 b = *reinterpret_cast<const int4*>(count + base + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = base + j < n ? count[base + j] : 0;
+  }
+}
+__device__ __forceinline__ void scan_store8(int* __restrict__ start, size_t base, size_t n, const int v[8]) {
+  if (base + 8 <= n) {
+    *reinterpret_cast<int4*>(start + base) = make_int4(v[0], v[1], v[2], v[3]);
+    *reinterpret_cast<int4*>(start + base + 4) = make_int4(v[4], v[5], v[6], v[7]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (base + j < n) start[base + j] = v[j];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_bsum(const int* __restrict__ count, const int* __restrict__ bflag,
+                                                   int* __restrict__ bsum, size_t n) {
+  __shared__ int s_w[4];
+  const size_t b = blockIdx.x;
+  if (bflag[b] == 0) {  // uniform over the block
+    if (threadIdx.x == 0) bsum[b] = 0;
+    return;
+  }
+  int v[8];
+  scan_load8(count, b * kScanBins + threadIdx.x * 8, n, v);
+  int t = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) bsum[b] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+}
+
+__global__ __launch_bounds__(256) void k_scan_write(const int* __restrict__ count, int* __restrict__ bflag,
+                                                    const int* __restrict__ boff, int* __restrict__ start, size_t n) {
+  __shared__ int s_w[4];
+  const size_t b = blockIdx.x;
+  const size_t base = b * kScanBins + threadIdx.x * 8;
+  const int off = boff[b];
+  int v[8];
+  if (bflag[b] == 0) {  // uniform over the block: no particle in it
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = off;
+    scan_store8(start, base, n, v);
+    return;
+  }
+  scan_load8(count, base, n, v);
+  int t = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int c = v[j];
+    v[j] = t;
+    t += c;
+  }
+  // inclusive scan of the thread totals over the wave, then over the 4 waves
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int inc = t;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(inc, o);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) s_w[wave] = inc;
+  __syncthreads();  // every thread has read bflag[b] before this point
+  int pre = off + inc - t;
+  for (int w = 0; w < wave; ++w) pre += s_w[w];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] += pre;
+  scan_store8(start, base, n, v);
+  if (threadIdx.x == 0) bflag[b] = 0;
 }
 
 // ------------------------------------------------------------------ P2G
@@ -794,21 +881,23 @@ __global__ void k_correct_commit(float* __restrict__ pb, int64_t cap, int64_t P,
 
 }  // namespace
 
-// ------------------------------------------------------------------ bin scan
-// bin_start = exclusive sum of bin_count over nbins + 1 entries (bin_count[nbins] stays 0,
-// so bin_start[nbins] is the particle count): one decoupled look-back pass, which reads the
-// 512^3 counts and writes the starts once each.
-size_t bin_scan_bytes(size_t nbins) {
+// ------------------------------------------------------------------ bin scan (host)
+// hipCUB scans the block sums: nbins / kScanBins + 1 entries
+size_t bin_scan_bytes(size_t nblocks) {
   size_t bytes = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int*)nullptr, (int*)nullptr, (int)(nbins + 1));
+  hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int*)nullptr, (int*)nullptr, (int)nblocks);
   return bytes;
 }
 int bin_scan(Ctx& c) {
+  const size_t n = c.nbins + 1;
+  hipLaunchKernelGGL(k_scan_bsum, dim3((unsigned)c.nbblk), dim3(256), 0, c.s, c.bin_count, c.bin_bflag, c.bin_bsum, n);
   size_t bytes = c.scan_bytes;
-  return hip_ok(c, hipcub::DeviceScan::ExclusiveSum(c.scan_tmp, bytes, c.bin_count, c.bin_start, (int)(c.nbins + 1), c.s),
-                "bin scan")
-             ? LH_OK
-             : LH_ERR_DEVICE;
+  if (!hip_ok(c, hipcub::DeviceScan::ExclusiveSum(c.scan_tmp, bytes, c.bin_bsum, c.bin_boff, (int)c.nbblk, c.s),
+              "bin scan"))
+    return LH_ERR_DEVICE;
+  hipLaunchKernelGGL(k_scan_write, dim3((unsigned)c.nbblk), dim3(256), 0, c.s, c.bin_count, c.bin_bflag, c.bin_boff,
+                     c.bin_start, n);
+  return LH_OK;
 }
 
 int run_binning(Ctx& c) {
@@ -823,7 +912,7 @@ int run_binning(Ctx& c) {
   hipMemsetAsync(c.dflag + 1, 0, sizeof(int), c.s);
   unsigned gp = grid_for((size_t)c.P);
   hipLaunchKernelGGL(k_bin_key, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, b, c.pkey, c.bin_count,
-                     c.dflag + 1);
+                     c.bin_bflag, c.dflag + 1);
   int rc = bin_scan(c);
   if (rc) return rc;
   hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, c.s, c.pkey, c.P, c.bin_start, c.bin_count, c.pslot);

@@ -204,8 +204,10 @@ int alloc_ctx(Ctx& c) {
     c.err = "grid too large for 32-bit bin offsets";
     return LH_ERR_CONFIG;
   }
-  c.scan_bytes = bin_scan_bytes(c.nbins);
+  c.nbblk = div_up(c.nbins + 1, (size_t)kScanBins);
+  c.scan_bytes = bin_scan_bytes(c.nbblk);
   ok = ok && dalloc(c, c.bin_count, c.nbins + 1) && dalloc(c, c.bin_start, c.nbins + 1) &&
+       dalloc(c, c.bin_bflag, c.nbblk) && dalloc(c, c.bin_bsum, c.nbblk) && dalloc(c, c.bin_boff, c.nbblk) &&
        dalloc(c, c.scan_tmp, c.scan_bytes);
   // multigrid levels
   int L = std::max(1, c.cfg.num_level);
@@ -245,6 +247,7 @@ int alloc_ctx(Ctx& c) {
   }
   hipMemcpyAsync(c.d_sdf, c.cfg.sdf.data(), c.cfg.sdf.size() * sizeof(float), hipMemcpyHostToDevice, c.s);
   hipMemsetAsync(c.bin_count, 0, (c.nbins + 1) * sizeof(int), c.s);  // [nbins] stays 0 (the scan total)
+  hipMemsetAsync(c.bin_bflag, 0, c.nbblk * sizeof(int), c.s);
   hipMemsetAsync(c.is_valid, 0, c.nnode, c.s);
   hipMemsetAsync(c.dflag, 0, 16 * sizeof(int), c.s);
   launch_fill_f32(c, c.solid_phi, c.nnode, 0.f);
@@ -281,7 +284,7 @@ void free_ctx(Ctx& c) {
     F(c.lw[a]);
   }
   F(c.pressure); F(c.phi); F(c.rhs); F(c.solid_phi); F(c.is_valid); F(c.xstate); F(c.d_sdf);
-  F(c.bin_count); F(c.bin_start); F(c.scan_tmp); F(c.pkey); F(c.pslot);
+  F(c.bin_count); F(c.bin_start); F(c.bin_bflag); F(c.bin_bsum); F(c.bin_boff); F(c.scan_tmp); F(c.pkey); F(c.pslot);
   F(c.pbuf[0]); F(c.pbuf[1]); F(c.ppack); F(c.cv_flag);
   for (auto& q : c.lv) {
     F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2); F(q.rows); F(q.nrows);

@@ -128,7 +128,13 @@ struct Ctx {
   int* bin_start = nullptr;  // nbins + 1
   int* pkey = nullptr;       // per particle bin key
   int* pslot = nullptr;      // per particle provisional slot
-  uint8_t* scan_tmp = nullptr;  // temporary storage of the bin scan
+  // the bin scan runs over blocks of kScanBins bins: a flag per block (set by the key kernel
+  // for every block that holds a particle, cleared by the scan), the block sums and offsets
+  int* bin_bflag = nullptr;
+  int* bin_bsum = nullptr;
+  int* bin_boff = nullptr;
+  size_t nbblk = 0;
+  uint8_t* scan_tmp = nullptr;  // temporary storage of the scan of the block sums
   size_t scan_bytes = 0;
   bool bins_valid = false;
   // solver (CGSolver<3>)
@@ -223,7 +229,8 @@ int run_extrapolate(Ctx& c);
 void launch_divergence(Ctx& c, double* h_mean, double* h_linf, int* h_count);
 void launch_gravity(Ctx& c, float dt);
 // ---- launchers (k_particles.hip)
-size_t bin_scan_bytes(size_t nbins);  // temporary storage of the bin scan
+constexpr int kScanBins = 2048;        // bins per block of the bin scan
+size_t bin_scan_bytes(size_t nblocks);  // temporary storage of the scan of nblocks block sums
 int run_binning(Ctx& c);  // LH_OK or LH_ERR_DOMAIN
 void launch_liquid_phi(Ctx& c);
 void launch_p2g(Ctx& c, bool fuse_body_forces, float dt);
