@@ -139,27 +139,49 @@ __global__ void k_row_flags(const uint8_t* __restrict__ cls, int* __restrict__ f
   const bool hit = __ballot(any) != 0;
   if (lane == 0) flag[row] = hit ? 1 : 0;
 }
-// ascending list of the flagged rows (one block of 1024 threads; deterministic order)
+// ascending list of the flagged rows (one block of 1024 threads; deterministic order).  Wave w
+// owns a contiguous run of 64-flag segments; each segment is one coalesced load.  Pass 1 counts
+// the wave's flags, the block forms the waves' offsets, pass 2 writes each flagged row at its
+// offset plus its rank among the flagged lanes of the segment (ballot), loading 8 segments ahead.
 __global__ __launch_bounds__(1024) void k_row_compact(const int* __restrict__ flag, int nrow, int* __restrict__ rows,
                                                       int* __restrict__ nrows) {
-  __shared__ int s[1024];
-  const int t = threadIdx.x;
-  const int chunk = (nrow + 1023) / 1024;
-  const int b = t * chunk, e = min(nrow, b + chunk);
+  __shared__ int s_w[16];
+  const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+  const int nseg = (nrow + 63) >> 6;
+  const int per = (nseg + 15) >> 4;
+  const int s0 = min(nseg, w * per), s1 = min(nseg, s0 + per);
   int cnt = 0;
-  for (int i = b; i < e; ++i) cnt += flag[i];
-  s[t] = cnt;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = (t >= off) ? s[t - off] : 0;
-    __syncthreads();
-    s[t] += v;
-    __syncthreads();
+#pragma unroll 8
+  for (int sg = s0; sg < s1; ++sg) {
+    const int i = sg * 64 + lane;
+    cnt += (i < nrow && flag[i] != 0) ? 1 : 0;
   }
-  int pos = s[t] - cnt;
-  for (int i = b; i < e; ++i)
-    if (flag[i]) rows[pos++] = i;
-  if (t == 1023) *nrows = s[1023];
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+  if (lane == 0) s_w[w] = cnt;
+  __syncthreads();
+  int pos = 0;
+  for (int k = 0; k < w; ++k) pos += s_w[k];
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int sg = s0; sg < s1; sg += 8) {
+    int fv[8];  // 8 segments' flags loaded before the ballots
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = (sg + u) * 64 + lane;
+      fv[u] = (sg + u < s1 && i < nrow) ? flag[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const bool f = fv[u] != 0;
+      const unsigned long long m = __ballot(f);
+      if (f) rows[pos + __popcll(m & below)] = (sg + u) * 64 + lane;
+      pos += __popcll(m);
+    }
+  }
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int k = 0; k < 16; ++k) tot += s_w[k];
+    *nrows = tot;
+  }
 }
 
 // qflag[X, Y] of a coarse level: its own liquid row, or a child row (2X+i, 2Y+j) that is a
