@@ -62,7 +62,9 @@ def algorithmic_bytes(kernel, n0, P, active, fused, rows_sweep=frozenset()):
     smooth_l1, smooth_l2: the same per cell of the rows holding liquid on level 1, and on
                level 2 (the coarsest of the three-level hierarchy): 14 B per colour pass, 28 B
                per k_sweep_rows or k_tile_sweep launch (a whole sweep);
-    spmv:      p read, q write, 4 coefficient floats -> 24 B per cell of those rows;
+    spmv:      the p update fused with q = A p (k_cg_p_spmv, all but the first SpMV of a solve):
+               z and p read, new p and q written, 4 coefficient floats -> 32 B per cell of
+               those rows (the plain first SpMV moves 24 B);
     p2g:       one component: 20 B/particle (position, velocity component,
                affine column) + 4 B per face written (~n0 faces);
     g2p:       72 B/particle (read position, write position, velocity, C) + 3
@@ -74,7 +76,7 @@ def algorithmic_bytes(kernel, n0, P, active, fused, rows_sweep=frozenset()):
         "smooth_l0": 28.0 * n0 if fused else 14.0 * active[0],
         "smooth_l1": (28.0 if 1 in rows_sweep else 14.0) * active.get(1, 0),
         "smooth_l2": (28.0 if 2 in rows_sweep else 14.0) * active.get(2, 0),
-        "spmv": 24.0 * active[0],
+        "spmv": 32.0 * active[0],
         "p2g": 20.0 * P + 4.0 * n0,
         "g2p": 72.0 * P + 12.0 * n0,
         "liquid_phi": 12.0 * P + 4.0 * n0,
@@ -83,7 +85,7 @@ def algorithmic_bytes(kernel, n0, P, active, fused, rows_sweep=frozenset()):
 
 # instrumented kernel -> its symbol in the library (rocprofv3 kernel names); the coarse levels'
 # symbols depend on the configuration (level_symbol)
-KERNEL_SYMBOL = {"smooth_l0": "k_smooth<true>", "spmv": "k_cg_spmv", "p2g": "k_p2g", "g2p": "k_g2p_advect",
+KERNEL_SYMBOL = {"smooth_l0": "k_smooth<true>", "spmv": "k_cg_p_spmv", "p2g": "k_p2g", "g2p": "k_g2p_advect",
                  "liquid_phi": "k_liquid_phi"}
 
 

@@ -983,9 +983,22 @@ __global__ void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r, con
   block_partials<1>(v, partial);
 }
 
-// partial r.z (r is zero off the liquid cells)
+// partial r.z (r is zero off the liquid cells).  Block 0 also forms the residual norm from the
+// r.r partials of the x/r update, sc[4] = ||r|| (:903-906), and copies sc[0..7] and the NaN flag
+// into pinned host memory: the host's per-iteration read.
 __global__ void k_cg_dot_rz(const float* __restrict__ r, const float* __restrict__ z, const int* __restrict__ rows,
-                            const int* __restrict__ nrows, int d2, double* __restrict__ partial) {
+                            const int* __restrict__ nrows, int d2, double* __restrict__ partial,
+                            const double* __restrict__ part_rr, int npart, float* __restrict__ sc,
+                            const int* __restrict__ flag, float* __restrict__ host_sc, int* __restrict__ host_flag) {
+  if (blockIdx.x == 0) {  // uniform over the block
+    const double rr = sum_partials_block(part_rr, npart);
+    if (threadIdx.x == 0) {
+      sc[4] = sqrtf((float)rr);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) host_sc[k] = sc[k];
+      *host_flag = *flag;
+    }
+  }
   double v[1] = {0.0};
   const RowIter it = row_iter(d2);
   const int nr = *nrows;
@@ -997,44 +1010,80 @@ __global__ void k_cg_dot_rz(const float* __restrict__ r, const float* __restrict
   block_partials<1>(v, partial);
 }
 
-// p = z + beta p on liquid cells (:913-921)
-// beta = rho_new / rho (:913-922) from the r.z partials in every block, then p = z + beta p on the
-// liquid cells.  Block 0 also forms the residual norm from the r.r partials and stores rho_new in
-// the other slot (no block reads it here), sc[0] = sc[1] = rho_new, sc[3] = beta, sc[4] = ||r||,
-// the NaN flag bit 2, and copies sc[0..7] and the flag into pinned host memory for the host.
-__global__ void k_cg_update_p(float* __restrict__ p, const float* __restrict__ z, const uint8_t* __restrict__ cls,
-                              float* __restrict__ sc, int slot, int* __restrict__ flag,
-                              const double* __restrict__ part_rz, const double* __restrict__ part_rr, int npart,
-                              float* __restrict__ host_sc, int* __restrict__ host_flag, const int* __restrict__ rows,
-                              const int* __restrict__ nrows, int d2) {
+// p = z + beta p (:913-921) fused with the next iteration's q = A p and p.q (:879-881).
+// beta = rho_new / rho from the r.z partials in every block; block 0 stores rho_new in the other
+// slot (no block reads it here), sc[0] = sc[1] = rho_new (m_oldRZ = m_newRZ, :922), sc[3] = beta
+// and the NaN flag bit 2.  The new p goes to pout (pin is read, so no thread sees another's
+// write): on every liquid cell of the rows, z + beta pin; a cell's SpMV takes its neighbours' new
+// values from z and pin the same way (a cell with cls == 0 keeps pin, which no later stage reads
+// with a nonzero weight).  q and the p.q partials are those of k_cg_spmv on the new p, with the
+// same thread mapping and order.
+__global__ void k_cg_p_spmv(float* __restrict__ pout, float* __restrict__ q, const float* __restrict__ pin,
+                            const float* __restrict__ z, LvP L, const int* __restrict__ rows,
+                            const int* __restrict__ nrows, float* __restrict__ sc, int slot, int* __restrict__ flag,
+                            const double* __restrict__ part_rz, int npart, double* __restrict__ partial) {
   const float newRZ = (float)sum_partials_block(part_rz, npart);
   const float beta = newRZ / sc[6 + slot];
-  if (blockIdx.x == 0) {
-    const double rr = sum_partials_block(part_rr, npart);
-    if (threadIdx.x == 0) {
-      sc[6 + (slot ^ 1)] = newRZ;
-      sc[1] = newRZ;
-      sc[3] = beta;
-      sc[0] = newRZ;  // m_oldRZ = m_newRZ (:922)
-      sc[4] = sqrtf((float)rr);
-      int f = *flag;
-      if (isnan(beta) || isinf(beta)) f |= 2;
-      *flag = f;
-#pragma unroll
-      for (int k = 0; k < 8; ++k) host_sc[k] = sc[k];
-      *host_flag = f;
-    }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[6 + (slot ^ 1)] = newRZ;
+    sc[1] = newRZ;
+    sc[3] = beta;
+    sc[0] = newRZ;
+    if (isnan(beta) || isinf(beta)) atomicOr(flag, 2);
   }
+  const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
+  const size_t sx = (size_t)d1 * d2, sy = (size_t)d2;
+  double v[1] = {0.0};
   const RowIter it = row_iter(d2);
+  const int kcap = d0 * d1 - 1, kstep = gridDim.x * it.R;
+  int k = blockIdx.x * it.R + it.sub;
+  int row_next = rows[min(k, kcap)];
   const int nr = *nrows;
-  if (!it.on) return;
-  for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
-    const size_t base = (size_t)rows[k] * d2;
-    for (int zi = it.z0; zi < d2; zi += it.zs) {
-      const size_t i = base + zi;
-      if (cls[i] != 0) p[i] = z[i] + beta * p[i];
+  if (it.on)
+    for (; k < nr; k += kstep) {
+      const int row = row_next;
+      row_next = rows[min(k + kstep, kcap)];
+      const int xi = row / d1, yi = row - xi * d1;
+      const bool irow = xi >= 1 && xi <= d0 - 2 && yi >= 1 && yi <= d1 - 2;
+      const size_t base = (size_t)row * d2;
+      for (int zi = it.z0; zi < d2; zi += it.zs) {
+        const size_t i = base + zi;
+        if (!(irow && zi >= 1 && zi <= d2 - 2)) {  // a cell on the box faces: the p update only
+          if (L.cls[i] != 0) pout[i] = z[i] + beta * pin[i];
+          continue;
+        }
+        // interior cell: every neighbour exists
+        int kc = L.cls[i], kxp = L.cls[i + sx], kxm = L.cls[i - sx], kyp = L.cls[i + sy], kym = L.cls[i - sy],
+            kzp = L.cls[i + 1], kzm = L.cls[i - 1];
+        float zc = z[i], zxp = z[i + sx], zxm = z[i - sx], zyp = z[i + sy], zym = z[i - sy], zzp = z[i + 1],
+              zzm = z[i - 1];
+        float pc = pin[i], pxp = pin[i + sx], pxm = pin[i - sx], pyp = pin[i + sy], pym = pin[i - sy],
+              pzp = pin[i + 1], pzm = pin[i - 1];
+        float ad = L.adiag[i], txp = L.t0[i], txm = L.t0[i - sx], typ = L.t1[i], tym = L.t1[i - sy], tzp = L.t2[i],
+              tzm = L.t2[i - 1];
+        // the loads complete here, together (see get_ax_full)
+        asm volatile("" : "+v"(kc), "+v"(kxp), "+v"(kxm), "+v"(kyp), "+v"(kym), "+v"(kzp), "+v"(kzm), "+v"(zc),
+                     "+v"(zxp), "+v"(zxm), "+v"(zyp), "+v"(zym), "+v"(zzp), "+v"(zzm));
+        asm volatile("" : "+v"(pc), "+v"(pxp), "+v"(pxm), "+v"(pyp), "+v"(pym), "+v"(pzp), "+v"(pzm), "+v"(ad),
+                     "+v"(txp), "+v"(txm), "+v"(typ), "+v"(tym), "+v"(tzp), "+v"(tzm));
+        if (kc == 0) continue;
+        const float nc = zc + beta * pc;
+        const float nxp = kxp != 0 ? zxp + beta * pxp : pxp, nxm = kxm != 0 ? zxm + beta * pxm : pxm;
+        const float nyp = kyp != 0 ? zyp + beta * pyp : pyp, nym = kym != 0 ? zym + beta * pym : pym;
+        const float nzp = kzp != 0 ? zzp + beta * pzp : pzp, nzm = kzm != 0 ? zzm + beta * pzm : pzm;
+        float res = nc * ad;
+        res = kxp == 2 ? res + nxp * (-txp) : res;
+        res = kxm == 2 ? res + nxm * (-txm) : res;
+        res = kyp == 2 ? res + nyp * (-typ) : res;
+        res = kym == 2 ? res + nym * (-tym) : res;
+        res = kzp == 2 ? res + nzp * (-tzp) : res;
+        res = kzm == 2 ? res + nzm * (-tzm) : res;
+        pout[i] = nc;
+        q[i] = res;
+        v[0] += (double)nc * (double)res;
+      }
     }
-  }
+  block_partials<1>(v, partial);
 }
 
 
@@ -1410,15 +1459,18 @@ int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, 
                      c.partial);
   hipLaunchKernelGGL(k_sc_first, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc, c.dsc64);
   hipMemcpyAsync(c.h_pin, c.dsc, 8 * sizeof(float), hipMemcpyDeviceToHost, c.s);
-  // An iteration's q = A p is queued before the host has read the previous residual, so the GPU
-  // runs it while the host waits: when that residual shows convergence it is discarded (it
-  // writes only q and the partial sums; x, r, p are untouched).
-  auto spmv_next = [&]() {
-    ProfScope ps(c, LH_PROF_SPMV);
-    hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.q, c.p, P0, L0.rows, L0.nrows, c.partial);
-  };
+  // partials: r.z in [0, nb), p.q in [nb, 2 nb), r.r in [2 nb, 3 nb) (no kernel reads the range
+  // it writes); rho alternates between sc[6] and sc[7].  An iteration's p update and the next
+  // q = A p are queued before the host has read the residual, so the GPU runs them while the host
+  // waits: when that residual shows convergence they are discarded (they write only p2, q, the
+  // partial sums, sc[0, 1, 3], the other rho slot and the NaN flag; x and r are untouched).
+  double* part_pq = c.partial + (size_t)nb;
+  double* part_rr = c.partial + 2 * (size_t)nb;
   hipEventRecord(c.ev_iter, c.s);
-  spmv_next();
+  {
+    ProfScope ps(c, LH_PROF_SPMV);
+    hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.q, c.p, P0, L0.rows, L0.nrows, part_pq);
+  }
   if (!hip_ok(c, hipEventSynchronize(c.ev_iter), "solve init")) return LH_ERR_DEVICE;
   float hsc[8];
   std::memcpy(hsc, c.h_pin, sizeof(hsc));
@@ -1438,17 +1490,19 @@ int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, 
     }
     // q = A p of this iteration is already queued
     const int sym = c.cfg.precond == LH_PRECOND_MULTIGRID && c.cfg.smoother == LH_SMOOTHER_SYMMETRIC;
-    // partials: p.q and then r.z in [0, nb), r.r in [2 nb, 3 nb); rho alternates between sc[6], sc[7]
     const int slot = it & 1;
-    double* part_rr = c.partial + 2 * (size_t)nb;
     hipLaunchKernelGGL(k_cg_update_xr, dim3(nb), dim3(kBlock), 0, c.s, c.x, L0.r, c.p, c.q, L0.cls, c.dsc, slot,
-                       c.dflag + 2, c.partial, nb, L0.d[0], L0.d[1], L0.d[2], L0.rows, L0.nrows, part_rr, L0.z, sym);
+                       c.dflag + 2, part_pq, nb, L0.d[0], L0.d[1], L0.d[2], L0.rows, L0.nrows, part_rr, L0.z, sym);
     apply_precond(c, sym != 0);
-    hipLaunchKernelGGL(k_cg_dot_rz, dim3(nb), dim3(kBlock), 0, c.s, L0.r, L0.z, L0.rows, L0.nrows, L0.d[2], c.partial);
-    hipLaunchKernelGGL(k_cg_update_p, dim3(nb), dim3(kBlock), 0, c.s, c.p, L0.z, L0.cls, c.dsc, slot, c.dflag + 2,
-                       c.partial, part_rr, nb, c.h_pin_dev, c.h_flag_dev, L0.rows, L0.nrows, L0.d[2]);
+    hipLaunchKernelGGL(k_cg_dot_rz, dim3(nb), dim3(kBlock), 0, c.s, L0.r, L0.z, L0.rows, L0.nrows, L0.d[2], c.partial,
+                       part_rr, nb, c.dsc, c.dflag + 2, c.h_pin_dev, c.h_flag_dev);
     hipEventRecord(c.ev_iter, c.s);
-    spmv_next();  // the next iteration's, running while the host waits for the event and decides
+    {  // p = z + beta p and the next iteration's q = A p, while the host waits for the event and decides
+      ProfScope ps(c, LH_PROF_SPMV);
+      hipLaunchKernelGGL(k_cg_p_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.p2, c.q, c.p, L0.z, P0, L0.rows, L0.nrows,
+                         c.dsc, slot, c.dflag + 2, c.partial, nb, part_pq);
+    }
+    std::swap(c.p, c.p2);
     if (!hip_ok(c, hipEventSynchronize(c.ev_iter), "solve iteration")) return LH_ERR_DEVICE;
     std::memcpy(hsc, c.h_pin, sizeof(hsc));
     residual = hsc[4];

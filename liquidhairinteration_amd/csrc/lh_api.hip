@@ -231,7 +231,7 @@ int alloc_ctx(Ctx& c) {
          dalloc(c, q.tiles, div_up(q.d[0], 4) * div_up(q.d[1], 4)) && dalloc(c, q.ntiles, 1);
   }
   c.npartial = 1024;
-  ok = ok && dalloc(c, c.p, c.ncell) && dalloc(c, c.x, c.ncell) && dalloc(c, c.q, c.ncell) &&
+  ok = ok && dalloc(c, c.p, c.ncell) && dalloc(c, c.p2, c.ncell) && dalloc(c, c.x, c.ncell) && dalloc(c, c.q, c.ncell) &&
        dalloc(c, c.partial, 4 * (size_t)c.npartial) && dalloc(c, c.dsc, 16) && dalloc(c, c.dsc64, 16) &&
        dalloc(c, c.dmaxv, 4) && dalloc(c, c.dflag, 16);
   if (!ok) return LH_ERR_OOM;
@@ -248,6 +248,7 @@ int alloc_ctx(Ctx& c) {
   hipMemcpyAsync(c.d_sdf, c.cfg.sdf.data(), c.cfg.sdf.size() * sizeof(float), hipMemcpyHostToDevice, c.s);
   hipMemsetAsync(c.bin_count, 0, (c.nbins + 1) * sizeof(int), c.s);  // [nbins] stays 0 (the scan total)
   hipMemsetAsync(c.bin_bflag, 0, c.nbblk * sizeof(int), c.s);
+  launch_fill_f32(c, c.p2, c.ncell, 0.f);
   hipMemsetAsync(c.is_valid, 0, c.nnode, c.s);
   hipMemsetAsync(c.dflag, 0, 16 * sizeof(int), c.s);
   launch_fill_f32(c, c.solid_phi, c.nnode, 0.f);
@@ -290,7 +291,7 @@ void free_ctx(Ctx& c) {
     F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2); F(q.rows); F(q.nrows);
     F(q.aflag); F(q.qrows); F(q.nqrows); F(q.qflag); F(q.zrows); F(q.nzrows); F(q.zflag); F(q.tflag); F(q.tiles); F(q.ntiles);
   }
-  F(c.p); F(c.x); F(c.q); F(c.partial); F(c.dsc); F(c.dsc64); F(c.dmaxv); F(c.dflag);
+  F(c.p); F(c.p2); F(c.x); F(c.q); F(c.partial); F(c.dsc); F(c.dsc64); F(c.dmaxv); F(c.dflag);
   F(c.x64); F(c.r64); F(c.rf); F(c.b_user); F(c.x0_user);
   F(c.xv_snap); F(c.xv_final); F(c.xcount); F(c.xlist[0]); F(c.xlist[1]); F(c.xlist[2]);
   nn_free(c.net);

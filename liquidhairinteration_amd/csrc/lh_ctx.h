@@ -140,6 +140,7 @@ struct Ctx {
   // solver (CGSolver<3>)
   std::vector<Level> lv;
   float *p = nullptr, *x = nullptr, *q = nullptr;
+  float* p2 = nullptr;  // ping-pong partner of p: the fused p update + SpMV writes the new p here
   double *x64 = nullptr, *r64 = nullptr;  // FP64 refinement solution and true residual
   float* rf = nullptr;                     // FP32 right-hand side of a refinement correction
   float* b_user = nullptr;                 // right-hand side handed in through lh_solve_rhs
