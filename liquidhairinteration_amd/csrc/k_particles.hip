@@ -477,35 +477,81 @@ __global__ void __launch_bounds__(256) k_liquid_phi(float* __restrict__ phi, con
     const int hz0 = max(0, 2 * (zi - dW2)), hz1 = min(H2 - 1, 2 * (zi + dW2) + 1);
     // bound slack: 1e-3 dx plus the float error of world coordinates
     const float margin = 1e-3f * g.dx[0] + 1e-6f * (fabsf(cpx) + fabsf(cpy) + fabsf(cpz));
-    for (int pass = 0; pass < 2; ++pass)
-      for (int hx = hx0; hx <= hx1; ++hx)
-        for (int hy = hy0; hy <= hy1; ++hy) {
+    auto phi_of = [&](int s) {
+      const float4 A = rec[2 * (size_t)s];  // x y z of the bin-ordered particle record
+      return norm3(cpx - A.x, cpy - A.y, cpz - A.z) - radius;
+    };
+    auto run = [&](int s, int s1) {
+      for (; s + 3 < s1; s += 4) {  // four loads in flight; the min is exact in any order
+        const float p0 = phi_of(s), p1 = phi_of(s + 1), p2 = phi_of(s + 2), p3 = phi_of(s + 3);
+        const float m01 = (p0 < p1) ? p0 : p1, m23 = (p2 < p3) ? p2 : p3;
+        const float m = (m01 < m23) ? m01 : m23;
+        best = (best < m) ? best : m;
+      }
+      for (; s < s1; ++s) {
+        const float ph = phi_of(s);
+        best = (best < ph) ? best : ph;
+      }
+    };
+    // the cell's own 8 half-cells first (a tight bound early), then the rest of its own 2 x 2
+    // half-cell rows, cut in z as below; starts loaded together
+    {
+      int S0[4], S1[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const size_t k0 = ((size_t)(2 * xi + (j >> 1)) * H1 + 2 * yi + (j & 1)) * H2;
+        S0[j] = start[k0 + 2 * zi];
+        S1[j] = start[k0 + 2 * zi + 2];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) run(S0[j], S1[j]);
+      const float lim = best + margin + radius;
+      const int w2 = (int)floorf(2.f * lim / g.dx[2]);
+      const int lo = max(hz0, 2 * zi - w2), hi = min(hz1, 2 * zi + 1 + w2);
+      int A0[4], A1[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const size_t k0 = ((size_t)(2 * xi + (j >> 1)) * H1 + 2 * yi + (j & 1)) * H2;
+        A0[j] = start[k0 + lo];
+        S0[j] = start[k0 + 2 * zi];
+        A1[j] = start[k0 + 2 * zi + 2];
+        S1[j] = start[k0 + hi + 1];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        run(A0[j], S0[j]);
+        run(A1[j], S1[j]);
+      }
+    }
+    // the other rows, pruned by the distance bound against the running minimum (it only
+    // decreases, so a row pruned once stays prunable); per x half-cell, the starts of up to 10
+    // rows are loaded together
+    constexpr int kRowBatch = 10;
+    for (int hx = hx0; hx <= hx1; ++hx) {
+      const float gx = fmaxf(0.f, fmaxf(0.5f * hx - (xi + 0.5f), (xi + 0.5f) - 0.5f * (hx + 1))) * g.dx[0];
+      for (int hyb = hy0; hyb <= hy1; hyb += kRowBatch) {
+        int S0[kRowBatch], S1[kRowBatch];
+        // a particle lowers the minimum only within lim of the cell centre: rows farther than lim
+        // in (x, y) are skipped, and a row's z half-cells are cut to those within
+        // sqrt(lim^2 - gx^2 - gy^2) in z
+        const float lim = best + margin + radius;
+#pragma unroll
+        for (int j = 0; j < kRowBatch; ++j) {
+          const int hy = hyb + j;
+          const float gy = fmaxf(0.f, fmaxf(0.5f * hy - (yi + 0.5f), (yi + 0.5f) - 0.5f * (hy + 1))) * g.dx[1];
           const bool own = (hx >> 1) == xi && (hy >> 1) == yi;
-          if (own != (pass == 0)) continue;
-          if (pass == 1) {
-            // gap from the cell centre (grid units x+0.5) to half-cell [h/2, (h+1)/2] per axis
-            float gx = fmaxf(0.f, fmaxf(0.5f * hx - (xi + 0.5f), (xi + 0.5f) - 0.5f * (hx + 1))) * g.dx[0];
-            float gy = fmaxf(0.f, fmaxf(0.5f * hy - (yi + 0.5f), (yi + 0.5f) - 0.5f * (hy + 1))) * g.dx[1];
-            if (sqrtf(gx * gx + gy * gy) - radius > best + margin) continue;
-          }
-          size_t k0 = ((size_t)hx * H1 + hy) * H2;
-          int s0 = start[k0 + hz0], s1 = start[k0 + hz1 + 1];
-          auto phi_of = [&](int s) {
-            const float4 A = rec[2 * (size_t)s];  // x y z of the bin-ordered particle record
-            return norm3(cpx - A.x, cpy - A.y, cpz - A.z) - radius;
-          };
-          int s = s0;
-          for (; s + 3 < s1; s += 4) {  // four loads in flight; the min is exact in any order
-            const float p0 = phi_of(s), p1 = phi_of(s + 1), p2 = phi_of(s + 2), p3 = phi_of(s + 3);
-            const float m01 = (p0 < p1) ? p0 : p1, m23 = (p2 < p3) ? p2 : p3;
-            const float m = (m01 < m23) ? m01 : m23;
-            best = (best < m) ? best : m;
-          }
-          for (; s < s1; ++s) {
-            const float ph = phi_of(s);
-            best = (best < ph) ? best : ph;
-          }
+          const float rem = lim * lim - gx * gx - gy * gy;
+          const bool use = hy <= hy1 && !own && rem >= 0.f;
+          const int w2 = (int)floorf(2.f * sqrtf(fmaxf(rem, 0.f)) / g.dx[2]);
+          const int lo = max(hz0, 2 * zi - w2), hi = min(hz1, 2 * zi + 1 + w2);
+          const size_t k0 = ((size_t)hx * H1 + min(hy, hy1)) * H2;
+          S0[j] = use ? start[k0 + lo] : 0;
+          S1[j] = use ? start[k0 + hi + 1] : 0;
         }
+#pragma unroll
+        for (int j = 0; j < kRowBatch; ++j) run(S0[j], S1[j]);
+      }
+    }
   }
   phi[((size_t)xi * N1 + yi) * N2 + zi] = best;
 }
