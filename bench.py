@@ -93,7 +93,7 @@ def smooth_grid(d):
     """work-items of one k_smooth launch on a level of dims d (the library's smooth_blocks)"""
     H = (d[2] + 1) // 2
     R = 1 if H >= 256 else 256 // H
-    return max(1, min(-(-d[0] * d[1] // R), 8192)) * 256
+    return -(-max(1, min(-(-d[0] * d[1] // R), 8192)) // 8) * 8 * 256
 
 
 def row_sweep_grid(d):

@@ -129,6 +129,22 @@ __device__ __forceinline__ RowIter row_iter(int d2) {
   return it;
 }
 
+// A row list split into 8 contiguous runs, one per XCD: workgroups are dispatched to the 8 XCDs
+// round-robin, so workgroup b serves run b % 8, and a row's x/y neighbours (adjacent and nearby
+// list entries) are mostly read through the same XCD's L2.  The grid is a multiple of 8; R list
+// entries per workgroup pass, sub = the thread's entry among them.
+struct XcdRun {
+  int k, end, step;
+};
+__device__ __forceinline__ XcdRun xcd_run(int nr, int R, int sub) {
+  const int xcd = (int)blockIdx.x & 7;
+  XcdRun q;
+  q.end = (int)(((long long)nr * (xcd + 1)) >> 3);
+  q.step = ((int)gridDim.x >> 3) * R;
+  q.k = (int)(((long long)nr * xcd) >> 3) + ((int)blockIdx.x >> 3) * R + sub;
+  return q;
+}
+
 // flag[row] = 1 when the z-row holds a cell with cls != 0; one wave per row
 __global__ void k_row_flags(const uint8_t* __restrict__ cls, int* __restrict__ flag, int nrow, int d2) {
   const int lane = threadIdx.x & 63;
@@ -338,7 +354,8 @@ __global__ void k_build_rhs(float* __restrict__ b, const float* __restrict__ phi
 // Smooth, PressureSolver.cpp:821-837: mode 0 is (1. - w) * z + w * term with the
 // reference's double/float promotion (quirk Q4); mode 1 is z + w * term.
 // one colour pass over the active rows; kLevel0 only gives the finest level its own
-// symbol in kernel traces.  Thread -> (row, z of the pass's colour in that row).
+// symbol in kernel traces.  Thread -> (row, z of the pass's colour in that row); the rows in
+// per-XCD runs (xcd_run).
 template <bool kLevel0>
 __global__ __launch_bounds__(kBlock) void k_smooth(float* __restrict__ z, const float* __restrict__ r, LvP L,
                                                    const int* __restrict__ rows, const int* __restrict__ nrows,
@@ -349,12 +366,12 @@ __global__ __launch_bounds__(kBlock) void k_smooth(float* __restrict__ z, const 
   const int sub = H >= kBlock ? 0 : (int)threadIdx.x / H;
   if (sub >= R) return;
   const int h0 = (int)threadIdx.x - sub * (H >= kBlock ? 0 : H), hs = H >= kBlock ? kBlock : H;
-  // the row id is fetched together with the row count (rows holds d0 * d1 entries)
-  const int kcap = L.d[0] * d1 - 1, kstep = gridDim.x * R;
-  int k = blockIdx.x * R + sub;
+  const int kcap = L.d[0] * d1 - 1;
+  const XcdRun run = xcd_run(*nrows, R, sub);
+  const int kstep = run.step;
+  int k = run.k;
   int row_next = rows[min(k, kcap)];
-  const int nr = *nrows;
-  for (; k < nr; k += kstep) {
+  for (; k < run.end; k += kstep) {
     const int row = row_next;
     row_next = rows[min(k + kstep, kcap)];
     const int xi = row / d1, yi = row - xi * d1;
@@ -1035,12 +1052,13 @@ __global__ void k_cg_p_spmv(float* __restrict__ pout, float* __restrict__ q, con
   const size_t sx = (size_t)d1 * d2, sy = (size_t)d2;
   double v[1] = {0.0};
   const RowIter it = row_iter(d2);
-  const int kcap = d0 * d1 - 1, kstep = gridDim.x * it.R;
-  int k = blockIdx.x * it.R + it.sub;
+  const int kcap = d0 * d1 - 1;
+  const XcdRun run = xcd_run(*nrows, it.R, it.sub);  // the grid (npartial blocks) is a multiple of 8
+  const int kstep = run.step;
+  int k = run.k;
   int row_next = rows[min(k, kcap)];
-  const int nr = *nrows;
   if (it.on)
-    for (; k < nr; k += kstep) {
+    for (; k < run.end; k += kstep) {
       const int row = row_next;
       row_next = rows[min(k + kstep, kcap)];
       const int xi = row / d1, yi = row - xi * d1;
@@ -1175,7 +1193,7 @@ unsigned smooth_blocks(const Level& L) {
   const int H = (L.d[2] + 1) / 2;
   const int R = H >= kBlock ? 1 : kBlock / H;
   const size_t need = div_up((size_t)L.d[0] * L.d[1], (size_t)R);
-  return (unsigned)std::max<size_t>(1, std::min<size_t>(need, 8192));
+  return (unsigned)(div_up(std::max<size_t>(1, std::min<size_t>(need, 8192)), 8) * 8);  // xcd_run: 8 | grid
 }
 
 // blocks for a kernel over the cells of a level's row list (RowIter mapping; the kernel strides)

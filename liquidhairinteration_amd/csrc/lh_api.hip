@@ -230,7 +230,7 @@ int alloc_ctx(Ctx& c) {
          dalloc(c, q.zflag, (size_t)q.d[0] * q.d[1]) && dalloc(c, q.tflag, div_up(q.d[0], 4) * div_up(q.d[1], 4)) &&
          dalloc(c, q.tiles, div_up(q.d[0], 4) * div_up(q.d[1], 4)) && dalloc(c, q.ntiles, 1);
   }
-  c.npartial = 1024;
+  c.npartial = 1024;  // also the PCG vector kernels' grid: a multiple of 8 (per-XCD row runs)
   ok = ok && dalloc(c, c.p, c.ncell) && dalloc(c, c.p2, c.ncell) && dalloc(c, c.x, c.ncell) && dalloc(c, c.q, c.ncell) &&
        dalloc(c, c.partial, 4 * (size_t)c.npartial) && dalloc(c, c.dsc, 16) && dalloc(c, c.dsc64, 16) &&
        dalloc(c, c.dmaxv, 4) && dalloc(c, c.dflag, 16);
