@@ -1432,14 +1432,17 @@ __global__ void k_f32_to_f64(double* __restrict__ out, const float* __restrict__
 
 }  // namespace
 
-// Init (:666-680: r and z of every level zeroed), BuildRHS (:682-698), BuildLHS (:544-664)
+// Init (:666-680: r and z of every level zeroed), BuildRHS (:682-698), BuildLHS (:544-664).
+// Level 0's r and z are written over the whole grid by every solve's first kernel (k_cg_init,
+// or k_cg_set_x0 and k_cg_warm_residual) before anything reads them, so only levels >= 1 are
+// cleared here.
 void build_system(Ctx& c, float dt) {
-  for (size_t l = 0; l < c.lv.size(); ++l) {
+  for (size_t l = 1; l < c.lv.size(); ++l) {
     Level& L = c.lv[l];
     launch_fill_f32(c, L.r, L.n, 0.f);
     launch_fill_f32(c, L.z, L.n, 0.f);
     // the tiled sweep's ping-pong partner: zero outside the tiles for the whole solve, as z
-    if (l > 0 && c.cfg.coarse_sweep == kCoarseTiles) launch_fill_f32(c, L.z2, L.n, 0.f);
+    if (c.cfg.coarse_sweep == kCoarseTiles) launch_fill_f32(c, L.z2, L.n, 0.f);
   }
   launch_build_rhs(c);
   launch_build_lhs(c, dt);
