@@ -820,8 +820,10 @@ __global__ void k_prolong(float* __restrict__ zf, const float* __restrict__ zc, 
   const RowIter it = row_iter(c2);
   const int nr = *nrows;
   if (!it.on) return;
-  for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
-    const int row = rows[k];
+  for (int k = blockIdx.x * it.R + it.sub, kst = gridDim.x * it.R, rn = k < nr ? rows[k] : 0; k < nr;
+       k += kst) {  // the next row id loads with this row's operands
+    const int row = rn;
+    rn = k + kst < nr ? rows[k + kst] : 0;
     const int xi = row / c1, yi = row - xi * c1;
     for (int zi = it.z0; zi < c2; zi += it.zs) {
       const float v = zc[(size_t)row * c2 + zi];
@@ -840,8 +842,10 @@ __global__ void k_zero_rows(float* __restrict__ z, const int* __restrict__ rows,
   const RowIter it = row_iter(d2);
   const int nr = *nrows;
   if (!it.on) return;
-  for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
-    const size_t base = (size_t)rows[k] * d2;
+  for (int k = blockIdx.x * it.R + it.sub, kst = gridDim.x * it.R, rn = k < nr ? rows[k] : 0; k < nr;
+       k += kst) {  // the next row id loads with this row's operands
+    const size_t base = (size_t)rn * d2;
+    rn = k + kst < nr ? rows[k + kst] : 0;
     for (int zi = it.z0; zi < d2; zi += it.zs) z[base + zi] = 0.0f;
   }
 }
@@ -916,8 +920,10 @@ __global__ void k_cg_first(float* __restrict__ p, const float* __restrict__ z, c
   const RowIter it = row_iter(d2);
   const int nr = *nrows;
   if (it.on)
-    for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
-      const size_t base = (size_t)rows[k] * d2;
+    for (int k = blockIdx.x * it.R + it.sub, kst = gridDim.x * it.R, rn = k < nr ? rows[k] : 0; k < nr;
+         k += kst) {  // the next row id loads with this row's operands
+      const size_t base = (size_t)rn * d2;
+      rn = k + kst < nr ? rows[k + kst] : 0;
       for (int zi = it.z0; zi < d2; zi += it.zs) {
         const size_t i = base + zi;
         float zv = z[i], rv = r[i];
@@ -978,10 +984,16 @@ __global__ void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r, con
   }
   double v[1] = {0.0};
   const RowIter it = row_iter(d2);
+  // the next row id is fetched with the current row's operands; a cell's class and its four
+  // operands are loaded together
+  const int kcap = d0 * d1 - 1, kstep = gridDim.x * it.R;
+  int k = blockIdx.x * it.R + it.sub;
+  int row_next = rows[min(k, kcap)];
   const int nr = *nrows;
   if (it.on)
-    for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
-      const int row = rows[k];
+    for (; k < nr; k += kstep) {
+      const int row = row_next;
+      row_next = rows[min(k + kstep, kcap)];
       const int xi = row / d1, yi = row - xi * d1;
       const bool inner = xi >= 1 && xi <= d0 - 2 && yi >= 1 && yi <= d1 - 2;
       if (!inner && !zero_z) continue;
@@ -990,9 +1002,12 @@ __global__ void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r, con
         const size_t i = base + zi;
         if (zero_z) z[i] = 0.0f;
         if (!inner || zi < 1 || zi > d2 - 2) continue;
-        if (cls[i] == 0) continue;
-        x[i] += alpha * p[i];
-        float rn = r[i] - alpha * q[i];
+        int kc = cls[i];
+        float xv = x[i], pv = p[i], rv = r[i], qv = q[i];
+        asm volatile("" : "+v"(kc), "+v"(xv), "+v"(pv), "+v"(rv), "+v"(qv));
+        if (kc == 0) continue;
+        x[i] = xv + alpha * pv;
+        const float rn = rv - alpha * qv;
         r[i] = rn;
         v[0] += (double)rn * (double)rn;
       }
@@ -1018,10 +1033,13 @@ __global__ void k_cg_dot_rz(const float* __restrict__ r, const float* __restrict
   }
   double v[1] = {0.0};
   const RowIter it = row_iter(d2);
+  const int kstep = gridDim.x * it.R;
+  int k = blockIdx.x * it.R + it.sub;
   const int nr = *nrows;
   if (it.on)
-    for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
-      const size_t base = (size_t)rows[k] * d2;
+    for (int row_next = k < nr ? rows[k] : 0; k < nr; k += kstep) {
+      const size_t base = (size_t)row_next * d2;
+      row_next = k + kstep < nr ? rows[k + kstep] : 0;
       for (int zi = it.z0; zi < d2; zi += it.zs) v[0] += (double)r[base + zi] * (double)z[base + zi];
     }
   block_partials<1>(v, partial);
@@ -1380,8 +1398,10 @@ __global__ void k_ir_residual(double* __restrict__ r64, const double* __restrict
   const RowIter it = row_iter(d2);
   const int nr = *nrows;
   if (it.on)
-    for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
-      const size_t base = (size_t)rows[k] * d2;
+    for (int k = blockIdx.x * it.R + it.sub, kst = gridDim.x * it.R, rn = k < nr ? rows[k] : 0; k < nr;
+         k += kst) {  // the next row id loads with this row's operands
+      const size_t base = (size_t)rn * d2;
+      rn = k + kst < nr ? rows[k + kst] : 0;
       for (int zi = it.z0; zi < d2; zi += it.zs) {
         const size_t i = base + zi;
         int xi, yi, zz;
@@ -1404,8 +1424,10 @@ __global__ void k_ir_rhs(float* __restrict__ rf, const double* __restrict__ r64,
   const RowIter it = row_iter(d2);
   const int nr = *nrows;
   if (!it.on) return;
-  for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
-    const size_t base = (size_t)rows[k] * d2;
+  for (int k = blockIdx.x * it.R + it.sub, kst = gridDim.x * it.R, rn = k < nr ? rows[k] : 0; k < nr;
+       k += kst) {  // the next row id loads with this row's operands
+    const size_t base = (size_t)rn * d2;
+    rn = k + kst < nr ? rows[k + kst] : 0;
     for (int zi = it.z0; zi < d2; zi += it.zs) rf[base + zi] = (float)(r64[base + zi] * inv);
   }
 }
@@ -1416,8 +1438,10 @@ __global__ void k_ir_update(double* __restrict__ x64, const float* __restrict__ 
   const RowIter it = row_iter(d2);
   const int nr = *nrows;
   if (!it.on) return;
-  for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
-    const size_t base = (size_t)rows[k] * d2;
+  for (int k = blockIdx.x * it.R + it.sub, kst = gridDim.x * it.R, rn = k < nr ? rows[k] : 0; k < nr;
+       k += kst) {  // the next row id loads with this row's operands
+    const size_t base = (size_t)rn * d2;
+    rn = k + kst < nr ? rows[k + kst] : 0;
     for (int zi = it.z0; zi < d2; zi += it.zs) x64[base + zi] += s * (double)e[base + zi];
   }
 }
