@@ -335,7 +335,8 @@ void clear_grid(Model& m) {
 
 // LiquidModel<3>::ComputeLiquidPhi, LiquidModel.cpp:539-560.  The min is
 // order-free, so a serial loop gives the reference's result bit for bit.
-// pinned by: brute-force min over all particles on small cases in tests.
+// pinned by: an FP64 brute-force min over the particles whose cell window holds
+// the cell (tests/test_oracle_pins_3d.py), the LiquidData frame-1 known answer.
 void compute_liquid_phi(Model& m) {
   if (m.P == 0) return;
   const int* N = m.N;
@@ -362,8 +363,10 @@ void compute_liquid_phi(Model& m) {
 }
 
 // ---- P2G, LiquidModel.cpp:613-721.
-// pinned by: uniform-field reproduction, FP64 brute force on small particle
-// sets and the LiquidData frame-1 known answer (tests).
+// pinned by (tests/test_oracle_pins_3d.py): both orders against an FP64 brute
+// force over every particle-face pair (random particles, particles 0.05 cells
+// inside half-cell boundaries, boundary half-cells), the orders against each
+// other, uniform-field reproduction; the LiquidData frame-1 known answer.
 struct P2GContrib {
   float w, vw;
 };
@@ -511,6 +514,9 @@ void p2g_normalize(Model& m) {
 // LiquidModel<3>::ApplyGravity, LiquidModel.cpp:898-910 (all faces, Q8), plus
 // the optional horizontal tank acceleration a(t) = A sin(2 pi f t) on the
 // interior faces of the chosen axis (same index ranges rotated to that axis).
+// pinned by: the LiquidData frame-1 known answer (gravity); a floating block's
+// exact one-step displacement and the rotated face mask (tank acceleration,
+// tests/test_oracle_pins_3d.py).
 void apply_gravity(Model& m, float dt) {
   const int* N = m.N;
   float gdt = m.prm.gravity * dt;
@@ -908,8 +914,8 @@ void velocity_update(Model& m, float dt) {
 }
 
 // LiquidModel<3>::Extrapolate, LiquidModel.cpp:937-1063 (sequential, Q6).
-// pinned by: hand-checked small cases (validity chains along the scan order)
-// in tests.
+// pinned by: hand-derived values of a validity chain along the scan order
+// (tests/test_oracle_pins_3d.py).
 void extrapolate(Model& m) {
   if (m.P == 0) return;
   const int* N = m.N;
@@ -950,7 +956,8 @@ void extrapolate(Model& m) {
 }
 
 // LiquidModel<3>::Grid2Particle, LiquidModel.cpp:723-823.
-// pinned by: uniform field -> exact v; FP64 brute force; frame-1 KAT.
+// pinned by: an FP64 brute force of v and C over every face
+// (tests/test_oracle_pins_3d.py); uniform field -> exact v; frame-1 KAT.
 void g2p(Model& m) {
   if (m.P == 0) return;
   const int* N = m.N;
@@ -1141,10 +1148,15 @@ float get_cfl(const Model& m, float* maxv_out) {
   return clampf(m.dx[0] / maxVel, 0.001f, 0.1f);
 }
 
-// LiquidModel<3>::ComputeDivergence, LiquidModel.cpp:1065-1083 (+ L-inf).
+// LiquidModel<3>::ComputeDivergence, LiquidModel.cpp:1065-1083 (+ L-inf).  The
+// mean's sum runs in FP32 in index order (dot_mode 0, the reference's serial loop)
+// or in FP64 (dot_mode 1, the HIP library's reduction), like the solver's dots.
+// pinned by: closed-form fields (linear velocities, a single face spike, liquid
+// test phi <= 0, openness weights) in tests/test_oracle_pins_3d.py.
 void divergence(const Model& m, double* mean, double* linf, int* count) {
   const int* N = m.N;
   float dv = 0.0f;
+  double dv64 = 0.0;
   int cnt = 0;
   float mx = 0.0f;
   for (int xi = 0; xi < N[0]; ++xi)
@@ -1157,10 +1169,11 @@ void divergence(const Model& m, double* mean, double* linf, int* count) {
         d += m.lwz(xi, yi, zi + 1) * m.vz(xi, yi, zi + 1) - m.lwz(xi, yi, zi) * m.vz(xi, yi, zi);
         d /= m.dx[0];
         dv += d;
+        dv64 += (double)d;
         float ad = std::fabs(d);
         mx = (ad > mx) ? ad : mx;
       }
-  if (mean) *mean = (double)(dv / (float)cnt);
+  if (mean) *mean = (m.prm.dot_mode == 1) ? (cnt ? dv64 / (double)cnt : 0.0) : (double)(dv / (float)cnt);
   if (linf) *linf = (double)mx;
   if (count) *count = cnt;
 }
