@@ -17,6 +17,7 @@
 #pragma once
 
 #include <array>
+#include <fstream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -63,18 +64,20 @@ class ParticleSystem {
     detail::check(ctx_, lh_write_positions(ctx_, filename.c_str(), isAppend ? 1 : 0));
   }
   void LoadFromLine(const std::string& line) { detail::check(ctx_, lh_load_positions_line(ctx_, line.c_str())); }
+  void Resize(int64_t size) { detail::check(ctx_, lh_resize_particles(ctx_, size)); }
 
  private:
   lh_ctx* ctx_;
 };
 
-// CGSolver<3>: Solve(rho, dt) writes Grid<3>::pressure; the density is the scene's
+// CGSolver<3>: Solve(liquidDensity, dt) builds the system with that density and writes
+// Grid<3>::pressure (PressureSolver.h:104)
 class CGSolver {
  public:
   explicit CGSolver(lh_ctx* ctx) : ctx_(ctx) {}
-  lh_solve_stats Solve(float /*liquidDensity*/, float dt) {
+  lh_solve_stats Solve(float liquidDensity, float dt) {
     lh_solve_stats st{};
-    int rc = lh_solve_pressure(ctx_, dt, &st);
+    int rc = lh_solve_pressure(ctx_, liquidDensity, dt, &st);
     if (rc != LH_OK && rc != LH_ERR_NOT_CONVERGED) detail::check(ctx_, rc);
     return st;
   }
@@ -125,8 +128,11 @@ class LiquidModel {
   void Particle2Grid() { detail::check(ctx_, lh_p2g(ctx_)); }
   void Grid2Particle() { detail::check(ctx_, lh_g2p(ctx_)); }
   void ApplyGravity(float dt) { detail::check(ctx_, lh_apply_gravity(ctx_, dt)); }
+  // LiquidModel<3>::SolvePressure (LiquidModel.cpp:825-890): Solve(scene density, dt), velocity update
   void SolvePressure(float dt) {
-    solver_.Solve(0.f, dt);
+    float rho = 0.f;
+    detail::check(ctx_, lh_liquid_density(ctx_, &rho));
+    solver_.Solve(rho, dt);
     detail::check(ctx_, lh_velocity_update(ctx_, dt));
   }
   void Projection(float dt) { SolvePressure(dt); }
@@ -179,6 +185,33 @@ class Scene {
     std::array<float, 3> mn{}, dx{};
     detail::check(ctx_.get(), lh_grid_info(ctx_.get(), d.data(), mn.data(), dx.data()));
     return d;
+  }
+  float GetLiquidDensity() const {  // Scene.h:23
+    float rho = 0.f;
+    detail::check(ctx_.get(), lh_liquid_density(ctx_.get(), &rho));
+    return rho;
+  }
+  // Scene<3>::PlaySimulation (Scene.cpp:26-45) with the renderer replaced by onFrame: the
+  // first line's count resizes the particle system, then every following line goes through
+  // LoadFromLine (count lines included, as in the reference).  Returns the lines replayed.
+  template <class OnFrame>
+  long PlaySimulation(const std::string& filename, OnFrame&& onFrame) {
+    std::ifstream reader(filename);
+    if (!reader.is_open()) throw Error(LH_ERR_ARG, "Failed to open file " + filename);
+    std::string line;
+    if (!std::getline(reader, line)) throw Error(LH_ERR_ARG, "empty trajectory file " + filename);
+    ParticleSystem& ps = model_->GetParticleSystem();
+    ps.Resize((int64_t)std::stoul(line));
+    long frames = 0;
+    while (std::getline(reader, line)) {
+      ps.LoadFromLine(line);
+      onFrame(ps);
+      ++frames;
+    }
+    return frames;
+  }
+  long PlaySimulation(const std::string& filename) {
+    return PlaySimulation(filename, [](ParticleSystem&) {});
   }
   lh_ctx* handle() const { return ctx_.get(); }
 

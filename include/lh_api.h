@@ -124,7 +124,11 @@ int lh_clear_grid(lh_ctx* ctx);                      /* Grid.cpp:157-169 */
 int lh_compute_liquid_phi(lh_ctx* ctx);              /* LiquidModel.cpp:539-560 */
 int lh_p2g(lh_ctx* ctx);                             /* LiquidModel.cpp:613-721 */
 int lh_apply_gravity(lh_ctx* ctx, float dt);         /* LiquidModel.cpp:898-910 */
-int lh_solve_pressure(lh_ctx* ctx, float dt, lh_solve_stats* stats); /* CGSolver<3>::Solve, PressureSolver.cpp:839-934 */
+/* CGSolver<3>::Solve(liquidDensity, dt), PressureSolver.h:104 / PressureSolver.cpp:839-934:
+ * BuildRHS, BuildLHS(rho, dt) on every level, MG-PCG; writes the pressure field.  The
+ * step (lh_step) passes the scene's density, as LiquidModel.cpp:830-831 does.  rho must be
+ * positive and finite (LH_ERR_ARG otherwise). */
+int lh_solve_pressure(lh_ctx* ctx, float rho, float dt, lh_solve_stats* stats);
 int lh_velocity_update(lh_ctx* ctx, float dt);       /* LiquidModel.cpp:834-889 */
 int lh_extrapolate(lh_ctx* ctx);                     /* LiquidModel.cpp:937-1063 */
 int lh_g2p(lh_ctx* ctx);                             /* LiquidModel.cpp:723-823 */
@@ -140,8 +144,9 @@ int lh_correct_volume(lh_ctx* ctx, float dt);
 
 /* ---- solver-level access (CGSolver<3>::BuildRHS / BuildLHS / Solve split,
  * PressureSolver.cpp:682-698, 544-664, 839-934) ---------------------------- */
-/* BuildRHS + BuildLHS (all levels) from the current liquidPhi / liquidW / v */
-int lh_build_system(lh_ctx* ctx, float dt);
+/* BuildRHS + BuildLHS(liquidDensity, dt) (PressureSolver.h:97-99) on all levels from the
+ * current liquidPhi / liquidW / v */
+int lh_build_system(lh_ctx* ctx, float rho, float dt);
 /* solve A x = b on the built operator in the configured precision (FP32 MGPCG,
  * or FP32 MGPCG under the FP64 refinement loop with "precision": "refined");
  * b == NULL uses the built right-hand side; x0 (optional) is an initial guess,
@@ -200,6 +205,13 @@ int lh_num_particles(const lh_ctx* ctx, int64_t* n);
  * (ParticleSystem.cpp:10-21, ParticleSystem.h:44-49); vel / C may be NULL (zero). */
 int lh_set_particles(lh_ctx* ctx, const float* pos, const float* vel, const float* C, int64_t n);
 int lh_get_particles(lh_ctx* ctx, float* pos, float* vel, float* C, int64_t n);
+/* ParticleSystem<3>::Resize (ParticleSystem.cpp:10-21), as Scene<3>::PlaySimulation calls it with
+ * the count line of a trajectory file (Scene.cpp:34-35): the first min(count, n) particles keep
+ * their state, new particles start with zero position, velocity and affine matrix (the
+ * reference leaves them uninitialised) and the default radius 2 dx. */
+int lh_resize_particles(lh_ctx* ctx, int64_t n);
+/* Scene<3>::GetLiquidDensity (Scene.h:23): the config's LiquidParameter.density */
+int lh_liquid_density(const lh_ctx* ctx, float* rho);
 /* count = number of elements of the field (uint8 for LH_FIELD_IS_VALID, float otherwise) */
 int lh_get_field(lh_ctx* ctx, int field, void* host, size_t count);
 int lh_set_field(lh_ctx* ctx, int field, const void* host, size_t count);

@@ -29,6 +29,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_NAME = "liblh.so"
 
 LH_OK = 0
+LH_ERR_ARG = 6
 ERROR_NAMES = {1: "config", 2: "out of memory", 3: "not converged", 4: "NaN detected", 5: "device", 6: "argument",
                7: "domain"}
 
@@ -102,7 +103,7 @@ def load_library():
         "lh_compute_liquid_phi": ([vp], C.c_int),
         "lh_p2g": ([vp], C.c_int),
         "lh_apply_gravity": ([vp, C.c_float], C.c_int),
-        "lh_solve_pressure": ([vp, C.c_float, C.POINTER(SolveStats)], C.c_int),
+        "lh_solve_pressure": ([vp, C.c_float, C.c_float, C.POINTER(SolveStats)], C.c_int),
         "lh_velocity_update": ([vp, C.c_float], C.c_int),
         "lh_extrapolate": ([vp], C.c_int),
         "lh_g2p": ([vp], C.c_int),
@@ -118,7 +119,9 @@ def load_library():
         "lh_profile_enable": ([vp, C.c_int], C.c_int),
         "lh_profile_read": ([vp, C.c_int, ip, dp], C.c_int),
         "lh_get_time": ([vp, dp], C.c_int),
-        "lh_build_system": ([vp, C.c_float], C.c_int),
+        "lh_build_system": ([vp, C.c_float, C.c_float], C.c_int),
+        "lh_resize_particles": ([vp, C.c_int64], C.c_int),
+        "lh_liquid_density": ([vp, fp], C.c_int),
         "lh_solve_rhs": ([vp, fp, fp, C.POINTER(SolveStats)], C.c_int),
         "lh_get_solution64": ([vp, dp, C.c_size_t], C.c_int),
         "lh_set_warm_start": ([vp, fp, C.c_size_t], C.c_int),
@@ -233,6 +236,11 @@ class ParticleSystem:
         """ParticleSystem::LoadFromLine (ParticleSystem.cpp:86-101), first token skipped as in the reference."""
         self._h.check(self._h.L.lh_load_positions_line(self._h.h, line.encode()))
 
+    def Resize(self, size):
+        """ParticleSystem::Resize (ParticleSystem.cpp:10-21): the first min(size, n) particles keep their
+        state, new ones start at zero with the default radius."""
+        self._h.check(self._h.L.lh_resize_particles(self._h.h, int(size)))
+
 
 class CGSolver:
     """CGSolver<3> (PressureSolver.h:81-145) bound to a scene's grid."""
@@ -241,23 +249,28 @@ class CGSolver:
         self._h = handle
         self.last_stats = None
 
-    def Solve(self, liquidDensity=None, dt=0.01):
-        """Solve(rho, dt): builds RHS/LHS and runs MG-PCG; writes grid.pressure.
+    def _density(self, liquidDensity):
+        if liquidDensity is not None:
+            return float(liquidDensity)
+        rho = C.c_float()
+        self._h.check(self._h.L.lh_liquid_density(self._h.h, C.byref(rho)))
+        return rho.value
 
-        The density comes from the scene config (the reference passes
-        m_scene.GetLiquidDensity(), LiquidModel.cpp:830); liquidDensity is accepted
-        for signature parity and must match it when given.
-        """
+    def Solve(self, liquidDensity=None, dt=0.01):
+        """Solve(liquidDensity, dt) (PressureSolver.h:104): BuildRHS, BuildLHS(rho, dt), MG-PCG;
+        writes grid.pressure.  liquidDensity=None takes the scene's density, which is what
+        LiquidModel<3>::SolvePressure passes (LiquidModel.cpp:830-831)."""
         st = SolveStats()
-        rc = self._h.L.lh_solve_pressure(self._h.h, float(dt), C.byref(st))
+        rc = self._h.L.lh_solve_pressure(self._h.h, self._density(liquidDensity), float(dt), C.byref(st))
         self.last_stats = st.as_dict()
         if rc not in (LH_OK, 3):
             self._h.check(rc)
         return self.last_stats
 
     # split access: BuildRHS + BuildLHS, then solves on that operator
-    def BuildSystem(self, dt):
-        self._h.check(self._h.L.lh_build_system(self._h.h, float(dt)))
+    def BuildSystem(self, dt, liquidDensity=None):
+        """BuildRHS + BuildLHS(liquidDensity, dt) on every level (PressureSolver.h:97-99)"""
+        self._h.check(self._h.L.lh_build_system(self._h.h, self._density(liquidDensity), float(dt)))
 
     def SolveRHS(self, b=None, x0=None):
         """Solve A x = b (b=None: the built RHS) from x0 (optional); x -> pressure. Returns stats."""
@@ -453,6 +466,7 @@ class LiquidModel:
         self._h.check(self._h.L.lh_apply_gravity(self._h.h, float(dt)))
 
     def SolvePressure(self, dt):
+        # LiquidModel<3>::SolvePressure: Solve(scene density, dt), then the velocity update
         st = self._solver.Solve(None, dt)
         self._h.check(self._h.L.lh_velocity_update(self._h.h, float(dt)))
         return st
@@ -548,6 +562,30 @@ class Scene:
 
     def GetGridDimension(self):
         return self.grid_dimension
+
+    def GetLiquidDensity(self):
+        """Scene<3>::GetLiquidDensity (Scene.h:23)"""
+        return self._model.GetSolver()._density(None)
+
+    def PlaySimulation(self, filename, on_frame=None):
+        """Scene<3>::PlaySimulation (Scene.cpp:26-45) without the renderer: the first line's count
+        resizes the particle system, then every following line goes through LoadFromLine (count
+        lines too, as in the reference: they change nothing past the skipped first token) and
+        on_frame(particle_system) is called where the reference renders.  Returns the number of
+        lines replayed."""
+        ps = self._model.GetParticleSystem()
+        with open(filename) as f:
+            first = f.readline()
+            if not first:
+                raise LHError(LH_ERR_ARG, "empty trajectory file %s" % filename)
+            ps.Resize(int(first.strip()))      # std::stoul(line)
+            n = 0
+            for line in f:
+                ps.LoadFromLine(line.rstrip("\n"))
+                if on_frame is not None:
+                    on_frame(ps)
+                n += 1
+        return n
 
     def GetMinCoord(self):
         return self.min_coord

@@ -467,8 +467,7 @@ __global__ void k_xcommit(uint8_t* __restrict__ valid, const uint8_t* __restrict
 __global__ void k_divergence(const float* __restrict__ vx, const float* __restrict__ vy,
                              const float* __restrict__ vz, const float* __restrict__ lwx,
                              const float* __restrict__ lwy, const float* __restrict__ lwz,
-                             const float* __restrict__ phi, GridP g, double* __restrict__ partial,
-                             unsigned* __restrict__ maxbits, int* __restrict__ count) {
+                             const float* __restrict__ phi, GridP g, double* __restrict__ partial) {
   const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
   size_t total = (size_t)N0 * N1 * N2;
   double acc = 0.0;
@@ -504,25 +503,13 @@ __global__ void k_divergence(const float* __restrict__ vx, const float* __restri
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0) {  // the partial layout of the row-list divergence's final reduction
     partial[blockIdx.x] = sacc[0];
-    atomicMax(maxbits, __float_as_uint(smx[0]));
-    atomicAdd(count, scnt[0]);
+    partial[gridDim.x + blockIdx.x] = (double)smx[0];
+    partial[2 * gridDim.x + blockIdx.x] = (double)scnt[0];
   }
 }
 
-__global__ void k_sum_partials(const double* __restrict__ partial, int n, double* __restrict__ out) {
-  __shared__ double s[kBlock];
-  double a = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) a += partial[i];
-  s[threadIdx.x] = a;
-  __syncthreads();
-  for (int k = kBlock / 2; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = s[0];
-}
 
 }  // namespace
 
@@ -722,25 +709,20 @@ int run_extrapolate(Ctx& c) {
   return extrapolate_record(c);
 }
 
-void launch_divergence(Ctx& c, double* h_mean, double* h_linf, int* h_count) {
+void launch_divergence_dense(Ctx& c, double* out) {
   const int nb = c.npartial;
-  hipMemsetAsync(c.dmaxv, 0, sizeof(unsigned), c.s);
-  hipMemsetAsync(c.dflag, 0, sizeof(int), c.s);
   hipLaunchKernelGGL(k_divergence, dim3(nb), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], c.lw[0], c.lw[1], c.lw[2],
-                     c.phi, c.g, c.partial, c.dmaxv, c.dflag);
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, c.dsc64 + 7);
-  hipMemcpyAsync(c.h_pin + 7, c.dsc64 + 7, sizeof(double), hipMemcpyDeviceToHost, c.s);
-  hipMemcpyAsync(c.h_flag, c.dflag, sizeof(int), hipMemcpyDeviceToHost, c.s);
-  unsigned mb = 0;
-  hipMemcpyAsync(c.h_pin + 8, c.dmaxv, sizeof(unsigned), hipMemcpyDeviceToHost, c.s);
+                     c.phi, c.g, c.partial);
+  launch_div_final(c, nb, out);
+}
+
+void launch_divergence(Ctx& c, double* h_mean, double* h_linf, int* h_count) {
+  launch_divergence_dense(c, reinterpret_cast<double*>(c.h_pin_dev) + kPinDiv);
   hipStreamSynchronize(c.s);
-  std::memcpy(&mb, c.h_pin + 8, sizeof(unsigned));
-  int cnt = *c.h_flag;
+  const int cnt = (int)c.h_pin[kPinDiv + 2];
   if (h_count) *h_count = cnt;
-  if (h_mean) *h_mean = cnt ? c.h_pin[7] / (double)cnt : 0.0;
-  float mf;
-  std::memcpy(&mf, &mb, sizeof(float));
-  if (h_linf) *h_linf = (double)mf;
+  if (h_mean) *h_mean = cnt ? c.h_pin[kPinDiv] / (double)cnt : 0.0;
+  if (h_linf) *h_linf = c.h_pin[kPinDiv + 1];
 }
 
 }  // namespace lh

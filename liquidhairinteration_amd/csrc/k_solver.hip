@@ -1137,14 +1137,107 @@ __global__ void k_sc_first(const double* __restrict__ part, int nb, float* __res
   }
 }
 
+// ComputeDivergence (LiquidModel.cpp:1065-1083) over the level-0 liquid rows: the rows holding a
+// cell with cls != 0 are exactly those holding a cell with liquidPhi <= 0 (the cells the reference
+// counts; a NaN phi has cls 1 and is counted as the reference's "> 0.0" test lets it through).
+// Per block: the FP64 sum, the max |div| and the count, at partial[b], [nb + b], [2 nb + b].
+__global__ void k_div_rows(const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz,
+                           const float* __restrict__ lwx, const float* __restrict__ lwy,
+                           const float* __restrict__ lwz, const float* __restrict__ phi,
+                           const int* __restrict__ rows, const int* __restrict__ nrows, GridP g,
+                           double* __restrict__ partial) {
+  const int N1 = g.N[1], N2 = g.N[2];
+  const size_t plane = (size_t)N1 * N2;
+  const RowIter it = row_iter(N2);
+  const int kstep = gridDim.x * it.R;
+  const int nr = *nrows;
+  double acc = 0.0;
+  float mx = 0.f;
+  int cnt = 0;
+  if (it.on)
+    for (int k = blockIdx.x * it.R + it.sub; k < nr; k += kstep) {
+      const int row = rows[k];
+      const int xi = row / N1, yi = row - xi * N1;
+      const size_t base = (size_t)row * N2;
+      const size_t fy0 = ((size_t)xi * (N1 + 1) + yi) * N2, fz0 = base + (size_t)row;  // (x*N1+y)*(N2+1)
+      for (int zi = it.z0; zi < N2; zi += it.zs) {
+        if (phi[base + zi] > 0.0f) continue;
+        const size_t fx0 = base + zi, fx1 = fx0 + plane;
+        const size_t y0 = fy0 + zi, y1 = y0 + N2;
+        const size_t z0 = fz0 + zi, z1 = z0 + 1;
+        float d = lwx[fx1] * vx[fx1] - lwx[fx0] * vx[fx0];
+        d += lwy[y1] * vy[y1] - lwy[y0] * vy[y0];
+        d += lwz[z1] * vz[z1] - lwz[z0] * vz[z0];
+        d /= g.dx[0];
+        acc += (double)d;
+        const float ad = fabsf(d);
+        mx = ad > mx ? ad : mx;
+        ++cnt;
+      }
+    }
+  __shared__ double s_acc[kBlock];
+  __shared__ float s_mx[kBlock];
+  __shared__ int s_cnt[kBlock];
+  s_acc[threadIdx.x] = acc;
+  s_mx[threadIdx.x] = mx;
+  s_cnt[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int st = kBlock / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      s_acc[threadIdx.x] += s_acc[threadIdx.x + st];
+      s_mx[threadIdx.x] = fmaxf(s_mx[threadIdx.x], s_mx[threadIdx.x + st]);
+      s_cnt[threadIdx.x] += s_cnt[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s_acc[0];
+    partial[gridDim.x + blockIdx.x] = (double)s_mx[0];
+    partial[2 * gridDim.x + blockIdx.x] = (double)s_cnt[0];
+  }
+}
+
+// sums of k_div_rows' partials in a fixed order: out[0] = sum, out[1] = max, out[2] = count
+// (out may be pinned host memory read after the step's wait)
+__global__ void k_div_final(const double* __restrict__ partial, int nb, double* __restrict__ out) {
+  const double sum = sum_partials_block(partial, nb);
+  const double cnt = sum_partials_block(partial + 2 * nb, nb);
+  __shared__ double s_mx[kBlock];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) m = fmax(m, partial[nb + i]);
+  s_mx[threadIdx.x] = m;
+  __syncthreads();
+  for (int st = kBlock / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) s_mx[threadIdx.x] = fmax(s_mx[threadIdx.x], s_mx[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sum;
+    out[1] = s_mx[0];
+    out[2] = cnt;
+  }
+}
+
 }  // namespace
+
+void launch_divergence_rows(Ctx& c, double* out) {
+  const Level& L = c.lv[0];
+  const int nb = c.npartial;
+  hipLaunchKernelGGL(k_div_rows, dim3(nb), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], c.lw[0], c.lw[1], c.lw[2],
+                     c.phi, L.rows, L.nrows, c.g, c.partial);
+  hipLaunchKernelGGL(k_div_final, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, out);
+}
+
+void launch_div_final(Ctx& c, int nb, double* out) {
+  hipLaunchKernelGGL(k_div_final, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, out);
+}
 
 void launch_build_rhs(Ctx& c) {
   hipLaunchKernelGGL(k_build_rhs, dim3(grid_for(c.ncell)), dim3(kBlock), 0, c.s, c.rhs, c.phi, c.v[0], c.v[1], c.v[2],
                      c.lw[0], c.lw[1], c.lw[2], c.g);
 }
 
-void launch_build_lhs(Ctx& c, float dt) {
+void launch_build_lhs(Ctx& c, float rho, float dt) {
   FineP F;
   for (int k = 0; k < 3; ++k) F.N[k] = c.g.N[k];
   F.phi = c.phi;
@@ -1155,7 +1248,7 @@ void launch_build_lhs(Ctx& c, float dt) {
     Level& L = c.lv[l];
     float f = (float)(1 << l);
     hipLaunchKernelGGL(k_build_lhs, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.adiag, L.t[0], L.t[1], L.t[2], L.cls,
-                       F, (int)l, L.d[0], L.d[1], L.d[2], dt, c.cfg.density, f * c.g.dx[0], f * c.g.dx[1],
+                       F, (int)l, L.d[0], L.d[1], L.d[2], dt, rho, f * c.g.dx[0], f * c.g.dx[1],
                        f * c.g.dx[2]);
     const int nrow = L.d[0] * L.d[1];
     hipLaunchKernelGGL(k_row_flags, dim3((unsigned)div_up(nrow, kBlock / 64)), dim3(kBlock), 0, c.s, L.cls, L.aflag,
@@ -1460,7 +1553,7 @@ __global__ void k_f32_to_f64(double* __restrict__ out, const float* __restrict__
 // Level 0's r and z are written over the whole grid by every solve's first kernel (k_cg_init,
 // or k_cg_set_x0 and k_cg_warm_residual) before anything reads them, so only levels >= 1 are
 // cleared here.
-void build_system(Ctx& c, float dt) {
+void build_system(Ctx& c, float rho, float dt) {
   for (size_t l = 1; l < c.lv.size(); ++l) {
     Level& L = c.lv[l];
     launch_fill_f32(c, L.r, L.n, 0.f);
@@ -1469,7 +1562,7 @@ void build_system(Ctx& c, float dt) {
     if (c.cfg.coarse_sweep == kCoarseTiles) launch_fill_f32(c, L.z2, L.n, 0.f);
   }
   launch_build_rhs(c);
-  launch_build_lhs(c, dt);
+  launch_build_lhs(c, rho, dt);
 }
 
 // The PCG loop of CGSolver<3>::Solve (PressureSolver.cpp:845-931) on the operator
@@ -1748,9 +1841,9 @@ void launch_warm_ritz(Ctx& c, float* x0, const float* b) {
 
 // CGSolver<3>::Solve, PressureSolver.cpp:839-934: Init, BuildRHS, BuildLHS, PCG
 // from x = 0 (or the installed / learned initial guess), pressure = x.
-int run_solve(Ctx& c, float dt, lh_solve_stats* st) {
+int run_solve(Ctx& c, float rho, float dt, lh_solve_stats* st) {
   hipEventRecord(c.ev0, c.s);
-  build_system(c, dt);
+  build_system(c, rho, dt);
   c.last_dt = dt;
   const float* x0 = c.warm_x0;
   if (c.warm_learned && c.net) {

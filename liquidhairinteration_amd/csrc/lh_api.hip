@@ -512,7 +512,9 @@ int do_pre_step(Ctx& c, bool in_step = false) {
   return LH_OK;
 }
 
-int do_step(Ctx& c, float dt, lh_solve_stats* st) {
+// one Step; with div != nullptr also PostStep's ComputeDivergence (LiquidModel.cpp:530-537,
+// 1065-1083) inside the step's one wait: div[0] = mean, div[1] = L-inf, div[2] = count
+int do_step(Ctx& c, float dt, lh_solve_stats* st, double* div = nullptr) {
   lh_solve_stats local;
   std::memset(&local, 0, sizeof(local));
   if (!c.bins_valid && c.P > 0) {
@@ -523,7 +525,8 @@ int do_step(Ctx& c, float dt, lh_solve_stats* st) {
   if (c.P == 0) launch_gravity(c, dt);
   int solve_rc = LH_OK;
   if (c.P > 0) {  // SolvePressure returns early without particles (LiquidModel.cpp:828)
-    solve_rc = run_solve(c, dt, &local);
+    // SolvePressure passes the scene's density (LiquidModel.cpp:830-831)
+    solve_rc = run_solve(c, c.cfg.density, dt, &local);
     if (solve_rc == LH_ERR_DEVICE || solve_rc == LH_ERR_NAN) {
       if (st) *st = local;
       return solve_rc;
@@ -539,7 +542,13 @@ int do_step(Ctx& c, float dt, lh_solve_stats* st) {
   }
   c.time += (double)dt;
   c.last_solve = local;
-  // the domain flag and max |v| with one wait
+  if (div) {
+    double* out = reinterpret_cast<double*>(c.h_pin_dev) + kPinDiv;
+    // with particles the solve has just built the level-0 liquid rows from this liquidPhi
+    if (c.P > 0) launch_divergence_rows(c, out);
+    else launch_divergence_dense(c, out);
+  }
+  // the domain flag and max |v| with one wait (and the divergence's host-mapped results)
   hipMemcpyAsync(c.h_flag + 1, c.dflag + 1, sizeof(int), hipMemcpyDeviceToHost, c.s);
   hipMemcpyAsync(c.h_flag + 4, c.dmaxv, sizeof(unsigned), hipMemcpyDeviceToHost, c.s);
   if (!hip_ok(c, hipStreamSynchronize(c.s), "step end")) return LH_ERR_DEVICE;
@@ -549,6 +558,12 @@ int do_step(Ctx& c, float dt, lh_solve_stats* st) {
     return LH_ERR_DOMAIN;
   }
   std::memcpy(&c.last_maxv, c.h_flag + 4, sizeof(float));
+  if (div) {
+    const double cnt = c.h_pin[kPinDiv + 2];
+    div[0] = cnt > 0 ? c.h_pin[kPinDiv] / cnt : 0.0;
+    div[1] = c.h_pin[kPinDiv + 1];
+    div[2] = cnt;
+  }
   return LH_OK;
 }
 
@@ -728,13 +743,17 @@ int lh_apply_gravity(lh_ctx* ctx, float dt) {
   });
 }
 
-int lh_solve_pressure(lh_ctx* ctx, float dt, lh_solve_stats* stats) {
-  return guarded(ctx, [dt, stats](Ctx& c) {
+int lh_solve_pressure(lh_ctx* ctx, float rho, float dt, lh_solve_stats* stats) {
+  return guarded(ctx, [rho, dt, stats](Ctx& c) {
+    if (!(rho > 0.f) || !std::isfinite(rho)) {
+      c.err = "lh_solve_pressure: liquidDensity must be positive and finite";
+      return LH_ERR_ARG;
+    }
     if (c.P == 0) {
       if (stats) std::memset(stats, 0, sizeof(*stats));
       return LH_OK;
     }
-    return run_solve(c, dt, stats);
+    return run_solve(c, rho, dt, stats);
   });
 }
 
@@ -819,8 +838,9 @@ int lh_run_steps(lh_ctx* ctx, int nsteps, float* dt, lh_step_stats* last) {
       hipEventCreate(&e0);
       hipEventCreate(&e1);
       hipEventRecord(e0, c.s);
+      double div[3] = {0.0, 0.0, 0.0};
       int rc = do_pre_step(c, true);
-      if (rc == LH_OK) rc = do_step(c, *dt, &st.solve);
+      if (rc == LH_OK) rc = do_step(c, *dt, &st.solve, div);  // PostStep's divergence every step
       hipEventRecord(e1, c.s);
       hipEventSynchronize(e1);
       float ms = 0.f;
@@ -829,9 +849,9 @@ int lh_run_steps(lh_ctx* ctx, int nsteps, float* dt, lh_step_stats* last) {
       hipEventDestroy(e1);
       if (rc != LH_OK && rc != LH_ERR_NOT_CONVERGED) return rc;
       st.step_ms = ms;
-      if (last && k == nsteps - 1) {
-        launch_divergence(c, &st.divergence_mean, &st.divergence_linf, &st.fluid_cells);
-      }
+      st.divergence_mean = div[0];
+      st.divergence_linf = div[1];
+      st.fluid_cells = (int)div[2];
       st.max_velocity = c.last_maxv;
       st.cfl_dt = cfl_from(c);
       st.sim_time = c.time;
@@ -889,6 +909,34 @@ int lh_get_particles(lh_ctx* ctx, float* pos, float* vel, float* C, int64_t n) {
   });
 }
 
+int lh_resize_particles(lh_ctx* ctx, int64_t n) {
+  return guarded(ctx, [=](Ctx& c) {
+    if (n < 0 || n >= (int64_t)0x7fffffff) {
+      c.err = "lh_resize_particles: count out of range";
+      return LH_ERR_ARG;
+    }
+    // ParticleSystem<3>::Resize (ParticleSystem.cpp:10-21): the first min(P, n) particles keep
+    // their state (in id order), new ones start at zero
+    const int64_t keep = std::min<int64_t>(c.P, n);
+    std::vector<float> pos(3 * (size_t)n, 0.f), vel(3 * (size_t)n, 0.f), Cm(9 * (size_t)n, 0.f);
+    if (keep > 0) {
+      std::vector<float> p0(3 * (size_t)c.P), v0(3 * (size_t)c.P), c0(9 * (size_t)c.P);
+      int rc = lh_get_particles(ctx, p0.data(), v0.data(), c0.data(), c.P);
+      if (rc) return rc;
+      std::copy(p0.begin(), p0.begin() + 3 * keep, pos.begin());
+      std::copy(v0.begin(), v0.begin() + 3 * keep, vel.begin());
+      std::copy(c0.begin(), c0.begin() + 9 * keep, Cm.begin());
+    }
+    return upload_particles(c, pos.data(), vel.data(), Cm.data(), n);
+  });
+}
+
+int lh_liquid_density(const lh_ctx* ctx, float* rho) {
+  if (!ctx || !rho) return LH_ERR_ARG;
+  *rho = ctx->c.cfg.density;
+  return LH_OK;
+}
+
 int lh_get_field(lh_ctx* ctx, int field, void* host, size_t count) {
   return guarded(ctx, [=](Ctx& c) {
     size_t n = field_count(c, field, nullptr);
@@ -939,9 +987,13 @@ int lh_profile_read(lh_ctx* ctx, int kernel, int* launches, double* total_ms) {
   return LH_OK;
 }
 
-int lh_build_system(lh_ctx* ctx, float dt) {
-  return guarded(ctx, [dt](Ctx& c) {
-    build_system(c, dt);
+int lh_build_system(lh_ctx* ctx, float rho, float dt) {
+  return guarded(ctx, [rho, dt](Ctx& c) {
+    if (!(rho > 0.f) || !std::isfinite(rho)) {
+      c.err = "lh_build_system: liquidDensity must be positive and finite";
+      return LH_ERR_ARG;
+    }
+    build_system(c, rho, dt);
     c.last_dt = dt;
     return hip_ok(c, hipStreamSynchronize(c.s), "build system") ? LH_OK : LH_ERR_DEVICE;
   });

@@ -65,6 +65,7 @@ struct GridP {
 };
 
 enum { kCoarsePasses = 0, kCoarseRows = 1, kCoarseTiles = 2 };
+constexpr int kPinDiv = 24;  // first of the three pinned host doubles the divergence reduction writes
 
 struct Level {
   int d[3] = {0, 0, 0};
@@ -156,7 +157,7 @@ struct Ctx {
   double* dsc64 = nullptr;    // device FP64 scalars
   unsigned* dmaxv = nullptr;  // max |v| bits
   int* dflag = nullptr;       // misc device flags (extrapolation progress, domain errors)
-  double* h_pin = nullptr;    // pinned host scalars
+  double* h_pin = nullptr;    // pinned host scalars (64; [kPinDiv, kPinDiv + 3): divergence sum, L-inf, count)
   int* h_flag = nullptr;
   // device-side addresses of h_pin and h_flag: the PCG's last scalar kernel of an iteration
   // stores the scalars and the NaN flag there itself instead of two copies per iteration
@@ -227,7 +228,11 @@ void launch_clear(Ctx& c, bool in_step = false);
 void launch_velocity_update(Ctx& c, float dt);
 // returns number of propagation rounds used; <0 on device error
 int run_extrapolate(Ctx& c);
+// ComputeDivergence (LiquidModel.cpp:1065-1083) over every cell; waits and returns the mean, the
+// L-inf norm and the count of cells with liquidPhi <= 0
 void launch_divergence(Ctx& c, double* h_mean, double* h_linf, int* h_count);
+// its launches alone (no wait): out[0] = FP64 sum, out[1] = L-inf, out[2] = count
+void launch_divergence_dense(Ctx& c, double* out);
 void launch_gravity(Ctx& c, float dt);
 // ---- launchers (k_particles.hip)
 constexpr int kScanBins = 2048;        // bins per block of the bin scan
@@ -238,12 +243,17 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt);
 void launch_g2p_advect(Ctx& c, bool do_g2p, bool do_advect, float dt);
 int run_correct_volume(Ctx& c, float dt);  // LiquidModel<3>::CorrectVolume; LH_OK or LH_ERR_DOMAIN
 // ---- launchers (k_solver.hip)
-int run_solve(Ctx& c, float dt, lh_solve_stats* st);  // CGSolver<3>::Solve
-void build_system(Ctx& c, float dt);                  // Init + BuildRHS + BuildLHS
+int run_solve(Ctx& c, float rho, float dt, lh_solve_stats* st);  // CGSolver<3>::Solve(liquidDensity, dt)
+void build_system(Ctx& c, float rho, float dt);       // Init + BuildRHS + BuildLHS(rho, dt)
 // solve A x = b on the built operator (configured precision), x -> pressure
 int solve_built(Ctx& c, const float* b, const float* x0, lh_solve_stats* st, bool record_start = true);
 void launch_build_rhs(Ctx& c);
-void launch_build_lhs(Ctx& c, float dt);
+void launch_build_lhs(Ctx& c, float rho, float dt);
+// the same over the level-0 liquid rows of the operator built from the current liquidPhi, no
+// wait: out[0] = FP64 sum, out[1] = L-inf, out[2] = count (out may be pinned host memory)
+void launch_divergence_rows(Ctx& c, double* out);
+// final reduction of per-block divergence partials (sum, max, count at [b], [nb + b], [2 nb + b])
+void launch_div_final(Ctx& c, int nb, double* out);
 
 // device error check helper
 bool hip_ok(Ctx& c, hipError_t e, const char* what);

@@ -873,10 +873,11 @@ void init_solver(Model& m) {  // Init, :666-680
   m.p.fill(0.f); m.x.fill(0.f); m.rhs.fill(0.f);
 }
 
-int solve_pressure_only(Model& m, float dt, orc_solve_stats* st) {
+// CGSolver<3>::Solve(liquidDensity, dt), PressureSolver.cpp:839-934
+int solve_pressure_only(Model& m, float rho, float dt, orc_solve_stats* st) {
   init_solver(m);
   build_rhs(m);
-  build_lhs(m, m.prm.density, dt);
+  build_lhs(m, rho, dt);
   SolveVecs<float> v;
   std::vector<float> xo;
   pcg<float>(m, v, m.rhs.v, xo, m.prm.smoother, m.prm.dot_mode, m.prm.tol_mode, (double)m.prm.tol, m.prm.max_iter, st);
@@ -1373,10 +1374,10 @@ int orc_p2g(void* h, int order) {
   return 0;
 }
 int orc_apply_gravity(void* h, float dt) { apply_gravity(*(Model*)h, dt); return 0; }
-int orc_solve_pressure(void* h, float dt, orc_solve_stats* st) {
+int orc_solve_pressure(void* h, float rho, float dt, orc_solve_stats* st) {
   Model& m = *(Model*)h;
   if (m.P == 0) return 0;
-  return solve_pressure_only(m, dt, st);
+  return solve_pressure_only(m, rho, dt, st);
 }
 int orc_velocity_update(void* h, float dt) {
   Model& m = *(Model*)h;
@@ -1402,7 +1403,7 @@ int orc_step(void* h, float dt, orc_solve_stats* st, double* div_mean, double* d
   if (rc) return rc;
   apply_gravity(m, dt);
   if (m.P > 0) {
-    solve_pressure_only(m, dt, st);
+    solve_pressure_only(m, m.prm.density, dt, st);  // the scene's density, LiquidModel.cpp:830-831
     velocity_update(m, dt);
   }
   extrapolate(m);
@@ -1417,11 +1418,11 @@ int orc_step(void* h, float dt, orc_solve_stats* st, double* div_mean, double* d
 // Solver-level access for the refined / FP64 checks: build A from the
 // current liquidPhi / liquidW, then solve A x = b (b given) in FP32 (reference
 // loop) or FP64 (mode 1: symmetric MG, FP64 vectors, FP64 dots).
-int orc_build_system(void* h, float dt) {
+int orc_build_system(void* h, float rho, float dt) {
   Model& m = *(Model*)h;
   init_solver(m);
   build_rhs(m);
-  build_lhs(m, m.prm.density, dt);
+  build_lhs(m, rho, dt);
   return 0;
 }
 int orc_get_level(void* h, int level, float* adiag, float* acoef6, int dims[3]) {

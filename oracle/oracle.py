@@ -75,7 +75,7 @@ def lib():
             getattr(L, name).argtypes = [vp]
         L.orc_p2g.argtypes = [vp, C.c_int]
         L.orc_apply_gravity.argtypes = [vp, C.c_float]
-        L.orc_solve_pressure.argtypes = [vp, C.c_float, C.POINTER(SolveStats)]
+        L.orc_solve_pressure.argtypes = [vp, C.c_float, C.c_float, C.POINTER(SolveStats)]
         L.orc_velocity_update.argtypes = [vp, C.c_float]
         L.orc_advect.argtypes = [vp, C.c_float]
         L.orc_correct_volume.argtypes = [vp, C.c_float]
@@ -86,7 +86,7 @@ def lib():
         L.orc_solid_phi_at.argtypes = [vp, C.c_float, C.c_float, C.c_float]
         L.orc_solid_gradient.argtypes = [vp, C.c_float, C.c_float, C.c_float, fp]
         L.orc_step.argtypes = [vp, C.c_float, C.POINTER(SolveStats), dp, dp]
-        L.orc_build_system.argtypes = [vp, C.c_float]
+        L.orc_build_system.argtypes = [vp, C.c_float, C.c_float]
         L.orc_get_level.argtypes = [vp, C.c_int, fp, fp, ip]
         L.orc_num_levels.restype = C.c_int
         L.orc_num_levels.argtypes = [vp]
@@ -305,9 +305,10 @@ class Oracle:
     def apply_gravity(self, dt):
         self.L.orc_apply_gravity(self.h, dt)
 
-    def solve_pressure(self, dt):
+    def solve_pressure(self, dt, rho=None):
+        """CGSolver<3>::Solve(rho, dt); rho=None: the scene's density"""
         st = SolveStats()
-        self.L.orc_solve_pressure(self.h, dt, C.byref(st))
+        self.L.orc_solve_pressure(self.h, self.prm.density if rho is None else rho, dt, C.byref(st))
         return st
 
     def velocity_update(self, dt):
@@ -349,8 +350,8 @@ class Oracle:
         return st, m.value, l.value
 
     # --- solver access
-    def build_system(self, dt):
-        self.L.orc_build_system(self.h, dt)
+    def build_system(self, dt, rho=None):
+        self.L.orc_build_system(self.h, self.prm.density if rho is None else rho, dt)
 
     def level(self, l):
         dims = (C.c_int * 3)()

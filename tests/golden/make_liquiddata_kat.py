@@ -3,8 +3,10 @@
 Source: LiquidData.csv of the reference (26 frames x 5000 particle positions at
 6 significant digits, written by ParticleSystem::WriteToFile, ParticleSystem.cpp:66-84,
 from the headless loop of main.cpp:36-44 on Config/PureLiquid3D.json).
-Writes tests/golden/liquiddata_kat.npz with frames 0-2 (float32) and the
-per-frame centroid y of all 26 frames (float64).  Data only; run once here:
+Writes tests/golden/liquiddata_kat.npz with frames 0-2 (float32), the
+per-frame centroid y of all 26 frames (float64), and the file's text of the
+first count line + frame-0 line and of the frame-1 line, byte for byte (uint8),
+which pin the text format of WriteToFile / LoadFromLine.  Data only; run once here:
     python tests/golden/make_liquiddata_kat.py /root/reference/LiquidData.csv
 """
 import os
@@ -14,7 +16,8 @@ import numpy as np
 
 
 def main(src):
-    lines = open(src).read().split("\n")
+    raw = open(src, "rb").read()
+    lines = raw.decode().split("\n")
     frames = []
     k = 0
     while 2 * k + 1 < len(lines) and lines[2 * k].strip():
@@ -26,7 +29,9 @@ def main(src):
     np.savez_compressed(out, frame0=frames[0].astype(np.float32), frame1=frames[1].astype(np.float32),
                         frame2=frames[2].astype(np.float32),
                         centroid_y=np.array([f[:, 1].mean() for f in frames]),
-                        max_step=np.array([np.abs(frames[i + 1] - frames[i]).max() for i in range(len(frames) - 1)]))
+                        max_step=np.array([np.abs(frames[i + 1] - frames[i]).max() for i in range(len(frames) - 1)]),
+                        text_frame0=np.frombuffer(("\n".join(lines[:2]) + "\n").encode(), np.uint8),
+                        text_frame1_line=np.frombuffer(lines[3].encode(), np.uint8))
     print("wrote", out, len(frames), "frames")
 
 

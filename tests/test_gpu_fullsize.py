@@ -3,9 +3,9 @@ CPU oracle is too slow to run step by step: size-independent properties of the s
 
 * determinism: two runs from the same config give bit-identical particles;
 * every refined solve converges, with the FP64 true residual at or below 1e-9 relative;
-* projection: the velocity update leaves a discrete divergence that is the PCG residual,
-  so its L-infinity norm drops by at least 1e5 from the pre-projection divergence
-  (L-inf(after) <= ||r||_2 <= 1e-9 ||b||_2 <= 1e-9 sqrt(n) L-inf(before), n ~ 1e6 liquid cells);
+* projection: the velocity update leaves a discrete divergence that is the PCG residual plus
+  the rounding of the FP32 pressure and velocity fields it is computed from; its L-infinity norm
+  stays under that floor (conftest.projection_floor);
 * every particle stays inside the box and out of the walls' interior.
 """
 import json
@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import liquidhairinteration_amd as lh
+from conftest import projection_floor
 
 pytestmark = pytest.mark.gpu
 
@@ -36,6 +37,7 @@ def run(cfg, steps):
     before = m.ComputeDivergence()
     st = m.SolvePressure(dt)
     after = m.ComputeDivergence()
+    after = (after, projection_floor(m, st, dt))
     pos, vel, _ = m.GetParticleSystem().get()
     sc.close()
     return pos, vel, stats, before, after, st
@@ -53,8 +55,9 @@ def test_collision_256_properties():
         sv = st_["solve"]
         assert sv["converged"] == 1
         assert sv["true_residual"] <= 1e-9 * (1 + 1e-6)
-    print("divergence L-inf before / after projection:", before, after)
-    assert before[1] > 0 and after[1] <= 1e-5 * before[1]
+    after, floor = after
+    print("divergence L-inf before / after projection: %.3e / %.3e (FP32 floor %.3e)" % (before[1], after[1], floor))
+    assert before[1] > 0 and after[1] <= floor
     N = cfg["gridDimension"][0]
     h = 1.0 / N
     assert np.isfinite(p1).all() and np.isfinite(v1).all()

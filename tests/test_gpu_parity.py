@@ -64,6 +64,14 @@ def coarse_tiles(cfg):
     return cfg
 
 
+def precond(cfg, kind):
+    """CGSolver<3>'s other preconditioners (PressureSolver.cpp:721-743): None (z = r) and Jacobi
+    (z = r / Adiag on cells with a nonzero diagonal)"""
+    cfg["SolverParameter"]["preconditionerType"] = kind
+    cfg["SolverParameter"]["maxIterations"] = 400
+    return cfg
+
+
 def cv_cycle(cfg, k):
     cfg["LiquidParameter"]["correctCycle"] = k
     return cfg
@@ -98,7 +106,20 @@ CASES = {
     # (overlap product); every second particle id moves per call at 40^3
     "dam24_correct_volume": lambda: dict(small_dam(24, "symmetric"), correctVolume=True),
     "dam40_correct_volume": lambda: cv_cycle(dict(small_dam(40, "symmetric", fused=True), correctVolume=True), 2),
+    "dam24_precond_none": lambda: precond(small_dam(24, "reference"), "None"),
+    "dam24_precond_jacobi": lambda: precond(small_dam(24, "reference"), "Jacobi"),
+    "dam32_tank_sphere_jacobi": lambda: precond(small_dam(32, "symmetric", tank=True), "Jacobi"),
 }
+
+
+def same_divergence(g, o, what):
+    """ComputeDivergence (LiquidModel.cpp:1065-1083): count and L-inf bit-exact; the mean's FP64
+    sums differ only in order, |d mean| <= 2 eps64 L-inf"""
+    mg, lg, cg = g
+    mo, lo, co = o
+    assert cg == co, "%s: count %d vs %d" % (what, cg, co)
+    assert lg == lo, "%s: L-inf %r vs %r" % (what, lg, lo)
+    assert abs(mg - mo) <= 1e-12 * max(lo, 1e-30), "%s: mean %r vs %r" % (what, mg, mo)
 
 
 @pytest.mark.parametrize("case", list(CASES))
@@ -155,6 +176,7 @@ def test_stages_bit_exact(case):
         orc.advect(dt)
         m.AdvectParticle(dt)
         same(ps.get()[0], orc.particles()[0], "advected positions step %d" % step)
+        same_divergence(m.ComputeDivergence(), orc.divergence(), "divergence step %d" % step)
         if cfg.get("correctVolume"):
             orc.correct_volume(dt)
             m.CorrectVolume(dt)
@@ -168,7 +190,8 @@ def test_stages_bit_exact(case):
 
 @pytest.mark.parametrize("case", ["dam24_reference_smoother", "dam24_tank_sphere", "dam40_symmetric_fused_sweep",
                                   "dam24_correct_volume", "tall_16x24x160_row_segments",
-                                  "dam40_symmetric_tile_sweep", "tall_16x24x160_tile_sweep"])
+                                  "dam40_symmetric_tile_sweep", "tall_16x24x160_tile_sweep",
+                                  "dam24_precond_none", "dam32_tank_sphere_jacobi"])
 def test_free_running_steps_bit_exact(case):
     print("case", case, flush=True)
     cfg = CASES[case]()
@@ -185,6 +208,9 @@ def test_free_running_steps_bit_exact(case):
         dtg, sst = m.RunSteps(1, dtg)
         assert sst["solve"]["iterations"] == st.iterations
         assert np.float32(dtg) == dt
+        # PostStep's divergence, computed inside every step of the native loop over the liquid rows
+        same_divergence((sst["divergence_mean"], sst["divergence_linf"], sst["fluid_cells"]), orc.divergence(),
+                        "per-step divergence")
     pg, vg, cg = m.GetParticleSystem().get()
     po, vo, co = orc.particles()
     same(pg, po, "positions after 6 steps")
@@ -342,4 +368,46 @@ def test_moving_sphere_bit_exact():
         po, vo, _ = orc.particles()
         same(pg, po, "positions, step %d" % k)
         same(vg, vo, "velocities, step %d" % k)
+    scene.close()
+
+
+def test_solve_density_argument():
+    """CGSolver<3>::Solve(liquidDensity, dt) (PressureSolver.h:104) with a density other than the
+    scene's: bit-exact against the oracle's Solve(500, dt).  Every operator term W dt / (rho dx^2)
+    doubles exactly when rho halves, so the PCG iterates scale by exact powers of two: the
+    iteration count is the same and the pressure is exactly half of the rho = 1000 solve's."""
+    print("case dam24 solve density", flush=True)
+    cfg = small_dam(24, "symmetric")
+    orc = Oracle(cfg, dot_mode=1, p2g_order=1)
+    scene = lh.Scene(cfg)
+    m = scene.GetLiquidModel()
+    orc.init()
+    m.Init()
+    assert scene.GetLiquidDensity() == 1000.0
+    dt = np.float32(0.01)
+    for _ in range(3):   # a few steps so the pressure has more than a hydrostatic profile
+        orc.step(dt)
+        dt = np.float32(orc.get_cfl()[0])
+    po, vo, co = orc.particles()
+    m.GetParticleSystem().set(po, vo, co)
+    m.set_field("isValid", orc.get(F["isValid"]))
+    orc.clear_grid()
+    orc.compute_liquid_phi()
+    orc.p2g()
+    orc.apply_gravity(dt)
+    m.PreStep(dt)
+    m.Particle2Grid()
+    m.ApplyGravity(dt)
+    st_o = orc.solve_pressure(dt, rho=500.0)
+    st_g = m.GetSolver().Solve(500.0, dt)
+    assert st_g["iterations"] == st_o.iterations and st_g["converged"]
+    p500 = m.get_field("pressure")
+    same(p500, orc.get(F["pressure"]), "pressure with rho = 500")
+    st_1000 = m.GetSolver().Solve(None, dt)
+    p1000 = m.get_field("pressure")
+    assert st_1000["iterations"] == st_g["iterations"]
+    assert np.abs(p1000).max() > 0
+    same(p500, p1000 * np.float32(0.5), "rho = 500 pressure vs half the rho = 1000 pressure")
+    with pytest.raises(lh.LHError):
+        m.GetSolver().Solve(-1.0, dt)
     scene.close()

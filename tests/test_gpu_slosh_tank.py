@@ -1,0 +1,158 @@
+"""The sloshing-tank workload: a tank 40 % full (y < 0.4) with a static sphere of radius 0.15 at its
+centre (a Union of the walls' box and the sphere, Boundary.h:27-62), the tank accelerated along x
+by 3 sin(2 pi t) m/s^2 (a body force beside ApplyGravity, LiquidModel.cpp:898-910), the pressure
+solved by FP32 MGPCG under the FP64 refinement loop from the learned initial guess.
+
+* reduced size (32^3 and 48^3, the same scene): after free-running steps on the GPU, the oracle
+  takes the GPU's particles; liquid phi and P2G with gravity and the tank acceleration at that
+  time must be bit-identical, the right-hand sides too, and the GPU's refined solution from the
+  network's guess must be within 1e-9 of the oracle's FP64 PCG (outer tolerance 1e-10; at the
+  workload's own 1e-9 the error stays under 5e-9, the contract of a 1e-9 residual);
+* full size (256^3, 49 M particles): a few steps twice, bit-identical; every refined solve
+  converged to an FP64 true residual <= 1e-9 with the learned guess kept or dropped (counted);
+  particles inside the walls and outside the sphere; the projection's divergence at the FP32
+  storage floor (projection_floor).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import projection_floor
+from oracle import Oracle
+
+lh = pytest.importorskip("liquidhairinteration_amd")
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(lh.__file__), "weights", "warmstart.lhws")
+F = {"vx": 0, "vy": 1, "vz": 2, "liquidPhi": 10, "rhs": 13}
+
+
+def slosh(N, refined_tol=None):
+    cfg = json.load(open(lh.config_path("slosh_tank_256")))
+    cfg["gridDimension"] = [N, N, N]
+    cfg["Boundary"]["left"]["halfExtent"] = [0.5 - 3.0 / N] * 3
+    if refined_tol is not None:
+        cfg["refinedTolerance"] = refined_tol
+    return cfg
+
+
+def determined(orc):
+    ad, _ = orc.level(0)
+    phi = orc.get(10)
+    m = np.zeros(phi.shape, bool)
+    m[1:-1, 1:-1, 1:-1] = True
+    return m & (phi <= 0) & (ad != 0)
+
+
+def contained(pos, N):
+    """particles within a cell of the open region (the walls' solid starts 3 cells in; advection
+    pushes particles out along the interpolated SDF's gradient) and their distance to the sphere's
+    centre"""
+    h = 1.0 / N
+    r = np.linalg.norm(pos.astype(np.float64) - 0.5, axis=1)
+    inside_walls = bool((pos > 2 * h).all() and (pos < 1 - 2 * h).all())
+    return inside_walls, float(r.min())
+
+
+@pytest.mark.parametrize("N", [32, 48])
+def test_slosh_tank_reduced_against_oracle(N):
+    print("case slosh tank %d^3" % N, flush=True)
+    contract = {}
+    for tol in (1e-9, 1e-10):
+        cfg = slosh(N, tol)
+        scene = lh.Scene(cfg)
+        m = scene.GetLiquidModel()
+        m.Init()
+        ws = m.GetWarmStart()
+        ws.load(WEIGHTS)
+        ws.enable(True)
+        dt = float(cfg["firstDt"])
+        dt, st = m.RunSteps(12, dt)      # t ~ 0.1 s: the tank acceleration is well away from zero
+        assert st["solve"]["converged"] == 1
+        pos, vel, Cm = m.GetParticleSystem().get()
+        inside, rmin = contained(pos, N)
+        assert inside and rmin > 0.15 - 0.5 / N, rmin
+        orc = Oracle(cfg, dot_mode=1, p2g_order=1)
+        orc.init()                       # solidPhi / liquidW of the sphere and walls
+        orc.set_particles(pos, vel, Cm)
+        orc.set_time(m.time)
+        orc.clear_grid()
+        orc.compute_liquid_phi()
+        orc.p2g()
+        orc.apply_gravity(np.float32(dt))
+        m.PreStep(dt)
+        m.Particle2Grid()
+        m.ApplyGravity(dt)
+        assert np.array_equal(m.get_field("liquidPhi"), orc.get(F["liquidPhi"]))
+        for f in ("vx", "vy", "vz"):
+            assert np.array_equal(m.get_field(f), orc.get(F[f])), f
+        assert np.abs(orc.get(F["vx"])).max() > 0
+        orc.build_system(np.float32(dt))
+        b = orc.get(F["rhs"])
+        sv = m.GetSolver().Solve(None, dt)        # rebuilds the same system; learned guess, refined
+        assert np.array_equal(m.get_field("rhs"), b)
+        assert sv["converged"] == 1 and sv["true_residual"] <= tol * (1 + 1e-6)
+        assert sv["warm_start_used"] in (0, 1) and sv["outer_iterations"] >= 1
+        x64, _ = orc.solve_fp64(b.astype(np.float64), rel_tol=1e-13, max_iter=2000)
+        xg = m.GetSolver().GetSolution64(b.shape)
+        d = determined(orc)
+        err = float(np.linalg.norm(xg[d] - x64[d]) / np.linalg.norm(x64[d]))
+        print("N %d refinedTolerance %g: error vs FP64 oracle %.3e, warm start used %d, %d PCG iterations"
+              % (N, tol, err, sv["warm_start_used"], sv["iterations"]), flush=True)
+        contract[tol] = err
+        scene.close()
+    assert contract[1e-10] <= 1e-9
+    assert contract[1e-9] <= 5e-9
+
+
+def run_full(cfg, steps):
+    sc = lh.Scene(cfg)
+    m = sc.GetLiquidModel()
+    m.Init()
+    ws = m.GetWarmStart()
+    ws.load(WEIGHTS)
+    ws.enable(True)
+    dt = float(cfg["firstDt"])
+    stats = []
+    for _ in range(steps):
+        m.PreStep(dt)
+        m.Step(dt)
+        stats.append(m.PostStep(dt))
+        dt = m.GetCFL()
+    m.PreStep(dt)
+    m.Particle2Grid()
+    m.ApplyGravity(dt)
+    before = m.ComputeDivergence()
+    st = m.SolvePressure(dt)
+    after = m.ComputeDivergence()
+    floor = projection_floor(m, st, dt)
+    pos, vel, _ = m.GetParticleSystem().get()
+    sc.close()
+    return pos, vel, stats, before, (after, floor)
+
+
+def test_slosh_tank_256_properties():
+    cfg = json.load(open(lh.config_path("slosh_tank_256")))
+    N = cfg["gridDimension"][0]
+    print("full-size slosh tank 256^3, run 1", flush=True)
+    p1, v1, s1, before, after = run_full(cfg, 3)
+    print("full-size slosh tank 256^3, run 2", flush=True)
+    p2, v2, _, _, _ = run_full(cfg, 3)
+    assert p1.shape[0] > 40_000_000
+    np.testing.assert_array_equal(p1, p2)
+    np.testing.assert_array_equal(v1, v2)
+    used = 0
+    for st in s1:
+        sv = st["solve"]
+        assert sv["converged"] == 1 and sv["true_residual"] <= 1e-9 * (1 + 1e-6)
+        assert sv["warm_start_used"] in (0, 1)
+        used += sv["warm_start_used"]
+    after, floor = after
+    print("learned guess kept in %d of %d solves; divergence L-inf %.3e -> %.3e (FP32 floor %.3e)"
+          % (used, len(s1), before[1], after[1], floor))
+    assert before[1] > 0 and after[1] <= floor
+    assert np.isfinite(p1).all() and np.isfinite(v1).all()
+    inside, rmin = contained(p1, N)
+    assert inside and rmin > 0.15 - 0.5 / N, rmin
