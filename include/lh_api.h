@@ -97,6 +97,8 @@ typedef struct lh_ctx lh_ctx;
 int lh_create(const char* json_path, lh_ctx** out);
 int lh_create_from_string(const char* json_text, lh_ctx** out);
 void lh_destroy(lh_ctx* ctx);
+/* message of the last failure on ctx; with ctx == NULL, of the last failed context-free call
+ * (lh_divergence_noalloc) on the calling thread */
 const char* lh_last_error(const lh_ctx* ctx);
 /* grid geometry: Grid<3> ctor, Grid.cpp:86-98 */
 int lh_grid_info(const lh_ctx* ctx, int dims[3], float min_coord[3], float dx[3]);
@@ -256,6 +258,23 @@ int lh_get_time(const lh_ctx* ctx, double* t);
  * are bit-identical to the saved run's.  Little-endian binary, magic "LHSNAP01". */
 int lh_save(lh_ctx* ctx, const char* path);
 int lh_load(lh_ctx* ctx, const char* path);
+
+/* ---- allocation-free variant ----------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  lh_divergence's
+ * ComputeDivergence (LiquidModel.cpp:1065-1083) on caller-owned device arrays of a grid of
+ * dims[0] x dims[1] x dims[2] cells in the Array3 layout (faces (N0+1,N1,N2), (N0,N1+1,N2),
+ * (N0,N1,N2+1); phi per cell), dx the cell size.  It is enqueued on stream (a hipStream_t; NULL is
+ * the null stream) and neither allocates nor waits: when the stream reaches it, out (device memory,
+ * 3 doubles) holds the mean over the cells with liquidPhi <= 0, the L-inf norm and that count,
+ * bit-identical to lh_divergence on the same fields.  The per-block partial sums live in one
+ * 24 KiB device array per GPU (looked up on the first call on each device, kept until the process
+ * ends) that every call on that GPU shares: calls that overlap on one GPU (other streams, other
+ * threads) race.  Limits: 1 <= dims[a] <= 4096 and at most 2^31 - 1 cells; LH_ERR_ARG otherwise,
+ * with the message in lh_last_error(NULL). */
+int lh_divergence_noalloc(const float* vx, const float* vy, const float* vz, const float* lwx, const float* lwy,
+                          const float* lwz, const float* phi, const int dims[3], float dx, double* out,
+                          void* stream);
 
 /* ---- the 2D model: LiquidModel<2> / CGSolver<2> --------------------------
  * The reference's other instantiation of the same classes (LiquidModel.cpp:10-474,

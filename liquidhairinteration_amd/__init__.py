@@ -121,6 +121,7 @@ def load_library():
         "lh_get_time": ([vp, dp], C.c_int),
         "lh_build_system": ([vp, C.c_float, C.c_float], C.c_int),
         "lh_resize_particles": ([vp, C.c_int64], C.c_int),
+        "lh_divergence_noalloc": ([vp, vp, vp, vp, vp, vp, vp, ip, C.c_float, vp, vp], C.c_int),
         "lh_liquid_density": ([vp, fp], C.c_int),
         "lh_solve_rhs": ([vp, fp, fp, C.POINTER(SolveStats)], C.c_int),
         "lh_get_solution64": ([vp, dp, C.c_size_t], C.c_int),

@@ -709,11 +709,17 @@ int run_extrapolate(Ctx& c) {
   return extrapolate_record(c);
 }
 
+void launch_divergence_arrays(hipStream_t s, const float* const v[3], const float* const lw[3], const float* phi,
+                              const GridP& g, double* partial, int nb, double* out, int as_mean) {
+  hipLaunchKernelGGL(k_divergence, dim3(nb), dim3(kBlock), 0, s, v[0], v[1], v[2], lw[0], lw[1], lw[2], phi, g,
+                     partial);
+  launch_div_final_on(s, partial, nb, out, as_mean);
+}
+
 void launch_divergence_dense(Ctx& c, double* out) {
-  const int nb = c.npartial;
-  hipLaunchKernelGGL(k_divergence, dim3(nb), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], c.lw[0], c.lw[1], c.lw[2],
-                     c.phi, c.g, c.partial);
-  launch_div_final(c, nb, out);
+  const float* v[3] = {c.v[0], c.v[1], c.v[2]};
+  const float* lw[3] = {c.lw[0], c.lw[1], c.lw[2]};
+  launch_divergence_arrays(c.s, v, lw, c.phi, c.g, c.partial, c.npartial, out, 0);
 }
 
 void launch_divergence(Ctx& c, double* h_mean, double* h_linf, int* h_count) {

@@ -1197,9 +1197,9 @@ __global__ void k_div_rows(const float* __restrict__ vx, const float* __restrict
   }
 }
 
-// sums of k_div_rows' partials in a fixed order: out[0] = sum, out[1] = max, out[2] = count
-// (out may be pinned host memory read after the step's wait)
-__global__ void k_div_final(const double* __restrict__ partial, int nb, double* __restrict__ out) {
+// sums of the divergence partials in a fixed order: out[0] = sum (as_mean: sum / count, 0 without
+// cells), out[1] = max, out[2] = count (out may be pinned host memory read after the step's wait)
+__global__ void k_div_final(const double* __restrict__ partial, int nb, double* __restrict__ out, int as_mean) {
   const double sum = sum_partials_block(partial, nb);
   const double cnt = sum_partials_block(partial + 2 * nb, nb);
   __shared__ double s_mx[kBlock];
@@ -1212,7 +1212,7 @@ __global__ void k_div_final(const double* __restrict__ partial, int nb, double* 
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = sum;
+    out[0] = as_mean ? (cnt > 0.0 ? sum / cnt : 0.0) : sum;
     out[1] = s_mx[0];
     out[2] = cnt;
   }
@@ -1225,11 +1225,11 @@ void launch_divergence_rows(Ctx& c, double* out) {
   const int nb = c.npartial;
   hipLaunchKernelGGL(k_div_rows, dim3(nb), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], c.lw[0], c.lw[1], c.lw[2],
                      c.phi, L.rows, L.nrows, c.g, c.partial);
-  hipLaunchKernelGGL(k_div_final, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, out);
+  hipLaunchKernelGGL(k_div_final, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, out, 0);
 }
 
-void launch_div_final(Ctx& c, int nb, double* out) {
-  hipLaunchKernelGGL(k_div_final, dim3(1), dim3(kBlock), 0, c.s, c.partial, nb, out);
+void launch_div_final_on(hipStream_t s, const double* partial, int nb, double* out, int as_mean) {
+  hipLaunchKernelGGL(k_div_final, dim3(1), dim3(kBlock), 0, s, partial, nb, out, as_mean);
 }
 
 void launch_build_rhs(Ctx& c) {

@@ -643,7 +643,7 @@ void lh_destroy(lh_ctx* ctx) {
   delete ctx;
 }
 
-const char* lh_last_error(const lh_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+const char* lh_last_error(const lh_ctx* ctx) { return ctx ? ctx->c.err.c_str() : noalloc_last_error(); }
 
 int lh_grid_info(const lh_ctx* ctx, int dims[3], float min_coord[3], float dx[3]) {
   if (!ctx) return LH_ERR_ARG;

@@ -233,6 +233,10 @@ int run_extrapolate(Ctx& c);
 void launch_divergence(Ctx& c, double* h_mean, double* h_linf, int* h_count);
 // its launches alone (no wait): out[0] = FP64 sum, out[1] = L-inf, out[2] = count
 void launch_divergence_dense(Ctx& c, double* out);
+// the same on any arrays of the grid g, with nb blocks whose partials (3 nb doubles) go to
+// partial; out[0] is the mean instead of the sum when as_mean
+void launch_divergence_arrays(hipStream_t s, const float* const v[3], const float* const lw[3], const float* phi,
+                              const GridP& g, double* partial, int nb, double* out, int as_mean);
 void launch_gravity(Ctx& c, float dt);
 // ---- launchers (k_particles.hip)
 constexpr int kScanBins = 2048;        // bins per block of the bin scan
@@ -253,7 +257,10 @@ void launch_build_lhs(Ctx& c, float rho, float dt);
 // wait: out[0] = FP64 sum, out[1] = L-inf, out[2] = count (out may be pinned host memory)
 void launch_divergence_rows(Ctx& c, double* out);
 // final reduction of per-block divergence partials (sum, max, count at [b], [nb + b], [2 nb + b])
-void launch_div_final(Ctx& c, int nb, double* out);
+void launch_div_final_on(hipStream_t s, const double* partial, int nb, double* out, int as_mean);
+
+// message of the last failed context-free call on this thread (lh_noalloc.hip)
+const char* noalloc_last_error();
 
 // device error check helper
 bool hip_ok(Ctx& c, hipError_t e, const char* what);
