@@ -5,14 +5,16 @@ two 0.3^3 blocks at opposite corners of the unit cube, 8 jittered particles per
 cell, v = +-(2,0,0) + N(0, 0.05^2), solid walls 3 cells thick, MGPCG (3 levels,
 2/2 red-black Gauss-Seidel sweeps, 12 + 12 on the coarsest, symmetric V-cycle) to 1e-6
 relative residual inside an FP64 residual loop to 1e-9.
-One step = PreStep + Step + PostStep + GetCFL (main.cpp:50-53) run by the library's
-native loop; particles and grid stay resident in HBM for the whole timed region.
+One step = PreStep + Step + PostStep (its ComputeDivergence, mean and L-inf) + GetCFL
+(main.cpp:50-53) run by the library's native loop; particles and grid stay resident in HBM for
+the whole timed region.
 
 Prints one JSON line: steps/s over K timed steps with nothing else in the timed region; per
 kernel, the average launch time from HIP events around every launch over the next K steps (the
 events stay out of the timed region: each pair costs microseconds of queue time) and, for the
-dominant one by total time, its roofline; the CPU baseline (the oracle, timed on this host for
-one step of the same workload) and, under "warm_start", ms and iterations per solve to 1e-6
+dominant one by total time, its roofline; the CPU baseline (the oracle, timed on this host for two
+steps from each of three states of the GPU run: after the warm-up, after the timed steps and after
+the profiled steps) and, under "warm_start", ms and iterations per solve to 1e-6
 relative residual of the learned initial guess versus plain MGPCG on identical systems of
 held-out 128^3 scenes.
 """
@@ -196,23 +198,65 @@ def measured_traffic(kernel, workload):
     return None if rec is None else rec["bytes_per_dispatch"]
 
 
-def cpu_baseline(cfg, steps=1, threads=0):
+def host_cpu():
+    """the host CPU's model name (lscpu) and logical CPU count"""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
+
+
+def snapshot(model):
+    """the state the next step depends on: particles, the persistent isValid field, time"""
+    pos, vel, Cm = model.GetParticleSystem().get()
+    return {"pos": pos, "vel": vel, "C": Cm, "isValid": model.get_field("isValid"), "time": model.time}
+
+
+def cpu_baseline(cfg, snapshots, steps=2, threads=0):
+    """The oracle (FP32 restatement of the reference's step, OpenMP) on this host: `steps` steps
+    from each snapshot of the GPU run (early, middle and late states of the benchmark window, each
+    with the dt the GPU run had there); value = steps / total time.  Without snapshots (--cpu-only)
+    the steps start from the workload's initial state."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     from oracle import Oracle
 
     threads = threads or os.cpu_count() or 1
     orc = Oracle(cfg, dot_mode=1, p2g_order=1, threads=threads)
-    orc.init()
-    dt = np.float32(cfg.get("firstDt", 0.01))
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        orc.step(dt)
-        dt = np.float32(orc.get_cfl()[0])
-    el = time.perf_counter() - t0
-    return {"value": steps / el, "unit": "steps/s", "cores": int(orc.L.orc_max_threads()), "kind": "port",
-            "sample": "%d step(s) of the same workload from its initial state, FP32 oracle restatement, "
-                      "OpenMP over %d host thread(s) (%.1f s)" % (steps, int(orc.L.orc_max_threads()), el)}
+    per, total, n = [], 0.0, 0
+    src = "states of the GPU run"
+    if not snapshots:
+        orc.init()
+        snapshots = [("initial", None, cfg.get("firstDt", 0.01))]
+        src = "the workload's"
+    else:
+        orc.compute_solid_phi()
+    for name, snap, dt in snapshots:
+        if snap is not None:
+            orc.set_particles(snap["pos"], snap["vel"], snap["C"])
+            orc.set(12, snap["isValid"])
+            orc.set_time(snap["time"])
+        dt = np.float32(dt)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            orc.step(dt)
+            dt = np.float32(orc.get_cfl()[0])
+        el = time.perf_counter() - t0
+        per.append({"state": name, "sim_time": round(snap["time"], 5) if snap is not None else 0.0,
+                    "steps": steps, "s_per_step": round(el / steps, 3)})
+        total += el
+        n += steps
+    cores = int(orc.L.orc_max_threads())
+    return {"value": n / total, "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": "%d steps of the same workload, %d from each of %d %s (%s), FP32 oracle "
+                      "restatement, OpenMP over %d host thread(s) (%.1f s)"
+                      % (n, steps, len(snapshots), src, ", ".join(p["state"] for p in per), cores, total),
+            "per_state": per, "host": host_cpu()}
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA peak
@@ -295,13 +339,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default=os.path.join(ROOT, "liquidhairinteration_amd", "configs", "collision_256.json"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps from each sampled state")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads for cpu_baseline (0: every core)")
     ap.add_argument("--cpu-only", action="store_true",
                     help="time only the CPU baseline of the workload (no GPU) and print it as one JSON line")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned-vs-plain solve comparison")
     ap.add_argument("--learned", action="store_true",
                     help="start every pressure solve of the timed workload from the learned guess")
+    ap.add_argument("--smoother", choices=("config", "reference"), default="config",
+                    help="reference: the reference's Q4/Q5 smoother with its own solver parameters "
+                         "(Config/PureLiquid3D.json: 4 levels, 5/5/20 sweeps, omega 0.5) in place of the config's")
     ap.add_argument("--ws-grid", type=int, default=128)
     ap.add_argument("--ws-scenes", type=int, default=2)
     ap.add_argument("--ws-steps", type=int, default=8)
@@ -316,7 +363,8 @@ def main():
         return
     if args.cpu_only:
         cfg = json.load(open(args.config))
-        print(json.dumps({"workload": workload, "cpu_baseline": cpu_baseline(cfg, args.cpu_steps, args.cpu_threads)}))
+        print(json.dumps({"workload": workload, "cpu_baseline": cpu_baseline(cfg, None, args.cpu_steps,
+                                                                             args.cpu_threads)}))
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
@@ -331,6 +379,11 @@ def main():
     if len(cfg["gridDimension"]) == 2:
         bench_2d(args, cfg)
         return
+    if args.smoother == "reference":
+        cfg["SolverParameter"].update({"smoother": "reference", "smoothingFactor": 0.5, "numLevel": 4,
+                                       "numPreSmooth": 5, "numPostSmooth": 5, "numFinalSmooth": 20,
+                                       "maxIterations": 400})
+        workload += "_reference_smoother"
     scene = lh.Scene(cfg)
     model = scene.GetLiquidModel()
     model.Init()
@@ -346,10 +399,14 @@ def main():
     dt = float(cfg.get("firstDt", 0.01))
     if args.warmup > 0:
         dt, _ = model.RunSteps(args.warmup, dt)
+    want_cpu = not args.no_cpu_baseline
+    snaps = [("early", snapshot(model), dt)] if want_cpu else []
     # the timed region: K steps with nothing but the step in it
     t0 = time.perf_counter()
     dt, last = model.RunSteps(args.steps, dt)
     el = time.perf_counter() - t0
+    if want_cpu:
+        snaps.append(("middle", snapshot(model), dt))
     # kernel timings: the next K steps with HIP events around every launch of the listed kernels
     # (each event pair costs several microseconds of queue time, so they stay out of the timed region)
     kern = ("smooth_l0", "smooth_l1", "smooth_l2", "p2g", "g2p", "liquid_phi", "spmv")
@@ -359,6 +416,8 @@ def main():
     el_prof = time.perf_counter() - t1
     prof = {k: model.profile_read(k) for k in kern}
     model.profile_enable(False)
+    if want_cpu:
+        snaps.append(("late", snapshot(model), dt))
     prof = {k: v for k, v in prof.items() if v[0]}
     # cells of the rows holding liquid per level in the last step's operator (the smoother and
     # SpMV visit those)
@@ -407,7 +466,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded jittered particle blocks)",
-        "config": {"workload": os.path.basename(args.config).replace(".json", ""),
+        "config": {"workload": workload,
                    "grid": list(scene.GetGridDimension()), "particles": P, "active_rows_per_level": rows,
                    "solver": "MGPCG L=%d pre/post=%d/%d coarsest=%d %s, tol %g %s" % (
                        cfg["SolverParameter"]["numLevel"], cfg["SolverParameter"]["numPreSmooth"],
@@ -429,8 +488,8 @@ def main():
     scene.close()
     if not args.no_warm_start:
         out["warm_start"] = warm_start_eval(args.ws_grid, args.ws_scenes, args.ws_steps)
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps, args.cpu_threads)
+    if want_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, snaps, args.cpu_steps, args.cpu_threads)
     print(json.dumps(out))
 
 
