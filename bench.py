@@ -49,12 +49,13 @@ def coarse_form(cfg, l):
     return "rows" if d[0] * d[1] * d[2] <= sp.get("rowSweepMaxCells", 1 << 18) else "passes"
 
 
-def row_sweep_levels(cfg):
-    """coarse levels swept by one launch per sweep (both colours) instead of two passes"""
-    return {l for l in range(1, cfg["SolverParameter"]["numLevel"]) if coarse_form(cfg, l) != "passes"}
+def sweeps_per_launch(cfg, l):
+    """red-black sweeps per smoother launch on coarse level l: one for k_sweep_rows and k_tile_sweep,
+    half of one for a k_smooth colour pass"""
+    return 0.5 if coarse_form(cfg, l) == "passes" else 1.0
 
 
-def algorithmic_bytes(kernel, n0, P, active, fused, rows_sweep=frozenset()):
+def algorithmic_bytes(kernel, n0, P, active, fused, per_launch=None):
     """Bytes per launch counted once per array element the kernel must touch.
 
     smooth_l0: one colour pass = half of a red-black sweep, whose minimum is 28 B/cell
@@ -62,8 +63,9 @@ def algorithmic_bytes(kernel, n0, P, active, fused, rows_sweep=frozenset()):
                cell of the level-0 rows holding liquid (the pass visits only those); the
                fused one-launch sweep, when selected, is dense: 28 B/cell;
     smooth_l1, smooth_l2: the same per cell of the rows holding liquid on level 1, and on
-               level 2 (the coarsest of the three-level hierarchy): 14 B per colour pass, 28 B
-               per k_sweep_rows or k_tile_sweep launch (a whole sweep);
+               level 2 (the coarsest of the three-level hierarchy): 28 B per sweep, times the
+               sweeps of one launch (per_launch[l]: 1/2 for a colour pass, 1 for k_sweep_rows or
+               k_tile_sweep);
     spmv:      the p update fused with q = A p (k_cg_p_spmv, all but the first SpMV of a solve):
                z and p read, new p and q written, 4 coefficient floats -> 32 B per cell of
                those rows (the plain first SpMV moves 24 B);
@@ -74,10 +76,11 @@ def algorithmic_bytes(kernel, n0, P, active, fused, rows_sweep=frozenset()):
     liquid_phi: 12 B/particle + 4 B/cell.
     active[l] = cells of the level-l rows that hold liquid.
     """
+    per_launch = per_launch or {}
     return {
         "smooth_l0": 28.0 * n0 if fused else 14.0 * active[0],
-        "smooth_l1": (28.0 if 1 in rows_sweep else 14.0) * active.get(1, 0),
-        "smooth_l2": (28.0 if 2 in rows_sweep else 14.0) * active.get(2, 0),
+        "smooth_l1": 28.0 * per_launch.get(1, 0.5) * active.get(1, 0),
+        "smooth_l2": 28.0 * per_launch.get(2, 0.5) * active.get(2, 0),
         "spmv": 32.0 * active[0],
         "p2g": 20.0 * P + 4.0 * n0,
         "g2p": 72.0 * P + 12.0 * n0,
@@ -187,15 +190,15 @@ def pmc_reduce(fetch_csv, write_csv, workload, cfg):
     print(json.dumps({workload: out}, indent=1))
 
 
-def measured_traffic(kernel, workload):
-    """HBM bytes per dispatch of `kernel` on `workload` from the committed rocprofv3 PMC runs
-    (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a copy kernel of
-    known size), or None when that pair was not measured"""
+def measured_traffic(kernel, workload, symbol):
+    """HBM bytes per dispatch of `kernel` (library symbol `symbol`) on `workload` from the committed
+    rocprofv3 PMC runs (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a copy
+    kernel of known size), or None when that pair was not measured for this symbol"""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if not os.path.exists(path):
         return None
     rec = json.load(open(path)).get(workload, {}).get(kernel)
-    return None if rec is None else rec["bytes_per_dispatch"]
+    return None if rec is None or rec.get("kernel") != symbol else rec["bytes_per_dispatch"]
 
 
 def host_cpu():
@@ -346,6 +349,8 @@ def main():
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned-vs-plain solve comparison")
     ap.add_argument("--learned", action="store_true",
                     help="start every pressure solve of the timed workload from the learned guess")
+    ap.add_argument("--coarse-sweep", choices=("config", "rows", "tiles", "passes"), default="config",
+                    help="override SolverParameter.coarseSweep (how the coarse levels are swept)")
     ap.add_argument("--smoother", choices=("config", "reference"), default="config",
                     help="reference: the reference's Q4/Q5 smoother with its own solver parameters "
                          "(Config/PureLiquid3D.json: 4 levels, 5/5/20 sweeps, omega 0.5) in place of the config's")
@@ -384,6 +389,8 @@ def main():
                                        "numPreSmooth": 5, "numPostSmooth": 5, "numFinalSmooth": 20,
                                        "maxIterations": 400})
         workload += "_reference_smoother"
+    if args.coarse_sweep != "config":
+        cfg["SolverParameter"]["coarseSweep"] = args.coarse_sweep
     scene = lh.Scene(cfg)
     model = scene.GetLiquidModel()
     model.Init()
@@ -426,10 +433,10 @@ def main():
         (_, _, d2), nr = model.GetSolver().LevelInfo(lvl)
         active[lvl], rows[lvl] = nr * d2, nr
     fused = cfg["SolverParameter"].get("fusedSweepMinCells", 1 << 62) <= n0
-    rsw = row_sweep_levels(cfg)
+    per_launch = {l: sweeps_per_launch(cfg, l) for l in range(1, cfg["SolverParameter"]["numLevel"])}
 
     def abytes(k):
-        return algorithmic_bytes(k, n0, P, active, fused, rsw)
+        return algorithmic_bytes(k, n0, P, active, fused, per_launch)
 
     def rate(k):
         n, ms = prof[k]
@@ -443,7 +450,7 @@ def main():
     symbol = level_symbol(cfg, int(dom[-1]))[0] if dom in ("smooth_l1", "smooth_l2") else KERNEL_SYMBOL[dom]
     roof = {"kernel": dom, "symbol": symbol, "bound": "hbm", "achieved": round(ach, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(dom, workload), "launches": launches,
+            "traffic": measured_traffic(dom, workload, symbol), "launches": launches,
             "avg_launch_us": round(avg_s * 1e6, 2),
             "algorithmic_bytes_per_launch": abytes(dom)}
     kernels = {k: {"launches": v[0], "ms": round(v[1], 3), "pct_of_pass": round(100.0 * v[1] / (el_prof * 1e3), 1),
