@@ -49,13 +49,26 @@ def coarse_form(cfg, l):
     return "rows" if d[0] * d[1] * d[2] <= sp.get("rowSweepMaxCells", 1 << 18) else "passes"
 
 
+def coarsest_fp64(cfg):
+    """the coarsest level's sweeps in FP64 (SolverParameter.coarsestPrecision; default with "precision":
+    "refined")"""
+    s = cfg["SolverParameter"]
+    return s.get("coarsestPrecision", "fp64" if cfg.get("precision") == "refined" else "fp32") == "fp64"
+
+
+def sweep_bytes(cfg, l):
+    """algorithmic bytes per cell of one red-black sweep on level l: adiag + 3 face terms 16 B, r 4 B,
+    z read + written 8 B (16 B when the coarsest level's z is FP64)"""
+    return 36.0 if l == cfg["SolverParameter"]["numLevel"] - 1 and l > 0 and coarsest_fp64(cfg) else 28.0
+
+
 def sweeps_per_launch(cfg, l):
     """red-black sweeps per smoother launch on coarse level l: one for k_sweep_rows and k_tile_sweep,
     half of one for a k_smooth colour pass"""
     return 0.5 if coarse_form(cfg, l) == "passes" else 1.0
 
 
-def algorithmic_bytes(kernel, n0, P, active, fused, per_launch=None):
+def algorithmic_bytes(kernel, n0, P, active, fused, per_launch=None, sweep_b=None):
     """Bytes per launch counted once per array element the kernel must touch.
 
     smooth_l0: one colour pass = half of a red-black sweep, whose minimum is 28 B/cell
@@ -65,7 +78,7 @@ def algorithmic_bytes(kernel, n0, P, active, fused, per_launch=None):
     smooth_l1, smooth_l2: the same per cell of the rows holding liquid on level 1, and on
                level 2 (the coarsest of the three-level hierarchy): 28 B per sweep, times the
                sweeps of one launch (per_launch[l]: 1/2 for a colour pass, 1 for k_sweep_rows or
-               k_tile_sweep);
+               k_tile_sweep), 36 B per sweep when the coarsest level's z is FP64 (sweep_b[l]);
     spmv:      the p update fused with q = A p (k_cg_p_spmv, all but the first SpMV of a solve):
                z and p read, new p and q written, 4 coefficient floats -> 32 B per cell of
                those rows (the plain first SpMV moves 24 B);
@@ -77,10 +90,11 @@ def algorithmic_bytes(kernel, n0, P, active, fused, per_launch=None):
     active[l] = cells of the level-l rows that hold liquid.
     """
     per_launch = per_launch or {}
+    sweep_b = sweep_b or {}
     return {
         "smooth_l0": 28.0 * n0 if fused else 14.0 * active[0],
-        "smooth_l1": 28.0 * per_launch.get(1, 0.5) * active.get(1, 0),
-        "smooth_l2": 28.0 * per_launch.get(2, 0.5) * active.get(2, 0),
+        "smooth_l1": sweep_b.get(1, 28.0) * per_launch.get(1, 0.5) * active.get(1, 0),
+        "smooth_l2": sweep_b.get(2, 28.0) * per_launch.get(2, 0.5) * active.get(2, 0),
         "spmv": 32.0 * active[0],
         "p2g": 20.0 * P + 4.0 * n0,
         "g2p": 72.0 * P + 12.0 * n0,
@@ -118,6 +132,8 @@ def level_symbol(cfg, l):
     mode = 1 if cfg["SolverParameter"].get("smoother", "reference") == "symmetric" else 0
     form = coarse_form(cfg, l)
     if form == "rows":
+        if l == cfg["SolverParameter"]["numLevel"] - 1 and coarsest_fp64(cfg):
+            return "k_sweep_rows<%d, double>" % mode, row_sweep_grid(d)
         return "k_sweep_rows<%d>" % mode, row_sweep_grid(d)
     if form == "tiles":
         return "k_tile_sweep<%d>" % mode, tile_sweep_grid(d)
@@ -434,9 +450,10 @@ def main():
         active[lvl], rows[lvl] = nr * d2, nr
     fused = cfg["SolverParameter"].get("fusedSweepMinCells", 1 << 62) <= n0
     per_launch = {l: sweeps_per_launch(cfg, l) for l in range(1, cfg["SolverParameter"]["numLevel"])}
+    sweep_b = {l: sweep_bytes(cfg, l) for l in range(1, cfg["SolverParameter"]["numLevel"])}
 
     def abytes(k):
-        return algorithmic_bytes(k, n0, P, active, fused, per_launch)
+        return algorithmic_bytes(k, n0, P, active, fused, per_launch, sweep_b)
 
     def rate(k):
         n, ms = prof[k]

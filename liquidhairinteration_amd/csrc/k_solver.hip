@@ -437,14 +437,19 @@ __device__ __forceinline__ float get_ax7(const LvP& L, size_t c, int x, int y, i
 // The operands of one cell's update, loaded without any test (out-of-range neighbours read
 // the cell itself): z and class of the cell and its six neighbours, its diagonal, its three
 // +face terms and the three -face terms of its minus neighbours, and r.
+// a cell's stencil for the row sweep; T is the type of z (float, or double for the coarsest level
+// in FP64), the coefficients stay FP32 and are promoted
+template <class T>
 struct Sten {
-  float zc, ad, rv, zxp, zxm, zyp, zym, zzp, zzm, txp, txm, typ, tym, tzp, tzm;
+  T zc, zxp, zxm, zyp, zym, zzp, zzm;
+  float ad, rv, txp, txm, typ, tym, tzp, tzm;
   int k, kxp, kxm, kyp, kym, kzp, kzm;
   bool hxp, hxm, hyp, hym, hzp, hzm;
 };
-__device__ __forceinline__ Sten sten_load(const LvP& L, const float* __restrict__ z, const float* __restrict__ r,
-                                          int x, int y, int w, size_t c) {
-  Sten s;
+template <class T>
+__device__ __forceinline__ Sten<T> sten_load(const LvP& L, const T* __restrict__ z, const float* __restrict__ r, int x,
+                                             int y, int w, size_t c) {
+  Sten<T> s;
   const size_t sx = (size_t)L.d[1] * L.d[2], sy = (size_t)L.d[2];
   s.hxp = x + 1 < L.d[0];
   s.hxm = x >= 1;
@@ -479,34 +484,35 @@ __device__ __forceinline__ Sten sten_load(const LvP& L, const float* __restrict_
   return s;
 }
 // the loaded values are complete here (see get_ax_full)
-__device__ __forceinline__ void sten_pin(Sten& s) {
+template <class T>
+__device__ __forceinline__ void sten_pin(Sten<T>& s) {
   asm volatile("" : "+v"(s.zc), "+v"(s.ad), "+v"(s.rv), "+v"(s.k), "+v"(s.kxp), "+v"(s.kxm), "+v"(s.kyp),
                "+v"(s.kym), "+v"(s.kzp), "+v"(s.kzm));
   asm volatile("" : "+v"(s.zxp), "+v"(s.zxm), "+v"(s.zyp), "+v"(s.zym), "+v"(s.zzp), "+v"(s.zzm), "+v"(s.txp),
                "+v"(s.txm), "+v"(s.typ), "+v"(s.tym), "+v"(s.tzp), "+v"(s.tzm));
 }
 // GetAx of the cell with the given neighbour z values (order and conditions of get_ax)
-__device__ __forceinline__ float sten_ax(const Sten& s, float zc, float zxp, float zxm, float zyp, float zym,
-                                         float zzp, float zzm) {
-  const float res0 = zc * s.ad;
-  float res = res0;
-  res = (s.hxp && s.kxp == 2) ? res + zxp * (-s.txp) : res;
-  res = (s.hxm && s.kxm == 2) ? res + zxm * (-s.txm) : res;
-  res = (s.hyp && s.kyp == 2) ? res + zyp * (-s.typ) : res;
-  res = (s.hym && s.kym == 2) ? res + zym * (-s.tym) : res;
-  res = (s.hzp && s.kzp == 2) ? res + zzp * (-s.tzp) : res;
-  res = (s.hzm && s.kzm == 2) ? res + zzm * (-s.tzm) : res;
+template <class T>
+__device__ __forceinline__ T sten_ax(const Sten<T>& s, T zc, T zxp, T zxm, T zyp, T zym, T zzp, T zzm) {
+  const T res0 = zc * (T)s.ad;
+  T res = res0;
+  res = (s.hxp && s.kxp == 2) ? res + zxp * (T)(-s.txp) : res;
+  res = (s.hxm && s.kxm == 2) ? res + zxm * (T)(-s.txm) : res;
+  res = (s.hyp && s.kyp == 2) ? res + zyp * (T)(-s.typ) : res;
+  res = (s.hym && s.kym == 2) ? res + zym * (T)(-s.tym) : res;
+  res = (s.hzp && s.kzp == 2) ? res + zzp * (T)(-s.tzp) : res;
+  res = (s.hzm && s.kzm == 2) ? res + zzm * (T)(-s.tzm) : res;
   return s.k == 0 ? res0 : res;
 }
-template <int MODE>
-__device__ __forceinline__ float sten_relax(const Sten& s, float ax, float zc, float w) {
+template <int MODE, class T>
+__device__ __forceinline__ T sten_relax(const Sten<T>& s, T ax, T zc, float w) {
   if (is_small(s.ad)) return zc;
-  const float term = (s.rv - ax) / s.ad;
+  const T term = ((T)s.rv - ax) / (T)s.ad;
   if (MODE == 0) {
-    const float wt = w * term;
-    return (float)((1.0 - (double)w) * (double)zc + (double)wt);
+    const T wt = (T)w * term;
+    return (T)((1.0 - (double)w) * (double)zc + (double)wt);
   }
-  return zc + w * term;
+  return zc + (T)w * term;
 }
 
 // One red-black sweep on the listed 4 x 4 x 64 tiles of a level (tiles holding a restriction
@@ -656,8 +662,8 @@ __global__ void k_tile_flags(const int* __restrict__ rowflag, int* __restrict__ 
 // operation.  The rows are the level's z-support (liquid rows and the rows a coarser
 // correction was prolonged into, whose cells are copied): every value read later from z lies
 // there; other rows are only ever multiplied by a zero diagonal.
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_sweep_rows(float* __restrict__ zout, const float* __restrict__ zin,
+template <int MODE, class T = float>
+__global__ __launch_bounds__(kBlock) void k_sweep_rows(T* __restrict__ zout, const T* __restrict__ zin,
                                                        const float* __restrict__ r, LvP L,
                                                        const int* __restrict__ rows, const int* __restrict__ nrows,
                                                        int first, float w) {
@@ -684,12 +690,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep_rows(float* __restrict__ zout,
     // the z neighbour in the next / previous segment (second-colour lanes at a segment end)
     const bool nzp = !first_colour && lane == 63 && wz + 1 < d2;
     const bool nzm = !first_colour && lane == 0 && wz >= 1;
-    Sten own = sten_load(L, zin, r, x, y, wz, c);
-    Sten s0 = sten_load(L, zin, r, nxp ? x + 1 : x, y, wz, nxp ? c + sx : c);
-    Sten s1 = sten_load(L, zin, r, nxm ? x - 1 : x, y, wz, nxm ? c - sx : c);
-    Sten s2 = sten_load(L, zin, r, x, nyp ? y + 1 : y, wz, nyp ? c + sy : c);
-    Sten s3 = sten_load(L, zin, r, x, nym ? y - 1 : y, wz, nym ? c - sy : c);
-    Sten sb = sten_load(L, zin, r, x, y, nzp ? wz + 1 : (nzm ? wz - 1 : wz), nzp ? c + 1 : (nzm ? c - 1 : c));
+    Sten<T> own = sten_load<T>(L, zin, r, x, y, wz, c);
+    Sten<T> s0 = sten_load<T>(L, zin, r, nxp ? x + 1 : x, y, wz, nxp ? c + sx : c);
+    Sten<T> s1 = sten_load<T>(L, zin, r, nxm ? x - 1 : x, y, wz, nxm ? c - sx : c);
+    Sten<T> s2 = sten_load<T>(L, zin, r, x, nyp ? y + 1 : y, wz, nyp ? c + sy : c);
+    Sten<T> s3 = sten_load<T>(L, zin, r, x, nym ? y - 1 : y, wz, nym ? c - sy : c);
+    Sten<T> sb = sten_load<T>(L, zin, r, x, y, nzp ? wz + 1 : (nzm ? wz - 1 : wz), nzp ? c + 1 : (nzm ? c - 1 : c));
     sten_pin(own);
     sten_pin(s0);
     sten_pin(s1);
@@ -697,22 +703,22 @@ __global__ __launch_bounds__(kBlock) void k_sweep_rows(float* __restrict__ zout,
     sten_pin(s3);
     sten_pin(sb);
     // first colour: own update; second colour: the neighbours' updates
-    const float v_own = sten_relax<MODE>(own, sten_ax(own, own.zc, own.zxp, own.zxm, own.zyp, own.zym, own.zzp,
-                                                      own.zzm), own.zc, w);
-    const float v0 = sten_relax<MODE>(s0, sten_ax(s0, s0.zc, s0.zxp, s0.zxm, s0.zyp, s0.zym, s0.zzp, s0.zzm), s0.zc, w);
-    const float v1 = sten_relax<MODE>(s1, sten_ax(s1, s1.zc, s1.zxp, s1.zxm, s1.zyp, s1.zym, s1.zzp, s1.zzm), s1.zc, w);
-    const float v2 = sten_relax<MODE>(s2, sten_ax(s2, s2.zc, s2.zxp, s2.zxm, s2.zyp, s2.zym, s2.zzp, s2.zzm), s2.zc, w);
-    const float v3 = sten_relax<MODE>(s3, sten_ax(s3, s3.zc, s3.zxp, s3.zxm, s3.zyp, s3.zym, s3.zzp, s3.zzm), s3.zc, w);
-    const float vb = sten_relax<MODE>(sb, sten_ax(sb, sb.zc, sb.zxp, sb.zxm, sb.zyp, sb.zym, sb.zzp, sb.zzm), sb.zc, w);
+    const T v_own = sten_relax<MODE>(own, sten_ax(own, own.zc, own.zxp, own.zxm, own.zyp, own.zym, own.zzp,
+                                                  own.zzm), own.zc, w);
+    const T v0 = sten_relax<MODE>(s0, sten_ax(s0, s0.zc, s0.zxp, s0.zxm, s0.zyp, s0.zym, s0.zzp, s0.zzm), s0.zc, w);
+    const T v1 = sten_relax<MODE>(s1, sten_ax(s1, s1.zc, s1.zxp, s1.zxm, s1.zyp, s1.zym, s1.zzp, s1.zzm), s1.zc, w);
+    const T v2 = sten_relax<MODE>(s2, sten_ax(s2, s2.zc, s2.zxp, s2.zxm, s2.zyp, s2.zym, s2.zzp, s2.zzm), s2.zc, w);
+    const T v3 = sten_relax<MODE>(s3, sten_ax(s3, s3.zc, s3.zxp, s3.zxm, s3.zyp, s3.zym, s3.zzp, s3.zzm), s3.zc, w);
+    const T vb = sten_relax<MODE>(sb, sten_ax(sb, sb.zc, sb.zxp, sb.zxm, sb.zyp, sb.zym, sb.zzp, sb.zzm), sb.zc, w);
     // the segment after the first colour: first-colour lanes hold their update, the others zin
-    const float after1 = first_colour ? v_own : own.zc;
+    const T after1 = first_colour ? v_own : own.zc;
     // the shuffles run with every lane of the wave active (a lane outside a shuffle supplies
     // no value to the others); segment-end lanes then take their recomputed neighbour
-    const float sl = __shfl(after1, lane - 1 < 0 ? 0 : lane - 1);
-    const float sr = __shfl(after1, lane + 1 > 63 ? 63 : lane + 1);
-    const float zl = lane == 0 ? vb : sl;
-    const float zr = lane == 63 ? vb : sr;
-    float out = after1;
+    const T sl = __shfl(after1, lane - 1 < 0 ? 0 : lane - 1);
+    const T sr = __shfl(after1, lane + 1 > 63 ? 63 : lane + 1);
+    const T zl = lane == 0 ? vb : sl;
+    const T zr = lane == 63 ? vb : sr;
+    T out = after1;
     if (!first_colour) out = sten_relax<MODE>(own, sten_ax(own, own.zc, v0, v1, v2, v3, zr, zl), own.zc, w);
     if (zr0 < d2) zout[c] = out;
   }
@@ -782,7 +788,8 @@ __global__ void __launch_bounds__(256) k_rbgs_sweep(float* __restrict__ zout, co
 __global__ __launch_bounds__(kBlock) void k_restrict(float* __restrict__ rc, float* __restrict__ zc,
                                                           const float* __restrict__ rf, const float* __restrict__ zf,
                                                           LvP Lf, int c1, int c2, const int* __restrict__ rows,
-                                                          const int* __restrict__ nrows, int nrow_cap) {
+                                                          const int* __restrict__ nrows, int nrow_cap,
+                                                          double* __restrict__ zc64) {
   static_assert(kBlock == 256, "4 fine rows x 64 fine z per block");
   __shared__ float s_d[8][32];
   const int nch = (c2 + 31) >> 5;
@@ -808,6 +815,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(float* __restrict__ rc, flo
       const size_t i = (size_t)row * c2 + ch * 32 + tid;
       rc[i] = acc;
       zc[i] = 0.0f;
+      if (zc64) zc64[i] = 0.0;  // the coarsest level's FP64 z
     }
     __syncthreads();
   }
@@ -833,6 +841,19 @@ __global__ void k_prolong(float* __restrict__ zf, const float* __restrict__ zc, 
         zf[((size_t)x * f1 + y) * f2 + w] += v;
       }
     }
+  }
+}
+
+// z = (float) z64 on the rows of a level's row list (the FP64 coarsest level's result, for the
+// prolongation)
+__global__ void k_z64_to_f32(float* __restrict__ z, const double* __restrict__ z64, const int* __restrict__ rows,
+                             const int* __restrict__ nrows, int d2) {
+  const RowIter it = row_iter(d2);
+  const int nr = *nrows;
+  if (!it.on) return;
+  for (int k = blockIdx.x * it.R + it.sub; k < nr; k += gridDim.x * it.R) {
+    const size_t base = (size_t)rows[k] * d2;
+    for (int zi = it.z0; zi < d2; zi += it.zs) z[base + zi] = (float)z64[base + zi];
   }
 }
 
@@ -1381,6 +1402,31 @@ void sweeps(Ctx& c, int l, int first, int fwd, int rev, int mode) {
   for (int it = 0; it < rev; ++it) sweep(c, l, 1 - first, mode);
 }
 
+// The coarsest level's sweeps in FP64 (coarsest_fp64): `fwd` row sweeps starting with colour 0,
+// then `rev` starting with colour 1, on z64 (zeroed by the restriction) with r promoted, then z =
+// (float) z64 on the z-support rows for the prolongation
+void coarsest_sweeps64(Ctx& c, int fwd, int rev, int mode) {
+  const int l = (int)c.lv.size() - 1;
+  Level& L = c.lv[l];
+  const size_t nseg = div_up((size_t)L.d[2], (size_t)64);
+  const unsigned nb = (unsigned)std::max<size_t>(
+      1, std::min<size_t>(div_up((size_t)L.d[0] * L.d[1] * nseg, (size_t)(kBlock / 64)), 8192));
+  {
+    ProfScope ps(c, l == 1 ? LH_PROF_SMOOTH_L1 : LH_PROF_SMOOTH_L2, fwd + rev);
+    for (int it = 0; it < fwd + rev; ++it) {
+      const int first = it < fwd ? 0 : 1;
+      if (mode == 0)
+        hipLaunchKernelGGL((k_sweep_rows<0, double>), dim3(nb), dim3(kBlock), 0, c.s, L.z64b, L.z64, L.r, lvp(L),
+                           L.zrows, L.nzrows, first, c.cfg.omega);
+      else
+        hipLaunchKernelGGL((k_sweep_rows<1, double>), dim3(nb), dim3(kBlock), 0, c.s, L.z64b, L.z64, L.r, lvp(L),
+                           L.zrows, L.nzrows, first, c.cfg.omega);
+      std::swap(L.z64, L.z64b);
+    }
+  }
+  hipLaunchKernelGGL(k_z64_to_f32, dim3(row_blocks(L)), dim3(kBlock), 0, c.s, L.z, L.z64, L.zrows, L.nzrows, L.d[2]);
+}
+
 // ApplyPreconditioner, PressureSolver.cpp:719-772 (symmetric mode: z[0] = 0 first,
 // post-smoothing and the second half of the coarsest sweeps in reverse colour order)
 void apply_precond(Ctx& c, bool z_zeroed) {
@@ -1404,9 +1450,12 @@ void apply_precond(Ctx& c, bool z_zeroed) {
     const size_t nch = div_up((size_t)Lc.d[2], (size_t)32);
     const unsigned nb = (unsigned)std::min<size_t>((size_t)Lc.d[0] * Lc.d[1] * nch, 16384);
     hipLaunchKernelGGL(k_restrict, dim3(nb), dim3(kBlock), 0, c.s, Lc.r, Lc.z, Lf.r, Lf.z, lvp(Lf), Lc.d[1], Lc.d[2],
-                       Lc.qrows, Lc.nqrows, Lc.d[0] * Lc.d[1]);
+                       Lc.qrows, Lc.nqrows, Lc.d[0] * Lc.d[1], Lc.z64);
   }
-  sweeps(c, L - 1, 0, c.cfg.num_final, sym ? c.cfg.num_final : 0, mode);
+  if (c.lv[L - 1].z64)
+    coarsest_sweeps64(c, c.cfg.num_final, sym ? c.cfg.num_final : 0, mode);
+  else
+    sweeps(c, L - 1, 0, c.cfg.num_final, sym ? c.cfg.num_final : 0, mode);
   for (int l = L - 2; l >= 0; --l) {
     Level& Lc = c.lv[l + 1];
     Level& Lf = c.lv[l];

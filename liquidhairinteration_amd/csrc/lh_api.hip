@@ -144,6 +144,9 @@ Config parse_config(const std::string& text) {
   if (prec == "refined") c.refined = 1;
   else if (prec != "fp32") throw std::runtime_error("Unknown precision " + prec);
   c.refined_tol = j.get("refinedTolerance", 1e-9);
+  std::string cp = s.get("coarsestPrecision", c.refined ? "fp64" : "fp32");
+  if (cp == "fp64") c.coarsest_fp64 = 1;
+  else if (cp != "fp32") throw std::runtime_error("Unknown coarsestPrecision " + cp);
   c.first_dt = (float)j.get("firstDt", 0.01);
   c.correct_volume = (j.has("correctVolume") && j.at("correctVolume").as_bool()) ? 1 : 0;
   c.steps = (int)j.get("steps", 0.0);
@@ -230,6 +233,20 @@ int alloc_ctx(Ctx& c) {
          dalloc(c, q.zflag, (size_t)q.d[0] * q.d[1]) && dalloc(c, q.tflag, div_up(q.d[0], 4) * div_up(q.d[1], 4)) &&
          dalloc(c, q.tiles, div_up(q.d[0], 4) * div_up(q.d[1], 4)) && dalloc(c, q.ntiles, 1);
   }
+  if (c.cfg.coarsest_fp64 && c.cfg.precond == LH_PRECOND_MULTIGRID) {
+    // the FP64 coarsest sweeps are row sweeps: the coarsest level must be a coarse level swept by rows
+    Level& q = c.lv[L - 1];
+    if (L < 2 || c.cfg.coarse_sweep != kCoarseRows || (long long)q.n > c.cfg.row_sweep_max_cells) {
+      c.err = "coarsestPrecision fp64 needs numLevel >= 2, coarseSweep rows and a coarsest level of at most "
+              "rowSweepMaxCells cells";
+      return LH_ERR_CONFIG;
+    }
+    ok = ok && dalloc(c, q.z64, q.n) && dalloc(c, q.z64b, q.n);
+    if (ok) {
+      hipMemsetAsync(q.z64, 0, q.n * sizeof(double), c.s);
+      hipMemsetAsync(q.z64b, 0, q.n * sizeof(double), c.s);
+    }
+  }
   c.npartial = 1024;  // also the PCG vector kernels' grid: a multiple of 8 (per-XCD row runs)
   ok = ok && dalloc(c, c.p, c.ncell) && dalloc(c, c.p2, c.ncell) && dalloc(c, c.x, c.ncell) && dalloc(c, c.q, c.ncell) &&
        dalloc(c, c.partial, 4 * (size_t)c.npartial) && dalloc(c, c.dsc, 16) && dalloc(c, c.dsc64, 16) &&
@@ -288,7 +305,8 @@ void free_ctx(Ctx& c) {
   F(c.bin_count); F(c.bin_start); F(c.bin_bflag); F(c.bin_bsum); F(c.bin_boff); F(c.scan_tmp); F(c.pkey); F(c.pslot);
   F(c.pbuf[0]); F(c.pbuf[1]); F(c.ppack); F(c.cv_flag);
   for (auto& q : c.lv) {
-    F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2); F(q.rows); F(q.nrows);
+    F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2); F(q.z64); F(q.z64b);
+    F(q.rows); F(q.nrows);
     F(q.aflag); F(q.qrows); F(q.nqrows); F(q.qflag); F(q.zrows); F(q.nzrows); F(q.zflag); F(q.tflag); F(q.tiles); F(q.ntiles);
   }
   F(c.p); F(c.p2); F(c.x); F(c.q); F(c.partial); F(c.dsc); F(c.dsc64); F(c.dmaxv); F(c.dflag);

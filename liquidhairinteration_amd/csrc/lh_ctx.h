@@ -43,6 +43,9 @@ struct Config {
   long long row_sweep_max_cells = 1LL << 18;
   // mixed-precision refinement (FP32 inner MGPCG under an FP64 residual loop)
   int refined = 0;
+  // the multigrid coarsest level's red-black sweeps in FP64 (SolverParameter.coarsestPrecision;
+  // default: on with refined precision): r and z promoted, z rounded back for the prolongation
+  int coarsest_fp64 = 0;
   double refined_tol = 1e-9;
   int max_outer = 50;
   // boundary SDF program (postfix, 12 floats per node)
@@ -76,6 +79,8 @@ struct Level {
   float* r = nullptr;
   float* z = nullptr;
   float* z2 = nullptr;  // ping-pong partner of z for the fused red-black sweep
+  double* z64 = nullptr;   // the coarsest level's z in FP64 (coarsest_fp64) ...
+  double* z64b = nullptr;  // ... and its ping-pong partner
   // z-rows (x, y) holding at least one liquid cell (cls != 0), ascending; rebuilt with the operator
   int* rows = nullptr;   // capacity d[0] * d[1]
   int* nrows = nullptr;  // device count

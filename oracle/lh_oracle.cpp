@@ -66,6 +66,7 @@ struct orc_params {
   int correct_cycle;   // LiquidParameter.correctCycle
   unsigned long long seed;
   long long num_particles;  // LiquidParameter.numParticles (the 2D lattice spawn's cap)
+  int coarsest_fp64;        // 1: the multigrid coarsest level's sweeps in FP64 (refined mode's default)
 };
 struct orc_solve_stats {
   int iterations, converged;
@@ -718,6 +719,39 @@ void prolong(const Model& m, SolveVecs<T>& v, int to) {  // :774-794
       }
 }
 
+// The coarsest level of the V-cycle, PressureSolver.cpp:754-757: num_final red-black sweeps
+// (symmetric mode: then num_final in reverse colour order).  With coarsest_fp64 the sweeps run in
+// FP64: r and z (zero after the restriction) promoted, the same formulas and order on the FP32
+// coefficients promoted, and z rounded back to T for the prolongation.
+template <class T>
+void coarsest_sweeps(const Model& m, SolveVecs<T>& v, int smoother) {
+  const int L = m.max_level;
+  auto run = [&](auto& vv) {
+    for (int it = 0; it < m.prm.num_final; ++it) {
+      smooth(m, vv, L - 1, 0, smoother);
+      smooth(m, vv, L - 1, 1, smoother);
+    }
+    if (smoother == 1) {
+      for (int it = 0; it < m.prm.num_final; ++it) {
+        smooth(m, vv, L - 1, 1, smoother);
+        smooth(m, vv, L - 1, 0, smoother);
+      }
+    }
+  };
+  if (!m.prm.coarsest_fp64) {
+    run(v);
+    return;
+  }
+  SolveVecs<double> d;
+  d.r.resize(L);
+  d.z.resize(L);
+  d.r[L - 1].assign(v.r[L - 1].begin(), v.r[L - 1].end());
+  d.z[L - 1].assign(v.z[L - 1].begin(), v.z[L - 1].end());
+  run(d);
+  std::vector<T>& z = v.z[L - 1];
+  for (size_t c = 0; c < z.size(); ++c) z[c] = (T)d.z[L - 1][c];
+}
+
 template <class T>
 void apply_precond(const Model& m, SolveVecs<T>& v, int smoother) {  // :719-772
   const A3& phi = m.liquidPhi;
@@ -744,16 +778,7 @@ void apply_precond(const Model& m, SolveVecs<T>& v, int smoother) {  // :719-772
     }
     restrict_(m, v, l);
   }
-  for (int it = 0; it < m.prm.num_final; ++it) {
-    smooth(m, v, L - 1, 0, smoother);
-    smooth(m, v, L - 1, 1, smoother);
-  }
-  if (smoother == 1) {
-    for (int it = 0; it < m.prm.num_final; ++it) {
-      smooth(m, v, L - 1, 1, smoother);
-      smooth(m, v, L - 1, 0, smoother);
-    }
-  }
+  coarsest_sweeps(m, v, smoother);
   for (int l = L - 2; l >= 0; --l) {
     prolong(m, v, l);
     for (int it = 0; it < m.prm.num_post; ++it) {

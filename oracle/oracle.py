@@ -30,7 +30,7 @@ class Params(C.Structure):
         ("tank_amp", C.c_float), ("tank_freq", C.c_float), ("tank_axis", C.c_int),
         ("nsdf", C.c_int), ("sdf", C.POINTER(C.c_float)),
         ("correct_volume", C.c_int), ("correct_cycle", C.c_int), ("seed", C.c_ulonglong),
-        ("num_particles", C.c_longlong),
+        ("num_particles", C.c_longlong), ("coarsest_fp64", C.c_int),
     ]
 
 
@@ -181,7 +181,14 @@ def params_from_config(cfg, dot_mode=1, p2g_order=1, smoother=None):
     p.correct_cycle = int(lq["correctCycle"])
     p.seed = int(cfg.get("seed", 0))
     p.num_particles = int(lq["numParticles"])
+    p.coarsest_fp64 = 1 if coarsest_precision(cfg) == "fp64" else 0
     return p
+
+
+def coarsest_precision(cfg):
+    """SolverParameter.coarsestPrecision; default fp64 with "precision": "refined", else fp32"""
+    s = cfg["SolverParameter"]
+    return s.get("coarsestPrecision", "fp64" if cfg.get("precision") == "refined" else "fp32")
 
 
 class Oracle:

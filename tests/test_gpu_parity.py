@@ -64,6 +64,13 @@ def coarse_tiles(cfg):
     return cfg
 
 
+def coarsest64(cfg):
+    """the multigrid coarsest level's sweeps in FP64 (the refined mode's default), here under the
+    FP32 PCG so that the oracle's restatement is compared bit for bit"""
+    cfg["SolverParameter"]["coarsestPrecision"] = "fp64"
+    return cfg
+
+
 def precond(cfg, kind):
     """CGSolver<3>'s other preconditioners (PressureSolver.cpp:721-743): None (z = r) and Jacobi
     (z = r / Adiag on cells with a nonzero diagonal)"""
@@ -106,6 +113,8 @@ CASES = {
     # (overlap product); every second particle id moves per call at 40^3
     "dam24_correct_volume": lambda: dict(small_dam(24, "symmetric"), correctVolume=True),
     "dam40_correct_volume": lambda: cv_cycle(dict(small_dam(40, "symmetric", fused=True), correctVolume=True), 2),
+    "dam24_symmetric_coarsest_fp64": lambda: coarsest64(small_dam(24, "symmetric")),
+    "dam24_reference_coarsest_fp64": lambda: coarsest64(small_dam(24, "reference")),
     "dam24_precond_none": lambda: precond(small_dam(24, "reference"), "None"),
     "dam24_precond_jacobi": lambda: precond(small_dam(24, "reference"), "Jacobi"),
     "dam32_tank_sphere_jacobi": lambda: precond(small_dam(32, "symmetric", tank=True), "Jacobi"),
@@ -191,7 +200,8 @@ def test_stages_bit_exact(case):
 @pytest.mark.parametrize("case", ["dam24_reference_smoother", "dam24_tank_sphere", "dam40_symmetric_fused_sweep",
                                   "dam24_correct_volume", "tall_16x24x160_row_segments",
                                   "dam40_symmetric_tile_sweep", "tall_16x24x160_tile_sweep",
-                                  "dam24_precond_none", "dam32_tank_sphere_jacobi"])
+                                  "dam24_precond_none", "dam32_tank_sphere_jacobi",
+                                  "dam24_symmetric_coarsest_fp64", "dam24_reference_coarsest_fp64"])
 def test_free_running_steps_bit_exact(case):
     print("case", case, flush=True)
     cfg = CASES[case]()

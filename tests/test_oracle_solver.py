@@ -15,17 +15,25 @@ from conftest import config_file
 from oracle import Oracle
 
 
-def advanced(N=16, smoother="symmetric", steps=3):
+def advanced(N=16, smoother="symmetric", steps=3, coarsest="fp32"):
+    """the system of the 4th step of a 16^3 dam break (the first 3 stepped with the FP32 coarsest
+    level, so every coarsest precision builds the same system)"""
     cfg = json.load(open(config_file("dam_break_48.json")))
     cfg["gridDimension"] = [N, N, N]
     cfg["Boundary"]["child"]["halfExtent"] = [0.5 - 3.0 / N] * 3
     cfg["SolverParameter"].update({"smoother": smoother, "numLevel": 2, "maxIterations": 400})
-    orc = Oracle(cfg, dot_mode=1, p2g_order=1)
-    orc.init()
+    run = Oracle(cfg, dot_mode=1, p2g_order=1)
+    run.init()
     dt = np.float32(0.01)
     for _ in range(steps):
-        orc.step(dt)
-        dt = np.float32(orc.get_cfl()[0])
+        run.step(dt)
+        dt = np.float32(run.get_cfl()[0])
+    cfg["SolverParameter"]["coarsestPrecision"] = coarsest
+    orc = Oracle(cfg, dot_mode=1, p2g_order=1)
+    orc.compute_solid_phi()
+    orc.set_particles(*run.particles())
+    orc.set(12, run.get(12))
+    orc.set_time(run.time)
     orc.clear_grid()
     orc.compute_liquid_phi()
     orc.p2g()
@@ -73,9 +81,12 @@ def test_operator_is_symmetric_and_diagonally_dominant():
     assert np.any(d > off * (1 + 1e-3))           # free-surface (ghost fluid) rows are strictly dominant
 
 
-@pytest.mark.parametrize("smoother", ["symmetric", "reference"])
-def test_pcg_matches_sparse_direct_solve(smoother):
-    orc = advanced(smoother=smoother)
+@pytest.mark.parametrize("smoother,coarsest", [("symmetric", "fp32"), ("reference", "fp32"), ("symmetric", "fp64"),
+                                               ("reference", "fp64")])
+def test_pcg_matches_sparse_direct_solve(smoother, coarsest):
+    """with the coarsest level's sweeps in FP32 or FP64 (coarsestPrecision), the FP32 MGPCG and the
+    FP64 PCG both converge to the direct solution"""
+    orc = advanced(smoother=smoother, coarsest=coarsest)
     A, rows = assemble(orc)
     b = orc.get(13).reshape(-1)
     x_direct = spla.spsolve(A.tocsc(), b[rows].astype(np.float64))
@@ -93,3 +104,19 @@ def test_pcg_matches_sparse_direct_solve(smoother):
     upd[1:-1, 1:-1, 1:-1] = True
     upd &= phi <= 0
     assert np.all(x32.reshape(phi.shape)[~upd] == 0)
+
+
+@pytest.mark.parametrize("smoother", ["symmetric", "reference"])
+def test_fp64_coarsest_changes_only_rounding(smoother):
+    """the coarsest level in FP64 is the same preconditioner up to FP32 rounding: the FP32 MGPCG's
+    iteration count moves by at most one and the solutions agree to the solve's tolerance"""
+    o32 = advanced(smoother=smoother, coarsest="fp32")
+    o64 = advanced(smoother=smoother, coarsest="fp64")
+    b = o32.get(13)
+    assert np.array_equal(b, o64.get(13))
+    x32, s32 = o32.solve_fp32(b)
+    x64, s64 = o64.solve_fp32(b)
+    assert s32.converged and s64.converged
+    assert abs(s32.iterations - s64.iterations) <= 1
+    assert not np.array_equal(x32, x64)           # the FP64 sweeps did run
+    np.testing.assert_allclose(x64, x32, rtol=0, atol=1e-4 * np.abs(x32).max())
