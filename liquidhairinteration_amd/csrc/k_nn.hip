@@ -149,6 +149,150 @@ __global__ __launch_bounds__(kBlk) void k_conv(const bf16_t* __restrict__ in, co
   }
 }
 
+// The same convolution over LDS tiles: a workgroup (4 waves) computes TX x TY x 16 output voxels.
+// The input over the tile and a one-voxel halo, (TX + 2) (TY + 2) 18 voxels x CIN channels, is
+// staged in LDS once (voxel stride CIN + 8 BF16: an odd multiple of 16 bytes, so the 16-byte reads
+// of 16 consecutive voxels hit distinct banks), and every tap reads its B fragments from there
+// instead of gathering them from global memory 27 times.  The weights pass through LDS in chunks of
+// KC columns of K, double-buffered: each thread fetches its share of chunk c + 1 into registers
+// before the MFMAs of chunk c and stores it after them, so the global latency hides behind the
+// chunk's arithmetic.  Each wave owns TX TY / 4 z-runs of 16 voxels (NVB B blocks sharing each A
+// fragment).  The MFMA mapping, the K order (k = tap * CIN + ci) and the epilogues are those of
+// k_conv, so the output is identical.  Tiles are numbered z fastest and dealt to the 8 XCDs in
+// contiguous runs, so tiles that share halo voxels share an L2.
+template <int CIN, int COUT, int TX, int TY, int KC, int EPI>
+__global__ __launch_bounds__(256) void k_conv_tile(const bf16_t* __restrict__ in, const bf16_t* __restrict__ W,
+                                                   const float* __restrict__ bias, const bf16_t* __restrict__ act,
+                                                   bf16_t* __restrict__ out, int d0, int d1, int d2) {
+  static_assert(COUT % 16 == 0 && CIN % 8 == 0 && (TX * TY) % 4 == 0 && KC % 32 == 0, "channel counts / tile");
+  constexpr int TZ = 16, NVB = TX * TY / 4, MB = COUT / 16, KP = kpad32(27 * CIN), KS = KP / 32;
+  constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, NH = HX * HY * HZ, VS = CIN + 8;
+  constexpr int WS = KC + 8;                         // LDS row stride of a weight chunk (odd x 16 bytes)
+  constexpr int NCH = (KP + KC - 1) / KC;            // weight chunks
+  constexpr int WP = COUT * KC / 8;                  // 16-byte pieces of a chunk
+  constexpr int WPT = (WP + 255) / 256;              // pieces per thread
+  __shared__ __attribute__((aligned(16))) bf16_t s_in[NH * VS];
+  __shared__ __attribute__((aligned(16))) bf16_t s_w[2][COUT * WS];
+  // per K step and lane group: the LDS offset of the B fragment from the lane's halo-corner voxel
+  // (tap (ox, oy, oz) -> ((ox + 1) HY + oy + 1) HZ + oz + 1 voxels, plus the channel); past 27 taps
+  // the weights are zero and the offset 0 reads a finite in-tile value
+  __shared__ uint16_t s_off[KS * 4];
+  const int ntz = (d2 + TZ - 1) / TZ, nty = (d1 + TY - 1) / TY, ntx = (d0 + TX - 1) / TX;
+  const int nt = ntx * nty * ntz;
+  const int run = (nt + 7) >> 3;
+  const int t = ((int)blockIdx.x & 7) * run + ((int)blockIdx.x >> 3);
+  if ((int)blockIdx.x >= 8 * run || t >= nt) return;  // uniform over the block
+  const int tz = t % ntz, ty = (t / ntz) % nty, tx = t / (ntz * nty);
+  const int X0 = tx * TX, Y0 = ty * TY, Z0 = tz * TZ;
+  // weight chunk c: rows co, columns [c KC, c KC + KC) of W (zero past KP)
+  auto fetch_w = [&](int c, u32x4 (&q)[WPT]) {
+#pragma unroll
+    for (int j = 0; j < WPT; ++j) {
+      const int p = (int)threadIdx.x + 256 * j;
+      const int co = p / (KC / 8), k8 = p - co * (KC / 8);
+      const int k = c * KC + k8 * 8;
+      q[j] = (p < WP && k < KP) ? *(const u32x4*)(W + (size_t)co * KP + k) : u32x4{0u, 0u, 0u, 0u};
+    }
+  };
+  auto store_w = [&](int buf, const u32x4 (&q)[WPT]) {
+#pragma unroll
+    for (int j = 0; j < WPT; ++j) {
+      const int p = (int)threadIdx.x + 256 * j;
+      const int co = p / (KC / 8), k8 = p - co * (KC / 8);
+      if (p < WP) *(u32x4*)(&s_w[buf][co * WS + k8 * 8]) = q[j];
+    }
+  };
+  {
+    u32x4 q[WPT];
+    fetch_w(0, q);
+    store_w(0, q);
+  }
+  for (int i = threadIdx.x; i < KS * 4; i += 256) {
+    const int k0 = (i >> 2) * 32 + (i & 3) * 8;
+    const int tap = k0 / CIN, ci = k0 - tap * CIN;
+    int ox, oy, oz;
+    tap_off(tap, ox, oy, oz);
+    s_off[i] = tap < 27 ? (uint16_t)((((ox + 1) * HY + (oy + 1)) * HZ + (oz + 1)) * VS + ci) : (uint16_t)0;
+  }
+  // stage the input tile with its halo (zero outside the grid), 8 channels per 16-byte load
+  constexpr int C8 = CIN / 8;
+  for (int i = threadIdx.x; i < NH * C8; i += 256) {
+    const int hv = i / C8, c8 = i - hv * C8;
+    const int hz = hv % HZ, hy = (hv / HZ) % HY, hx = hv / (HZ * HY);
+    const int x = X0 - 1 + hx, y = Y0 - 1 + hy, z = Z0 - 1 + hz;
+    u32x4 q = u32x4{0u, 0u, 0u, 0u};
+    if ((unsigned)x < (unsigned)d0 && (unsigned)y < (unsigned)d1 && (unsigned)z < (unsigned)d2)
+      q = *(const u32x4*)(in + (size_t)((x * d1 + y) * d2 + z) * CIN + c8 * 8);
+    *(u32x4*)(s_in + hv * VS + c8 * 8) = q;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, row = lane & 15, kq = lane >> 4, wv = threadIdx.x >> 6;
+  int rx[NVB], ry[NVB], base[NVB];
+#pragma unroll
+  for (int nb = 0; nb < NVB; ++nb) {
+    const int rid = wv * NVB + nb;
+    rx[nb] = rid / TY;
+    ry[nb] = rid - rx[nb] * TY;
+    base[nb] = ((rx[nb] * HY + ry[nb]) * HZ + row) * VS;  // the halo-corner voxel of the lane's output voxel
+  }
+  f32x4 acc[MB][NVB];
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < NVB; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int c = 0; c < NCH; ++c) {
+    u32x4 q[WPT];
+    if (c + 1 < NCH) fetch_w(c + 1, q);
+    const bf16_t* wl = &s_w[c & 1][row * WS + kq * 8];
+#pragma unroll
+    for (int kk = 0; kk < KC / 32; ++kk) {
+      const int ks = c * (KC / 32) + kk;
+      if (ks < KS) {  // uniform
+        const int o = s_off[ks * 4 + kq];
+        bf16x8 a[MB], b[NVB];
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) a[mb] = __builtin_bit_cast(bf16x8, *(const u32x4*)(wl + mb * 16 * WS + kk * 32));
+#pragma unroll
+        for (int nb = 0; nb < NVB; ++nb) b[nb] = __builtin_bit_cast(bf16x8, *(const u32x4*)(s_in + base[nb] + o));
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+          for (int nb = 0; nb < NVB; ++nb)
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], b[nb], acc[mb][nb], 0, 0, 0);
+      }
+    }
+    if (c + 1 < NCH) {
+      store_w((c + 1) & 1, q);  // the other buffer: its last readers passed the previous barrier
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int nb = 0; nb < NVB; ++nb) {
+    const int x = X0 + rx[nb], y = Y0 + ry[nb], z = Z0 + row;
+    if (x >= d0 || y >= d1 || z >= d2) continue;
+    const size_t v = (size_t)((x * d1 + y) * d2 + z);
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      const int co = mb * 16 + kq * 4;
+      bf16_t o[4];
+      if constexpr (EPI == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o[i] = f2bf(fmaxf(acc[mb][nb][i] + bias[co + i], 0.f));
+      } else if constexpr (EPI == 1) {
+        const u32x2 am = *(const u32x2*)(act + v * COUT + co);
+        const bf16_t h[4] = {(bf16_t)(am[0] & 0xffffu), (bf16_t)(am[0] >> 16), (bf16_t)(am[1] & 0xffffu),
+                             (bf16_t)(am[1] >> 16)};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o[i] = bf_pos(h[i]) ? f2bf(acc[mb][nb][i]) : (bf16_t)0;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o[i] = f2bf(acc[mb][nb][i]);
+      }
+      *(u32x2*)(out + v * COUT + co) = u32x2{(unsigned)o[0] | ((unsigned)o[1] << 16), (unsigned)o[2] | ((unsigned)o[3] << 16)};
+    }
+  }
+}
+
 // dW[co][k] += sum_v g[v][co] * in[v + off(tap(k))][ci(k)], k < 27 CIN.  MFMA with the
 // voxels as the reduction: A = g^T (16 co x 32 voxels), B = im2col (32 voxels x 16 k).
 // Grid: x voxel chunks, y groups of NBW 16-column blocks of k, z groups of MBW 16-row blocks of co.
@@ -747,6 +891,11 @@ void padded_to_logical(const Net& n, const std::vector<float>& pd, float* lg) {
 #define CONV(CI, CO, NVB, EPI, IN, WP, BIAS, ACT, OUT, D)                                                          \
   hipLaunchKernelGGL((k_conv<CI, CO, NVB, EPI>), dim3((unsigned)div_up((size_t)(D)[0] * (D)[1] * (D)[2], 64 * (NVB))), \
                      dim3(kBlk), 0, c.s, IN, WP, BIAS, ACT, OUT, (D)[0], (D)[1], (D)[2])
+// the LDS-tiled form: TX x TY x 16 voxels per workgroup, the grid 8 x the per-XCD run of tiles
+#define CONVT(CI, CO, TX, TY, KC, EPI, IN, WP, BIAS, ACT, OUT, D)                                                \
+  hipLaunchKernelGGL((k_conv_tile<CI, CO, TX, TY, KC, EPI>),                                                     \
+                     dim3((unsigned)(8 * div_up(div_up((D)[0], TX) * div_up((D)[1], TY) * div_up((D)[2], 16), 8))), \
+                     dim3(256), 0, c.s, IN, WP, BIAS, ACT, OUT, (D)[0], (D)[1], (D)[2])
 
 void forward(Ctx& c, Net& n) {
   ProfScope ps(c, LH_PROF_NN_FWD);
@@ -758,24 +907,24 @@ void forward(Ctx& c, Net& n) {
   CONV(2, 16, 4, 0, n.x0, h + n.off[0], w + n.boff[0], nullptr, n.a0, D[0]);
   hipLaunchKernelGGL(k_pool<16>, dim3(nblocks((size_t)D[1][0] * D[1][1] * D[1][2] * 2)), dim3(kBlk), 0, c.s, n.a0, n.p0,
                      D[1][0], D[1][1], D[1][2]);
-  CONV(16, 32, 4, 0, n.p0, h + n.off[1], w + n.boff[1], nullptr, n.a1, D[1]);
+  CONVT(16, 32, 4, 4, 128, 0, n.p0, h + n.off[1], w + n.boff[1], nullptr, n.a1, D[1]);
   hipLaunchKernelGGL(k_pool<32>, dim3(nblocks((size_t)D[2][0] * D[2][1] * D[2][2] * 4)), dim3(kBlk), 0, c.s, n.a1, n.p1,
                      D[2][0], D[2][1], D[2][2]);
-  CONV(32, 64, 4, 0, n.p1, h + n.off[2], w + n.boff[2], nullptr, n.a2, D[2]);
+  CONVT(32, 64, 4, 4, 64, 0, n.p1, h + n.off[2], w + n.boff[2], nullptr, n.a2, D[2]);
   hipLaunchKernelGGL(k_pool<64>, dim3(nblocks((size_t)D[3][0] * D[3][1] * D[3][2] * 8)), dim3(kBlk), 0, c.s, n.a2, n.p2,
                      D[3][0], D[3][1], D[3][2]);
-  CONV(64, 96, 2, 0, n.p2, h + n.off[3], w + n.boff[3], nullptr, n.a3, D[3]);
+  CONVT(64, 96, 2, 2, 64, 0, n.p2, h + n.off[3], w + n.boff[3], nullptr, n.a3, D[3]);
   hipLaunchKernelGGL((k_upcat<96, 64>), dim3(nblocks((size_t)D[2][0] * D[2][1] * D[2][2] * 20)), dim3(kBlk), 0, c.s,
                      n.a3, n.a2, n.c2, D[2][0], D[2][1], D[2][2]);
-  CONV(160, 64, 4, 0, n.c2, h + n.off[4], w + n.boff[4], nullptr, n.d2, D[2]);
+  CONVT(160, 64, 2, 2, 128, 0, n.c2, h + n.off[4], w + n.boff[4], nullptr, n.d2, D[2]);
   hipLaunchKernelGGL((k_upcat<64, 32>), dim3(nblocks((size_t)D[1][0] * D[1][1] * D[1][2] * 12)), dim3(kBlk), 0, c.s,
                      n.d2, n.a1, n.c1, D[1][0], D[1][1], D[1][2]);
-  CONV(96, 32, 4, 0, n.c1, h + n.off[5], w + n.boff[5], nullptr, n.d1, D[1]);
+  CONVT(96, 32, 2, 2, 128, 0, n.c1, h + n.off[5], w + n.boff[5], nullptr, n.d1, D[1]);
   hipLaunchKernelGGL((k_upcat<32, 16>), dim3(nblocks((size_t)D[0][0] * D[0][1] * D[0][2] * 6)), dim3(kBlk), 0, c.s,
                      n.d1, n.a0, n.c0, D[0][0], D[0][1], D[0][2]);
   {
     ProfScope pd(c, LH_PROF_CONV_D0);
-    CONV(48, 16, 4, 0, n.c0, h + n.off[6], w + n.boff[6], nullptr, n.d0, D[0]);
+    CONVT(48, 16, 4, 4, 128, 0, n.c0, h + n.off[6], w + n.boff[6], nullptr, n.d0, D[0]);
   }
   hipLaunchKernelGGL(k_out_fwd, dim3(nblocks((size_t)D[0][0] * D[0][1] * D[0][2])), dim3(kBlk), 0, c.s, n.d0,
                      h + n.off[7], w + n.boff[7], n.y, D[0][0], D[0][1], D[0][2]);
@@ -815,32 +964,32 @@ void backward(Ctx& c, Net& n) {
   }
   // D0 (48 -> 16) at level 0
   WGRAD(48, 16, 1, 4, n.g_d0, n.c0, 6, D[0]);
-  CONV(16, 48, 4, 1, n.g_d0, wt + n.wtoff[6], nullptr, n.c0, n.g_c0, D[0]);
+  CONVT(16, 48, 4, 4, 128, 1, n.g_d0, wt + n.wtoff[6], nullptr, n.c0, n.g_c0, D[0]);
   hipLaunchKernelGGL((k_upcat_bwd<32, 48>), dim3(nblocks(Vl[1] * 4)), dim3(kBlk), 0, c.s, n.g_c0, n.g_d1, D[1][0],
                      D[1][1], D[1][2]);
   // D1 (96 -> 32) at level 1
   WGRAD(96, 32, 2, 4, n.g_d1, n.c1, 5, D[1]);
-  CONV(32, 96, 2, 1, n.g_d1, wt + n.wtoff[5], nullptr, n.c1, n.g_c1, D[1]);
+  CONVT(32, 96, 4, 4, 64, 1, n.g_d1, wt + n.wtoff[5], nullptr, n.c1, n.g_c1, D[1]);
   hipLaunchKernelGGL((k_upcat_bwd<64, 96>), dim3(nblocks(Vl[2] * 8)), dim3(kBlk), 0, c.s, n.g_c1, n.g_d2, D[2][0],
                      D[2][1], D[2][2]);
   // D2 (160 -> 64) at level 2
   WGRAD(160, 64, 2, 4, n.g_d2, n.c2, 4, D[2]);
-  CONV(64, 160, 1, 1, n.g_d2, wt + n.wtoff[4], nullptr, n.c2, n.g_c2, D[2]);
+  CONVT(64, 160, 2, 2, 64, 1, n.g_d2, wt + n.wtoff[4], nullptr, n.c2, n.g_c2, D[2]);
   hipLaunchKernelGGL((k_upcat_bwd<96, 160>), dim3(nblocks(Vl[3] * 12)), dim3(kBlk), 0, c.s, n.g_c2, n.g_a3, D[3][0],
                      D[3][1], D[3][2]);
   // E3 (64 -> 96) at level 3
   WGRAD(64, 96, 2, 4, n.g_a3, n.p2, 3, D[3]);
-  CONV(96, 64, 4, 2, n.g_a3, wt + n.wtoff[3], nullptr, nullptr, n.g_p2, D[3]);
+  CONVT(96, 64, 2, 2, 64, 2, n.g_a3, wt + n.wtoff[3], nullptr, nullptr, n.g_p2, D[3]);
   hipLaunchKernelGGL((k_skip_combine<64, 96, 160>), dim3(nblocks(Vl[2] * 8)), dim3(kBlk), 0, c.s, n.g_c2, n.a2, n.g_p2,
                      n.g_a2, D[2][0], D[2][1], D[2][2]);
   // E2 (32 -> 64) at level 2
   WGRAD(32, 64, 2, 4, n.g_a2, n.p1, 2, D[2]);
-  CONV(64, 32, 4, 2, n.g_a2, wt + n.wtoff[2], nullptr, nullptr, n.g_p1, D[2]);
+  CONVT(64, 32, 2, 4, 64, 2, n.g_a2, wt + n.wtoff[2], nullptr, nullptr, n.g_p1, D[2]);
   hipLaunchKernelGGL((k_skip_combine<32, 64, 96>), dim3(nblocks(Vl[1] * 4)), dim3(kBlk), 0, c.s, n.g_c1, n.a1, n.g_p1,
                      n.g_a1, D[1][0], D[1][1], D[1][2]);
   // E1 (16 -> 32) at level 1
   WGRAD(16, 32, 2, 4, n.g_a1, n.p0, 1, D[1]);
-  CONV(32, 16, 4, 2, n.g_a1, wt + n.wtoff[1], nullptr, nullptr, n.g_p0, D[1]);
+  CONVT(32, 16, 4, 4, 128, 2, n.g_a1, wt + n.wtoff[1], nullptr, nullptr, n.g_p0, D[1]);
   hipLaunchKernelGGL((k_skip_combine<16, 32, 48>), dim3(nblocks(Vl[0] * 2)), dim3(kBlk), 0, c.s, n.g_c0, n.a0, n.g_p0,
                      n.g_a0, D[0][0], D[0][1], D[0][2]);
   // E0 (2 -> 16) at level 0
