@@ -653,74 +653,90 @@ __global__ void k_tile_flags(const int* __restrict__ rowflag, int* __restrict__ 
 }
 
 // One red-black sweep (Smooth(l, first) then Smooth(l, 1 - first)) on the listed rows of a
-// coarse level in one launch.  A wave owns a 64-cell segment of a row, lane = z.  A cell of
-// the second colour needs the first-colour results of its six neighbours: the two in its row
-// come from the neighbouring lanes (or, at a segment end, are recomputed by the lane from zin),
-// the four in the rows x +- 1, y +- 1 are recomputed by the lane itself from zin (a first-colour
-// update depends on zin only).  zin is read, zout written (every cell of the segment), so no
-// wave sees another's writes; the values are those of two colour passes, operation for
-// operation.  The rows are the level's z-support (liquid rows and the rows a coarser
-// correction was prolonged into, whose cells are copied): every value read later from z lies
-// there; other rows are only ever multiplied by a zero diagonal.
+// coarse level in one launch.  A block of five waves owns a 64-cell segment of a row, lane = z.
+// A cell of the second colour needs the first-colour results of its six neighbours: the two in
+// its row come from the neighbouring lanes of wave 0 (or, at a segment end, are recomputed by
+// wave 0 from zin), the four in the rows x +- 1, y +- 1 are recomputed from zin by waves 1-4, one
+// row each, and handed to wave 0 through LDS (a first-colour update depends on zin only).  zin is
+// read, zout written (every cell of the segment), so no block sees another's writes; the values
+// are those of two colour passes, operation for operation.  Five short waves instead of one that
+// evaluates six stencils: every launch starts with cold L2s, so the cost is the latency of one
+// wave's chain of loads and dependent FP64/FP32 arithmetic, and one wave per SIMD hid none of it.
+// The rows are the level's z-support (liquid rows and the rows a coarser correction was prolonged
+// into, whose cells are copied): every value read later from z lies there; other rows are only
+// ever multiplied by a zero diagonal.
+constexpr int kSweepWaves = 5;
 template <int MODE, class T = float>
-__global__ __launch_bounds__(kBlock) void k_sweep_rows(T* __restrict__ zout, const T* __restrict__ zin,
-                                                       const float* __restrict__ r, LvP L,
-                                                       const int* __restrict__ rows, const int* __restrict__ nrows,
-                                                       int first, float w) {
-  const int d1 = L.d[1], d2 = L.d[2];
+__global__ __launch_bounds__(64 * kSweepWaves) void k_sweep_rows(T* __restrict__ zout, const T* __restrict__ zin,
+                                                                  const float* __restrict__ r, LvP L,
+                                                                  const int* __restrict__ rows,
+                                                                  const int* __restrict__ nrows, int first, float w) {
+  __shared__ T s_nb[4][64];  // first-colour updates of the rows x + 1, x - 1, y + 1, y - 1
+  const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
   const int nseg = (d2 + 63) >> 6;
-  const int lane = (int)threadIdx.x & 63;
-  const int kcap = L.d[0] * d1 - 1, kstep = gridDim.x * (kBlock / 64);
-  int k = blockIdx.x * (kBlock / 64) + ((int)threadIdx.x >> 6);  // (list index, segment) = (k / nseg, k % nseg)
-  int row_next = rows[min(k / nseg, kcap)];
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
   const int nr = *nrows;
   const size_t sx = (size_t)d1 * d2, sy = (size_t)d2;
-  for (; k < nr * nseg; k += kstep) {
-    const int row = row_next;
-    row_next = rows[min((k + kstep) / nseg, kcap)];
+  // workgroups go to the 8 XCDs in turn: XCD q takes the q-th eighth of the list's segments, so
+  // the rows x +- 1 of a row (a plane of the list away) are mostly read through the same L2
+  const int nwork = nr * nseg, chunk = (nwork + 7) >> 3;
+  const int q = (int)blockIdx.x & 7, per_xcd = ((int)gridDim.x - q + 7) >> 3;
+  for (int j = (int)blockIdx.x >> 3; j < chunk; j += per_xcd) {
+    const int k = q * chunk + j;  // (list index, segment) = (k / nseg, k % nseg)
+    if (k >= nwork) break;
+    const int row = rows[k / nseg];
     const int seg = k - (k / nseg) * nseg;
     const int x = row / d1, y = row - x * d1;
     const int zr0 = seg * 64 + lane;
     const int wz = min(zr0, d2 - 1);
     const size_t c = (size_t)row * d2 + wz;
     const bool first_colour = ((x + y + wz) & 1) == first;
-    // the four x/y neighbours (second-colour lanes) or the cell itself (first-colour lanes)
-    const bool nxp = !first_colour && x + 1 < L.d[0], nxm = !first_colour && x >= 1;
-    const bool nyp = !first_colour && y + 1 < d1, nym = !first_colour && y >= 1;
-    // the z neighbour in the next / previous segment (second-colour lanes at a segment end)
-    const bool nzp = !first_colour && lane == 63 && wz + 1 < d2;
-    const bool nzm = !first_colour && lane == 0 && wz >= 1;
-    Sten<T> own = sten_load<T>(L, zin, r, x, y, wz, c);
-    Sten<T> s0 = sten_load<T>(L, zin, r, nxp ? x + 1 : x, y, wz, nxp ? c + sx : c);
-    Sten<T> s1 = sten_load<T>(L, zin, r, nxm ? x - 1 : x, y, wz, nxm ? c - sx : c);
-    Sten<T> s2 = sten_load<T>(L, zin, r, x, nyp ? y + 1 : y, wz, nyp ? c + sy : c);
-    Sten<T> s3 = sten_load<T>(L, zin, r, x, nym ? y - 1 : y, wz, nym ? c - sy : c);
-    Sten<T> sb = sten_load<T>(L, zin, r, x, y, nzp ? wz + 1 : (nzm ? wz - 1 : wz), nzp ? c + 1 : (nzm ? c - 1 : c));
-    sten_pin(own);
-    sten_pin(s0);
-    sten_pin(s1);
-    sten_pin(s2);
-    sten_pin(s3);
-    sten_pin(sb);
-    // first colour: own update; second colour: the neighbours' updates
-    const T v_own = sten_relax<MODE>(own, sten_ax(own, own.zc, own.zxp, own.zxm, own.zyp, own.zym, own.zzp,
-                                                  own.zzm), own.zc, w);
-    const T v0 = sten_relax<MODE>(s0, sten_ax(s0, s0.zc, s0.zxp, s0.zxm, s0.zyp, s0.zym, s0.zzp, s0.zzm), s0.zc, w);
-    const T v1 = sten_relax<MODE>(s1, sten_ax(s1, s1.zc, s1.zxp, s1.zxm, s1.zyp, s1.zym, s1.zzp, s1.zzm), s1.zc, w);
-    const T v2 = sten_relax<MODE>(s2, sten_ax(s2, s2.zc, s2.zxp, s2.zxm, s2.zyp, s2.zym, s2.zzp, s2.zzm), s2.zc, w);
-    const T v3 = sten_relax<MODE>(s3, sten_ax(s3, s3.zc, s3.zxp, s3.zxm, s3.zyp, s3.zym, s3.zzp, s3.zzm), s3.zc, w);
-    const T vb = sten_relax<MODE>(sb, sten_ax(sb, sb.zc, sb.zxp, sb.zxm, sb.zyp, sb.zym, sb.zzp, sb.zzm), sb.zc, w);
-    // the segment after the first colour: first-colour lanes hold their update, the others zin
-    const T after1 = first_colour ? v_own : own.zc;
-    // the shuffles run with every lane of the wave active (a lane outside a shuffle supplies
-    // no value to the others); segment-end lanes then take their recomputed neighbour
-    const T sl = __shfl(after1, lane - 1 < 0 ? 0 : lane - 1);
-    const T sr = __shfl(after1, lane + 1 > 63 ? 63 : lane + 1);
-    const T zl = lane == 0 ? vb : sl;
-    const T zr = lane == 63 ? vb : sr;
-    T out = after1;
-    if (!first_colour) out = sten_relax<MODE>(own, sten_ax(own, own.zc, v0, v1, v2, v3, zr, zl), own.zc, w);
-    if (zr0 < d2) zout[c] = out;
+    Sten<T> own;
+    T after1 = 0, zl = 0, zr = 0;
+    if (wv > 0) {
+      // wave a + 1: the first-colour update of the neighbour in row x + 1, x - 1, y + 1 or y - 1
+      // (second-colour lanes whose neighbour exists; the others load their own cell, unused)
+      const int a = wv - 1;
+      const int nx = a == 0 ? x + 1 : (a == 1 ? x - 1 : x), ny = a == 2 ? y + 1 : (a == 3 ? y - 1 : y);
+      const bool has = !first_colour && nx >= 0 && nx < d0 && ny >= 0 && ny < d1;
+      const size_t cn = has ? (a == 0 ? c + sx : (a == 1 ? c - sx : (a == 2 ? c + sy : c - sy))) : c;
+      Sten<T> sn = sten_load<T>(L, zin, r, has ? nx : x, has ? ny : y, wz, cn);
+      sten_pin(sn);
+      s_nb[a][lane] = sten_relax<MODE>(sn, sten_ax(sn, sn.zc, sn.zxp, sn.zxm, sn.zyp, sn.zym, sn.zzp, sn.zzm), sn.zc, w);
+    } else {
+      own = sten_load<T>(L, zin, r, x, y, wz, c);
+      sten_pin(own);
+      // the z neighbour in the next / previous segment (second-colour lanes at a segment end; with
+      // one segment per row no lane reads it)
+      T vb = own.zc;
+      if (nseg > 1) {
+        const bool nzp = !first_colour && lane == 63 && wz + 1 < d2;
+        const bool nzm = !first_colour && lane == 0 && wz >= 1;
+        Sten<T> sb = sten_load<T>(L, zin, r, x, y, nzp ? wz + 1 : (nzm ? wz - 1 : wz), nzp ? c + 1 : (nzm ? c - 1 : c));
+        sten_pin(sb);
+        vb = sten_relax<MODE>(sb, sten_ax(sb, sb.zc, sb.zxp, sb.zxm, sb.zyp, sb.zym, sb.zzp, sb.zzm), sb.zc, w);
+      }
+      // first colour: own update
+      const T v_own = sten_relax<MODE>(own, sten_ax(own, own.zc, own.zxp, own.zxm, own.zyp, own.zym, own.zzp,
+                                                    own.zzm), own.zc, w);
+      // the segment after the first colour: first-colour lanes hold their update, the others zin
+      after1 = first_colour ? v_own : own.zc;
+      // the shuffles run with every lane of the wave active (a lane outside a shuffle supplies
+      // no value to the others); segment-end lanes then take their recomputed neighbour
+      const T sl = __shfl(after1, lane - 1 < 0 ? 0 : lane - 1);
+      const T sr = __shfl(after1, lane + 1 > 63 ? 63 : lane + 1);
+      zl = lane == 0 ? vb : sl;
+      zr = lane == 63 ? vb : sr;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      T out = after1;
+      if (!first_colour)
+        out = sten_relax<MODE>(own, sten_ax(own, own.zc, s_nb[0][lane], s_nb[1][lane], s_nb[2][lane], s_nb[3][lane],
+                                            zr, zl), own.zc, w);
+      if (zr0 < d2) zout[c] = out;
+    }
+    __syncthreads();  // s_nb is rewritten by the next segment
   }
 }
 
@@ -1335,22 +1351,27 @@ unsigned row_blocks(const Level& L) {
   return (unsigned)std::max<size_t>(1, std::min<size_t>(need, 8192));
 }
 
+// blocks for k_sweep_rows: one per row segment of the level (the list's capacity; blocks past the
+// list's length exit at once), striding beyond 65536
+unsigned row_sweep_blocks(const Level& L) {
+  const size_t nseg = div_up((size_t)L.d[2], (size_t)64);
+  return (unsigned)std::max<size_t>(1, std::min<size_t>((size_t)L.d[0] * L.d[1] * nseg, 65536));
+}
+
 // one red-black sweep on level l: Smooth(l, first) then Smooth(l, 1 - first).
 // Large levels use the fused tile kernel; small ones (launch-bound) two colour passes.
 void sweep(Ctx& c, int l, int first, int mode) {
   Level& L = c.lv[l];
   if (l > 0 && c.cfg.coarse_sweep == kCoarseRows && (long long)L.n <= c.cfg.row_sweep_max_cells) {
-    const size_t nseg = div_up((size_t)L.d[2], (size_t)64);
-    const unsigned nb = (unsigned)std::max<size_t>(
-        1, std::min<size_t>(div_up((size_t)L.d[0] * L.d[1] * nseg, (size_t)(kBlock / 64)), 8192));
+    const unsigned nb = row_sweep_blocks(L);
     // over the z-support rows (liquid rows plus the rows the coarser correction reaches): the
     // output buffer must hold every value a later prolongation reads
     if (mode == 0)
-      hipLaunchKernelGGL(k_sweep_rows<0>, dim3(nb), dim3(kBlock), 0, c.s, L.z2, L.z, L.r, lvp(L), L.zrows, L.nzrows,
-                         first, c.cfg.omega);
+      hipLaunchKernelGGL(k_sweep_rows<0>, dim3(nb), dim3(64 * kSweepWaves), 0, c.s, L.z2, L.z, L.r, lvp(L), L.zrows,
+                         L.nzrows, first, c.cfg.omega);
     else
-      hipLaunchKernelGGL(k_sweep_rows<1>, dim3(nb), dim3(kBlock), 0, c.s, L.z2, L.z, L.r, lvp(L), L.zrows, L.nzrows,
-                         first, c.cfg.omega);
+      hipLaunchKernelGGL(k_sweep_rows<1>, dim3(nb), dim3(64 * kSweepWaves), 0, c.s, L.z2, L.z, L.r, lvp(L), L.zrows,
+                         L.nzrows, first, c.cfg.omega);
     std::swap(L.z, L.z2);
     return;
   }
@@ -1408,19 +1429,17 @@ void sweeps(Ctx& c, int l, int first, int fwd, int rev, int mode) {
 void coarsest_sweeps64(Ctx& c, int fwd, int rev, int mode) {
   const int l = (int)c.lv.size() - 1;
   Level& L = c.lv[l];
-  const size_t nseg = div_up((size_t)L.d[2], (size_t)64);
-  const unsigned nb = (unsigned)std::max<size_t>(
-      1, std::min<size_t>(div_up((size_t)L.d[0] * L.d[1] * nseg, (size_t)(kBlock / 64)), 8192));
+  const unsigned nb = row_sweep_blocks(L);
   {
     ProfScope ps(c, l == 1 ? LH_PROF_SMOOTH_L1 : LH_PROF_SMOOTH_L2, fwd + rev);
     for (int it = 0; it < fwd + rev; ++it) {
       const int first = it < fwd ? 0 : 1;
       if (mode == 0)
-        hipLaunchKernelGGL((k_sweep_rows<0, double>), dim3(nb), dim3(kBlock), 0, c.s, L.z64b, L.z64, L.r, lvp(L),
-                           L.zrows, L.nzrows, first, c.cfg.omega);
+        hipLaunchKernelGGL((k_sweep_rows<0, double>), dim3(nb), dim3(64 * kSweepWaves), 0, c.s, L.z64b, L.z64, L.r,
+                           lvp(L), L.zrows, L.nzrows, first, c.cfg.omega);
       else
-        hipLaunchKernelGGL((k_sweep_rows<1, double>), dim3(nb), dim3(kBlock), 0, c.s, L.z64b, L.z64, L.r, lvp(L),
-                           L.zrows, L.nzrows, first, c.cfg.omega);
+        hipLaunchKernelGGL((k_sweep_rows<1, double>), dim3(nb), dim3(64 * kSweepWaves), 0, c.s, L.z64b, L.z64, L.r,
+                           lvp(L), L.zrows, L.nzrows, first, c.cfg.omega);
       std::swap(L.z64, L.z64b);
     }
   }
