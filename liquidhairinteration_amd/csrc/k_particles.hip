@@ -76,30 +76,43 @@ __global__ void k_bin_scatter(const int* __restrict__ key, int64_t P, const int*
 }
 
 // place particle i at start[key] + (number of particles in its bin with a smaller id)
-__global__ void k_bin_permute(const float* __restrict__ src, float* __restrict__ dst, int64_t cap, int64_t P,
-                              const int* __restrict__ key, const int* __restrict__ start,
-                              const int* __restrict__ slot_owner, float4* __restrict__ pack) {
+__global__ void k_bin_rank(const float* __restrict__ src, int64_t cap, int64_t P, const int* __restrict__ key,
+                           const int* __restrict__ start, const int* __restrict__ slot_owner, int* __restrict__ order) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
-    int k = key[i];
-    int s0 = start[k], s1 = start[k + 1];
-    int myid = __float_as_int(src[(size_t)PA_ID * cap + i]);
+    const int k = key[i];
+    const int s0 = start[k], s1 = start[k + 1];
     int rank = 0;
-    for (int s = s0; s < s1; ++s) {
-      int j = slot_owner[s];
-      int idj = __float_as_int(src[(size_t)PA_ID * cap + j]);
-      rank += (idj < myid) ? 1 : 0;
+    if (s1 - s0 > 1) {  // a bin of one particle needs no comparison
+      const int myid = __float_as_int(src[(size_t)PA_ID * cap + i]);
+      for (int s = s0; s < s1; ++s) {
+        const int j = slot_owner[s];
+        const int idj = __float_as_int(src[(size_t)PA_ID * cap + j]);
+        rank += (idj < myid) ? 1 : 0;
+      }
     }
-    size_t d = (size_t)(s0 + rank);
+    order[s0 + rank] = (int)i;
+  }
+}
+
+// the particle buffer in bin order: destination d takes source order[d] (coalesced writes of the
+// 16 attributes and of the P2G records: per axis a, (x, y, z, v_a) and (row a of C, 0))
+__global__ void k_bin_gather(const float* __restrict__ src, float* __restrict__ dst, int64_t cap, int64_t P,
+                             const int* __restrict__ order, float4* __restrict__ pack) {
+  for (int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; d < P; d += (int64_t)gridDim.x * blockDim.x) {
+    // particle ids are 0..P-1, so the ranks of a bin are distinct and every slot is written; the
+    // guard only keeps a corrupt order inside the buffer
+    size_t i = (size_t)(unsigned)order[d];
+    if (i >= (size_t)P) i = (size_t)d;
+    float v[kParticleAttrs];
 #pragma unroll
-    for (int a = 0; a < kParticleAttrs; ++a) dst[(size_t)a * cap + d] = src[(size_t)a * cap + i];
-    const float x = src[(size_t)PA_X * cap + i], y = src[(size_t)(PA_X + 1) * cap + i],
-                z = src[(size_t)(PA_X + 2) * cap + i];
+    for (int a = 0; a < kParticleAttrs; ++a) v[a] = src[(size_t)a * cap + i];
+#pragma unroll
+    for (int a = 0; a < kParticleAttrs; ++a) dst[(size_t)a * cap + d] = v[a];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       float4* r = pack + 2 * ((size_t)a * cap + d);
-      r[0] = make_float4(x, y, z, src[(size_t)(PA_VX + a) * cap + i]);
-      r[1] = make_float4(src[(size_t)(PA_C + 3 * a) * cap + i], src[(size_t)(PA_C + 3 * a + 1) * cap + i],
-                         src[(size_t)(PA_C + 3 * a + 2) * cap + i], 0.f);
+      r[0] = make_float4(v[PA_X], v[PA_X + 1], v[PA_X + 2], v[PA_VX + a]);
+      r[1] = make_float4(v[PA_C + 3 * a], v[PA_C + 3 * a + 1], v[PA_C + 3 * a + 2], 0.f);
     }
   }
 }
@@ -962,8 +975,10 @@ int run_binning(Ctx& c) {
   int rc = bin_scan(c);
   if (rc) return rc;
   hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, c.s, c.pkey, c.P, c.bin_start, c.bin_count, c.pslot);
-  hipLaunchKernelGGL(k_bin_permute, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.pbuf[1 - c.cur], c.cap, c.P,
-                     c.pkey, c.bin_start, c.pslot, (float4*)c.ppack);
+  hipLaunchKernelGGL(k_bin_rank, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, c.pkey, c.bin_start,
+                     c.pslot, c.porder);
+  hipLaunchKernelGGL(k_bin_gather, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.pbuf[1 - c.cur], c.cap, c.P,
+                     c.porder, (float4*)c.ppack);
   c.cur = 1 - c.cur;
   c.bins_valid = true;
   return LH_OK;

@@ -302,7 +302,7 @@ void free_ctx(Ctx& c) {
     F(c.lw[a]);
   }
   F(c.pressure); F(c.phi); F(c.rhs); F(c.solid_phi); F(c.is_valid); F(c.xstate); F(c.d_sdf);
-  F(c.bin_count); F(c.bin_start); F(c.bin_bflag); F(c.bin_bsum); F(c.bin_boff); F(c.scan_tmp); F(c.pkey); F(c.pslot);
+  F(c.bin_count); F(c.bin_start); F(c.bin_bflag); F(c.bin_bsum); F(c.bin_boff); F(c.scan_tmp); F(c.pkey); F(c.pslot); F(c.porder);
   F(c.pbuf[0]); F(c.pbuf[1]); F(c.ppack); F(c.cv_flag);
   for (auto& q : c.lv) {
     F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2); F(q.z64); F(q.z64b);
@@ -330,13 +330,15 @@ int ensure_particle_capacity(Ctx& c, int64_t n) {
   if (c.pbuf[1]) hipFree(c.pbuf[1]);
   if (c.pkey) hipFree(c.pkey);
   if (c.pslot) hipFree(c.pslot);
+  if (c.porder) hipFree(c.porder);
   if (c.ppack) hipFree(c.ppack);
   c.pbuf[0] = c.pbuf[1] = nullptr;
-  c.pkey = c.pslot = nullptr;
+  c.pkey = c.pslot = c.porder = nullptr;
   c.ppack = nullptr;
   int64_t cap = std::max<int64_t>(n, 1);
   bool ok = dalloc(c, c.pbuf[0], (size_t)kParticleAttrs * cap) && dalloc(c, c.pbuf[1], (size_t)kParticleAttrs * cap) &&
-            dalloc(c, c.pkey, cap) && dalloc(c, c.pslot, cap) && dalloc(c, c.ppack, (size_t)24 * cap);
+            dalloc(c, c.pkey, cap) && dalloc(c, c.pslot, cap) && dalloc(c, c.porder, cap) &&
+            dalloc(c, c.ppack, (size_t)24 * cap);
   if (!ok) {
     c.cap = 0;
     return LH_ERR_OOM;

@@ -134,6 +134,7 @@ struct Ctx {
   int* bin_start = nullptr;  // nbins + 1
   int* pkey = nullptr;       // per particle bin key
   int* pslot = nullptr;      // per particle provisional slot
+  int* porder = nullptr;     // per bin-order slot, the particle (source index) that takes it
   // the bin scan runs over blocks of kScanBins bins: a flag per block (set by the key kernel
   // for every block that holds a particle, cleared by the scan), the block sums and offsets
   int* bin_bflag = nullptr;
