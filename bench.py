@@ -56,15 +56,33 @@ def coarsest_fp64(cfg):
     return s.get("coarsestPrecision", "fp64" if cfg.get("precision") == "refined" else "fp32") == "fp64"
 
 
+def coarsest_chebyshev(cfg):
+    """Chebyshev steps of the coarsest level's solve (SolverParameter.coarsestSolver "chebyshev",
+    coarsestSteps; 0: the reference's numFinalSmooth sweeps)"""
+    s = cfg["SolverParameter"]
+    return int(s.get("coarsestSteps", 12)) if s.get("coarsestSolver", "sweeps") == "chebyshev" else 0
+
+
+def is_coarsest(cfg, l):
+    return l == cfg["SolverParameter"]["numLevel"] - 1 and l > 0
+
+
 def sweep_bytes(cfg, l):
     """algorithmic bytes per cell of one red-black sweep on level l: adiag + 3 face terms 16 B, r 4 B,
-    z read + written 8 B (16 B when the coarsest level's z is FP64)"""
-    return 36.0 if l == cfg["SolverParameter"]["numLevel"] - 1 and l > 0 and coarsest_fp64(cfg) else 28.0
+    z read + written 8 B (16 B when the coarsest level's z is FP64); for the coarsest level's
+    Chebyshev launch (two steps): z and d read and written in FP64 32 B, r 4 B, adiag + 3 face
+    terms 16 B = 52 B"""
+    if is_coarsest(cfg, l) and coarsest_chebyshev(cfg):
+        return 52.0
+    return 36.0 if is_coarsest(cfg, l) and coarsest_fp64(cfg) else 28.0
 
 
 def sweeps_per_launch(cfg, l):
     """red-black sweeps per smoother launch on coarse level l: one for k_sweep_rows and k_tile_sweep,
-    half of one for a k_smooth colour pass"""
+    half of one for a k_smooth colour pass (1 for the coarsest level's Chebyshev launches, whose
+    bytes sweep_bytes counts per launch)"""
+    if is_coarsest(cfg, l) and coarsest_chebyshev(cfg):
+        return 1.0
     return 0.5 if coarse_form(cfg, l) == "passes" else 1.0
 
 
@@ -78,7 +96,9 @@ def algorithmic_bytes(kernel, n0, P, active, fused, per_launch=None, sweep_b=Non
     smooth_l1, smooth_l2: the same per cell of the rows holding liquid on level 1, and on
                level 2 (the coarsest of the three-level hierarchy): 28 B per sweep, times the
                sweeps of one launch (per_launch[l]: 1/2 for a colour pass, 1 for k_sweep_rows or
-               k_tile_sweep), 36 B per sweep when the coarsest level's z is FP64 (sweep_b[l]);
+               k_tile_sweep), 36 B per sweep when the coarsest level's z is FP64, 52 B per
+               launch of the coarsest level's two Chebyshev steps (sweep_b[l]); with more than
+               three levels smooth_l2 times every level from 2 down, its bytes count level 2's;
     spmv:      the p update fused with q = A p (k_cg_p_spmv, all but the first SpMV of a solve):
                z and p read, new p and q written, 4 coefficient floats -> 32 B per cell of
                those rows (the plain first SpMV moves 24 B);
@@ -116,9 +136,10 @@ def smooth_grid(d):
 
 
 def row_sweep_grid(d):
-    """work-items of one k_sweep_rows launch on a level of dims d (a wave per 64-cell row segment)"""
+    """work-items of one k_sweep_rows / k_cheb_rows launch on a level of dims d (a block of five waves
+    per 64-cell row segment, capped at 65536 blocks)"""
     nseg = -(-d[2] // 64)
-    return max(1, min(-(-d[0] * d[1] * nseg // 4), 8192)) * 256
+    return max(1, min(d[0] * d[1] * nseg, 65536)) * 320
 
 
 def tile_sweep_grid(d):
@@ -131,13 +152,25 @@ def level_symbol(cfg, l):
     d = level_dims(cfg)[l]
     mode = 1 if cfg["SolverParameter"].get("smoother", "reference") == "symmetric" else 0
     form = coarse_form(cfg, l)
+    if is_coarsest(cfg, l) and coarsest_chebyshev(cfg):
+        return "k_cheb_rows<double>", row_sweep_grid(d)
     if form == "rows":
-        if l == cfg["SolverParameter"]["numLevel"] - 1 and coarsest_fp64(cfg):
+        if is_coarsest(cfg, l) and coarsest_fp64(cfg):
             return "k_sweep_rows<%d, double>" % mode, row_sweep_grid(d)
         return "k_sweep_rows<%d>" % mode, row_sweep_grid(d)
     if form == "tiles":
         return "k_tile_sweep<%d>" % mode, tile_sweep_grid(d)
     return "k_smooth<false>", smooth_grid(d)
+
+
+def coarsest_label(cfg):
+    """the coarsest level's solve: its sweep count, or its Chebyshev steps and interval"""
+    s = cfg["SolverParameter"]
+    k = coarsest_chebyshev(cfg)
+    prec = "FP64" if coarsest_fp64(cfg) else "FP32"
+    if k:
+        return "chebyshev %d on [%g, 2] %s" % (k, float(s.get("chebyshevLower", 0.01)), prec)
+    return "%d sweeps %s" % (int(s["numFinalSmooth"]), prec)
 
 
 COPY_BYTES = 256 * 1024 * 1024  # the calibration copy: bytes read and bytes written per launch
@@ -492,9 +525,9 @@ def main():
         "data": "synthetic (seeded jittered particle blocks)",
         "config": {"workload": workload,
                    "grid": list(scene.GetGridDimension()), "particles": P, "active_rows_per_level": rows,
-                   "solver": "MGPCG L=%d pre/post=%d/%d coarsest=%d %s, tol %g %s" % (
+                   "solver": "MGPCG L=%d pre/post=%d/%d coarsest=%s %s, tol %g %s" % (
                        cfg["SolverParameter"]["numLevel"], cfg["SolverParameter"]["numPreSmooth"],
-                       cfg["SolverParameter"]["numPostSmooth"], cfg["SolverParameter"]["numFinalSmooth"],
+                       cfg["SolverParameter"]["numPostSmooth"], coarsest_label(cfg),
                        cfg["SolverParameter"].get("smoother", "reference"),
                        cfg["SolverParameter"]["tolerance"], cfg["SolverParameter"].get("toleranceMode", "absolute")),
                    "precision": cfg.get("precision", "fp32") + (

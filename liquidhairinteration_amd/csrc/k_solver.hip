@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "lh_ctx.h"
 #include "lh_nn.h"
@@ -740,6 +741,121 @@ __global__ __launch_bounds__(64 * kSweepWaves) void k_sweep_rows(T* __restrict__
   }
 }
 
+// One Chebyshev step of the coarsest level's solve at a cell (the oracle's coarsest_chebyshev):
+// d = a d + b (r - A z) / ad (the solve's first step: d = b (r - A z) / ad), z + d; a cell with a
+// negligible diagonal keeps z (its d, read by no one, is 0)
+template <class T>
+__device__ __forceinline__ void cheb_cell(const Sten<T>& s, T ax, T zc, T dprev, double a, double b, bool first,
+                                          T& zn, T& dn) {
+  if (is_small(s.ad)) {
+    zn = zc;
+    dn = (T)0;
+    return;
+  }
+  const T t = ((T)s.rv - ax) / (T)s.ad;
+  dn = first ? (T)b * t : (T)a * dprev + (T)b * t;
+  zn = zc + dn;
+}
+
+struct ChebSteps {
+  double a0, b0, a1, b1;  // coefficients of the launch's first and second step
+  int first;              // the first step is the solve's step 0 (no previous direction)
+  int two;                // two steps in this launch
+};
+
+// Chebyshev steps on the coarsest level's rows (its z-support), one or two per launch, with the
+// layout of k_sweep_rows: a block of five waves per 64-cell row segment.  A step updates every
+// cell from the previous iterate, so the second step of a cell needs the first step's values at
+// its six neighbours: the two in its row come from wave 0's neighbouring lanes (at a segment end
+// recomputed by wave 0), the four in the rows x +- 1, y +- 1 are recomputed by waves 1-4 from
+// zin / din and handed over through LDS.  zin / din are read and zout / dout written, so no block
+// sees another's writes, and every value is that of two separate steps, operation for operation.
+template <class T>
+__global__ __launch_bounds__(64 * kSweepWaves) void k_cheb_rows(T* __restrict__ zout, T* __restrict__ dout,
+                                                                 const T* __restrict__ zin, const T* __restrict__ din,
+                                                                 const float* __restrict__ r, LvP L,
+                                                                 const int* __restrict__ rows,
+                                                                 const int* __restrict__ nrows, ChebSteps q) {
+  __shared__ T s_nb[4][64];  // first-step z of the rows x + 1, x - 1, y + 1, y - 1
+  const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
+  const int nseg = (d2 + 63) >> 6;
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+  const int nr = *nrows;
+  const size_t sx = (size_t)d1 * d2, sy = (size_t)d2;
+  const bool first = q.first != 0, two = q.two != 0;
+  // XCD q takes the q-th eighth of the list's segments (as k_sweep_rows)
+  const int nwork = nr * nseg, chunk = (nwork + 7) >> 3;
+  const int xq = (int)blockIdx.x & 7, per_xcd = ((int)gridDim.x - xq + 7) >> 3;
+  for (int j = (int)blockIdx.x >> 3; j < chunk; j += per_xcd) {
+    const int k = xq * chunk + j;  // (list index, segment) = (k / nseg, k % nseg)
+    if (k >= nwork) break;
+    const int row = rows[k / nseg];
+    const int seg = k - (k / nseg) * nseg;
+    const int x = row / d1, y = row - x * d1;
+    const int zr0 = seg * 64 + lane;
+    const int wz = min(zr0, d2 - 1);
+    const size_t c = (size_t)row * d2 + wz;
+    Sten<T> own;
+    T zA = 0, dA = 0, zl = 0, zr = 0;
+    if (wv > 0) {
+      if (two) {
+        // wave a + 1: the first step at the neighbour in row x + 1, x - 1, y + 1 or y - 1 (lanes
+        // whose neighbour lies outside the grid load their own cell, unused)
+        const int a = wv - 1;
+        const int nx = a == 0 ? x + 1 : (a == 1 ? x - 1 : x), ny = a == 2 ? y + 1 : (a == 3 ? y - 1 : y);
+        const bool has = nx >= 0 && nx < d0 && ny >= 0 && ny < d1;
+        const size_t cn = has ? (a == 0 ? c + sx : (a == 1 ? c - sx : (a == 2 ? c + sy : c - sy))) : c;
+        Sten<T> sn = sten_load<T>(L, zin, r, has ? nx : x, has ? ny : y, wz, cn);
+        const T dp = first ? (T)0 : din[cn];
+        sten_pin(sn);
+        T zn, dn;
+        cheb_cell<T>(sn, sten_ax(sn, sn.zc, sn.zxp, sn.zxm, sn.zyp, sn.zym, sn.zzp, sn.zzm), sn.zc, dp, q.a0, q.b0,
+                     first, zn, dn);
+        s_nb[a][lane] = zn;
+      }
+    } else {
+      own = sten_load<T>(L, zin, r, x, y, wz, c);
+      const T dp = first ? (T)0 : din[c];
+      sten_pin(own);
+      cheb_cell<T>(own, sten_ax(own, own.zc, own.zxp, own.zxm, own.zyp, own.zym, own.zzp, own.zzm), own.zc, dp,
+                   q.a0, q.b0, first, zA, dA);
+      if (two) {
+        // the z neighbour in the next / previous segment (lane 63 / lane 0; with one segment per
+        // row no lane reads it)
+        T vb = zA;
+        if (nseg > 1) {
+          const bool nzp = lane == 63 && wz + 1 < d2, nzm = lane == 0 && wz >= 1;
+          const int wb = nzp ? wz + 1 : (nzm ? wz - 1 : wz);
+          const size_t cb = nzp ? c + 1 : (nzm ? c - 1 : c);
+          Sten<T> sb = sten_load<T>(L, zin, r, x, y, wb, cb);
+          const T db = first ? (T)0 : din[cb];
+          sten_pin(sb);
+          T dn;
+          cheb_cell<T>(sb, sten_ax(sb, sb.zc, sb.zxp, sb.zxm, sb.zyp, sb.zym, sb.zzp, sb.zzm), sb.zc, db, q.a0, q.b0,
+                       first, vb, dn);
+        }
+        // the shuffles run with every lane of the wave active
+        const T sl = __shfl(zA, lane - 1 < 0 ? 0 : lane - 1);
+        const T sr = __shfl(zA, lane + 1 > 63 ? 63 : lane + 1);
+        zl = lane == 0 ? vb : sl;
+        zr = lane == 63 ? vb : sr;
+      }
+    }
+    __syncthreads();
+    if (wv == 0) {
+      T zo = zA, dO = dA;
+      if (two)
+        cheb_cell<T>(own, sten_ax(own, zA, s_nb[0][lane], s_nb[1][lane], s_nb[2][lane], s_nb[3][lane], zr, zl), zA, dA,
+                     q.a1, q.b1, false, zo, dO);
+      if (zr0 < d2) {
+        zout[c] = zo;
+        dout[c] = dO;
+      }
+    }
+    __syncthreads();  // s_nb is rewritten by the next segment
+  }
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(256) k_rbgs_sweep(float* __restrict__ zout, const float* __restrict__ zin,
                                                      const float* __restrict__ r, LvP L, int first, float w) {
@@ -1446,6 +1562,50 @@ void coarsest_sweeps64(Ctx& c, int fwd, int rev, int mode) {
   hipLaunchKernelGGL(k_z64_to_f32, dim3(row_blocks(L)), dim3(kBlock), 0, c.s, L.z, L.z64, L.zrows, L.nzrows, L.d[2]);
 }
 
+// The coarsest level solved by coarsest_cheb Chebyshev steps on its Jacobi-scaled system over the
+// interval [cheb_lo, 2] (in FP64 on z64 / d64, from z64 = 0 as the restriction leaves it), two
+// steps per launch, then z = (float) z64 on the z-support rows for the prolongation.  The step
+// coefficients follow the oracle's recurrence in the same double operations: theta = (2 + lo) / 2,
+// delta = (2 - lo) / 2, sigma = theta / delta, rho_0 = 1 / sigma; step 0: b = 1 / theta; step k:
+// rho = 1 / (2 sigma - rho_prev), a = rho rho_prev, b = 2 rho / delta.
+void coarsest_chebyshev64(Ctx& c) {
+  const int l = (int)c.lv.size() - 1;
+  Level& L = c.lv[l];
+  const int steps = c.cfg.coarsest_cheb;
+  const double lo = (double)c.cfg.cheb_lo, hi = 2.0;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+  double rho_prev = 1.0 / sigma;
+  std::vector<double> A(steps), B(steps);
+  for (int k = 0; k < steps; ++k) {
+    A[k] = 0.0;
+    B[k] = 1.0 / theta;
+    if (k > 0) {
+      const double rho = 1.0 / (2.0 * sigma - rho_prev);
+      A[k] = rho * rho_prev;
+      B[k] = 2.0 * rho / delta;
+      rho_prev = rho;
+    }
+  }
+  const unsigned nb = row_sweep_blocks(L);
+  {
+    ProfScope ps(c, l == 1 ? LH_PROF_SMOOTH_L1 : LH_PROF_SMOOTH_L2, (steps + 1) / 2);
+    for (int k = 0; k < steps; k += 2) {
+      ChebSteps q;
+      q.two = k + 1 < steps ? 1 : 0;
+      q.first = k == 0 ? 1 : 0;
+      q.a0 = A[k];
+      q.b0 = B[k];
+      q.a1 = q.two ? A[k + 1] : 0.0;
+      q.b1 = q.two ? B[k + 1] : 0.0;
+      hipLaunchKernelGGL(k_cheb_rows<double>, dim3(nb), dim3(64 * kSweepWaves), 0, c.s, L.z64b, L.d64b, L.z64, L.d64,
+                         L.r, lvp(L), L.zrows, L.nzrows, q);
+      std::swap(L.z64, L.z64b);
+      std::swap(L.d64, L.d64b);
+    }
+  }
+  hipLaunchKernelGGL(k_z64_to_f32, dim3(row_blocks(L)), dim3(kBlock), 0, c.s, L.z, L.z64, L.zrows, L.nzrows, L.d[2]);
+}
+
 // ApplyPreconditioner, PressureSolver.cpp:719-772 (symmetric mode: z[0] = 0 first,
 // post-smoothing and the second half of the coarsest sweeps in reverse colour order)
 void apply_precond(Ctx& c, bool z_zeroed) {
@@ -1471,7 +1631,9 @@ void apply_precond(Ctx& c, bool z_zeroed) {
     hipLaunchKernelGGL(k_restrict, dim3(nb), dim3(kBlock), 0, c.s, Lc.r, Lc.z, Lf.r, Lf.z, lvp(Lf), Lc.d[1], Lc.d[2],
                        Lc.qrows, Lc.nqrows, Lc.d[0] * Lc.d[1], Lc.z64);
   }
-  if (c.lv[L - 1].z64)
+  if (c.cfg.coarsest_cheb > 0)
+    coarsest_chebyshev64(c);
+  else if (c.lv[L - 1].z64)
     coarsest_sweeps64(c, c.cfg.num_final, sym ? c.cfg.num_final : 0, mode);
   else
     sweeps(c, L - 1, 0, c.cfg.num_final, sym ? c.cfg.num_final : 0, mode);

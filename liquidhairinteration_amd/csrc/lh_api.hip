@@ -147,6 +147,18 @@ Config parse_config(const std::string& text) {
   std::string cp = s.get("coarsestPrecision", c.refined ? "fp64" : "fp32");
   if (cp == "fp64") c.coarsest_fp64 = 1;
   else if (cp != "fp32") throw std::runtime_error("Unknown coarsestPrecision " + cp);
+  std::string csol = s.get("coarsestSolver", "sweeps");
+  if (csol == "chebyshev") {
+    const double steps = s.get("coarsestSteps", 12.0), lo = s.get("chebyshevLower", 0.01);
+    if (!(steps >= 1 && steps <= 1000) || steps != (double)(int)steps)
+      throw std::runtime_error("coarsestSteps must be an integer in [1, 1000]");
+    if (!(lo > 0.0 && lo < 2.0)) throw std::runtime_error("chebyshevLower must lie in (0, 2)");
+    if (!c.coarsest_fp64) throw std::runtime_error("coarsestSolver chebyshev needs coarsestPrecision fp64");
+    c.coarsest_cheb = (int)steps;
+    c.cheb_lo = (float)lo;
+  } else if (csol != "sweeps") {
+    throw std::runtime_error("Unknown coarsestSolver " + csol);
+  }
   c.first_dt = (float)j.get("firstDt", 0.01);
   c.correct_volume = (j.has("correctVolume") && j.at("correctVolume").as_bool()) ? 1 : 0;
   c.steps = (int)j.get("steps", 0.0);
@@ -242,9 +254,14 @@ int alloc_ctx(Ctx& c) {
       return LH_ERR_CONFIG;
     }
     ok = ok && dalloc(c, q.z64, q.n) && dalloc(c, q.z64b, q.n);
+    if (c.cfg.coarsest_cheb > 0) ok = ok && dalloc(c, q.d64, q.n) && dalloc(c, q.d64b, q.n);
     if (ok) {
       hipMemsetAsync(q.z64, 0, q.n * sizeof(double), c.s);
       hipMemsetAsync(q.z64b, 0, q.n * sizeof(double), c.s);
+      if (q.d64) {
+        hipMemsetAsync(q.d64, 0, q.n * sizeof(double), c.s);
+        hipMemsetAsync(q.d64b, 0, q.n * sizeof(double), c.s);
+      }
     }
   }
   c.npartial = 1024;  // also the PCG vector kernels' grid: a multiple of 8 (per-XCD row runs)
@@ -305,7 +322,7 @@ void free_ctx(Ctx& c) {
   F(c.bin_count); F(c.bin_start); F(c.bin_bflag); F(c.bin_bsum); F(c.bin_boff); F(c.scan_tmp); F(c.pkey); F(c.pslot); F(c.porder);
   F(c.pbuf[0]); F(c.pbuf[1]); F(c.ppack); F(c.cv_flag);
   for (auto& q : c.lv) {
-    F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2); F(q.z64); F(q.z64b);
+    F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2); F(q.z64); F(q.z64b); F(q.d64); F(q.d64b);
     F(q.rows); F(q.nrows);
     F(q.aflag); F(q.qrows); F(q.nqrows); F(q.qflag); F(q.zrows); F(q.nzrows); F(q.zflag); F(q.tflag); F(q.tiles); F(q.ntiles);
   }

@@ -46,6 +46,10 @@ struct Config {
   // the multigrid coarsest level's red-black sweeps in FP64 (SolverParameter.coarsestPrecision;
   // default: on with refined precision): r and z promoted, z rounded back for the prolongation
   int coarsest_fp64 = 0;
+  // coarsestSolver "chebyshev": the coarsest level solved by coarsest_cheb Chebyshev steps on
+  // [cheb_lo, 2] in FP64 instead of numFinalSmooth sweeps (0: sweeps)
+  int coarsest_cheb = 0;
+  float cheb_lo = 0.01f;
   double refined_tol = 1e-9;
   int max_outer = 50;
   // boundary SDF program (postfix, 12 floats per node)
@@ -81,6 +85,8 @@ struct Level {
   float* z2 = nullptr;  // ping-pong partner of z for the fused red-black sweep
   double* z64 = nullptr;   // the coarsest level's z in FP64 (coarsest_fp64) ...
   double* z64b = nullptr;  // ... and its ping-pong partner
+  double* d64 = nullptr;   // the coarsest level's Chebyshev direction (coarsest_cheb) ...
+  double* d64b = nullptr;  // ... and its ping-pong partner
   // z-rows (x, y) holding at least one liquid cell (cls != 0), ascending; rebuilt with the operator
   int* rows = nullptr;   // capacity d[0] * d[1]
   int* nrows = nullptr;  // device count

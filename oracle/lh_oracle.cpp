@@ -67,6 +67,8 @@ struct orc_params {
   unsigned long long seed;
   long long num_particles;  // LiquidParameter.numParticles (the 2D lattice spawn's cap)
   int coarsest_fp64;        // 1: the multigrid coarsest level's sweeps in FP64 (refined mode's default)
+  int coarsest_cheb;        // > 0: the coarsest level solved by that many Chebyshev steps instead of sweeps
+  float cheb_lo;            // the Chebyshev interval's lower end (the upper end is 2)
 };
 struct orc_solve_stats {
   int iterations, converged;
@@ -723,10 +725,57 @@ void prolong(const Model& m, SolveVecs<T>& v, int to) {  // :774-794
 // (symmetric mode: then num_final in reverse colour order).  With coarsest_fp64 the sweeps run in
 // FP64: r and z (zero after the restriction) promoted, the same formulas and order on the FP32
 // coefficients promoted, and z rounded back to T for the prolongation.
+// Chebyshev iteration on the Jacobi-scaled coarsest system D^-1 A z = D^-1 r from z = 0, for the
+// eigenvalue interval [lo, 2] (2 bounds the spectrum of D^-1 A for a diagonally dominant A by
+// Gershgorin).  Saad, Iterative Methods for Sparse Linear Systems, Alg. 12.1: theta = (hi + lo) / 2,
+// delta = (hi - lo) / 2, sigma = theta / delta; d_0 = D^-1 r / theta; rho_k = 1 / (2 sigma -
+// rho_{k-1}), d_k = rho_k rho_{k-1} d_{k-1} + 2 rho_k / delta D^-1 (r - A z_k); z_{k+1} = z_k + d_k.
+// The result is p(D^-1 A) D^-1 r for a fixed polynomial p positive on (0, 2]: a symmetric positive
+// definite preconditioner.  Cells with a negligible diagonal keep z = 0.
+template <class T>
+void coarsest_chebyshev(const Model& m, SolveVecs<T>& v, int steps, double lo) {
+  const int L = m.max_level;
+  const Level& q = m.lv[L - 1];
+  std::vector<T>& z = v.z[L - 1];
+  const std::vector<T>& r = v.r[L - 1];
+  const double hi = 2.0, theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+  std::vector<T> d(z.size(), (T)0), zn(z.size(), (T)0);
+  double rho_prev = 1.0 / sigma;
+  for (int k = 0; k < steps; ++k) {
+    double a = 0.0, b = 1.0 / theta;
+    if (k > 0) {
+      const double rho = 1.0 / (2.0 * sigma - rho_prev);
+      a = rho * rho_prev;
+      b = 2.0 * rho / delta;
+      rho_prev = rho;
+    }
+#pragma omp parallel for
+    for (int xi = 0; xi < q.d[0]; ++xi)
+      for (int yi = 0; yi < q.d[1]; ++yi)
+        for (int zi = 0; zi < q.d[2]; ++zi) {
+          const size_t c = ((size_t)xi * q.d[1] + yi) * q.d[2] + zi;
+          const float ad = q.adiag.v[c];
+          if (is_small(ad)) {
+            zn[c] = z[c];
+            continue;
+          }
+          const T res = r[c] - get_ax(q, z, xi, yi, zi);
+          // step 0 has no previous direction (no 0 * d term, whose sign of zero could differ)
+          d[c] = k == 0 ? (T)b * (res / (T)ad) : (T)a * d[c] + (T)b * (res / (T)ad);
+          zn[c] = z[c] + d[c];
+        }
+    z.swap(zn);
+  }
+}
+
 template <class T>
 void coarsest_sweeps(const Model& m, SolveVecs<T>& v, int smoother) {
   const int L = m.max_level;
   auto run = [&](auto& vv) {
+    if (m.prm.coarsest_cheb > 0) {
+      coarsest_chebyshev(m, vv, m.prm.coarsest_cheb, (double)m.prm.cheb_lo);
+      return;
+    }
     for (int it = 0; it < m.prm.num_final; ++it) {
       smooth(m, vv, L - 1, 0, smoother);
       smooth(m, vv, L - 1, 1, smoother);
@@ -1474,6 +1523,22 @@ int orc_solve_fp64(void* h, const double* b, double* x, double rel_tol, int max_
   std::vector<double> bb(b, b + m.liquidPhi.v.size()), xo;
   pcg<double>(m, v, bb, xo, 1, 1, 1, rel_tol, max_iter, st);
   std::memcpy(x, xo.data(), xo.size() * 8);
+  return 0;
+}
+// z = the coarsest level's solve of r (sweeps or Chebyshev, as configured) in FP64 from z = 0, on
+// the system of the last build; r and z hold the coarsest level's cells
+int orc_coarsest_solve64(void* h, const double* r, double* z) {
+  Model& m = *(Model*)h;
+  const int L = m.max_level;
+  if (L < 1 || m.lv.size() < (size_t)L) return 6;
+  const size_t n = m.lv[L - 1].adiag.v.size();
+  SolveVecs<double> v;
+  v.r.resize(L);
+  v.z.resize(L);
+  v.r[L - 1].assign(r, r + n);
+  v.z[L - 1].assign(n, 0.0);
+  coarsest_sweeps(m, v, m.prm.smoother);
+  std::memcpy(z, v.z[L - 1].data(), n * 8);
   return 0;
 }
 // y = A x at level 0 (GetAx per cell), FP64 arithmetic on the FP32 coefficients

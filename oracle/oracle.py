@@ -31,6 +31,7 @@ class Params(C.Structure):
         ("nsdf", C.c_int), ("sdf", C.POINTER(C.c_float)),
         ("correct_volume", C.c_int), ("correct_cycle", C.c_int), ("seed", C.c_ulonglong),
         ("num_particles", C.c_longlong), ("coarsest_fp64", C.c_int),
+        ("coarsest_cheb", C.c_int), ("cheb_lo", C.c_float),
     ]
 
 
@@ -93,6 +94,8 @@ def lib():
         L.orc_solve_fp32.argtypes = [vp, fp, fp, C.POINTER(SolveStats)]
         L.orc_solve_fp64.argtypes = [vp, dp, dp, C.c_double, C.c_int, C.POINTER(SolveStats)]
         L.orc_apply_a64.argtypes = [vp, dp, dp]
+        L.orc_coarsest_solve64.argtypes = [vp, dp, dp]
+        L.orc_coarsest_solve64.restype = C.c_int
         _lib = L
     return _lib
 
@@ -182,6 +185,9 @@ def params_from_config(cfg, dot_mode=1, p2g_order=1, smoother=None):
     p.seed = int(cfg.get("seed", 0))
     p.num_particles = int(lq["numParticles"])
     p.coarsest_fp64 = 1 if coarsest_precision(cfg) == "fp64" else 0
+    if s.get("coarsestSolver", "sweeps") == "chebyshev":
+        p.coarsest_cheb = int(s.get("coarsestSteps", 12))
+        p.cheb_lo = float(s.get("chebyshevLower", 0.01))
     return p
 
 
@@ -392,6 +398,14 @@ class Oracle:
         y = np.empty_like(x)
         self.L.orc_apply_a64(self.h, _dptr(x), _dptr(y))
         return y
+
+    def coarsest_solve64(self, r):
+        """the multigrid coarsest level's solve of r (its configured sweeps or Chebyshev steps) in
+        FP64 from z = 0, on the system of the last build"""
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        assert self.L.orc_coarsest_solve64(self.h, _dptr(r), _dptr(z)) == 0
+        return z
 
 
 def scalar_lib():

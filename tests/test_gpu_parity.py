@@ -71,6 +71,17 @@ def coarsest64(cfg):
     return cfg
 
 
+def chebyshev(cfg, steps, lo, levels=None):
+    """the coarsest level solved by Chebyshev steps on its Jacobi-scaled system (coarsestSolver
+    chebyshev, FP64; an option of this build, the reference's coarsest level is numFinalSmooth
+    sweeps, PressureSolver.cpp:754-757), two steps per launch"""
+    cfg["SolverParameter"].update({"coarsestPrecision": "fp64", "coarsestSolver": "chebyshev",
+                                   "coarsestSteps": steps, "chebyshevLower": lo})
+    if levels:
+        cfg["SolverParameter"]["numLevel"] = levels
+    return cfg
+
+
 def precond(cfg, kind):
     """CGSolver<3>'s other preconditioners (PressureSolver.cpp:721-743): None (z = r) and Jacobi
     (z = r / Adiag on cells with a nonzero diagonal)"""
@@ -118,6 +129,11 @@ CASES = {
     "dam24_precond_none": lambda: precond(small_dam(24, "reference"), "None"),
     "dam24_precond_jacobi": lambda: precond(small_dam(24, "reference"), "Jacobi"),
     "dam32_tank_sphere_jacobi": lambda: precond(small_dam(32, "symmetric", tank=True), "Jacobi"),
+    # even and odd step counts (an odd count ends with a one-step launch); two levels of the tall
+    # grid put the Chebyshev steps on 80-cell rows (two segments: the segment-end recomputation)
+    "dam24_chebyshev_coarsest": lambda: chebyshev(small_dam(24, "symmetric"), 12, 0.02),
+    "dam32_tank_sphere_chebyshev_odd": lambda: chebyshev(small_dam(32, "symmetric", tank=True), 7, 0.01),
+    "tall_16x24x160_chebyshev_segments": lambda: chebyshev(tall_dam(), 9, 0.005, levels=2),
 }
 
 
@@ -201,7 +217,9 @@ def test_stages_bit_exact(case):
                                   "dam24_correct_volume", "tall_16x24x160_row_segments",
                                   "dam40_symmetric_tile_sweep", "tall_16x24x160_tile_sweep",
                                   "dam24_precond_none", "dam32_tank_sphere_jacobi",
-                                  "dam24_symmetric_coarsest_fp64", "dam24_reference_coarsest_fp64"])
+                                  "dam24_symmetric_coarsest_fp64", "dam24_reference_coarsest_fp64",
+                                  "dam24_chebyshev_coarsest", "dam32_tank_sphere_chebyshev_odd",
+                                  "tall_16x24x160_chebyshev_segments"])
 def test_free_running_steps_bit_exact(case):
     print("case", case, flush=True)
     cfg = CASES[case]()
