@@ -434,9 +434,10 @@ def main():
         bench_2d(args, cfg)
         return
     if args.smoother == "reference":
+        # the reference's V-cycle throughout: its smoother, and numFinalSmooth sweeps on the coarsest level
         cfg["SolverParameter"].update({"smoother": "reference", "smoothingFactor": 0.5, "numLevel": 4,
                                        "numPreSmooth": 5, "numPostSmooth": 5, "numFinalSmooth": 20,
-                                       "maxIterations": 400})
+                                       "maxIterations": 400, "coarsestSolver": "sweeps"})
         workload += "_reference_smoother"
     if args.coarse_sweep != "config":
         cfg["SolverParameter"]["coarseSweep"] = args.coarse_sweep
