@@ -12,18 +12,19 @@
 namespace lh {
 
 // Kernel.h:12-17 (quadratic B-spline weight)
+// Both pieces are evaluated and one is selected: the same operations as the branches, without
+// the divergent branch (and its exec-mask bookkeeping) per call in the particle loops.
 LH_HD float quad_w(float d) {
   d = fabsf(d);
-  if (d < 0.5f) return 0.75f - d * d;
-  else if (d < 1.5f) return 0.5f * (1.5f - d) * (1.5f - d);
-  return 0.0f;
+  const float inner = 0.75f - d * d;
+  const float t = 1.5f - d;
+  const float outer = 0.5f * t * t;
+  return d < 0.5f ? inner : (d < 1.5f ? outer : 0.0f);
 }
 // Kernel.h:26-31 (derivative w.r.t. |d|, unsigned as in the reference)
 LH_HD float quad_gw(float d) {
   d = fabsf(d);
-  if (d < 0.5f) return -2.f * d;
-  else if (d < 1.5f) return d - 1.5f;
-  return 0.f;
+  return d < 0.5f ? -2.f * d : (d < 1.5f ? d - 1.5f : 0.f);
 }
 
 constexpr float kEps = 1e-8f;  // Definition.h:93 EPSILON
