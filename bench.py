@@ -102,8 +102,8 @@ def algorithmic_bytes(kernel, n0, P, active, fused, per_launch=None, sweep_b=Non
     spmv:      the p update fused with q = A p (k_cg_p_spmv, all but the first SpMV of a solve):
                z and p read, new p and q written, 4 coefficient floats -> 32 B per cell of
                those rows (the plain first SpMV moves 24 B);
-    p2g:       one component: 20 B/particle (position, velocity component,
-               affine column) + 4 B per face written (~n0 faces);
+    p2g:       the three components in one launch: 60 B/particle (position, velocity,
+               affine matrix) + 4 B per face written (~3 n0 faces);
     g2p:       72 B/particle (read position, write position, velocity, C) + 3
                velocity components read around the particles (~12 B/cell);
     liquid_phi: 12 B/particle + 4 B/cell.
@@ -116,7 +116,7 @@ def algorithmic_bytes(kernel, n0, P, active, fused, per_launch=None, sweep_b=Non
         "smooth_l1": sweep_b.get(1, 28.0) * per_launch.get(1, 0.5) * active.get(1, 0),
         "smooth_l2": sweep_b.get(2, 28.0) * per_launch.get(2, 0.5) * active.get(2, 0),
         "spmv": 32.0 * active[0],
-        "p2g": 20.0 * P + 4.0 * n0,
+        "p2g": 60.0 * P + 12.0 * n0,
         "g2p": 72.0 * P + 12.0 * n0,
         "liquid_phi": 12.0 * P + 4.0 * n0,
     }[kernel]

@@ -95,7 +95,8 @@ __global__ void k_bin_rank(const float* __restrict__ src, int64_t cap, int64_t P
 }
 
 // the particle buffer in bin order: destination d takes source order[d] (coalesced writes of the
-// 16 attributes and of the P2G records: per axis a, (x, y, z, v_a) and (row a of C, 0))
+// 16 attributes and of the P2G records: (x, y, z, v_x) at pack[d], (C.col(0), v_y), (C.col(1), v_z),
+// (C.col(2), 0) at pack[cap + 3 d ..])
 __global__ void k_bin_gather(const float* __restrict__ src, float* __restrict__ dst, int64_t cap, int64_t P,
                              const int* __restrict__ order, float4* __restrict__ pack) {
   for (int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; d < P; d += (int64_t)gridDim.x * blockDim.x) {
@@ -108,12 +109,11 @@ __global__ void k_bin_gather(const float* __restrict__ src, float* __restrict__ 
     for (int a = 0; a < kParticleAttrs; ++a) v[a] = src[(size_t)a * cap + i];
 #pragma unroll
     for (int a = 0; a < kParticleAttrs; ++a) dst[(size_t)a * cap + d] = v[a];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      float4* r = pack + 2 * ((size_t)a * cap + d);
-      r[0] = make_float4(v[PA_X], v[PA_X + 1], v[PA_X + 2], v[PA_VX + a]);
-      r[1] = make_float4(v[PA_C + 3 * a], v[PA_C + 3 * a + 1], v[PA_C + 3 * a + 2], 0.f);
-    }
+    pack[d] = make_float4(v[PA_X], v[PA_X + 1], v[PA_X + 2], v[PA_VX]);
+    float4* r = pack + cap + 3 * (size_t)d;
+    r[0] = make_float4(v[PA_C], v[PA_C + 1], v[PA_C + 2], v[PA_VX + 1]);
+    r[1] = make_float4(v[PA_C + 3], v[PA_C + 4], v[PA_C + 5], v[PA_VX + 2]);
+    r[2] = make_float4(v[PA_C + 6], v[PA_C + 7], v[PA_C + 8], 0.f);
   }
 }
 
@@ -206,7 +206,6 @@ __global__ __launch_bounds__(256) void k_scan_write(const int* __restrict__ coun
 // ------------------------------------------------------------------ P2G
 struct P2GP {
   GridP g;
-  int a;             // component
   int H[3];
   float gdt;         // gravity * dt, added to interior vy faces
   int tank_axis;
@@ -267,109 +266,132 @@ __device__ __forceinline__ bool tile_coords(int d0, int d1, int d2, int c[3], in
   return true;
 }
 
-// ------------------------------------------------------------------ x-marching P2G
-// A 256-thread workgroup owns kMX x-faces x 4 y x 64 z: each thread one (y, z) row of kMX
-// faces.  It marches over the x half-slabs of the tile's reach in ascending order and adds
-// each slab's particles to the (at most 3) consecutive x-faces whose reach holds the slab,
-// kept in a sliding window of registers: one read of a particle record serves those faces,
-// which share its y and z weights and the y and z part of the affine term.  Each face still
-// sums its slabs in ascending x and, within a slab, its y columns and z bins in order: the
-// bin order the oracle sums in.  At 256^3 (6.5 M particles) 4 x-faces per tile balance the
-// shared work against the number of busy waves: with 8 or 16 too few tiles hold particles
-// to fill the CUs.
+// ------------------------------------------------------------------ x-marching P2G, three components
+// The bin permutation writes, per particle in bin order, R0 = (x, y, z, v_x) into one array (the
+// positions liquid phi reads, 16 bytes apart) and R1 = (C.col(0), v_y), R2 = (C.col(1), v_z),
+// R3 = (C.col(2), 0) into a second (48 bytes per particle).
+//
+// A 256-thread workgroup owns kMX x positions x 4 y x 64 z of all three face grids: each thread
+// one (y, z) column with its x-, y- and z-faces at x = c0 .. c0 + kMX - 1 (those that exist: the
+// x-face grid is (N0 + 1) x N1 x N2, and so on).  A face of component a is reached by the
+// particles of the half-cells [2c - 3, 2c + 2] on axis a and [2c - 2, 2c + 3] on the other axes
+// (every particle outside has weight exactly zero), so the union [2c - 3, 2c + 3] per axis holds
+// the particles of all three faces of a column: the thread marches over the x half-slabs of the
+// tile's reach in ascending order, reads each slab's 7 x 7 union run once, and adds every record
+// to the (at most 3) consecutive faces of each component whose reach holds the slab, kept in
+// sliding windows of registers (the x-faces' window runs half a cell ahead of the y- and
+// z-faces').  The components share the record, its grid coordinates, the y and z weights of
+// the two face offsets (cell centre and cell face) and the x weights of the y- and z-faces.  A
+// face receives its nonzero contributions in ascending slabs and, within a slab, its y columns
+// and z bins in order, and zero weights add nothing: the bin order the oracle sums in, so the
+// result is bit-identical to one component per launch.  At 256^3 (6.5 M particles) 4 x
+// positions per tile balance the shared work against the number of busy waves.
 constexpr int kMX = 4;
 
-// record index of the t-th particle of a face's run of up to 6 bin columns: column j holds
+// record index of the t-th particle of a thread's run of up to 7 bin columns: column j holds
 // the t in [cum[j-1], cum[j]), at records t + off[j]
-__device__ __forceinline__ int p2g_rec(int t, const int off[6], const int cum[6]) {
-  int o = off[5];
+__device__ __forceinline__ int p2g_rec(int t, const int off[7], const int cum[7]) {
+  int o = off[6];
 #pragma unroll
-  for (int j = 4; j >= 0; --j) o = (t < cum[j]) ? off[j] : o;
+  for (int j = 5; j >= 0; --j) o = (t < cum[j]) ? off[j] : o;
   return t + o;
 }
 
-// one particle record (x y z v_a | C.col(a)) into the weight and momentum sums of the first
-// NF faces of the window (LiquidModel.cpp:613-721): weight quad(gx - ccx) quad(gy - ccy)
-// quad(gz - ccz) with grid coordinates (x - min) / dx (an exact power-of-two multiply when
-// kPow2), momentum weight * (v + C.col(a) . (x - face)); zero weights add nothing
-template <bool kPow2, int NF>
-__device__ __forceinline__ void p2g_acc(float4 A, float4 B, float mn0, float mn1, float mn2, float dx0, float dx1,
-                                        float dx2, float ccy, float ccz, float fy, float fz, const float ccx[3],
-                                        const float fx[3], float ws[3], float vs[3]) {
-  const float gx = kPow2 ? (A.x - mn0) * dx0 : (A.x - mn0) / dx0;
-  const float gy = kPow2 ? (A.y - mn1) * dx1 : (A.y - mn1) / dx1;
-  const float gz = kPow2 ? (A.z - mn2) * dx2 : (A.z - mn2) / dx2;
-  const float wy = quad_w(gy - ccy), wz = quad_w(gz - ccz);
-  const float yz = B.y * (A.y - fy) + B.z * (A.z - fz);  // sum3(a, b, c) == a + (b + c)
+// a thread's column: grid minimum, dx (1 / dx when kPow2), the face coordinates of its (y, z)
+// on the cell-face (i) and cell-centre (h) offsets, in grid units and in world units
+struct P2GCol {
+  float mn0, mn1, mn2, dx0, dx1, dx2;
+  float ccy_i, ccy_h, ccz_i, ccz_h;
+  float fy_i, fy_h, fz_i, fz_h;
+};
+
+// one particle record into the weight and momentum sums of the first NX x-faces and the first
+// NYZ y- and z-faces of the windows (LiquidModel.cpp:613-721): weight quad(gx - ccx) quad(gy -
+// ccy) quad(gz - ccz) with grid coordinates (x - min) / dx (an exact power-of-two multiply when
+// kPow2), momentum weight * (v_a + C.col(a) . (x - face)); zero weights add nothing
+template <bool kPow2, int NX, int NYZ>
+__device__ __forceinline__ void p2g_acc(float4 R0, float4 R1, float4 R2, float4 R3, const P2GCol& k,
+                                        const float ccx0[3], const float fx0[3], const float ccxh[3],
+                                        const float fxh[3], float ws[3][3], float vs[3][3]) {
+  const float gx = kPow2 ? (R0.x - k.mn0) * k.dx0 : (R0.x - k.mn0) / k.dx0;
+  const float gy = kPow2 ? (R0.y - k.mn1) * k.dx1 : (R0.y - k.mn1) / k.dx1;
+  const float gz = kPow2 ? (R0.z - k.mn2) * k.dx2 : (R0.z - k.mn2) / k.dx2;
+  const float wyi = quad_w(gy - k.ccy_i), wyh = quad_w(gy - k.ccy_h);
+  const float wzi = quad_w(gz - k.ccz_i), wzh = quad_w(gz - k.ccz_h);
+  // the y and z part of each component's affine term (sum3(a, b, c) == a + (b + c))
+  const float yz0 = R1.y * (R0.y - k.fy_h) + R1.z * (R0.z - k.fz_h);
+  const float yz1 = R2.y * (R0.y - k.fy_i) + R2.z * (R0.z - k.fz_h);
+  const float yz2 = R3.y * (R0.y - k.fy_h) + R3.z * (R0.z - k.fz_i);
 #pragma unroll
-  for (int k = 0; k < NF; ++k) {
-    const float wgt = quad_w(gx - ccx[k]) * wy * wz;
-    const float dot = B.x * (A.x - fx[k]) + yz;
+  for (int f = 0; f < NX; ++f) {
+    const float wgt = quad_w(gx - ccx0[f]) * wyh * wzh;
+    const float dot = R1.x * (R0.x - fx0[f]) + yz0;
     const bool add = wgt != 0.f;
-    ws[k] = add ? ws[k] + wgt : ws[k];
-    vs[k] = add ? vs[k] + wgt * (A.w + dot) : vs[k];
+    ws[0][f] = add ? ws[0][f] + wgt : ws[0][f];
+    vs[0][f] = add ? vs[0][f] + wgt * (R0.w + dot) : vs[0][f];
+  }
+#pragma unroll
+  for (int f = 0; f < NYZ; ++f) {
+    const float wx = quad_w(gx - ccxh[f]);
+    const float w1 = wx * wyi * wzh, w2 = wx * wyh * wzi;
+    const float d1 = R2.x * (R0.x - fxh[f]) + yz1, d2 = R3.x * (R0.x - fxh[f]) + yz2;
+    const bool a1 = w1 != 0.f, a2 = w2 != 0.f;
+    ws[1][f] = a1 ? ws[1][f] + w1 : ws[1][f];
+    vs[1][f] = a1 ? vs[1][f] + w1 * (R1.w + d1) : vs[1][f];
+    ws[2][f] = a2 ? ws[2][f] + w2 : ws[2][f];
+    vs[2][f] = a2 ? vs[2][f] + w2 * (R2.w + d2) : vs[2][f];
   }
 }
 
-// the records t in [0, total) of a lane's run of bin columns in one slab
-template <bool kPow2, int NF>
-__device__ __forceinline__ void p2g_slab(const float4* __restrict__ rec, const int off[6], const int cum[6], int total,
-                                         float mn0, float mn1, float mn2, float dx0, float dx1, float dx2, float ccy,
-                                         float ccz, float fy, float fz, const float ccx[3], const float fx[3],
-                                         float ws[3], float vs[3]) {
+// the records t in [0, total) of a thread's run of bin columns in one slab, two in flight
+template <bool kPow2, int NX, int NYZ>
+__device__ __forceinline__ void p2g_slab(const float4* __restrict__ pos, const float4* __restrict__ rest,
+                                         const int off[7], const int cum[7], int total, const P2GCol& k,
+                                         const float ccx0[3], const float fx0[3], const float ccxh[3],
+                                         const float fxh[3], float ws[3][3], float vs[3][3]) {
   int t = 0;
-  // four records in flight, then pairs
-  for (; t + 3 < total; t += 4) {
-    const int i0 = p2g_rec(t, off, cum), i1 = p2g_rec(t + 1, off, cum);
-    const int i2 = p2g_rec(t + 2, off, cum), i3 = p2g_rec(t + 3, off, cum);
-    const float4 A0 = rec[2 * (size_t)i0], B0 = rec[2 * (size_t)i0 + 1];
-    const float4 A1 = rec[2 * (size_t)i1], B1 = rec[2 * (size_t)i1 + 1];
-    const float4 A2 = rec[2 * (size_t)i2], B2 = rec[2 * (size_t)i2 + 1];
-    const float4 A3 = rec[2 * (size_t)i3], B3 = rec[2 * (size_t)i3 + 1];
-    p2g_acc<kPow2, NF>(A0, B0, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
-    p2g_acc<kPow2, NF>(A1, B1, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
-    p2g_acc<kPow2, NF>(A2, B2, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
-    p2g_acc<kPow2, NF>(A3, B3, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
-  }
   for (; t + 1 < total; t += 2) {
-    const int i0 = p2g_rec(t, off, cum), i1 = p2g_rec(t + 1, off, cum);
-    const float4 A0 = rec[2 * (size_t)i0], B0 = rec[2 * (size_t)i0 + 1];
-    const float4 A1 = rec[2 * (size_t)i1], B1 = rec[2 * (size_t)i1 + 1];
-    p2g_acc<kPow2, NF>(A0, B0, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
-    p2g_acc<kPow2, NF>(A1, B1, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
+    const size_t i0 = (size_t)p2g_rec(t, off, cum), i1 = (size_t)p2g_rec(t + 1, off, cum);
+    const float4 A0 = pos[i0], B0 = rest[3 * i0], C0 = rest[3 * i0 + 1], D0 = rest[3 * i0 + 2];
+    const float4 A1 = pos[i1], B1 = rest[3 * i1], C1 = rest[3 * i1 + 1], D1 = rest[3 * i1 + 2];
+    p2g_acc<kPow2, NX, NYZ>(A0, B0, C0, D0, k, ccx0, fx0, ccxh, fxh, ws, vs);
+    p2g_acc<kPow2, NX, NYZ>(A1, B1, C1, D1, k, ccx0, fx0, ccxh, fxh, ws, vs);
   }
   if (t < total) {
-    const int i0 = p2g_rec(t, off, cum);
-    p2g_acc<kPow2, NF>(rec[2 * (size_t)i0], rec[2 * (size_t)i0 + 1], mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz,
-                       ccx, fx, ws, vs);
+    const size_t i0 = (size_t)p2g_rec(t, off, cum);
+    p2g_acc<kPow2, NX, NYZ>(pos[i0], rest[3 * i0], rest[3 * i0 + 1], rest[3 * i0 + 2], k, ccx0, fx0, ccxh, fxh, ws,
+                            vs);
   }
 }
 
-// Gather form of Particle2Grid for one velocity component, each face summing the particles
-// of the half-cell bins around it in bin order.  Within a slab the (up to 6) y columns of a
-// face are concatenated into one run, so a wave's lanes wait for the longest run of 6
-// columns instead of the longest of each column.
+// Gather form of Particle2Grid for the three velocity components, each face summing the
+// particles of the half-cell bins around it in bin order.  Within a slab the (up to 7) y
+// columns of a thread's union window are concatenated into one run, so a wave's lanes wait for
+// the longest run of 7 columns instead of the longest of each column.
 template <bool kPow2>
-__global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const float4* __restrict__ pack, int64_t cap,
+__global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
+                                              const float4* __restrict__ pos, const float4* __restrict__ rest,
                                               const int* __restrict__ start, P2GP q) {
-  const int a = q.a;
-  const int d0 = q.g.N[0] + (a == 0), d1 = q.g.N[1] + (a == 1), d2 = q.g.N[2] + (a == 2);
+  const int N0 = q.g.N[0], N1 = q.g.N[1], N2 = q.g.N[2];
   int c[3], c0[3];
-  if (!tile_coords<kMX, kPTY, kPTZ>(d0, d1, d2, c, c0)) return;
+  // tiles over the union of the three face grids, (N0 + 1) x (N1 + 1) x (N2 + 1)
+  if (!tile_coords<kMX, kPTY, kPTZ>(N0 + 1, N1 + 1, N2 + 1, c, c0)) return;
   const int T[3] = {kMX, kPTY, kPTZ};
   int blo[3], bhi[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    blo[k] = max((k == a) ? 2 * c0[k] - 3 : 2 * c0[k] - 2, 0);
-    bhi[k] = min(2 * (c0[k] + T[k] - 1) + ((k == a) ? 2 : 3), q.H[k] - 1);
+    blo[k] = max(2 * c0[k] - 3, 0);
+    bhi[k] = min(2 * (c0[k] + T[k] - 1) + 3, q.H[k] - 1);
   }
   const bool any = tile_has_particles(start, blo, bhi, q.H[1], q.H[2]);
-  if (c[1] >= d1 || c[2] >= d2) return;
-  const int xend = min(c0[0] + kMX, d0);
-  // bins [xlo, xhi] of face x along axis 0
-  auto xlo = [&](int x) { return max((a == 0) ? 2 * x - 3 : 2 * x - 2, 0); };
-  auto xhi = [&](int x) { return min(((a == 0) ? 2 * x - 3 : 2 * x - 2) + 5, q.H[0] - 1); };
-  auto finish = [&](int x, float ws, float vs) {
+  if (c[1] > N1 || c[2] > N2) return;
+  // the faces of this column: x-faces (x <= N0, y < N1, z < N2), y-faces (x < N0, y <= N1,
+  // z < N2), z-faces (x < N0, y < N1, z <= N2)
+  const bool has[3] = {c[1] < N1 && c[2] < N2, c[1] <= N1 && c[2] < N2, c[1] < N1 && c[2] <= N2};
+  const int xend0 = min(c0[0] + kMX, N0 + 1), xend12 = min(c0[0] + kMX, N0);
+  float* const vout[3] = {vx, vy, vz};
+  auto finish = [&](int a, int x, float ws, float vs) {
+    if (!has[a]) return;
     float v = !is_small(ws) ? vs / ws : 0.0f;
     // interior ranges of ApplyGravity (a == 1) and of the tank acceleration
     if (a == 1 || a == q.tank_axis) {
@@ -377,7 +399,7 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const flo
       bool in = true;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        int h = (k == a) ? q.g.N[k] - 1 : q.g.N[k] - 2;
+        const int h = (k == a) ? q.g.N[k] - 1 : q.g.N[k] - 2;
         if (cc[k] < 1 || cc[k] > h) in = false;
       }
       if (in) {
@@ -385,72 +407,111 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vout, const flo
         if (a == q.tank_axis) v += q.adt;
       }
     }
-    vout[((size_t)x * d1 + c[1]) * d2 + c[2]] = v;
+    const int d1 = N1 + (a == 1), d2 = N2 + (a == 2);
+    vout[a][((size_t)x * d1 + c[1]) * d2 + c[2]] = v;
   };
   if (!any) {
-    for (int x = c0[0]; x < xend; ++x) finish(x, 0.f, 0.f);
+    for (int x = c0[0]; x < xend0; ++x) finish(0, x, 0.f, 0.f);
+    for (int x = c0[0]; x < xend12; ++x) {
+      finish(1, x, 0.f, 0.f);
+      finish(2, x, 0.f, 0.f);
+    }
     return;
   }
-  const float4* rec = pack + 2 * (size_t)a * cap;  // x y z v_a | C.col(a) 0
-  const float mn0 = q.g.mn[0], mn1 = q.g.mn[1], mn2 = q.g.mn[2];
-  const float dx0 = kPow2 ? q.inv[0] : q.g.dx[0], dx1 = kPow2 ? q.inv[1] : q.g.dx[1],
-              dx2 = kPow2 ? q.inv[2] : q.g.dx[2];
-  int lo[3], hi[3];
+  P2GCol k;
+  k.mn0 = q.g.mn[0];
+  k.mn1 = q.g.mn[1];
+  k.mn2 = q.g.mn[2];
+  k.dx0 = kPow2 ? q.inv[0] : q.g.dx[0];
+  k.dx1 = kPow2 ? q.inv[1] : q.g.dx[1];
+  k.dx2 = kPow2 ? q.inv[2] : q.g.dx[2];
+  k.ccy_i = (float)c[1];
+  k.ccy_h = (float)c[1] + 0.5f;
+  k.ccz_i = (float)c[2];
+  k.ccz_h = (float)c[2] + 0.5f;
+  k.fy_i = k.ccy_i * q.g.dx[1] + k.mn1;
+  k.fy_h = k.ccy_h * q.g.dx[1] + k.mn1;
+  k.fz_i = k.ccz_i * q.g.dx[2] + k.mn2;
+  k.fz_h = k.ccz_h * q.g.dx[2] + k.mn2;
+  // the union window in y and z
+  const int ylo = max(2 * c[1] - 3, 0), yhi = min(2 * c[1] + 3, q.H[1] - 1);
+  const int zlo = max(2 * c[2] - 3, 0), zhi = min(2 * c[2] + 3, q.H[2] - 1);
+  // x reach of an x-face (0) and of a y- or z-face (1)
+  auto xlo = [&](int m, int x) { return max(m == 0 ? 2 * x - 3 : 2 * x - 2, 0); };
+  auto xhi = [&](int m, int x) { return min((m == 0 ? 2 * x - 3 : 2 * x - 2) + 5, q.H[0] - 1); };
+  float ws[3][3], vs[3][3];
 #pragma unroll
-  for (int k = 1; k < 3; ++k) {
-    int l = (k == a) ? 2 * c[k] - 3 : 2 * c[k] - 2;
-    lo[k] = max(l, 0);
-    hi[k] = min(l + 5, q.H[k] - 1);
-  }
-  const float ccy = (a == 1) ? (float)c[1] : (float)c[1] + 0.5f;
-  const float ccz = (a == 2) ? (float)c[2] : (float)c[2] + 0.5f;
-  const float fy = ccy * q.g.dx[1] + mn1, fz = ccz * q.g.dx[2] + mn2;
-  float ws[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f};  // faces cw, cw + 1, cw + 2
-  int cw = c0[0];
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int f = 0; f < 3; ++f) ws[a][f] = vs[a][f] = 0.f;
+  int cw0 = c0[0], cw12 = c0[0];  // first face of each window
   for (int hx = blo[0]; hx <= bhi[0]; ++hx) {
-    // the faces whose reach holds slab hx: a prefix of the window (the reaches of
-    // consecutive faces are ordered and at most 3 overlap), the same for the whole tile
-    float ccx[3], fx[3];
-    int nf = 0;
+    // the faces whose reach holds slab hx: a prefix of each window (the reaches of consecutive
+    // faces are ordered and at most 3 overlap)
+    float ccx0[3], fx0[3], ccxh[3], fxh[3];
+    int nx = 0, nyz = 0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int x = cw + k;
-      nf += (x < xend && xlo(x) <= hx && hx <= xhi(x)) ? 1 : 0;
-      ccx[k] = (a == 0) ? (float)x : (float)x + 0.5f;
-      fx[k] = ccx[k] * q.g.dx[0] + mn0;
+    for (int f = 0; f < 3; ++f) {
+      const int x0 = cw0 + f, x1 = cw12 + f;
+      nx += (x0 < xend0 && xlo(0, x0) <= hx && hx <= xhi(0, x0)) ? 1 : 0;
+      nyz += (x1 < xend12 && xlo(1, x1) <= hx && hx <= xhi(1, x1)) ? 1 : 0;
+      ccx0[f] = (float)x0;
+      fx0[f] = ccx0[f] * q.g.dx[0] + k.mn0;
+      ccxh[f] = (float)x1 + 0.5f;
+      fxh[f] = ccxh[f] * q.g.dx[0] + k.mn0;
     }
-    if (nf > 0) {
-      int off[6], cum[6], run = 0;
+    if (nx + nyz > 0) {
+      int off[7], cum[7], run = 0;
 #pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        const int hy = lo[1] + j;
+      for (int j = 0; j < 7; ++j) {
+        const int hy = ylo + j;
         int s0 = 0, n = 0;
-        if (hy <= hi[1]) {
+        if (hy <= yhi) {
           const size_t k0 = ((size_t)hx * q.H[1] + hy) * q.H[2];
-          s0 = start[k0 + lo[2]];
-          n = start[k0 + hi[2] + 1] - s0;
+          s0 = start[k0 + zlo];
+          n = start[k0 + zhi + 1] - s0;
         }
         off[j] = s0 - run;
         run += n;
         cum[j] = run;
       }
-      if (nf == 3)
-        p2g_slab<kPow2, 3>(rec, off, cum, run, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
-      else if (nf == 2)
-        p2g_slab<kPow2, 2>(rec, off, cum, run, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
-      else
-        p2g_slab<kPow2, 1>(rec, off, cum, run, mn0, mn1, mn2, dx0, dx1, dx2, ccy, ccz, fy, fz, ccx, fx, ws, vs);
+#define LH_P2G_CASE(A, B) \
+  case A * 4 + B:         \
+    p2g_slab<kPow2, A, B>(pos, rest, off, cum, run, k, ccx0, fx0, ccxh, fxh, ws, vs); \
+    break;
+      switch (nx * 4 + nyz) {
+        LH_P2G_CASE(0, 1) LH_P2G_CASE(0, 2) LH_P2G_CASE(0, 3)
+        LH_P2G_CASE(1, 0) LH_P2G_CASE(1, 1) LH_P2G_CASE(1, 2) LH_P2G_CASE(1, 3)
+        LH_P2G_CASE(2, 0) LH_P2G_CASE(2, 1) LH_P2G_CASE(2, 2) LH_P2G_CASE(2, 3)
+        LH_P2G_CASE(3, 0) LH_P2G_CASE(3, 1) LH_P2G_CASE(3, 2) LH_P2G_CASE(3, 3)
+        default: break;
+      }
+#undef LH_P2G_CASE
     }
     // faces whose last slab this was
-    while (cw < xend && xhi(cw) <= hx) {
-      finish(cw, ws[0], vs[0]);
-      ws[0] = ws[1];
-      vs[0] = vs[1];
-      ws[1] = ws[2];
-      vs[1] = vs[2];
-      ws[2] = 0.f;
-      vs[2] = 0.f;
-      ++cw;
+    while (cw0 < xend0 && xhi(0, cw0) <= hx) {
+      finish(0, cw0, ws[0][0], vs[0][0]);
+      ws[0][0] = ws[0][1];
+      vs[0][0] = vs[0][1];
+      ws[0][1] = ws[0][2];
+      vs[0][1] = vs[0][2];
+      ws[0][2] = 0.f;
+      vs[0][2] = 0.f;
+      ++cw0;
+    }
+    while (cw12 < xend12 && xhi(1, cw12) <= hx) {
+      finish(1, cw12, ws[1][0], vs[1][0]);
+      finish(2, cw12, ws[2][0], vs[2][0]);
+#pragma unroll
+      for (int a = 1; a < 3; ++a) {
+        ws[a][0] = ws[a][1];
+        vs[a][0] = vs[a][1];
+        ws[a][1] = ws[a][2];
+        vs[a][1] = vs[a][2];
+        ws[a][2] = 0.f;
+        vs[a][2] = 0.f;
+      }
+      ++cw12;
     }
   }
 }
@@ -491,7 +552,7 @@ __global__ void __launch_bounds__(256) k_liquid_phi(float* __restrict__ phi, con
     // bound slack: 1e-3 dx plus the float error of world coordinates
     const float margin = 1e-3f * g.dx[0] + 1e-6f * (fabsf(cpx) + fabsf(cpy) + fabsf(cpz));
     auto phi_of = [&](int s) {
-      const float4 A = rec[2 * (size_t)s];  // x y z of the bin-ordered particle record
+      const float4 A = rec[s];  // x y z of the bin-ordered particle
       return norm3(cpx - A.x, cpy - A.y, cpz - A.z) - radius;
     };
     auto run = [&](int s, int s1) {
@@ -1016,20 +1077,15 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
     q.inv[k] = std::ldexp(1.0f, 1 - e);
     pow2 = pow2 && m == 0.5f && std::isnormal(q.inv[k]) && c.g.dx[k] * q.inv[k] == 1.0f;
   }
-  for (int a = 0; a < 3; ++a) {
-    q.a = a;
-    P2GP qa = q;
-    if (!fuse_body_forces || a != 1) qa.gdt = 0.f;
-    ProfScope ps(c, LH_PROF_P2G);
-    unsigned nt = tile_grid(div_up(c.g.N[0] + (a == 0), kMX) * div_up(c.g.N[1] + (a == 1), kPTY) *
-                            div_up(c.g.N[2] + (a == 2), kPTZ));
-    if (pow2)
-      hipLaunchKernelGGL(k_p2g<true>, dim3(nt), dim3(kBlock), 0, c.s, c.v[a], (const float4*)c.ppack, c.cap,
-                         c.bin_start, qa);
-    else
-      hipLaunchKernelGGL(k_p2g<false>, dim3(nt), dim3(kBlock), 0, c.s, c.v[a], (const float4*)c.ppack, c.cap,
-                         c.bin_start, qa);
-  }
+  if (!fuse_body_forces) q.gdt = 0.f;
+  ProfScope ps(c, LH_PROF_P2G);
+  const unsigned nt = tile_grid(div_up(c.g.N[0] + 1, kMX) * div_up(c.g.N[1] + 1, kPTY) * div_up(c.g.N[2] + 1, kPTZ));
+  if (pow2)
+    hipLaunchKernelGGL(k_p2g<true>, dim3(nt), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], (const float4*)c.ppack,
+                       (const float4*)c.ppack + c.cap, c.bin_start, q);
+  else
+    hipLaunchKernelGGL(k_p2g<false>, dim3(nt), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], (const float4*)c.ppack,
+                       (const float4*)c.ppack + c.cap, c.bin_start, q);
 }
 
 int run_correct_volume(Ctx& c, float dt) {

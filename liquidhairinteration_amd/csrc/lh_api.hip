@@ -355,7 +355,7 @@ int ensure_particle_capacity(Ctx& c, int64_t n) {
   int64_t cap = std::max<int64_t>(n, 1);
   bool ok = dalloc(c, c.pbuf[0], (size_t)kParticleAttrs * cap) && dalloc(c, c.pbuf[1], (size_t)kParticleAttrs * cap) &&
             dalloc(c, c.pkey, cap) && dalloc(c, c.pslot, cap) && dalloc(c, c.porder, cap) &&
-            dalloc(c, c.ppack, (size_t)24 * cap);
+            dalloc(c, c.ppack, (size_t)16 * cap);
   if (!ok) {
     c.cap = 0;
     return LH_ERR_OOM;
