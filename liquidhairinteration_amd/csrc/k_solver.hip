@@ -770,12 +770,15 @@ struct ChebSteps {
 // recomputed by wave 0), the four in the rows x +- 1, y +- 1 are recomputed by waves 1-4 from
 // zin / din and handed over through LDS.  zin / din are read and zout / dout written, so no block
 // sees another's writes, and every value is that of two separate steps, operation for operation.
+// With zf (the solve's last launch) it also writes z rounded to float for the prolongation, over
+// the same cells the conversion k_z64_to_f32 covers (the z-support rows).
 template <class T>
 __global__ __launch_bounds__(64 * kSweepWaves) void k_cheb_rows(T* __restrict__ zout, T* __restrict__ dout,
                                                                  const T* __restrict__ zin, const T* __restrict__ din,
                                                                  const float* __restrict__ r, LvP L,
                                                                  const int* __restrict__ rows,
-                                                                 const int* __restrict__ nrows, ChebSteps q) {
+                                                                 const int* __restrict__ nrows, ChebSteps q,
+                                                                 float* __restrict__ zf) {
   __shared__ T s_nb[4][64];  // first-step z of the rows x + 1, x - 1, y + 1, y - 1
   const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
   const int nseg = (d2 + 63) >> 6;
@@ -850,6 +853,7 @@ __global__ __launch_bounds__(64 * kSweepWaves) void k_cheb_rows(T* __restrict__ 
       if (zr0 < d2) {
         zout[c] = zo;
         dout[c] = dO;
+        if (zf) zf[c] = (float)zo;  // the solve's last launch: z for the prolongation
       }
     }
     __syncthreads();  // s_nb is rewritten by the next segment
@@ -1567,7 +1571,8 @@ void coarsest_sweeps64(Ctx& c, int fwd, int rev, int mode) {
 // steps per launch, then z = (float) z64 on the z-support rows for the prolongation.  The step
 // coefficients follow the oracle's recurrence in the same double operations: theta = (2 + lo) / 2,
 // delta = (2 - lo) / 2, sigma = theta / delta, rho_0 = 1 / sigma; step 0: b = 1 / theta; step k:
-// rho = 1 / (2 sigma - rho_prev), a = rho rho_prev, b = 2 rho / delta.
+// rho = 1 / (2 sigma - rho_prev), a = rho rho_prev, b = 2 rho / delta.  The last launch also
+// writes z = (float) z64 for the prolongation.
 void coarsest_chebyshev64(Ctx& c) {
   const int l = (int)c.lv.size() - 1;
   Level& L = c.lv[l];
@@ -1598,12 +1603,11 @@ void coarsest_chebyshev64(Ctx& c) {
       q.a1 = q.two ? A[k + 1] : 0.0;
       q.b1 = q.two ? B[k + 1] : 0.0;
       hipLaunchKernelGGL(k_cheb_rows<double>, dim3(nb), dim3(64 * kSweepWaves), 0, c.s, L.z64b, L.d64b, L.z64, L.d64,
-                         L.r, lvp(L), L.zrows, L.nzrows, q);
+                         L.r, lvp(L), L.zrows, L.nzrows, q, k + 2 >= steps ? L.z : (float*)nullptr);
       std::swap(L.z64, L.z64b);
       std::swap(L.d64, L.d64b);
     }
   }
-  hipLaunchKernelGGL(k_z64_to_f32, dim3(row_blocks(L)), dim3(kBlock), 0, c.s, L.z, L.z64, L.zrows, L.nzrows, L.d[2]);
 }
 
 // ApplyPreconditioner, PressureSolver.cpp:719-772 (symmetric mode: z[0] = 0 first,
