@@ -1040,21 +1040,29 @@ __device__ double sum_partials_block(const double* __restrict__ part, int n) {
   return t;
 }
 template <int NV>
+// The block's sum in a fixed pairwise order: thread i takes i + 128, then i + 64 (through LDS: the
+// four waves), then wave 0 continues with i + 32, 16, 8, 4, 2, 1 by wavefront shuffles, the same
+// pairs as a shared-memory tree.
 __device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partial) {
+  static_assert(kBlock == 256, "two cross-wave stages, then one wave");
   __shared__ double s[NV][kBlock];
+  const int t = (int)threadIdx.x;
 #pragma unroll
-  for (int k = 0; k < NV; ++k) s[k][threadIdx.x] = v[k];
+  for (int k = 0; k < NV; ++k) s[k][t] = v[k];
   __syncthreads();
-  for (int st = kBlock / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) {
+  if (t < 128) {
 #pragma unroll
-      for (int k = 0; k < NV; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + st];
-    }
-    __syncthreads();
+    for (int k = 0; k < NV; ++k) s[k][t] += s[k][t + 128];
   }
-  if (threadIdx.x == 0) {
+  __syncthreads();
+  if (t < 64) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) partial[(size_t)k * gridDim.x + blockIdx.x] = s[k][0];
+    for (int k = 0; k < NV; ++k) {
+      double a = s[k][t] + s[k][t + 64];
+#pragma unroll
+      for (int st = 32; st > 0; st >>= 1) a += __shfl_down(a, st);  // lanes < st keep the tree's pairs
+      if (t == 0) partial[(size_t)k * gridDim.x + blockIdx.x] = a;
+    }
   }
 }
 
