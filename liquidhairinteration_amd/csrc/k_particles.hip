@@ -384,6 +384,10 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vx, float* __re
     bhi[k] = min(2 * (c0[k] + T[k] - 1) + 3, q.H[k] - 1);
   }
   const bool any = tile_has_particles(start, blo, bhi, q.H[1], q.H[2]);
+  // the 64 lanes of a wave share their y (kPTZ == 64): y and everything derived from it is
+  // wave-uniform, and lives in scalar registers
+  static_assert(kPTZ == 64, "one y per wave");
+  c[1] = __builtin_amdgcn_readfirstlane(c[1]);
   if (c[1] > N1 || c[2] > N2) return;
   // the faces of this column: x-faces (x <= N0, y < N1, z < N2), y-faces (x < N0, y <= N1,
   // z < N2), z-faces (x < N0, y < N1, z <= N2)
