@@ -308,7 +308,10 @@ struct P2GCol {
 // one particle record into the weight and momentum sums of the first NX x-faces and the first
 // NYZ y- and z-faces of the windows (LiquidModel.cpp:613-721): weight quad(gx - ccx) quad(gy -
 // ccy) quad(gz - ccz) with grid coordinates (x - min) / dx (an exact power-of-two multiply when
-// kPow2), momentum weight * (v_a + C.col(a) . (x - face)); zero weights add nothing
+// kPow2), momentum weight * (v_a + C.col(a) . (x - face)).  A record outside a face's reach has
+// weight exactly +0 and adds exactly nothing: the weight sum stays as it is, and the momentum
+// term is +-0 (velocities and affine terms are finite), which leaves a sum that started at +0
+// unchanged (such a sum is never -0: x + -x and +0 + -0 are +0 in round-to-nearest).
 template <bool kPow2, int NX, int NYZ>
 __device__ __forceinline__ void p2g_acc(float4 R0, float4 R1, float4 R2, float4 R3, const P2GCol& k,
                                         const float ccx0[3], const float fx0[3], const float ccxh[3],
@@ -326,20 +329,19 @@ __device__ __forceinline__ void p2g_acc(float4 R0, float4 R1, float4 R2, float4 
   for (int f = 0; f < NX; ++f) {
     const float wgt = quad_w(gx - ccx0[f]) * wyh * wzh;
     const float dot = R1.x * (R0.x - fx0[f]) + yz0;
-    const bool add = wgt != 0.f;
-    ws[0][f] = add ? ws[0][f] + wgt : ws[0][f];
-    vs[0][f] = add ? vs[0][f] + wgt * (R0.w + dot) : vs[0][f];
+    ws[0][f] = ws[0][f] + wgt;
+    vs[0][f] = vs[0][f] + wgt * (R0.w + dot);
   }
 #pragma unroll
   for (int f = 0; f < NYZ; ++f) {
     const float wx = quad_w(gx - ccxh[f]);
     const float w1 = wx * wyi * wzh, w2 = wx * wyh * wzi;
-    const float d1 = R2.x * (R0.x - fxh[f]) + yz1, d2 = R3.x * (R0.x - fxh[f]) + yz2;
-    const bool a1 = w1 != 0.f, a2 = w2 != 0.f;
-    ws[1][f] = a1 ? ws[1][f] + w1 : ws[1][f];
-    vs[1][f] = a1 ? vs[1][f] + w1 * (R1.w + d1) : vs[1][f];
-    ws[2][f] = a2 ? ws[2][f] + w2 : ws[2][f];
-    vs[2][f] = a2 ? vs[2][f] + w2 * (R2.w + d2) : vs[2][f];
+    const float dxf = R0.x - fxh[f];
+    const float d1 = R2.x * dxf + yz1, d2 = R3.x * dxf + yz2;
+    ws[1][f] = ws[1][f] + w1;
+    vs[1][f] = vs[1][f] + w1 * (R1.w + d1);
+    ws[2][f] = ws[2][f] + w2;
+    vs[2][f] = vs[2][f] + w2 * (R2.w + d2);
   }
 }
 
@@ -638,14 +640,22 @@ __global__ void __launch_bounds__(256) k_liquid_phi(float* __restrict__ phi, con
 struct GP {
   GridP g;
   float dt;
-  int dW[3];
+  float inv[3];  // 1 / dx per axis when every dx is a power of two (then x / dx == x * inv exactly)
 };
 
-__device__ __forceinline__ float solid_phi_in(const float* __restrict__ sp, const GridP& g, float x, float y,
-                                              float z) {
-  float lx = (x - g.mn[0]) / g.dx[0];
-  float ly = (y - g.mn[1]) / g.dx[1];
-  float lz = (z - g.mn[2]) / g.dx[2];
+// grid coordinate (x - min) / dx on axis k: with kPow2 every dx is a power of two and the
+// division is the exact multiply by 1 / dx
+template <bool kPow2>
+__device__ __forceinline__ float to_grid(const GridP& g, const float* inv, int k, float x) {
+  return kPow2 ? (x - g.mn[k]) * inv[k] : (x - g.mn[k]) / g.dx[k];
+}
+
+template <bool kPow2>
+__device__ __forceinline__ float solid_phi_in(const float* __restrict__ sp, const GridP& g, const float* inv, float x,
+                                              float y, float z) {
+  float lx = to_grid<kPow2>(g, inv, 0, x);
+  float ly = to_grid<kPow2>(g, inv, 1, y);
+  float lz = to_grid<kPow2>(g, inv, 2, z);
   int xi, yi, zi;
   float fx, fy, fz;
   barycentric(lx, 0, g.N[0] + 1, xi, fx);
@@ -677,103 +687,148 @@ __device__ void grad_from_samples(float left, float right, float bottom, float t
   out[1] = b;
   out[2] = c;
 }
-__device__ void grad_inner(const float* sp, const GridP& g, float x, float y, float z, float out[3]) {
+template <bool kPow2>
+__device__ void grad_inner(const float* sp, const GridP& g, const float* inv, float x, float y, float z,
+                           float out[3]) {
   const float eps = 1e-4f;
-  grad_from_samples(solid_phi_in(sp, g, x - eps, y, z), solid_phi_in(sp, g, x + eps, y, z),
-                    solid_phi_in(sp, g, x, y - eps, z), solid_phi_in(sp, g, x, y + eps, z),
-                    solid_phi_in(sp, g, x, y, z + eps), solid_phi_in(sp, g, x, y, z - eps), out);
+  grad_from_samples(solid_phi_in<kPow2>(sp, g, inv, x - eps, y, z), solid_phi_in<kPow2>(sp, g, inv, x + eps, y, z),
+                    solid_phi_in<kPow2>(sp, g, inv, x, y - eps, z), solid_phi_in<kPow2>(sp, g, inv, x, y + eps, z),
+                    solid_phi_in<kPow2>(sp, g, inv, x, y, z + eps), solid_phi_in<kPow2>(sp, g, inv, x, y, z - eps),
+                    out);
 }
 // Grid<3>::GetSolidPhi (Grid.cpp:128-155) with the out-of-grid extrapolation
-__device__ float solid_phi_any(const float* sp, const GridP& g, const float mx[3], float x, float y, float z) {
-  if (in_grid(g, mx, x, y, z)) return solid_phi_in(sp, g, x, y, z);
+template <bool kPow2 = false>
+__device__ float solid_phi_any(const float* sp, const GridP& g, const float mx[3], float x, float y, float z,
+                               const float* inv = nullptr) {
+  if (in_grid(g, mx, x, y, z)) return solid_phi_in<kPow2>(sp, g, inv, x, y, z);
   float vx = clampf(x, g.mn[0] + g.dx[0], mx[0] - g.dx[0]);
   float vy = clampf(y, g.mn[1] + g.dx[1], mx[1] - g.dx[1]);
   float vz = clampf(z, g.mn[2] + g.dx[2], mx[2] - g.dx[2]);
   float gr[3];
-  grad_inner(sp, g, vx, vy, vz, gr);
-  return solid_phi_in(sp, g, vx, vy, vz) + gr[0] * (x - vx) + gr[1] * (y - vy) + gr[2] * (z - vz);
+  grad_inner<kPow2>(sp, g, inv, vx, vy, vz, gr);
+  return solid_phi_in<kPow2>(sp, g, inv, vx, vy, vz) + gr[0] * (x - vx) + gr[1] * (y - vy) + gr[2] * (z - vz);
 }
 // Grid<3>::GetSolidGradient (Grid.cpp:175-192)
+template <bool kPow2 = false>
 __device__ void solid_grad_any(const float* sp, const GridP& g, const float mx[3], float x, float y, float z,
-                               float out[3]) {
+                               float out[3], const float* inv = nullptr) {
   if (!in_grid(g, mx, x, y, z)) {
     float vx = clampf(x, g.mn[0] + g.dx[0], mx[0] - g.dx[0]);
     float vy = clampf(y, g.mn[1] + g.dx[1], mx[1] - g.dx[1]);
     float vz = clampf(z, g.mn[2] + g.dx[2], mx[2] - g.dx[2]);
-    grad_inner(sp, g, vx, vy, vz, out);
+    grad_inner<kPow2>(sp, g, inv, vx, vy, vz, out);
     return;
   }
   const float eps = 1e-4f;
-  grad_from_samples(solid_phi_any(sp, g, mx, x - eps, y, z), solid_phi_any(sp, g, mx, x + eps, y, z),
-                    solid_phi_any(sp, g, mx, x, y - eps, z), solid_phi_any(sp, g, mx, x, y + eps, z),
-                    solid_phi_any(sp, g, mx, x, y, z + eps), solid_phi_any(sp, g, mx, x, y, z - eps), out);
+  grad_from_samples(solid_phi_any<kPow2>(sp, g, mx, x - eps, y, z, inv),
+                    solid_phi_any<kPow2>(sp, g, mx, x + eps, y, z, inv),
+                    solid_phi_any<kPow2>(sp, g, mx, x, y - eps, z, inv),
+                    solid_phi_any<kPow2>(sp, g, mx, x, y + eps, z, inv),
+                    solid_phi_any<kPow2>(sp, g, mx, x, y, z + eps, inv),
+                    solid_phi_any<kPow2>(sp, g, mx, x, y, z - eps, inv), out);
 }
 
-__global__ void __launch_bounds__(256) k_g2p_advect(float* __restrict__ pb, int64_t cap, int64_t P, const float* __restrict__ vx,
-                             const float* __restrict__ vy, const float* __restrict__ vz,
-                             const float* __restrict__ sp, GP q, float mx0, float mx1, float mx2, int do_g2p,
-                             int do_advect, unsigned* __restrict__ maxbits) {
+// The G2P taps of one axis: the nodes j in [0, dim) whose centre c (j on the component's own
+// axis, j + 0.5 on the other two) has a nonzero B-spline weight, |g - c| < 1.5.  They are
+// consecutive and at most 3.  With b = floor(g) - 1 they lie among b .. b + 3 (integer centres;
+// b + 3 only when b has weight zero) or b .. b + 2 (half-integer centres); the reference scans
+// ceil(radius / dx) >= 2 nodes either side of (int)g, a window that holds every one of them.
+// Out: the first tap j0 and, for taps j0 .. j0 + 2, the weight and unsigned gradient computed
+// from d = g - c exactly as the reference does, zero for a tap outside [0, dim) or of weight
+// zero.  A zero weight adds exactly nothing to the sums (velocities are finite and a running
+// sum that starts at +0 is never -0), so the sums over the 27 taps equal the reference's over
+// the nonzero taps.
+template <bool kHalf>
+__device__ __forceinline__ void g2p_axis(float g, int dim, int& j0, float w[3], float gw[3]) {
+  constexpr int NC = kHalf ? 3 : 4;
+  const float fb = floorf(g) - 1.0f;
+  const int b = (int)fb;
+  float d[NC], wc[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    const float cc = kHalf ? (fb + (float)m) + 0.5f : fb + (float)m;  // (float)j (+ 0.5f), exactly
+    d[m] = g - cc;
+    wc[m] = quad_w(d[m]);
+  }
+  // integer centres: the taps start at b + 1 when b has weight zero
+  const bool sh = !kHalf && wc[0] == 0.f;
+  j0 = sh ? b + 1 : b;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    float wm = wc[m], dm = d[m];
+    if (!kHalf) {
+      wm = sh ? wc[m + 1] : wm;
+      dm = sh ? d[m + 1] : dm;
+    }
+    const bool in = (unsigned)(j0 + m) < (unsigned)dim;
+    w[m] = in ? wm : 0.f;
+    gw[m] = in ? quad_gw(dm) : 0.f;
+  }
+}
+
+__device__ __forceinline__ float ld_at(const float* __restrict__ base, unsigned byte_off) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+// Grid2Particle (LiquidModel.cpp:724-822) fused with AdvectParticle and the max |v| of GetCFL.
+// Each axis's taps of both centre kinds are found once and shared by the three components.
+template <bool kPow2>
+__global__ void __launch_bounds__(256) k_g2p_advect(float* __restrict__ pb, int64_t cap, int64_t P,
+                                                     const float* __restrict__ vx, const float* __restrict__ vy,
+                                                     const float* __restrict__ vz, const float* __restrict__ sp, GP q,
+                                                     float mx0, float mx1, float mx2, int do_g2p, int do_advect,
+                                                     unsigned* __restrict__ maxbits) {
   const float mx[3] = {mx0, mx1, mx2};
   float local_max = 0.f;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
     float pos[3] = {pb[(size_t)PA_X * cap + i], pb[(size_t)(PA_X + 1) * cap + i], pb[(size_t)(PA_X + 2) * cap + i]};
     float vel[3];
     if (do_g2p) {
-      float g[3];
-      int ci[3];
+      // per axis k: taps of the integer centres (the face axis, N_k + 1 nodes) and of the
+      // half-integer centres (N_k nodes)
+      int ji[3], jh[3];
+      float wi[3][3], gi[3][3], wh[3][3], gh[3][3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        g[k] = (pos[k] - q.g.mn[k]) / q.g.dx[k];
-        ci[k] = (int)g[k];
+        const float gk = to_grid<kPow2>(q.g, q.inv, k, pos[k]);
+        g2p_axis<false>(gk, q.g.N[k] + 1, ji[k], wi[k], gi[k]);
+        g2p_axis<true>(gk, q.g.N[k], jh[k], wh[k], gh[k]);
       }
       float Cm[9];
-#pragma unroll
-      for (int k = 0; k < 9; ++k) Cm[k] = 0.f;
       const float* V[3] = {vx, vy, vz};
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        int dims[3] = {q.g.N[0], q.g.N[1], q.g.N[2]};
-        dims[a] += 1;
-        // per axis: up to 3 taps with nonzero weight, in ascending index order; the taps with
-        // nonzero weight inside the grid are consecutive indices j0, j0 + 1, ...
+        const int d1 = q.g.N[1] + (a == 1), d2 = q.g.N[2] + (a == 2);
+        const int dims[3] = {q.g.N[0] + (a == 0), d1, d2};
+        // this component's taps: integer centres on axis a, half-integer on the others
         int j0[3];
-        float w[3][3], gw[3][3];
-        int nt[3];
+        const float* w[3];
+        const float* gw[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          nt[k] = 0;
-          j0[k] = 0;
-          for (int t = -q.dW[k]; t <= q.dW[k]; ++t) {
-            int j = ci[k] + t;
-            if (j < 0 || j > dims[k] - 1) continue;
-            float cc = (k == a) ? (float)j : (float)j + 0.5f;
-            if (quad_w(g[k] - cc) == 0.f || nt[k] >= 3) continue;
-            if (nt[k] == 0) j0[k] = j;
-            nt[k]++;
-          }
-          // the taps' weights (the same operations as in the scan)
-#pragma unroll
-          for (int m = 0; m < 3; ++m) {
-            const int j = j0[k] + m;
-            const float d = g[k] - ((k == a) ? (float)j : (float)j + 0.5f);
-            w[k][m] = quad_w(d);
-            gw[k][m] = quad_gw(d);
-          }
+          j0[k] = (k == a) ? ji[k] : jh[k];
+          w[k] = (k == a) ? wi[k] : wh[k];
+          gw[k] = (k == a) ? gi[k] : gh[k];
         }
-        // all 27 loads in flight (taps past nt read the first tap and are not summed), then
-        // the sums in tap order
+        // byte offsets of the 27 taps; a tap outside the grid (weight zero) reads the nearest node
+        unsigned ox[3], oy[3], oz[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          ox[m] = (unsigned)min(max(j0[0] + m, 0), dims[0] - 1) * ((unsigned)d1 * (unsigned)d2 * 4u);
+          oy[m] = (unsigned)min(max(j0[1] + m, 0), dims[1] - 1) * ((unsigned)d2 * 4u);
+          oz[m] = (unsigned)min(max(j0[2] + m, 0), dims[2] - 1) * 4u;
+        }
+        // all 27 loads in flight, then the sums in the reference's tap order (x, y, z ascending)
         const float* Va = V[a];
         float fv[3][3][3];
 #pragma unroll
         for (int ix = 0; ix < 3; ++ix)
 #pragma unroll
-          for (int iy = 0; iy < 3; ++iy)
+          for (int iy = 0; iy < 3; ++iy) {
+            const unsigned oxy = ox[ix] + oy[iy];
 #pragma unroll
-            for (int iz = 0; iz < 3; ++iz) {
-              const int jx = j0[0] + (ix < nt[0] ? ix : 0), jy = j0[1] + (iy < nt[1] ? iy : 0),
-                        jz = j0[2] + (iz < nt[2] ? iz : 0);
-              fv[ix][iy][iz] = Va[((size_t)jx * dims[1] + jy) * dims[2] + jz];
-            }
+            for (int iz = 0; iz < 3; ++iz) fv[ix][iy][iz] = ld_at(Va, oxy + oz[iz]);
+          }
         float acc = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
 #pragma unroll
         for (int ix = 0; ix < 3; ++ix)
@@ -781,16 +836,15 @@ __global__ void __launch_bounds__(256) k_g2p_advect(float* __restrict__ pb, int6
           for (int iy = 0; iy < 3; ++iy)
 #pragma unroll
             for (int iz = 0; iz < 3; ++iz) {
-              const bool on = ix < nt[0] && iy < nt[1] && iz < nt[2];
-              float wX = w[0][ix], wY = w[1][iy], wZ = w[2][iz];
-              float gX = gw[0][ix], gY = gw[1][iy], gZ = gw[2][iz];
-              float weight = wX * wY * wZ;
-              float g0 = gX * wY * wZ, g1 = wX * gY * wZ, g2 = wX * wY * gZ;
+              const float wX = w[0][ix], wY = w[1][iy], wZ = w[2][iz];
+              const float gX = gw[0][ix], gY = gw[1][iy], gZ = gw[2][iz];
+              const float weight = wX * wY * wZ;
+              const float g0 = gX * wY * wZ, g1 = wX * gY * wZ, g2 = wX * wY * gZ;
               const float f = fv[ix][iy][iz];
-              acc = on ? acc + f * weight : acc;
-              c0 = on ? c0 + f * g0 : c0;
-              c1 = on ? c1 + f * g1 : c1;
-              c2 = on ? c2 + f * g2 : c2;
+              acc = acc + f * weight;
+              c0 = c0 + f * g0;
+              c1 = c1 + f * g1;
+              c2 = c2 + f * g2;
             }
         vel[a] = acc;
         Cm[3 * a + 0] = c0;
@@ -809,10 +863,10 @@ __global__ void __launch_bounds__(256) k_g2p_advect(float* __restrict__ pb, int6
     local_max = (local_max > m3) ? local_max : m3;
     if (do_advect) {
       float p0 = pos[0] + vel[0] * q.dt, p1 = pos[1] + vel[1] * q.dt, p2 = pos[2] + vel[2] * q.dt;
-      float s = solid_phi_any(sp, q.g, mx, p0, p1, p2);
+      float s = solid_phi_any<kPow2>(sp, q.g, mx, p0, p1, p2, q.inv);
       if (s < 0.0f) {
         float gr[3];
-        solid_grad_any(sp, q.g, mx, p0, p1, p2, gr);
+        solid_grad_any<kPow2>(sp, q.g, mx, p0, p1, p2, gr, q.inv);
         float sh = s + kEps;
         p0 -= sh * gr[0];
         p1 -= sh * gr[1];
@@ -1057,6 +1111,18 @@ void launch_liquid_phi(Ctx& c) {
                      c.H[0], c.H[1], c.H[2], c.dW[0], c.dW[1], c.dW[2], c.radius, 3 * c.g.dx[0]);
 }
 
+// dx = 2^-k on every axis: (x - min) / dx == (x - min) * 2^k exactly; inv receives 2^k
+static bool pow2_inverse(const GridP& g, float inv[3]) {
+  bool pow2 = true;
+  for (int k = 0; k < 3; ++k) {
+    int e = 0;
+    const float m = std::frexp(g.dx[k], &e);
+    inv[k] = std::ldexp(1.0f, 1 - e);
+    pow2 = pow2 && m == 0.5f && std::isnormal(inv[k]) && g.dx[k] * inv[k] == 1.0f;
+  }
+  return pow2;
+}
+
 void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
   if (c.P == 0 || !c.bins_valid) return;
   P2GP q;
@@ -1073,14 +1139,7 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
       q.tank_axis = c.cfg.tank_axis;
     }
   }
-  // dx = 2^-k on every axis: (x - min) / dx == (x - min) * 2^k exactly
-  bool pow2 = true;
-  for (int k = 0; k < 3; ++k) {
-    int e = 0;
-    const float m = std::frexp(c.g.dx[k], &e);
-    q.inv[k] = std::ldexp(1.0f, 1 - e);
-    pow2 = pow2 && m == 0.5f && std::isnormal(q.inv[k]) && c.g.dx[k] * q.inv[k] == 1.0f;
-  }
+  const bool pow2 = pow2_inverse(c.g, q.inv);
   if (!fuse_body_forces) q.gdt = 0.f;
   ProfScope ps(c, LH_PROF_P2G);
   const unsigned nt = tile_grid(div_up(c.g.N[0] + 1, kMX) * div_up(c.g.N[1] + 1, kPTY) * div_up(c.g.N[2] + 1, kPTZ));
@@ -1146,12 +1205,18 @@ void launch_g2p_advect(Ctx& c, bool do_g2p, bool do_advect, float dt) {
   GP q;
   q.g = c.g;
   q.dt = dt;
-  for (int k = 0; k < 3; ++k) q.dW[k] = c.dW[k];
+  const bool pow2 = pow2_inverse(c.g, q.inv);
   unsigned gp = grid_for((size_t)c.P);
   ProfScope ps(c, LH_PROF_G2P);
-  hipLaunchKernelGGL(k_g2p_advect, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, c.v[0], c.v[1], c.v[2],
-                     c.solid_phi, q, c.cfg.mx[0], c.cfg.mx[1], c.cfg.mx[2], do_g2p ? 1 : 0, do_advect ? 1 : 0,
-                     c.dmaxv);
+  // the face arrays' byte offsets fit 32 bits: grids are under 2^31 half-cell bins (N < 645 per axis)
+  if (pow2)
+    hipLaunchKernelGGL(k_g2p_advect<true>, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, c.v[0], c.v[1],
+                       c.v[2], c.solid_phi, q, c.cfg.mx[0], c.cfg.mx[1], c.cfg.mx[2], do_g2p ? 1 : 0,
+                       do_advect ? 1 : 0, c.dmaxv);
+  else
+    hipLaunchKernelGGL(k_g2p_advect<false>, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, c.v[0], c.v[1],
+                       c.v[2], c.solid_phi, q, c.cfg.mx[0], c.cfg.mx[1], c.cfg.mx[2], do_g2p ? 1 : 0,
+                       do_advect ? 1 : 0, c.dmaxv);
 }
 
 }  // namespace lh

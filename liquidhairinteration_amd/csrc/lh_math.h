@@ -13,18 +13,25 @@ namespace lh {
 
 // Kernel.h:12-17 (quadratic B-spline weight)
 // Both pieces are evaluated and one is selected: the same operations as the branches, without
-// the divergent branch (and its exec-mask bookkeeping) per call in the particle loops.
+// the divergent branch (and its exec-mask bookkeeping) per call in the particle loops.  1.5 - d
+// is positive exactly when d < 1.5, so clamping it at +0 gives the outer piece below 1.5 and
+// +0 from 1.5 on, as the reference's third branch does.  The empty asm keeps the compiler from
+// sinking either piece back into a branch.
 LH_HD float quad_w(float d) {
   d = fabsf(d);
-  const float inner = 0.75f - d * d;
-  const float t = 1.5f - d;
-  const float outer = 0.5f * t * t;
-  return d < 0.5f ? inner : (d < 1.5f ? outer : 0.0f);
+  float inner = 0.75f - d * d;
+  const float t = fmaxf(1.5f - d, 0.0f);
+  float outer = 0.5f * t * t;
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(inner), "+v"(outer));
+#endif
+  return d < 0.5f ? inner : outer;
 }
-// Kernel.h:26-31 (derivative w.r.t. |d|, unsigned as in the reference)
+// Kernel.h:26-31 (derivative w.r.t. |d|, unsigned as in the reference); d - 1.5 is negative
+// exactly when d < 1.5
 LH_HD float quad_gw(float d) {
   d = fabsf(d);
-  return d < 0.5f ? -2.f * d : (d < 1.5f ? d - 1.5f : 0.f);
+  return d < 0.5f ? -2.f * d : fminf(d - 1.5f, 0.0f);
 }
 
 constexpr float kEps = 1e-8f;  // Definition.h:93 EPSILON
