@@ -604,29 +604,39 @@ __global__ void __launch_bounds__(256) k_liquid_phi(float* __restrict__ phi, con
       }
     }
     // the other rows, pruned by the distance bound against the running minimum (it only
-    // decreases, so a row pruned once stays prunable); per x half-cell, the starts of up to 10
-    // rows are loaded together
+    // decreases, so a row pruned once stays prunable): a particle lowers the minimum only within
+    // lim of the cell centre, so half-slabs and rows farther than lim in x, (x, y) are skipped and
+    // a row's z half-cells are cut to those within sqrt(lim^2 - gx^2 - gy^2) in z.  Per x
+    // half-cell, the bounds of up to 10 rows are formed first, and their starts loaded together
+    // only when one of them is in reach (for a cell inside the liquid none is).
     constexpr int kRowBatch = 10;
     for (int hx = hx0; hx <= hx1; ++hx) {
       const float gx = fmaxf(0.f, fmaxf(0.5f * hx - (xi + 0.5f), (xi + 0.5f) - 0.5f * (hx + 1))) * g.dx[0];
+      const float lim = best + margin + radius;
+      if (gx > lim) continue;
       for (int hyb = hy0; hyb <= hy1; hyb += kRowBatch) {
-        int S0[kRowBatch], S1[kRowBatch];
-        // a particle lowers the minimum only within lim of the cell centre: rows farther than lim
-        // in (x, y) are skipped, and a row's z half-cells are cut to those within
-        // sqrt(lim^2 - gx^2 - gy^2) in z
-        const float lim = best + margin + radius;
+        float rem[kRowBatch];
+        bool use[kRowBatch];
+        bool any_row = false;
 #pragma unroll
         for (int j = 0; j < kRowBatch; ++j) {
           const int hy = hyb + j;
           const float gy = fmaxf(0.f, fmaxf(0.5f * hy - (yi + 0.5f), (yi + 0.5f) - 0.5f * (hy + 1))) * g.dx[1];
           const bool own = (hx >> 1) == xi && (hy >> 1) == yi;
-          const float rem = lim * lim - gx * gx - gy * gy;
-          const bool use = hy <= hy1 && !own && rem >= 0.f;
-          const int w2 = (int)floorf(2.f * sqrtf(fmaxf(rem, 0.f)) / g.dx[2]);
+          rem[j] = lim * lim - gx * gx - gy * gy;
+          use[j] = hy <= hy1 && !own && rem[j] >= 0.f;
+          any_row = any_row || use[j];
+        }
+        if (!any_row) continue;
+        int S0[kRowBatch], S1[kRowBatch];
+#pragma unroll
+        for (int j = 0; j < kRowBatch; ++j) {
+          const int hy = hyb + j;
+          const int w2 = (int)floorf(2.f * sqrtf(fmaxf(rem[j], 0.f)) / g.dx[2]);
           const int lo = max(hz0, 2 * zi - w2), hi = min(hz1, 2 * zi + 1 + w2);
           const size_t k0 = ((size_t)hx * H1 + min(hy, hy1)) * H2;
-          S0[j] = use ? start[k0 + lo] : 0;
-          S1[j] = use ? start[k0 + hi + 1] : 0;
+          S0[j] = use[j] ? start[k0 + lo] : 0;
+          S1[j] = use[j] ? start[k0 + hi + 1] : 0;
         }
 #pragma unroll
         for (int j = 0; j < kRowBatch; ++j) run(S0[j], S1[j]);
