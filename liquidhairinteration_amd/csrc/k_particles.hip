@@ -219,7 +219,7 @@ struct P2GP {
 // that no particle can reach writes its closed-form value and leaves; the test
 // reads the two end offsets of every (hx, hy) bin row of the tile's reach.
 constexpr int kTX = 1, kTY = 4, kTZ = 64;
-// P2G tiles: kMX x kPTY x kPTZ faces, one (y, z) row of kMX x-faces per thread
+// P2G tiles: MX x kPTY x kPTZ faces, one (y, z) row of MX x-faces per thread
 constexpr int kPTY = 4, kPTZ = 64;
 
 __device__ bool tile_has_particles(const int* __restrict__ start, const int lo[3], const int hi[3], int H1, int H2) {
@@ -271,8 +271,8 @@ __device__ __forceinline__ bool tile_coords(int d0, int d1, int d2, int c[3], in
 // positions liquid phi reads, 16 bytes apart) and R1 = (C.col(0), v_y), R2 = (C.col(1), v_z),
 // R3 = (C.col(2), 0) into a second (48 bytes per particle).
 //
-// A 256-thread workgroup owns kMX x positions x 4 y x 64 z of all three face grids: each thread
-// one (y, z) column with its x-, y- and z-faces at x = c0 .. c0 + kMX - 1 (those that exist: the
+// A 256-thread workgroup owns MX x positions x 4 y x 64 z of all three face grids: each thread
+// one (y, z) column with its x-, y- and z-faces at x = c0 .. c0 + MX - 1 (those that exist: the
 // x-face grid is (N0 + 1) x N1 x N2, and so on).  A face of component a is reached by the
 // particles of the half-cells [2c - 3, 2c + 2] on axis a and [2c - 2, 2c + 3] on the other axes
 // (every particle outside has weight exactly zero), so the union [2c - 3, 2c + 3] per axis holds
@@ -284,9 +284,12 @@ __device__ __forceinline__ bool tile_coords(int d0, int d1, int d2, int c[3], in
 // the two face offsets (cell centre and cell face) and the x weights of the y- and z-faces.  A
 // face receives its nonzero contributions in ascending slabs and, within a slab, its y columns
 // and z bins in order, and zero weights add nothing: the bin order the oracle sums in, so the
-// result is bit-identical to one component per launch.  At 256^3 (6.5 M particles) 4 x
-// positions per tile balance the shared work against the number of busy waves.
-constexpr int kMX = 4;
+// result is bit-identical to one component per launch.  A thread reads 2 MX + 6 slabs for its MX
+// x positions, so a longer row re-reads each slab fewer times but leaves fewer busy waves: 4 x
+// positions suit liquid that fills a few percent of the box (the 256^3 collision, 0.4 particles
+// per cell: 1.54 ms against 1.60 / 1.64 ms with 8 / 12), 12 a grid whose liquid fills its tiles
+// (the 256^3 sloshing tank, 2.9 particles per cell: 5.62 ms against 6.87 ms with 4).
+constexpr int kMXSparse = 4, kMXDense = 12;
 
 // record index of the t-th particle of a thread's run of up to 7 bin columns: column j holds
 // the t in [cum[j-1], cum[j]), at records t + off[j]
@@ -370,15 +373,15 @@ __device__ __forceinline__ void p2g_slab(const float4* __restrict__ pos, const f
 // particles of the half-cell bins around it in bin order.  Within a slab the (up to 7) y
 // columns of a thread's union window are concatenated into one run, so a wave's lanes wait for
 // the longest run of 7 columns instead of the longest of each column.
-template <bool kPow2>
+template <bool kPow2, int MX>
 __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
                                               const float4* __restrict__ pos, const float4* __restrict__ rest,
                                               const int* __restrict__ start, P2GP q) {
   const int N0 = q.g.N[0], N1 = q.g.N[1], N2 = q.g.N[2];
   int c[3], c0[3];
   // tiles over the union of the three face grids, (N0 + 1) x (N1 + 1) x (N2 + 1)
-  if (!tile_coords<kMX, kPTY, kPTZ>(N0 + 1, N1 + 1, N2 + 1, c, c0)) return;
-  const int T[3] = {kMX, kPTY, kPTZ};
+  if (!tile_coords<MX, kPTY, kPTZ>(N0 + 1, N1 + 1, N2 + 1, c, c0)) return;
+  const int T[3] = {MX, kPTY, kPTZ};
   int blo[3], bhi[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -394,7 +397,7 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vx, float* __re
   // the faces of this column: x-faces (x <= N0, y < N1, z < N2), y-faces (x < N0, y <= N1,
   // z < N2), z-faces (x < N0, y < N1, z <= N2)
   const bool has[3] = {c[1] < N1 && c[2] < N2, c[1] <= N1 && c[2] < N2, c[1] < N1 && c[2] <= N2};
-  const int xend0 = min(c0[0] + kMX, N0 + 1), xend12 = min(c0[0] + kMX, N0);
+  const int xend0 = min(c0[0] + MX, N0 + 1), xend12 = min(c0[0] + MX, N0);
   float* const vout[3] = {vx, vy, vz};
   auto finish = [&](int a, int x, float ws, float vs) {
     if (!has[a]) return;
@@ -1152,13 +1155,24 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
   const bool pow2 = pow2_inverse(c.g, q.inv);
   if (!fuse_body_forces) q.gdt = 0.f;
   ProfScope ps(c, LH_PROF_P2G);
-  const unsigned nt = tile_grid(div_up(c.g.N[0] + 1, kMX) * div_up(c.g.N[1] + 1, kPTY) * div_up(c.g.N[2] + 1, kPTZ));
-  if (pow2)
-    hipLaunchKernelGGL(k_p2g<true>, dim3(nt), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], (const float4*)c.ppack,
-                       (const float4*)c.ppack + c.cap, c.bin_start, q);
+  // at least one particle per cell: the liquid fills its tiles, and longer rows pay
+  const bool dense = (size_t)c.P >= c.ncell;
+  const int mx = dense ? kMXDense : kMXSparse;
+  const unsigned nt = tile_grid(div_up(c.g.N[0] + 1, mx) * div_up(c.g.N[1] + 1, kPTY) * div_up(c.g.N[2] + 1, kPTZ));
+  const float4* pos = (const float4*)c.ppack;
+  const float4* rest = (const float4*)c.ppack + c.cap;
+  if (pow2 && dense)
+    hipLaunchKernelGGL((k_p2g<true, kMXDense>), dim3(nt), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], pos, rest,
+                       c.bin_start, q);
+  else if (pow2)
+    hipLaunchKernelGGL((k_p2g<true, kMXSparse>), dim3(nt), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], pos, rest,
+                       c.bin_start, q);
+  else if (dense)
+    hipLaunchKernelGGL((k_p2g<false, kMXDense>), dim3(nt), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], pos, rest,
+                       c.bin_start, q);
   else
-    hipLaunchKernelGGL(k_p2g<false>, dim3(nt), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], (const float4*)c.ppack,
-                       (const float4*)c.ppack + c.cap, c.bin_start, q);
+    hipLaunchKernelGGL((k_p2g<false, kMXSparse>), dim3(nt), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], pos, rest,
+                       c.bin_start, q);
 }
 
 int run_correct_volume(Ctx& c, float dt) {
