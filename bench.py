@@ -400,6 +400,8 @@ def main():
                     help="start every pressure solve of the timed workload from the learned guess")
     ap.add_argument("--coarse-sweep", choices=("config", "rows", "tiles", "passes"), default="config",
                     help="override SolverParameter.coarseSweep (how the coarse levels are swept)")
+    ap.add_argument("--pcg-graph", choices=("config", "on", "off"), default="config",
+                    help="SolverParameter.pcgGraph: each PCG iteration launched as one hipGraph")
     ap.add_argument("--smoother", choices=("config", "reference"), default="config",
                     help="reference: the reference's Q4/Q5 smoother with its own solver parameters "
                          "(Config/PureLiquid3D.json: 4 levels, 5/5/20 sweeps, omega 0.5) in place of the config's")
@@ -441,6 +443,8 @@ def main():
         workload += "_reference_smoother"
     if args.coarse_sweep != "config":
         cfg["SolverParameter"]["coarseSweep"] = args.coarse_sweep
+    if args.pcg_graph != "config":
+        cfg["SolverParameter"]["pcgGraph"] = args.pcg_graph == "on"
     scene = lh.Scene(cfg)
     model = scene.GetLiquidModel()
     model.Init()

@@ -611,9 +611,14 @@ __global__ void __launch_bounds__(256) k_liquid_phi(float* __restrict__ phi, con
     // lim of the cell centre, so half-slabs and rows farther than lim in x, (x, y) are skipped and
     // a row's z half-cells are cut to those within sqrt(lim^2 - gx^2 - gy^2) in z.  Per x
     // half-cell, the bounds of up to 10 rows are formed first, and their starts loaded together
-    // only when one of them is in reach (for a cell inside the liquid none is).
+    // only when one of them is in reach (for a cell inside the liquid none is).  The half-slabs
+    // are visited from the cell's own two outwards, so the nearest particles tighten the bound
+    // first.
     constexpr int kRowBatch = 10;
-    for (int hx = hx0; hx <= hx1; ++hx) {
+    for (int m = 0; m < 2 * (hx1 - hx0 + 1); ++m) {
+      // centre out: 2xi, 2xi + 1, 2xi - 1, 2xi + 2, ...
+      const int hx = (m & 1) ? 2 * xi + 1 + (m >> 1) : 2 * xi - (m >> 1);
+      if (hx < hx0 || hx > hx1) continue;
       const float gx = fmaxf(0.f, fmaxf(0.5f * hx - (xi + 0.5f), (xi + 0.5f) - 0.5f * (hx + 1))) * g.dx[0];
       const float lim = best + margin + radius;
       if (gx > lim) continue;

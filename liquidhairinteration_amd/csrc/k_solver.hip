@@ -1812,6 +1812,86 @@ void build_system(Ctx& c, float rho, float dt) {
 // guess x0 (device).  With x0, the PCG starts from x0 restricted to the cells it
 // updates and falls back to x = 0 when ||b - A x0|| > ||b||.  The stop rule is
 // the configured one; in relative mode it is ||r|| <= tol * ||b||.  Result in c.x.
+namespace {
+
+// One PCG iteration up to the r.z the host waits for (PressureSolver.cpp:879-906): x/r update,
+// preconditioner, r.z.  part_pq / part_rr: the p.q and r.r partials in c.partial.
+void pcg_iteration_launches(Ctx& c, int slot, double* part_pq, double* part_rr) {
+  Level& L0 = c.lv[0];
+  const int nb = c.npartial;
+  const int sym = c.cfg.precond == LH_PRECOND_MULTIGRID && c.cfg.smoother == LH_SMOOTHER_SYMMETRIC;
+  hipLaunchKernelGGL(k_cg_update_xr, dim3(nb), dim3(kBlock), 0, c.s, c.x, L0.r, c.p, c.q, L0.cls, c.dsc, slot,
+                     c.dflag + 2, part_pq, nb, L0.d[0], L0.d[1], L0.d[2], L0.rows, L0.nrows, part_rr, L0.z, sym);
+  apply_precond(c, sym != 0);
+  hipLaunchKernelGGL(k_cg_dot_rz, dim3(nb), dim3(kBlock), 0, c.s, L0.r, L0.z, L0.rows, L0.nrows, L0.d[2], c.partial,
+                     part_rr, nb, c.dsc, c.dflag + 2, c.h_pin_dev, c.h_flag_dev);
+}
+
+// the levels' ping-pong buffers, which the preconditioner's host code swaps as it launches
+std::vector<void*> level_buffers(const Ctx& c) {
+  std::vector<void*> v;
+  for (const Level& L : c.lv) {
+    v.push_back(L.z);
+    v.push_back(L.z2);
+    v.push_back(L.z64);
+    v.push_back(L.z64b);
+    v.push_back(L.d64);
+    v.push_back(L.d64b);
+  }
+  return v;
+}
+void set_level_buffers(Ctx& c, const std::vector<void*>& v) {
+  size_t k = 0;
+  for (Level& L : c.lv) {
+    L.z = (float*)v[k++];
+    L.z2 = (float*)v[k++];
+    L.z64 = (double*)v[k++];
+    L.z64b = (double*)v[k++];
+    L.d64 = (double*)v[k++];
+    L.d64b = (double*)v[k++];
+  }
+}
+
+// pcg_iteration_launches as an instantiated hipGraph (pcgGraph).  Every argument of those launches
+// is fixed for the context's lifetime except the p buffer (p ping-pongs with p2), the rho slot and
+// the levels' ping-pong buffers, so a graph is captured once per combination of them and replayed
+// in every later solve; the grids are sized from the row lists' capacities and the kernels read
+// the lists and their lengths on the device.  The captured launches contain no memset (this
+// runtime has given wrong results on replays of memset nodes) and no event (profiling uses the
+// direct launches).
+bool pcg_iteration_graph(Ctx& c, int slot, double* part_pq, double* part_rr) {
+  const std::vector<void*> before = level_buffers(c);
+  for (const Ctx::PcgGraph& g : c.pcg_graphs)
+    if (g.p == c.p && g.slot == slot && g.before == before) {
+      if (!hip_ok(c, hipGraphLaunch(g.exec, c.s), "PCG iteration graph launch")) return false;
+      set_level_buffers(c, g.after);
+      return true;
+    }
+  if (!hip_ok(c, hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal), "PCG iteration capture")) return false;
+  pcg_iteration_launches(c, slot, part_pq, part_rr);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(c.s, &graph);
+  const std::vector<void*> after = level_buffers(c);
+  set_level_buffers(c, before);  // the captured launches have not run yet
+  if (!hip_ok(c, e, "PCG iteration capture")) return false;
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (!hip_ok(c, ei, "PCG iteration graph instantiation")) return false;
+  Ctx::PcgGraph g;
+  g.exec = exec;
+  g.p = c.p;
+  g.slot = slot;
+  g.before = before;
+  g.after = after;
+  c.pcg_graphs.push_back(g);
+  if (!hip_ok(c, hipGraphLaunch(exec, c.s), "PCG iteration graph launch")) return false;
+  set_level_buffers(c, after);
+  return true;
+}
+
+}  // namespace
+
 int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, int max_iter, lh_solve_stats& S) {
   Level& L0 = c.lv[0];
   const size_t n = L0.n;
@@ -1869,13 +1949,12 @@ int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, 
       break;
     }
     // q = A p of this iteration is already queued
-    const int sym = c.cfg.precond == LH_PRECOND_MULTIGRID && c.cfg.smoother == LH_SMOOTHER_SYMMETRIC;
     const int slot = it & 1;
-    hipLaunchKernelGGL(k_cg_update_xr, dim3(nb), dim3(kBlock), 0, c.s, c.x, L0.r, c.p, c.q, L0.cls, c.dsc, slot,
-                       c.dflag + 2, part_pq, nb, L0.d[0], L0.d[1], L0.d[2], L0.rows, L0.nrows, part_rr, L0.z, sym);
-    apply_precond(c, sym != 0);
-    hipLaunchKernelGGL(k_cg_dot_rz, dim3(nb), dim3(kBlock), 0, c.s, L0.r, L0.z, L0.rows, L0.nrows, L0.d[2], c.partial,
-                       part_rr, nb, c.dsc, c.dflag + 2, c.h_pin_dev, c.h_flag_dev);
+    if (c.cfg.pcg_graph && !c.prof) {
+      if (!pcg_iteration_graph(c, slot, part_pq, part_rr)) return LH_ERR_DEVICE;
+    } else {
+      pcg_iteration_launches(c, slot, part_pq, part_rr);
+    }
     hipEventRecord(c.ev_iter, c.s);
     {  // p = z + beta p and the next iteration's q = A p, while the host waits for the event and decides
       ProfScope ps(c, LH_PROF_SPMV);

@@ -159,6 +159,7 @@ Config parse_config(const std::string& text) {
   } else if (csol != "sweeps") {
     throw std::runtime_error("Unknown coarsestSolver " + csol);
   }
+  c.pcg_graph = (s.has("pcgGraph") && s.at("pcgGraph").as_bool()) ? 1 : 0;
   c.first_dt = (float)j.get("firstDt", 0.01);
   c.correct_volume = (j.has("correctVolume") && j.at("correctVolume").as_bool()) ? 1 : 0;
   c.steps = (int)j.get("steps", 0.0);
@@ -336,6 +337,9 @@ void free_ctx(Ctx& c) {
   if (c.ev0) hipEventDestroy(c.ev0);
   if (c.ev1) hipEventDestroy(c.ev1);
   if (c.ev_iter) hipEventDestroy(c.ev_iter);
+  for (auto& g : c.pcg_graphs)
+    if (g.exec) hipGraphExecDestroy(g.exec);
+  c.pcg_graphs.clear();
   for (int k = 0; k < LH_PROF_COUNT; ++k)
     for (auto e : c.prof_ev[k]) hipEventDestroy(e);
   if (c.s) hipStreamDestroy(c.s);

@@ -50,6 +50,9 @@ struct Config {
   // [cheb_lo, 2] in FP64 instead of numFinalSmooth sweeps (0: sweeps)
   int coarsest_cheb = 0;
   float cheb_lo = 0.01f;
+  // SolverParameter.pcgGraph: each PCG iteration's x/r update, preconditioner and r.z launched as
+  // one instantiated hipGraph instead of one launch at a time
+  int pcg_graph = 0;
   double refined_tol = 1e-9;
   int max_outer = 50;
   // boundary SDF program (postfix, 12 floats per node)
@@ -195,6 +198,16 @@ struct Ctx {
   int* xcount = nullptr;        // device counters, component * 5 + sweep
   int xcnt[15] = {};
   bool xcache_ok = false;
+  // pcgGraph: instantiated graphs of one PCG iteration, keyed by the p buffer and rho slot they
+  // read and the levels' ping-pong buffers at capture; `after` is the buffer state the captured
+  // host code left (the preconditioner swaps ping-pong pointers as it launches)
+  struct PcgGraph {
+    hipGraphExec_t exec = nullptr;
+    const float* p = nullptr;
+    int slot = 0;
+    std::vector<void*> before, after;
+  };
+  std::vector<PcgGraph> pcg_graphs;
   // live kernel timing: event pairs per instrumented kernel
   bool prof = false;
   std::vector<hipEvent_t> prof_ev[LH_PROF_COUNT];
