@@ -42,15 +42,22 @@ inline unsigned grid_for(size_t n) {
 struct BinP {
   int H[3];
   float mn[3], dx[3];
+  float inv[3];  // 1 / dx per axis when every dx is a power of two (then x / dx == x * inv exactly)
 };
 
-__global__ void k_bin_key(const float* __restrict__ pb, int64_t cap, int64_t P, BinP b, int* __restrict__ key,
+// The particle records of a buffer (one particle: R0 at r0[i], R1..R3 at rs[3 i .. 3 i + 2]).
+__device__ __forceinline__ int rec_id(const float4* __restrict__ rs, size_t i) { return __float_as_int(rs[3 * i + 2].w); }
+
+template <bool kPow2>
+__global__ void k_bin_key(const float4* __restrict__ r0, int64_t P, BinP b, int* __restrict__ key,
                           int* __restrict__ count, int* __restrict__ bflag, int* __restrict__ err) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
+    const float4 R = r0[i];
+    const float x[3] = {R.x, R.y, R.z};
     int h[3];
     bool bad = false;
     for (int a = 0; a < 3; ++a) {
-      float g = (pb[(size_t)a * cap + i] - b.mn[a]) / b.dx[a];
+      float g = kPow2 ? (x[a] - b.mn[a]) * b.inv[a] : (x[a] - b.mn[a]) / b.dx[a];
       float g2 = 2.0f * g;
       if (!(g2 >= 0.0f) || !(g2 < (float)b.H[a])) {
         bad = true;
@@ -76,17 +83,17 @@ __global__ void k_bin_scatter(const int* __restrict__ key, int64_t P, const int*
 }
 
 // place particle i at start[key] + (number of particles in its bin with a smaller id)
-__global__ void k_bin_rank(const float* __restrict__ src, int64_t cap, int64_t P, const int* __restrict__ key,
+__global__ void k_bin_rank(const float4* __restrict__ rs, int64_t P, const int* __restrict__ key,
                            const int* __restrict__ start, const int* __restrict__ slot_owner, int* __restrict__ order) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
     const int k = key[i];
     const int s0 = start[k], s1 = start[k + 1];
     int rank = 0;
     if (s1 - s0 > 1) {  // a bin of one particle needs no comparison
-      const int myid = __float_as_int(src[(size_t)PA_ID * cap + i]);
+      const int myid = rec_id(rs, (size_t)i);
       for (int s = s0; s < s1; ++s) {
         const int j = slot_owner[s];
-        const int idj = __float_as_int(src[(size_t)PA_ID * cap + j]);
+        const int idj = rec_id(rs, (size_t)j);
         rank += (idj < myid) ? 1 : 0;
       }
     }
@@ -94,26 +101,22 @@ __global__ void k_bin_rank(const float* __restrict__ src, int64_t cap, int64_t P
   }
 }
 
-// the particle buffer in bin order: destination d takes source order[d] (coalesced writes of the
-// 16 attributes and of the P2G records: (x, y, z, v_x) at pack[d], (C.col(0), v_y), (C.col(1), v_z),
-// (C.col(2), 0) at pack[cap + 3 d ..])
-__global__ void k_bin_gather(const float* __restrict__ src, float* __restrict__ dst, int64_t cap, int64_t P,
-                             const int* __restrict__ order, float4* __restrict__ pack) {
+// the records in bin order: destination d takes the record of source order[d], four 16-byte loads
+// from two contiguous runs (R0; R1..R3) and four coalesced 16-byte stores
+__global__ void k_bin_gather(const float4* __restrict__ src, float4* __restrict__ dst, int64_t cap, int64_t P,
+                             const int* __restrict__ order) {
+  const float4* __restrict__ ss = src + cap;
+  float4* __restrict__ ds = dst + cap;
   for (int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; d < P; d += (int64_t)gridDim.x * blockDim.x) {
     // particle ids are 0..P-1, so the ranks of a bin are distinct and every slot is written; the
     // guard only keeps a corrupt order inside the buffer
     size_t i = (size_t)(unsigned)order[d];
     if (i >= (size_t)P) i = (size_t)d;
-    float v[kParticleAttrs];
-#pragma unroll
-    for (int a = 0; a < kParticleAttrs; ++a) v[a] = src[(size_t)a * cap + i];
-#pragma unroll
-    for (int a = 0; a < kParticleAttrs; ++a) dst[(size_t)a * cap + d] = v[a];
-    pack[d] = make_float4(v[PA_X], v[PA_X + 1], v[PA_X + 2], v[PA_VX]);
-    float4* r = pack + cap + 3 * (size_t)d;
-    r[0] = make_float4(v[PA_C], v[PA_C + 1], v[PA_C + 2], v[PA_VX + 1]);
-    r[1] = make_float4(v[PA_C + 3], v[PA_C + 4], v[PA_C + 5], v[PA_VX + 2]);
-    r[2] = make_float4(v[PA_C + 6], v[PA_C + 7], v[PA_C + 8], 0.f);
+    const float4 a = src[i], b = ss[3 * i], c = ss[3 * i + 1], e = ss[3 * i + 2];
+    dst[d] = a;
+    ds[3 * (size_t)d] = b;
+    ds[3 * (size_t)d + 1] = c;
+    ds[3 * (size_t)d + 2] = e;
   }
 }
 
@@ -791,15 +794,17 @@ __device__ __forceinline__ float ld_at(const float* __restrict__ base, unsigned 
 // Grid2Particle (LiquidModel.cpp:724-822) fused with AdvectParticle and the max |v| of GetCFL.
 // Each axis's taps of both centre kinds are found once and shared by the three components.
 template <bool kPow2>
-__global__ void __launch_bounds__(256) k_g2p_advect(float* __restrict__ pb, int64_t cap, int64_t P,
+__global__ void __launch_bounds__(256) k_g2p_advect(float4* __restrict__ r0, int64_t cap, int64_t P,
                                                      const float* __restrict__ vx, const float* __restrict__ vy,
                                                      const float* __restrict__ vz, const float* __restrict__ sp, GP q,
                                                      float mx0, float mx1, float mx2, int do_g2p, int do_advect,
                                                      unsigned* __restrict__ maxbits) {
   const float mx[3] = {mx0, mx1, mx2};
+  float4* __restrict__ rs = r0 + cap;
   float local_max = 0.f;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
-    float pos[3] = {pb[(size_t)PA_X * cap + i], pb[(size_t)(PA_X + 1) * cap + i], pb[(size_t)(PA_X + 2) * cap + i]};
+    const float4 R0 = r0[i];
+    float pos[3] = {R0.x, R0.y, R0.z};
     float vel[3];
     if (do_g2p) {
       // per axis k: taps of the integer centres (the face axis, N_k + 1 nodes) and of the
@@ -869,13 +874,14 @@ __global__ void __launch_bounds__(256) k_g2p_advect(float* __restrict__ pb, int6
         Cm[3 * a + 1] = c1;
         Cm[3 * a + 2] = c2;
       }
-#pragma unroll
-      for (int k = 0; k < 3; ++k) pb[(size_t)(PA_VX + k) * cap + i] = vel[k];
-#pragma unroll
-      for (int k = 0; k < 9; ++k) pb[(size_t)(PA_C + k) * cap + i] = Cm[k];
+      const float idbits = rs[3 * i + 2].w;
+      rs[3 * i] = make_float4(Cm[0], Cm[1], Cm[2], vel[1]);
+      rs[3 * i + 1] = make_float4(Cm[3], Cm[4], Cm[5], vel[2]);
+      rs[3 * i + 2] = make_float4(Cm[6], Cm[7], Cm[8], idbits);
     } else {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) vel[k] = pb[(size_t)(PA_VX + k) * cap + i];
+      vel[0] = R0.w;
+      vel[1] = rs[3 * i].w;
+      vel[2] = rs[3 * i + 1].w;
     }
     float m3 = max3(fabsf(vel[0]), fabsf(vel[1]), fabsf(vel[2]));
     local_max = (local_max > m3) ? local_max : m3;
@@ -890,10 +896,11 @@ __global__ void __launch_bounds__(256) k_g2p_advect(float* __restrict__ pb, int6
         p1 -= sh * gr[1];
         p2 -= sh * gr[2];
       }
-      pb[(size_t)PA_X * cap + i] = p0;
-      pb[(size_t)(PA_X + 1) * cap + i] = p1;
-      pb[(size_t)(PA_X + 2) * cap + i] = p2;
+      pos[0] = p0;
+      pos[1] = p1;
+      pos[2] = p2;
     }
+    if (do_g2p || do_advect) r0[i] = make_float4(pos[0], pos[1], pos[2], vel[0]);
   }
   // block max, then one atomic per block (max of non-negative floats is order-free)
   __shared__ float sm[kBlock];
@@ -975,16 +982,18 @@ __global__ void __launch_bounds__(256) k_cv_flags(uint8_t* __restrict__ flag, in
 }
 
 // new positions of the selected particles into out (3 x cap), bin-ordered particles
-__global__ void __launch_bounds__(256) k_correct_volume(const float* __restrict__ pb, int64_t cap, int64_t P,
+__global__ void __launch_bounds__(256) k_correct_volume(const float4* __restrict__ r0, int64_t cap, int64_t P,
                                                         const int* __restrict__ start,
                                                         const float* __restrict__ solid,
                                                         const uint8_t* __restrict__ flag, CVP q,
                                                         float* __restrict__ out) {
   const int* N = q.g.N;
   for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < P; s += (int64_t)gridDim.x * blockDim.x) {
-    const int pid = __float_as_int(pb[(size_t)PA_ID * cap + s]);
+    const float4* __restrict__ rs = r0 + cap;
+    const int pid = rec_id(rs, (size_t)s);
     if (pid % q.cycle != q.step) continue;
-    const float p[3] = {pb[(size_t)PA_X * cap + s], pb[(size_t)(PA_X + 1) * cap + s], pb[(size_t)(PA_X + 2) * cap + s]};
+    const float4 Rs = r0[s];
+    const float p[3] = {Rs.x, Rs.y, Rs.z};
     const float r = q.radius, r2 = r * r;
     int lo[3], hi[3], hlo[3], hhi[3];
 #pragma unroll
@@ -1010,10 +1019,10 @@ __global__ void __launch_bounds__(256) k_correct_volume(const float* __restrict_
         const size_t k0 = ((size_t)hx * q.H[1] + hy) * q.H[2];
         const int s1 = start[k0 + hhi[2] + 1];
         for (int t = start[k0 + hlo[2]]; t < s1; ++t) {
-          const int npid = __float_as_int(pb[(size_t)PA_ID * cap + t]);
+          const int npid = rec_id(rs, (size_t)t);
           if (npid == pid) continue;
-          const float qq[3] = {pb[(size_t)PA_X * cap + t], pb[(size_t)(PA_X + 1) * cap + t],
-                               pb[(size_t)(PA_X + 2) * cap + t]};
+          const float4 Rt = r0[t];
+          const float qq[3] = {Rt.x, Rt.y, Rt.z};
           const float d0 = p[0] - qq[0], d1 = p[1] - qq[1], d2 = p[2] - qq[2];
           const float sq = d0 * d0 + (d1 * d1 + d2 * d2);
           if (sq >= r2) continue;
@@ -1065,13 +1074,13 @@ __global__ void __launch_bounds__(256) k_correct_volume(const float* __restrict_
   }
 }
 // positions of the selected particles <- out
-__global__ void k_correct_commit(float* __restrict__ pb, int64_t cap, int64_t P, const float* __restrict__ out,
+__global__ void k_correct_commit(float4* __restrict__ r0, int64_t cap, int64_t P, const float* __restrict__ out,
                                  int cycle, int step) {
   for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < P; s += (int64_t)gridDim.x * blockDim.x) {
-    const int pid = __float_as_int(pb[(size_t)PA_ID * cap + s]);
+    const int pid = rec_id(r0 + cap, (size_t)s);
     if (pid % cycle != step) continue;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) pb[(size_t)(PA_X + k) * cap + s] = out[(size_t)k * cap + s];
+    const float4 R = r0[s];
+    r0[s] = make_float4(out[s], out[(size_t)cap + s], out[2 * (size_t)cap + s], R.w);
   }
 }
 
@@ -1096,39 +1105,6 @@ int bin_scan(Ctx& c) {
   return LH_OK;
 }
 
-int run_binning(Ctx& c) {
-  c.bins_valid = false;
-  if (c.P == 0) return LH_OK;
-  BinP b;
-  for (int a = 0; a < 3; ++a) {
-    b.H[a] = c.H[a];
-    b.mn[a] = c.g.mn[a];
-    b.dx[a] = c.g.dx[a];
-  }
-  hipMemsetAsync(c.dflag + 1, 0, sizeof(int), c.s);
-  unsigned gp = grid_for((size_t)c.P);
-  hipLaunchKernelGGL(k_bin_key, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, b, c.pkey, c.bin_count,
-                     c.bin_bflag, c.dflag + 1);
-  int rc = bin_scan(c);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, c.s, c.pkey, c.P, c.bin_start, c.bin_count, c.pslot);
-  hipLaunchKernelGGL(k_bin_rank, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, c.pkey, c.bin_start,
-                     c.pslot, c.porder);
-  hipLaunchKernelGGL(k_bin_gather, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.pbuf[1 - c.cur], c.cap, c.P,
-                     c.porder, (float4*)c.ppack);
-  c.cur = 1 - c.cur;
-  c.bins_valid = true;
-  return LH_OK;
-}
-
-void launch_liquid_phi(Ctx& c) {
-  if (c.P == 0 || !c.bins_valid) return;
-  ProfScope ps(c, LH_PROF_LIQUID_PHI);
-  unsigned nt = tile_grid(div_up(c.g.N[0], kTX) * div_up(c.g.N[1], kTY) * div_up(c.g.N[2], kTZ));
-  hipLaunchKernelGGL(k_liquid_phi, dim3(nt), dim3(kBlock), 0, c.s, c.phi, (const float4*)c.ppack, c.cap, c.bin_start, c.g,
-                     c.H[0], c.H[1], c.H[2], c.dW[0], c.dW[1], c.dW[2], c.radius, 3 * c.g.dx[0]);
-}
-
 // dx = 2^-k on every axis: (x - min) / dx == (x - min) * 2^k exactly; inv receives 2^k
 static bool pow2_inverse(const GridP& g, float inv[3]) {
   bool pow2 = true;
@@ -1140,6 +1116,46 @@ static bool pow2_inverse(const GridP& g, float inv[3]) {
   }
   return pow2;
 }
+
+int run_binning(Ctx& c) {
+  c.bins_valid = false;
+  if (c.P == 0) return LH_OK;
+  BinP b;
+  for (int a = 0; a < 3; ++a) {
+    b.H[a] = c.H[a];
+    b.mn[a] = c.g.mn[a];
+    b.dx[a] = c.g.dx[a];
+  }
+  const bool pow2 = pow2_inverse(c.g, b.inv);
+  hipMemsetAsync(c.dflag + 1, 0, sizeof(int), c.s);
+  unsigned gp = grid_for((size_t)c.P);
+  const float4* rec = (const float4*)c.prec[c.cur];
+  if (pow2)
+    hipLaunchKernelGGL(k_bin_key<true>, dim3(gp), dim3(kBlock), 0, c.s, rec, c.P, b, c.pkey, c.bin_count,
+                       c.bin_bflag, c.dflag + 1);
+  else
+    hipLaunchKernelGGL(k_bin_key<false>, dim3(gp), dim3(kBlock), 0, c.s, rec, c.P, b, c.pkey, c.bin_count,
+                       c.bin_bflag, c.dflag + 1);
+  int rc = bin_scan(c);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, c.s, c.pkey, c.P, c.bin_start, c.bin_count, c.pslot);
+  hipLaunchKernelGGL(k_bin_rank, dim3(gp), dim3(kBlock), 0, c.s, rec + c.cap, c.P, c.pkey, c.bin_start, c.pslot,
+                     c.porder);
+  hipLaunchKernelGGL(k_bin_gather, dim3(gp), dim3(kBlock), 0, c.s, rec, (float4*)c.prec[1 - c.cur], c.cap, c.P,
+                     c.porder);
+  c.cur = 1 - c.cur;
+  c.bins_valid = true;
+  return LH_OK;
+}
+
+void launch_liquid_phi(Ctx& c) {
+  if (c.P == 0 || !c.bins_valid) return;
+  ProfScope ps(c, LH_PROF_LIQUID_PHI);
+  unsigned nt = tile_grid(div_up(c.g.N[0], kTX) * div_up(c.g.N[1], kTY) * div_up(c.g.N[2], kTZ));
+  hipLaunchKernelGGL(k_liquid_phi, dim3(nt), dim3(kBlock), 0, c.s, c.phi, (const float4*)c.prec[c.cur], c.cap, c.bin_start, c.g,
+                     c.H[0], c.H[1], c.H[2], c.dW[0], c.dW[1], c.dW[2], c.radius, 3 * c.g.dx[0]);
+}
+
 
 void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
   if (c.P == 0 || !c.bins_valid) return;
@@ -1164,8 +1180,8 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
   const bool dense = (size_t)c.P >= c.ncell;
   const int mx = dense ? kMXDense : kMXSparse;
   const unsigned nt = tile_grid(div_up(c.g.N[0] + 1, mx) * div_up(c.g.N[1] + 1, kPTY) * div_up(c.g.N[2] + 1, kPTZ));
-  const float4* pos = (const float4*)c.ppack;
-  const float4* rest = (const float4*)c.ppack + c.cap;
+  const float4* pos = (const float4*)c.prec[c.cur];
+  const float4* rest = pos + c.cap;
   if (pow2 && dense)
     hipLaunchKernelGGL((k_p2g<true, kMXDense>), dim3(nt), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], pos, rest,
                        c.bin_start, q);
@@ -1216,12 +1232,12 @@ int run_correct_volume(Ctx& c, float dt) {
                        c.g.N[2], q.nslot);
     c.cv_nslot = q.nslot;
   }
-  float* out = c.pbuf[1 - c.cur];  // free between permutations: 3 x cap floats of it
+  float* out = c.prec[1 - c.cur];  // free between permutations: 3 x cap floats of it
   const unsigned gp = grid_for((size_t)c.P);
-  hipLaunchKernelGGL(k_correct_volume, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, c.bin_start,
-                     c.solid_phi, c.cv_flag, q, out);
-  hipLaunchKernelGGL(k_correct_commit, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, out, q.cycle,
-                     q.step);
+  hipLaunchKernelGGL(k_correct_volume, dim3(gp), dim3(kBlock), 0, c.s, (const float4*)c.prec[c.cur], c.cap, c.P,
+                     c.bin_start, c.solid_phi, c.cv_flag, q, out);
+  hipLaunchKernelGGL(k_correct_commit, dim3(gp), dim3(kBlock), 0, c.s, (float4*)c.prec[c.cur], c.cap, c.P, out,
+                     q.cycle, q.step);
   c.bins_valid = false;
   c.correct_step = (q.step + 1) % q.cycle;
   c.correct_count++;
@@ -1239,11 +1255,11 @@ void launch_g2p_advect(Ctx& c, bool do_g2p, bool do_advect, float dt) {
   ProfScope ps(c, LH_PROF_G2P);
   // the face arrays' byte offsets fit 32 bits: grids are under 2^31 half-cell bins (N < 645 per axis)
   if (pow2)
-    hipLaunchKernelGGL(k_g2p_advect<true>, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, c.v[0], c.v[1],
+    hipLaunchKernelGGL(k_g2p_advect<true>, dim3(gp), dim3(kBlock), 0, c.s, (float4*)c.prec[c.cur], c.cap, c.P, c.v[0], c.v[1],
                        c.v[2], c.solid_phi, q, c.cfg.mx[0], c.cfg.mx[1], c.cfg.mx[2], do_g2p ? 1 : 0,
                        do_advect ? 1 : 0, c.dmaxv);
   else
-    hipLaunchKernelGGL(k_g2p_advect<false>, dim3(gp), dim3(kBlock), 0, c.s, c.pbuf[c.cur], c.cap, c.P, c.v[0], c.v[1],
+    hipLaunchKernelGGL(k_g2p_advect<false>, dim3(gp), dim3(kBlock), 0, c.s, (float4*)c.prec[c.cur], c.cap, c.P, c.v[0], c.v[1],
                        c.v[2], c.solid_phi, q, c.cfg.mx[0], c.cfg.mx[1], c.cfg.mx[2], do_g2p ? 1 : 0,
                        do_advect ? 1 : 0, c.dmaxv);
 }

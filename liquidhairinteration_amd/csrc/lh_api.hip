@@ -321,7 +321,7 @@ void free_ctx(Ctx& c) {
   }
   F(c.pressure); F(c.phi); F(c.rhs); F(c.solid_phi); F(c.is_valid); F(c.xstate); F(c.d_sdf);
   F(c.bin_count); F(c.bin_start); F(c.bin_bflag); F(c.bin_bsum); F(c.bin_boff); F(c.scan_tmp); F(c.pkey); F(c.pslot); F(c.porder);
-  F(c.pbuf[0]); F(c.pbuf[1]); F(c.ppack); F(c.cv_flag);
+  F(c.prec[0]); F(c.prec[1]); F(c.cv_flag);
   for (auto& q : c.lv) {
     F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2); F(q.z64); F(q.z64b); F(q.d64); F(q.d64b);
     F(q.rows); F(q.nrows);
@@ -346,20 +346,17 @@ void free_ctx(Ctx& c) {
 }
 
 int ensure_particle_capacity(Ctx& c, int64_t n) {
-  if (n <= c.cap && c.pbuf[0]) return LH_OK;
-  if (c.pbuf[0]) hipFree(c.pbuf[0]);
-  if (c.pbuf[1]) hipFree(c.pbuf[1]);
+  if (n <= c.cap && c.prec[0]) return LH_OK;
+  if (c.prec[0]) hipFree(c.prec[0]);
+  if (c.prec[1]) hipFree(c.prec[1]);
   if (c.pkey) hipFree(c.pkey);
   if (c.pslot) hipFree(c.pslot);
   if (c.porder) hipFree(c.porder);
-  if (c.ppack) hipFree(c.ppack);
-  c.pbuf[0] = c.pbuf[1] = nullptr;
+  c.prec[0] = c.prec[1] = nullptr;
   c.pkey = c.pslot = c.porder = nullptr;
-  c.ppack = nullptr;
   int64_t cap = std::max<int64_t>(n, 1);
-  bool ok = dalloc(c, c.pbuf[0], (size_t)kParticleAttrs * cap) && dalloc(c, c.pbuf[1], (size_t)kParticleAttrs * cap) &&
-            dalloc(c, c.pkey, cap) && dalloc(c, c.pslot, cap) && dalloc(c, c.porder, cap) &&
-            dalloc(c, c.ppack, (size_t)16 * cap);
+  bool ok = dalloc(c, c.prec[0], (size_t)kRecFloats * cap) && dalloc(c, c.prec[1], (size_t)kRecFloats * cap) &&
+            dalloc(c, c.pkey, cap) && dalloc(c, c.pslot, cap) && dalloc(c, c.porder, cap);
   if (!ok) {
     c.cap = 0;
     return LH_ERR_OOM;
@@ -376,19 +373,22 @@ int upload_particles(Ctx& c, const float* pos, const float* vel, const float* C,
   c.P = n;
   c.bins_valid = false;
   if (n == 0) return LH_OK;
-  std::vector<float> soa((size_t)kParticleAttrs * n);
+  // the records: R0 for every particle, then R1..R3 for every particle
+  std::vector<float> r0((size_t)4 * n), rs((size_t)12 * n);
   for (int64_t i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) r0[4 * i + a] = pos[3 * i + a];
     for (int a = 0; a < 3; ++a) {
-      soa[(size_t)(PA_X + a) * n + i] = pos[3 * i + a];
-      soa[(size_t)(PA_VX + a) * n + i] = vel ? vel[3 * i + a] : 0.f;
+      const float va = vel ? vel[3 * i + a] : 0.f;
+      if (a == 0) r0[4 * i + 3] = va;
+      else rs[12 * i + 4 * (a - 1) + 3] = va;
+      for (int k = 0; k < 3; ++k) rs[12 * i + 4 * a + k] = C ? C[9 * i + 3 * a + k] : 0.f;
     }
-    for (int k = 0; k < 9; ++k) soa[(size_t)(PA_C + k) * n + i] = C ? C[9 * i + k] : 0.f;
-    int id = (int)i;
-    std::memcpy(&soa[(size_t)PA_ID * n + i], &id, 4);
+    const int id = (int)i;
+    std::memcpy(&rs[12 * i + 11], &id, 4);
   }
-  for (int k = 0; k < kParticleAttrs; ++k)
-    hipMemcpyAsync(c.pbuf[c.cur] + (size_t)k * c.cap, soa.data() + (size_t)k * n, n * sizeof(float),
-                   hipMemcpyHostToDevice, c.s);
+  hipMemcpyAsync(c.prec[c.cur], r0.data(), r0.size() * sizeof(float), hipMemcpyHostToDevice, c.s);
+  hipMemcpyAsync(c.prec[c.cur] + (size_t)4 * c.cap, rs.data(), rs.size() * sizeof(float), hipMemcpyHostToDevice,
+                 c.s);
   if (!hip_ok(c, hipStreamSynchronize(c.s), "upload particles")) return LH_ERR_DEVICE;
   return LH_OK;
 }
@@ -927,24 +927,24 @@ int lh_get_particles(lh_ctx* ctx, float* pos, float* vel, float* C, int64_t n) {
       return LH_ERR_ARG;
     }
     if (n == 0) return LH_OK;
-    std::vector<float> soa((size_t)kParticleAttrs * n);
-    for (int k = 0; k < kParticleAttrs; ++k)
-      hipMemcpyAsync(soa.data() + (size_t)k * n, c.pbuf[c.cur] + (size_t)k * c.cap, n * sizeof(float),
-                     hipMemcpyDeviceToHost, c.s);
+    std::vector<float> r0((size_t)4 * n), rs((size_t)12 * n);
+    hipMemcpyAsync(r0.data(), c.prec[c.cur], r0.size() * sizeof(float), hipMemcpyDeviceToHost, c.s);
+    hipMemcpyAsync(rs.data(), c.prec[c.cur] + (size_t)4 * c.cap, rs.size() * sizeof(float), hipMemcpyDeviceToHost,
+                   c.s);
     if (!hip_ok(c, hipStreamSynchronize(c.s), "download particles")) return LH_ERR_DEVICE;
     for (int64_t i = 0; i < n; ++i) {
       int id;
-      std::memcpy(&id, &soa[(size_t)PA_ID * n + i], 4);
+      std::memcpy(&id, &rs[12 * i + 11], 4);
       if (id < 0 || id >= n) {
         c.err = "corrupt particle id";
         return LH_ERR_DEVICE;
       }
       for (int a = 0; a < 3; ++a) {
-        if (pos) pos[3 * (size_t)id + a] = soa[(size_t)(PA_X + a) * n + i];
-        if (vel) vel[3 * (size_t)id + a] = soa[(size_t)(PA_VX + a) * n + i];
+        if (pos) pos[3 * (size_t)id + a] = r0[4 * i + a];
+        if (vel) vel[3 * (size_t)id + a] = a == 0 ? r0[4 * i + 3] : rs[12 * i + 4 * (a - 1) + 3];
+        if (C)
+          for (int k = 0; k < 3; ++k) C[9 * (size_t)id + 3 * a + k] = rs[12 * i + 4 * a + k];
       }
-      if (C)
-        for (int k = 0; k < 9; ++k) C[9 * (size_t)id + k] = soa[(size_t)(PA_C + k) * n + i];
     }
     return LH_OK;
   });

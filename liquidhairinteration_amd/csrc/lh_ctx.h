@@ -110,8 +110,11 @@ struct Level {
   int* ntiles = nullptr;
 };
 
-// attributes of one particle in the SoA particle buffer (kParticleAttrs x cap)
-enum { PA_X = 0, PA_VX = 3, PA_C = 6, PA_ID = 15, kParticleAttrs = 16 };
+// One particle is one 64-byte record of four float4: R0 = (x, y, z, v_x) at float4 index i of a
+// record buffer, and R1 = (C.col(0), v_y), R2 = (C.col(1), v_z), R3 = (C.col(2), id bits) at float4
+// index cap + 3 i .. + 2 (C column-major: C.col(a) holds the gradient of velocity component a).
+// kRecFloats floats per particle per buffer.
+constexpr int kRecFloats = 16;
 
 struct Ctx {
   Config cfg;
@@ -129,13 +132,11 @@ struct Ctx {
   size_t nface[3] = {0, 0, 0}, ncell = 0, nnode = 0;
   float* d_sdf = nullptr;
   int nsdf = 0;
-  // particles, SoA, double buffered for the per-step bin-order permutation
+  // particles: record buffers (the layout above), double buffered for the per-step bin-order
+  // permutation; prec[cur] holds the state, in bin order after each binning
   int64_t P = 0, cap = 0;
-  float* pbuf[2] = {nullptr, nullptr};
+  float* prec[2] = {nullptr, nullptr};
   int cur = 0;
-  // per component a, in bin order, one 32-byte record per particle: x, y, z, v_a, C.col(a), 0
-  // (written with the bin permutation; the transfers read it with two 16-byte loads)
-  float* ppack = nullptr;
   // half-cell bins: 2N per axis
   int H[3] = {0, 0, 0};
   size_t nbins = 0;
@@ -291,6 +292,5 @@ const char* noalloc_last_error();
 bool hip_ok(Ctx& c, hipError_t e, const char* what);
 
 inline size_t div_up(size_t a, size_t b) { return (a + b - 1) / b; }
-inline float* pattr(Ctx& c, int buf, int k) { return c.pbuf[buf] + (size_t)k * (size_t)c.cap; }
 
 }  // namespace lh
