@@ -45,8 +45,9 @@ struct BinP {
   float inv[3];  // 1 / dx per axis when every dx is a power of two (then x / dx == x * inv exactly)
 };
 
-// The particle records of a buffer (one particle: R0 at r0[i], R1..R3 at rs[3 i .. 3 i + 2]).
-__device__ __forceinline__ int rec_id(const float4* __restrict__ rs, size_t i) { return __float_as_int(rs[3 * i + 2].w); }
+// The particle records of a buffer (one particle: R0 at r0[i], R1..R3 at r0[cap + 3 i .. + 2]); the id is
+// R0.w, read 16 bytes apart by the kernels that need it beside or instead of the position.
+__device__ __forceinline__ int rec_id(const float4* __restrict__ r0, size_t i) { return __float_as_int(r0[i].w); }
 
 template <bool kPow2>
 __global__ void k_bin_key(const float4* __restrict__ r0, int64_t P, BinP b, int* __restrict__ key,
@@ -83,17 +84,17 @@ __global__ void k_bin_scatter(const int* __restrict__ key, int64_t P, const int*
 }
 
 // place particle i at start[key] + (number of particles in its bin with a smaller id)
-__global__ void k_bin_rank(const float4* __restrict__ rs, int64_t P, const int* __restrict__ key,
+__global__ void k_bin_rank(const float4* __restrict__ r0, int64_t P, const int* __restrict__ key,
                            const int* __restrict__ start, const int* __restrict__ slot_owner, int* __restrict__ order) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
     const int k = key[i];
     const int s0 = start[k], s1 = start[k + 1];
     int rank = 0;
     if (s1 - s0 > 1) {  // a bin of one particle needs no comparison
-      const int myid = rec_id(rs, (size_t)i);
+      const int myid = rec_id(r0, (size_t)i);
       for (int s = s0; s < s1; ++s) {
         const int j = slot_owner[s];
-        const int idj = rec_id(rs, (size_t)j);
+        const int idj = rec_id(r0, (size_t)j);
         rank += (idj < myid) ? 1 : 0;
       }
     }
@@ -270,9 +271,9 @@ __device__ __forceinline__ bool tile_coords(int d0, int d1, int d2, int c[3], in
 }
 
 // ------------------------------------------------------------------ x-marching P2G, three components
-// The bin permutation writes, per particle in bin order, R0 = (x, y, z, v_x) into one array (the
+// The particle records, in bin order after the permutation: R0 = (x, y, z, id) in one array (the
 // positions liquid phi reads, 16 bytes apart) and R1 = (C.col(0), v_y), R2 = (C.col(1), v_z),
-// R3 = (C.col(2), 0) into a second (48 bytes per particle).
+// R3 = (C.col(2), v_x) in a second (48 bytes per particle).
 //
 // A 256-thread workgroup owns MX x positions x 4 y x 64 z of all three face grids: each thread
 // one (y, z) column with its x-, y- and z-faces at x = c0 .. c0 + MX - 1 (those that exist: the
@@ -336,7 +337,7 @@ __device__ __forceinline__ void p2g_acc(float4 R0, float4 R1, float4 R2, float4 
     const float wgt = quad_w(gx - ccx0[f]) * wyh * wzh;
     const float dot = R1.x * (R0.x - fx0[f]) + yz0;
     ws[0][f] = ws[0][f] + wgt;
-    vs[0][f] = vs[0][f] + wgt * (R0.w + dot);
+    vs[0][f] = vs[0][f] + wgt * (R3.w + dot);
   }
 #pragma unroll
   for (int f = 0; f < NYZ; ++f) {
@@ -874,12 +875,11 @@ __global__ void __launch_bounds__(256) k_g2p_advect(float4* __restrict__ r0, int
         Cm[3 * a + 1] = c1;
         Cm[3 * a + 2] = c2;
       }
-      const float idbits = rs[3 * i + 2].w;
       rs[3 * i] = make_float4(Cm[0], Cm[1], Cm[2], vel[1]);
       rs[3 * i + 1] = make_float4(Cm[3], Cm[4], Cm[5], vel[2]);
-      rs[3 * i + 2] = make_float4(Cm[6], Cm[7], Cm[8], idbits);
+      rs[3 * i + 2] = make_float4(Cm[6], Cm[7], Cm[8], vel[0]);
     } else {
-      vel[0] = R0.w;
+      vel[0] = rs[3 * i + 2].w;
       vel[1] = rs[3 * i].w;
       vel[2] = rs[3 * i + 1].w;
     }
@@ -900,7 +900,7 @@ __global__ void __launch_bounds__(256) k_g2p_advect(float4* __restrict__ r0, int
       pos[1] = p1;
       pos[2] = p2;
     }
-    if (do_g2p || do_advect) r0[i] = make_float4(pos[0], pos[1], pos[2], vel[0]);
+    if (do_advect) r0[i] = make_float4(pos[0], pos[1], pos[2], R0.w);
   }
   // block max, then one atomic per block (max of non-negative floats is order-free)
   __shared__ float sm[kBlock];
@@ -989,8 +989,7 @@ __global__ void __launch_bounds__(256) k_correct_volume(const float4* __restrict
                                                         float* __restrict__ out) {
   const int* N = q.g.N;
   for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < P; s += (int64_t)gridDim.x * blockDim.x) {
-    const float4* __restrict__ rs = r0 + cap;
-    const int pid = rec_id(rs, (size_t)s);
+    const int pid = rec_id(r0, (size_t)s);
     if (pid % q.cycle != q.step) continue;
     const float4 Rs = r0[s];
     const float p[3] = {Rs.x, Rs.y, Rs.z};
@@ -1019,7 +1018,7 @@ __global__ void __launch_bounds__(256) k_correct_volume(const float4* __restrict
         const size_t k0 = ((size_t)hx * q.H[1] + hy) * q.H[2];
         const int s1 = start[k0 + hhi[2] + 1];
         for (int t = start[k0 + hlo[2]]; t < s1; ++t) {
-          const int npid = rec_id(rs, (size_t)t);
+          const int npid = rec_id(r0, (size_t)t);
           if (npid == pid) continue;
           const float4 Rt = r0[t];
           const float qq[3] = {Rt.x, Rt.y, Rt.z};
@@ -1077,7 +1076,7 @@ __global__ void __launch_bounds__(256) k_correct_volume(const float4* __restrict
 __global__ void k_correct_commit(float4* __restrict__ r0, int64_t cap, int64_t P, const float* __restrict__ out,
                                  int cycle, int step) {
   for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < P; s += (int64_t)gridDim.x * blockDim.x) {
-    const int pid = rec_id(r0 + cap, (size_t)s);
+    const int pid = rec_id(r0, (size_t)s);
     if (pid % cycle != step) continue;
     const float4 R = r0[s];
     r0[s] = make_float4(out[s], out[(size_t)cap + s], out[2 * (size_t)cap + s], R.w);
@@ -1139,7 +1138,7 @@ int run_binning(Ctx& c) {
   int rc = bin_scan(c);
   if (rc) return rc;
   hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, c.s, c.pkey, c.P, c.bin_start, c.bin_count, c.pslot);
-  hipLaunchKernelGGL(k_bin_rank, dim3(gp), dim3(kBlock), 0, c.s, rec + c.cap, c.P, c.pkey, c.bin_start, c.pslot,
+  hipLaunchKernelGGL(k_bin_rank, dim3(gp), dim3(kBlock), 0, c.s, rec, c.P, c.pkey, c.bin_start, c.pslot,
                      c.porder);
   hipLaunchKernelGGL(k_bin_gather, dim3(gp), dim3(kBlock), 0, c.s, rec, (float4*)c.prec[1 - c.cur], c.cap, c.P,
                      c.porder);

@@ -377,14 +377,13 @@ int upload_particles(Ctx& c, const float* pos, const float* vel, const float* C,
   std::vector<float> r0((size_t)4 * n), rs((size_t)12 * n);
   for (int64_t i = 0; i < n; ++i) {
     for (int a = 0; a < 3; ++a) r0[4 * i + a] = pos[3 * i + a];
+    const int id = (int)i;
+    std::memcpy(&r0[4 * i + 3], &id, 4);
     for (int a = 0; a < 3; ++a) {
-      const float va = vel ? vel[3 * i + a] : 0.f;
-      if (a == 0) r0[4 * i + 3] = va;
-      else rs[12 * i + 4 * (a - 1) + 3] = va;
+      // v_x in R3.w, v_y in R1.w, v_z in R2.w
+      rs[12 * i + 4 * ((a + 2) % 3) + 3] = vel ? vel[3 * i + a] : 0.f;
       for (int k = 0; k < 3; ++k) rs[12 * i + 4 * a + k] = C ? C[9 * i + 3 * a + k] : 0.f;
     }
-    const int id = (int)i;
-    std::memcpy(&rs[12 * i + 11], &id, 4);
   }
   hipMemcpyAsync(c.prec[c.cur], r0.data(), r0.size() * sizeof(float), hipMemcpyHostToDevice, c.s);
   hipMemcpyAsync(c.prec[c.cur] + (size_t)4 * c.cap, rs.data(), rs.size() * sizeof(float), hipMemcpyHostToDevice,
@@ -934,14 +933,14 @@ int lh_get_particles(lh_ctx* ctx, float* pos, float* vel, float* C, int64_t n) {
     if (!hip_ok(c, hipStreamSynchronize(c.s), "download particles")) return LH_ERR_DEVICE;
     for (int64_t i = 0; i < n; ++i) {
       int id;
-      std::memcpy(&id, &rs[12 * i + 11], 4);
+      std::memcpy(&id, &r0[4 * i + 3], 4);
       if (id < 0 || id >= n) {
         c.err = "corrupt particle id";
         return LH_ERR_DEVICE;
       }
       for (int a = 0; a < 3; ++a) {
         if (pos) pos[3 * (size_t)id + a] = r0[4 * i + a];
-        if (vel) vel[3 * (size_t)id + a] = a == 0 ? r0[4 * i + 3] : rs[12 * i + 4 * (a - 1) + 3];
+        if (vel) vel[3 * (size_t)id + a] = rs[12 * i + 4 * ((a + 2) % 3) + 3];
         if (C)
           for (int k = 0; k < 3; ++k) C[9 * (size_t)id + 3 * a + k] = rs[12 * i + 4 * a + k];
       }

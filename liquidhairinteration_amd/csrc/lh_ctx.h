@@ -110,8 +110,8 @@ struct Level {
   int* ntiles = nullptr;
 };
 
-// One particle is one 64-byte record of four float4: R0 = (x, y, z, v_x) at float4 index i of a
-// record buffer, and R1 = (C.col(0), v_y), R2 = (C.col(1), v_z), R3 = (C.col(2), id bits) at float4
+// One particle is one 64-byte record of four float4: R0 = (x, y, z, id bits) at float4 index i of a
+// record buffer, and R1 = (C.col(0), v_y), R2 = (C.col(1), v_z), R3 = (C.col(2), v_x) at float4
 // index cap + 3 i .. + 2 (C column-major: C.col(a) holds the gradient of velocity component a).
 // kRecFloats floats per particle per buffer.
 constexpr int kRecFloats = 16;
