@@ -49,9 +49,12 @@ struct BinP {
 // R0.w, read 16 bytes apart by the kernels that need it beside or instead of the position.
 __device__ __forceinline__ int rec_id(const float4* __restrict__ r0, size_t i) { return __float_as_int(r0[i].w); }
 
+// key[i] = the particle's half-cell bin, arrival[i] = its arrival rank among the bin's particles
+// (the atomic's order: any order serves, the rank kernel orders each bin by id)
 template <bool kPow2>
 __global__ void k_bin_key(const float4* __restrict__ r0, int64_t P, BinP b, int* __restrict__ key,
-                          int* __restrict__ count, int* __restrict__ bflag, int* __restrict__ err) {
+                          int* __restrict__ arrival, int* __restrict__ count, int* __restrict__ bflag,
+                          int* __restrict__ err) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
     const float4 R = r0[i];
     const float x[3] = {R.x, R.y, R.z};
@@ -69,17 +72,19 @@ __global__ void k_bin_key(const float4* __restrict__ r0, int64_t P, BinP b, int*
     if (bad) atomicOr(err, 1);
     int k = (h[0] * b.H[1] + h[1]) * b.H[2] + h[2];
     key[i] = k;
-    atomicAdd(&count[k], 1);
+    arrival[i] = atomicAdd(&count[k], 1);
     bflag[k / kScanBins] = 1;
   }
 }
 
-__global__ void k_bin_scatter(const int* __restrict__ key, int64_t P, const int* __restrict__ start,
-                              int* __restrict__ count, int* __restrict__ slot_owner) {
+// slot_owner[start[key] + arrival] = i; every particle of a bin clears its count for the next
+// binning (the scan has read the counts)
+__global__ void k_bin_scatter(const int* __restrict__ key, const int* __restrict__ arrival, int64_t P,
+                              const int* __restrict__ start, int* __restrict__ count, int* __restrict__ slot_owner) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
-    int k = key[i];
-    int s = start[k] + atomicSub(&count[k], 1) - 1;  // leaves count at zero for the next step
-    slot_owner[s] = (int)i;
+    const int k = key[i];
+    slot_owner[start[k] + arrival[i]] = (int)i;
+    count[k] = 0;
   }
 }
 
@@ -1130,14 +1135,16 @@ int run_binning(Ctx& c) {
   unsigned gp = grid_for((size_t)c.P);
   const float4* rec = (const float4*)c.prec[c.cur];
   if (pow2)
-    hipLaunchKernelGGL(k_bin_key<true>, dim3(gp), dim3(kBlock), 0, c.s, rec, c.P, b, c.pkey, c.bin_count,
+    hipLaunchKernelGGL(k_bin_key<true>, dim3(gp), dim3(kBlock), 0, c.s, rec, c.P, b, c.pkey, c.porder, c.bin_count,
                        c.bin_bflag, c.dflag + 1);
   else
-    hipLaunchKernelGGL(k_bin_key<false>, dim3(gp), dim3(kBlock), 0, c.s, rec, c.P, b, c.pkey, c.bin_count,
+    hipLaunchKernelGGL(k_bin_key<false>, dim3(gp), dim3(kBlock), 0, c.s, rec, c.P, b, c.pkey, c.porder, c.bin_count,
                        c.bin_bflag, c.dflag + 1);
   int rc = bin_scan(c);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, c.s, c.pkey, c.P, c.bin_start, c.bin_count, c.pslot);
+  // porder holds the arrival ranks until k_bin_rank writes the order over them
+  hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, c.s, c.pkey, c.porder, c.P, c.bin_start, c.bin_count,
+                     c.pslot);
   hipLaunchKernelGGL(k_bin_rank, dim3(gp), dim3(kBlock), 0, c.s, rec, c.P, c.pkey, c.bin_start, c.pslot,
                      c.porder);
   hipLaunchKernelGGL(k_bin_gather, dim3(gp), dim3(kBlock), 0, c.s, rec, (float4*)c.prec[1 - c.cur], c.cap, c.P,

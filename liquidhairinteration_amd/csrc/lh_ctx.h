@@ -140,10 +140,10 @@ struct Ctx {
   // half-cell bins: 2N per axis
   int H[3] = {0, 0, 0};
   size_t nbins = 0;
-  int* bin_count = nullptr;  // kept at zero between steps (decremented by the scatter)
+  int* bin_count = nullptr;  // kept at zero between steps (the scatter clears the counts it used)
   int* bin_start = nullptr;  // nbins + 1
   int* pkey = nullptr;       // per particle bin key
-  int* pslot = nullptr;      // per particle provisional slot
+  int* pslot = nullptr;      // per bin-order slot, the particle the scatter placed there (arrival order)
   int* porder = nullptr;     // per bin-order slot, the particle (source index) that takes it
   // the bin scan runs over blocks of kScanBins bins: a flag per block (set by the key kernel
   // for every block that holds a particle, cleared by the scan), the block sums and offsets
