@@ -47,6 +47,7 @@ struct LvP {
   const float* t1;
   const float* t2;
   const uint8_t* cls;
+  const uint16_t* cmask;
 };
 
 LvP lvp(const Level& L) {
@@ -57,6 +58,7 @@ LvP lvp(const Level& L) {
   p.t1 = L.t[1];
   p.t2 = L.t[2];
   p.cls = L.cls;
+  p.cmask = L.cmask;
   return p;
 }
 
@@ -64,13 +66,14 @@ LvP lvp(const Level& L) {
 // class 2, added in the order +x, -x, +y, -y, +z, -z.  Every load is issued before any test
 // (out-of-range neighbours read cell c instead) and the terms are selected rather than
 // branched around, so a launch waits for one round trip to memory, not one per test; the
-// value is the same as adding only the present terms.
+// value is the same as adding only the present terms.  The classes come from the cell's
+// coupling mask (one load instead of seven class bytes from five rows).
 struct AxAt {
   float ax;   // (A z)[c]
   float ad;   // adiag[c]
   float zc;   // z[c]
   float v;    // v[c] of the extra array handed in (r, b, ...)
-  int cls;    // cls[c]
+  int cls;    // 0 where cls[c] == 0, else 1
 };
 __device__ __forceinline__ AxAt get_ax_full(const LvP& L, const float* __restrict__ z, const float* __restrict__ v,
                                             int x, int y, int w, size_t c) {
@@ -80,28 +83,45 @@ __device__ __forceinline__ AxAt get_ax_full(const LvP& L, const float* __restric
   const size_t cxp = hxp ? c + sx : c, cxm = hxm ? c - sx : c, cyp = hyp ? c + sy : c, cym = hym ? c - sy : c,
                czp = hzp ? c + 1 : c, czm = hzm ? c - 1 : c;
   float zc = z[c], ad = L.adiag[c], vc = v[c];
-  int k = L.cls[c], kxp = L.cls[cxp], kxm = L.cls[cxm], kyp = L.cls[cyp], kym = L.cls[cym], kzp = L.cls[czp],
-      kzm = L.cls[czm];
+  int m = L.cmask[c];
   float zxp = z[cxp], zxm = z[cxm], zyp = z[cyp], zym = z[cym], zzp = z[czp], zzm = z[czm];
   float txp = L.t0[c], txm = L.t0[cxm], typ = L.t1[c], tym = L.t1[cym], tzp = L.t2[c], tzm = L.t2[czm];
   // an empty statement that takes every loaded value: the loads complete here, together,
   // and the compiler cannot move any of them behind the selects or the caller's tests
-  asm volatile("" : "+v"(zc), "+v"(ad), "+v"(vc), "+v"(k), "+v"(kxp), "+v"(kxm), "+v"(kyp), "+v"(kym), "+v"(kzp),
-               "+v"(kzm));
+  asm volatile("" : "+v"(zc), "+v"(ad), "+v"(vc), "+v"(m));
   asm volatile("" : "+v"(zxp), "+v"(zxm), "+v"(zyp), "+v"(zym), "+v"(zzp), "+v"(zzm), "+v"(txp), "+v"(txm),
                "+v"(typ), "+v"(tym), "+v"(tzp), "+v"(tzm));
   const float res0 = zc * ad;
   float res = res0;
-  res = (hxp && kxp == 2) ? res + zxp * (-txp) : res;
-  res = (hxm && kxm == 2) ? res + zxm * (-txm) : res;
-  res = (hyp && kyp == 2) ? res + zyp * (-typ) : res;
-  res = (hym && kym == 2) ? res + zym * (-tym) : res;
-  res = (hzp && kzp == 2) ? res + zzp * (-tzp) : res;
-  res = (hzm && kzm == 2) ? res + zzm * (-tzm) : res;
-  return {k == 0 ? res0 : res, ad, zc, vc, k};
+  res = (m & 1) ? res + zxp * (-txp) : res;
+  res = (m & 2) ? res + zxm * (-txm) : res;
+  res = (m & 4) ? res + zyp * (-typ) : res;
+  res = (m & 8) ? res + zym * (-tym) : res;
+  res = (m & 16) ? res + zzp * (-tzp) : res;
+  res = (m & 32) ? res + zzm * (-tzm) : res;
+  const bool air = (m & 64) != 0;
+  return {air ? res0 : res, ad, zc, vc, air ? 0 : 1};
 }
 __device__ __forceinline__ float get_ax(const LvP& L, const float* __restrict__ z, int x, int y, int w, size_t c) {
   return get_ax_full(L, z, z, x, y, w, c).ax;
+}
+
+// cmask of a level from its classes (layout in Level::cmask)
+__global__ void k_build_cmask(uint16_t* __restrict__ m, const uint8_t* __restrict__ cls, int d0, int d1, int d2) {
+  const size_t total = (size_t)d0 * d1 * d2, sx = (size_t)d1 * d2, sy = (size_t)d2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const int z = (int)(i % d2), y = (int)((i / d2) % d1), x = (int)(i / sx);
+    const bool has[6] = {x + 1 < d0, x >= 1, y + 1 < d1, y >= 1, z + 1 < d2, z >= 1};
+    const size_t nb[6] = {i + sx, i - sx, i + sy, i - sy, i + 1, i - 1};
+    int v = cls[i] == 0 ? 64 : 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int kn = has[k] ? cls[nb[k]] : 0;
+      v |= (kn == 2 ? 1 : 0) << k;
+      v |= (kn != 0 ? 1 : 0) << (8 + k);
+    }
+    m[i] = (uint16_t)v;
+  }
 }
 
 // ---------------------------------------------------------------- active z-rows
@@ -463,13 +483,16 @@ __device__ __forceinline__ Sten<T> sten_load(const LvP& L, const T* __restrict__
   s.zc = z[c];
   s.ad = L.adiag[c];
   s.rv = r[c];
-  s.k = L.cls[c];
-  s.kxp = L.cls[cxp];
-  s.kxm = L.cls[cxm];
-  s.kyp = L.cls[cyp];
-  s.kym = L.cls[cym];
-  s.kzp = L.cls[czp];
-  s.kzm = L.cls[czm];
+  // the classes the stencil tests, from the cell's coupling mask: 0 where cls == 0, 2 where a
+  // neighbour exists with cls 2 (the only values it compares against)
+  const int m = L.cmask[c];
+  s.k = (m & 64) ? 0 : 1;
+  s.kxp = (m & 1) ? 2 : 0;
+  s.kxm = (m & 2) ? 2 : 0;
+  s.kyp = (m & 4) ? 2 : 0;
+  s.kym = (m & 8) ? 2 : 0;
+  s.kzp = (m & 16) ? 2 : 0;
+  s.kzm = (m & 32) ? 2 : 0;
   s.zxp = z[cxp];
   s.zxm = z[cxm];
   s.zyp = z[cyp];
@@ -546,13 +569,16 @@ __device__ __forceinline__ Coef coef_load(const LvP& L, const float* __restrict_
                cym = s.hym ? c - sy : c, czp = s.hzp ? c + 1 : c, czm = s.hzm ? c - 1 : c;
   s.ad = L.adiag[c];
   s.rv = r[c];
-  s.k = L.cls[c];
-  s.kxp = L.cls[cxp];
-  s.kxm = L.cls[cxm];
-  s.kyp = L.cls[cyp];
-  s.kym = L.cls[cym];
-  s.kzp = L.cls[czp];
-  s.kzm = L.cls[czm];
+  // the classes the stencil tests, from the cell's coupling mask: 0 where cls == 0, 2 where a
+  // neighbour exists with cls 2 (the only values it compares against)
+  const int m = L.cmask[c];
+  s.k = (m & 64) ? 0 : 1;
+  s.kxp = (m & 1) ? 2 : 0;
+  s.kxm = (m & 2) ? 2 : 0;
+  s.kyp = (m & 4) ? 2 : 0;
+  s.kym = (m & 8) ? 2 : 0;
+  s.kzp = (m & 16) ? 2 : 0;
+  s.kzm = (m & 32) ? 2 : 0;
   s.txp = L.t0[c];
   s.txm = L.t0[cxm];
   s.typ = L.t1[c];
@@ -1254,8 +1280,7 @@ __global__ void k_cg_p_spmv(float* __restrict__ pout, float* __restrict__ q, con
           continue;
         }
         // interior cell: every neighbour exists
-        int kc = L.cls[i], kxp = L.cls[i + sx], kxm = L.cls[i - sx], kyp = L.cls[i + sy], kym = L.cls[i - sy],
-            kzp = L.cls[i + 1], kzm = L.cls[i - 1];
+        int m = L.cmask[i];
         float zc = z[i], zxp = z[i + sx], zxm = z[i - sx], zyp = z[i + sy], zym = z[i - sy], zzp = z[i + 1],
               zzm = z[i - 1];
         float pc = pin[i], pxp = pin[i + sx], pxm = pin[i - sx], pyp = pin[i + sy], pym = pin[i - sy],
@@ -1263,22 +1288,23 @@ __global__ void k_cg_p_spmv(float* __restrict__ pout, float* __restrict__ q, con
         float ad = L.adiag[i], txp = L.t0[i], txm = L.t0[i - sx], typ = L.t1[i], tym = L.t1[i - sy], tzp = L.t2[i],
               tzm = L.t2[i - 1];
         // the loads complete here, together (see get_ax_full)
-        asm volatile("" : "+v"(kc), "+v"(kxp), "+v"(kxm), "+v"(kyp), "+v"(kym), "+v"(kzp), "+v"(kzm), "+v"(zc),
-                     "+v"(zxp), "+v"(zxm), "+v"(zyp), "+v"(zym), "+v"(zzp), "+v"(zzm));
+        asm volatile("" : "+v"(m), "+v"(zc), "+v"(zxp), "+v"(zxm), "+v"(zyp), "+v"(zym), "+v"(zzp), "+v"(zzm));
         asm volatile("" : "+v"(pc), "+v"(pxp), "+v"(pxm), "+v"(pyp), "+v"(pym), "+v"(pzp), "+v"(pzm), "+v"(ad),
                      "+v"(txp), "+v"(txm), "+v"(typ), "+v"(tym), "+v"(tzp), "+v"(tzm));
-        if (kc == 0) continue;
+        if (m & 64) continue;  // cls == 0
+        // a neighbour's new p where its cls != 0 (mask bits 8-13; every neighbour of an interior cell
+        // exists), its off-diagonal term where its cls == 2 (bits 0-5)
         const float nc = zc + beta * pc;
-        const float nxp = kxp != 0 ? zxp + beta * pxp : pxp, nxm = kxm != 0 ? zxm + beta * pxm : pxm;
-        const float nyp = kyp != 0 ? zyp + beta * pyp : pyp, nym = kym != 0 ? zym + beta * pym : pym;
-        const float nzp = kzp != 0 ? zzp + beta * pzp : pzp, nzm = kzm != 0 ? zzm + beta * pzm : pzm;
+        const float nxp = (m & 256) ? zxp + beta * pxp : pxp, nxm = (m & 512) ? zxm + beta * pxm : pxm;
+        const float nyp = (m & 1024) ? zyp + beta * pyp : pyp, nym = (m & 2048) ? zym + beta * pym : pym;
+        const float nzp = (m & 4096) ? zzp + beta * pzp : pzp, nzm = (m & 8192) ? zzm + beta * pzm : pzm;
         float res = nc * ad;
-        res = kxp == 2 ? res + nxp * (-txp) : res;
-        res = kxm == 2 ? res + nxm * (-txm) : res;
-        res = kyp == 2 ? res + nyp * (-typ) : res;
-        res = kym == 2 ? res + nym * (-tym) : res;
-        res = kzp == 2 ? res + nzp * (-tzp) : res;
-        res = kzm == 2 ? res + nzm * (-tzm) : res;
+        res = (m & 1) ? res + nxp * (-txp) : res;
+        res = (m & 2) ? res + nxm * (-txm) : res;
+        res = (m & 4) ? res + nyp * (-typ) : res;
+        res = (m & 8) ? res + nym * (-tym) : res;
+        res = (m & 16) ? res + nzp * (-tzp) : res;
+        res = (m & 32) ? res + nzm * (-tzm) : res;
         pout[i] = nc;
         q[i] = res;
         v[0] += (double)nc * (double)res;
@@ -1415,6 +1441,8 @@ void launch_build_lhs(Ctx& c, float rho, float dt) {
     hipLaunchKernelGGL(k_build_lhs, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.adiag, L.t[0], L.t[1], L.t[2], L.cls,
                        F, (int)l, L.d[0], L.d[1], L.d[2], dt, rho, f * c.g.dx[0], f * c.g.dx[1],
                        f * c.g.dx[2]);
+    hipLaunchKernelGGL(k_build_cmask, dim3(grid_for(L.n)), dim3(kBlock), 0, c.s, L.cmask, L.cls, L.d[0], L.d[1],
+                       L.d[2]);
     const int nrow = L.d[0] * L.d[1];
     hipLaunchKernelGGL(k_row_flags, dim3((unsigned)div_up(nrow, kBlock / 64)), dim3(kBlock), 0, c.s, L.cls, L.aflag,
                        nrow, L.d[2]);

@@ -238,7 +238,8 @@ int alloc_ctx(Ctx& c) {
     }
     q.n = (size_t)q.d[0] * q.d[1] * q.d[2];
     ok = ok && dalloc(c, q.adiag, q.n) && dalloc(c, q.t[0], q.n) && dalloc(c, q.t[1], q.n) &&
-         dalloc(c, q.t[2], q.n) && dalloc(c, q.cls, q.n) && dalloc(c, q.r, q.n) && dalloc(c, q.z, q.n) &&
+         dalloc(c, q.t[2], q.n) && dalloc(c, q.cls, q.n) && dalloc(c, q.cmask, q.n) && dalloc(c, q.r, q.n) &&
+         dalloc(c, q.z, q.n) &&
          dalloc(c, q.z2, q.n) && dalloc(c, q.rows, (size_t)q.d[0] * q.d[1]) && dalloc(c, q.nrows, 1) &&
          dalloc(c, q.aflag, (size_t)q.d[0] * q.d[1]) && dalloc(c, q.qrows, (size_t)q.d[0] * q.d[1]) &&
          dalloc(c, q.nqrows, 1) && dalloc(c, q.qflag, (size_t)q.d[0] * q.d[1]) &&
@@ -323,7 +324,7 @@ void free_ctx(Ctx& c) {
   F(c.bin_count); F(c.bin_start); F(c.bin_bflag); F(c.bin_bsum); F(c.bin_boff); F(c.scan_tmp); F(c.pkey); F(c.pslot); F(c.porder);
   F(c.prec[0]); F(c.prec[1]); F(c.cv_flag);
   for (auto& q : c.lv) {
-    F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.r); F(q.z); F(q.z2); F(q.z64); F(q.z64b); F(q.d64); F(q.d64b);
+    F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.cmask); F(q.r); F(q.z); F(q.z2); F(q.z64); F(q.z64b); F(q.d64); F(q.d64b);
     F(q.rows); F(q.nrows);
     F(q.aflag); F(q.qrows); F(q.nqrows); F(q.qflag); F(q.zrows); F(q.nzrows); F(q.zflag); F(q.tflag); F(q.tiles); F(q.ntiles);
   }

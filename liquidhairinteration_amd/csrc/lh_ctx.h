@@ -83,6 +83,10 @@ struct Level {
   float* adiag = nullptr;
   float* t[3] = {nullptr, nullptr, nullptr};  // term of the +axis face of each cell
   uint8_t* cls = nullptr;                     // 0: phi > 0, 1: phi == 0 (or NaN), 2: phi < 0
+  // the classes a cell's stencil needs, in one load: bits 0-5 the +x, -x, +y, -y, +z, -z neighbour
+  // exists and has cls 2 (the off-diagonal terms), bit 6 the cell's own cls is 0, bits 8-13 the
+  // neighbour exists and has cls != 0 (built with cls by launch_build_lhs)
+  uint16_t* cmask = nullptr;
   float* r = nullptr;
   float* z = nullptr;
   float* z2 = nullptr;  // ping-pong partner of z for the fused red-black sweep
