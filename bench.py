@@ -3,8 +3,8 @@
 Workload (default): two-block collision on a 256^3 MAC grid (configs/collision_256.json):
 two 0.3^3 blocks at opposite corners of the unit cube, 8 jittered particles per
 cell, v = +-(2,0,0) + N(0, 0.05^2), solid walls 3 cells thick, MGPCG (3 levels,
-2/2 red-black Gauss-Seidel sweeps, 12 + 12 on the coarsest, symmetric V-cycle) to 1e-6
-relative residual inside an FP64 residual loop to 1e-9.
+2/2 red-black Gauss-Seidel sweeps, symmetric V-cycle, the coarsest level solved by 24 FP64
+Chebyshev steps) to 1e-6 relative residual inside an FP64 residual loop to 1e-9.
 One step = PreStep + Step + PostStep (its ComputeDivergence, mean and L-inf) + GetCFL
 (main.cpp:50-53) run by the library's native loop; particles and grid stay resident in HBM for
 the whole timed region.
