@@ -50,11 +50,12 @@ struct BinP {
 __device__ __forceinline__ int rec_id(const float4* __restrict__ r0, size_t i) { return __float_as_int(r0[i].w); }
 
 // key[i] = the particle's half-cell bin, arrival[i] = its arrival rank among the bin's particles
-// (the atomic's order: any order serves, the rank kernel orders each bin by id)
+// (the atomic's order: any order serves, the rank kernel orders each bin by id), id[i] = its id
+// (a dense copy of R0.w for the rank kernel's comparisons)
 template <bool kPow2>
 __global__ void k_bin_key(const float4* __restrict__ r0, int64_t P, BinP b, int* __restrict__ key,
-                          int* __restrict__ arrival, int* __restrict__ count, int* __restrict__ bflag,
-                          int* __restrict__ err) {
+                          int* __restrict__ arrival, int* __restrict__ id, int* __restrict__ count,
+                          int* __restrict__ bflag, int* __restrict__ err) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
     const float4 R = r0[i];
     const float x[3] = {R.x, R.y, R.z};
@@ -72,6 +73,7 @@ __global__ void k_bin_key(const float4* __restrict__ r0, int64_t P, BinP b, int*
     if (bad) atomicOr(err, 1);
     int k = (h[0] * b.H[1] + h[1]) * b.H[2] + h[2];
     key[i] = k;
+    id[i] = __float_as_int(R.w);
     arrival[i] = atomicAdd(&count[k], 1);
     bflag[k / kScanBins] = 1;
   }
@@ -89,17 +91,17 @@ __global__ void k_bin_scatter(const int* __restrict__ key, const int* __restrict
 }
 
 // place particle i at start[key] + (number of particles in its bin with a smaller id)
-__global__ void k_bin_rank(const float4* __restrict__ r0, int64_t P, const int* __restrict__ key,
+__global__ void k_bin_rank(const int* __restrict__ id, int64_t P, const int* __restrict__ key,
                            const int* __restrict__ start, const int* __restrict__ slot_owner, int* __restrict__ order) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
     const int k = key[i];
     const int s0 = start[k], s1 = start[k + 1];
     int rank = 0;
     if (s1 - s0 > 1) {  // a bin of one particle needs no comparison
-      const int myid = rec_id(r0, (size_t)i);
+      const int myid = id[i];
       for (int s = s0; s < s1; ++s) {
         const int j = slot_owner[s];
-        const int idj = rec_id(r0, (size_t)j);
+        const int idj = id[j];
         rank += (idj < myid) ? 1 : 0;
       }
     }
@@ -1135,17 +1137,17 @@ int run_binning(Ctx& c) {
   unsigned gp = grid_for((size_t)c.P);
   const float4* rec = (const float4*)c.prec[c.cur];
   if (pow2)
-    hipLaunchKernelGGL(k_bin_key<true>, dim3(gp), dim3(kBlock), 0, c.s, rec, c.P, b, c.pkey, c.porder, c.bin_count,
-                       c.bin_bflag, c.dflag + 1);
+    hipLaunchKernelGGL(k_bin_key<true>, dim3(gp), dim3(kBlock), 0, c.s, rec, c.P, b, c.pkey, c.porder, c.pid,
+                       c.bin_count, c.bin_bflag, c.dflag + 1);
   else
-    hipLaunchKernelGGL(k_bin_key<false>, dim3(gp), dim3(kBlock), 0, c.s, rec, c.P, b, c.pkey, c.porder, c.bin_count,
-                       c.bin_bflag, c.dflag + 1);
+    hipLaunchKernelGGL(k_bin_key<false>, dim3(gp), dim3(kBlock), 0, c.s, rec, c.P, b, c.pkey, c.porder, c.pid,
+                       c.bin_count, c.bin_bflag, c.dflag + 1);
   int rc = bin_scan(c);
   if (rc) return rc;
   // porder holds the arrival ranks until k_bin_rank writes the order over them
   hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, c.s, c.pkey, c.porder, c.P, c.bin_start, c.bin_count,
                      c.pslot);
-  hipLaunchKernelGGL(k_bin_rank, dim3(gp), dim3(kBlock), 0, c.s, rec, c.P, c.pkey, c.bin_start, c.pslot,
+  hipLaunchKernelGGL(k_bin_rank, dim3(gp), dim3(kBlock), 0, c.s, c.pid, c.P, c.pkey, c.bin_start, c.pslot,
                      c.porder);
   hipLaunchKernelGGL(k_bin_gather, dim3(gp), dim3(kBlock), 0, c.s, rec, (float4*)c.prec[1 - c.cur], c.cap, c.P,
                      c.porder);

@@ -321,7 +321,7 @@ void free_ctx(Ctx& c) {
     F(c.lw[a]);
   }
   F(c.pressure); F(c.phi); F(c.rhs); F(c.solid_phi); F(c.is_valid); F(c.xstate); F(c.d_sdf);
-  F(c.bin_count); F(c.bin_start); F(c.bin_bflag); F(c.bin_bsum); F(c.bin_boff); F(c.scan_tmp); F(c.pkey); F(c.pslot); F(c.porder);
+  F(c.bin_count); F(c.bin_start); F(c.bin_bflag); F(c.bin_bsum); F(c.bin_boff); F(c.scan_tmp); F(c.pkey); F(c.pslot); F(c.porder); F(c.pid);
   F(c.prec[0]); F(c.prec[1]); F(c.cv_flag);
   for (auto& q : c.lv) {
     F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.cmask); F(q.r); F(q.z); F(q.z2); F(q.z64); F(q.z64b); F(q.d64); F(q.d64b);
@@ -353,11 +353,12 @@ int ensure_particle_capacity(Ctx& c, int64_t n) {
   if (c.pkey) hipFree(c.pkey);
   if (c.pslot) hipFree(c.pslot);
   if (c.porder) hipFree(c.porder);
+  if (c.pid) hipFree(c.pid);
   c.prec[0] = c.prec[1] = nullptr;
-  c.pkey = c.pslot = c.porder = nullptr;
+  c.pkey = c.pslot = c.porder = c.pid = nullptr;
   int64_t cap = std::max<int64_t>(n, 1);
   bool ok = dalloc(c, c.prec[0], (size_t)kRecFloats * cap) && dalloc(c, c.prec[1], (size_t)kRecFloats * cap) &&
-            dalloc(c, c.pkey, cap) && dalloc(c, c.pslot, cap) && dalloc(c, c.porder, cap);
+            dalloc(c, c.pkey, cap) && dalloc(c, c.pslot, cap) && dalloc(c, c.porder, cap) && dalloc(c, c.pid, cap);
   if (!ok) {
     c.cap = 0;
     return LH_ERR_OOM;

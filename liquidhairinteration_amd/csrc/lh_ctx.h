@@ -149,6 +149,7 @@ struct Ctx {
   int* pkey = nullptr;       // per particle bin key
   int* pslot = nullptr;      // per bin-order slot, the particle the scatter placed there (arrival order)
   int* porder = nullptr;     // per bin-order slot, the particle (source index) that takes it
+  int* pid = nullptr;        // per particle, its id (R0.w), written by the key kernel for the rank kernel
   // the bin scan runs over blocks of kScanBins bins: a flag per block (set by the key kernel
   // for every block that holds a particle, cleared by the scan), the block sums and offsets
   int* bin_bflag = nullptr;
