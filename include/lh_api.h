@@ -223,15 +223,15 @@ int lh_set_field(lh_ctx* ctx, int field, const void* host, size_t count);
  * by hipEventRecord pairs; lh_profile_read returns launch count and summed ms. */
 enum lh_prof_kernel {
   LH_PROF_SMOOTH_L0 = 0,   /* red-black smoother on the finest level: one colour pass (one fused sweep when fused) */
-  LH_PROF_P2G = 1,         /* particle-to-grid gather (one component per launch) */
+  LH_PROF_P2G = 1,         /* particle-to-grid gather (the three components in one launch) */
   LH_PROF_G2P = 2,         /* grid-to-particle + advection */
   LH_PROF_LIQUID_PHI = 3,  /* liquid SDF gather */
   LH_PROF_SPMV = 4,        /* PCG A*p + dot */
   LH_PROF_NN_FWD = 5,      /* warm-start network forward (all layers) */
   LH_PROF_NN_BWD = 6,      /* warm-start network backward (all layers, gradients only) */
   LH_PROF_CONV_D0 = 7,     /* the network's largest convolution: 48 -> 16 channels at full resolution */
-  LH_PROF_SMOOTH_L1 = 8,   /* red-black smoother on multigrid level 1: one colour pass */
-  LH_PROF_SMOOTH_L2 = 9,   /* red-black smoother on levels 2 and coarser: one colour pass */
+  LH_PROF_SMOOTH_L1 = 8,   /* red-black smoother on multigrid level 1: one colour pass (or row sweep) */
+  LH_PROF_SMOOTH_L2 = 9,   /* levels 2 and coarser: a colour pass, a row sweep or two Chebyshev steps */
   LH_PROF_COUNT = 10
 };
 int lh_profile_enable(lh_ctx* ctx, int on);
