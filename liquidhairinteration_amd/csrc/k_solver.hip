@@ -947,11 +947,29 @@ __global__ void __launch_bounds__(256) k_rbgs_sweep(float* __restrict__ zout, co
 // and the coarse cell then sums its 8 children from LDS in the reference's order.  Eight times
 // the threads of a one-coarse-cell-per-thread form keep the loads in flight (finest level
 // 32.8 -> 21.6 us, level 1 12.4 -> 5.2 us per launch at 256^3).
+// The coarse level's first pre-smoothing colour pass (colour 0: cells with x + y + z even) from z = 0,
+// folded into the kernel that zeroes z: with every z zero, (A z)[c] is exactly +0 (0 * adiag and the
+// coupled terms 0 * -term are +0 or -0, and +0 + -0 = +0), so the pass leaves z = relax(0, r / adiag)
+// on the colour's cells of nonzero diagonal and 0 elsewhere, the operations of k_smooth with zv = 0.
+__device__ __forceinline__ float first_pass_from_zero(float rv, float ad, float w, int mode) {
+  if (is_small(ad)) return 0.0f;
+  const float zv = 0.0f;
+  const float term = (rv - 0.0f) / ad;
+  if (mode == 0) {
+    const float wt = w * term;
+    return (float)((1.0 - (double)w) * (double)zv + (double)wt);
+  }
+  return zv + w * term;
+}
+
+// adc / first: the coarse level's adiag and, when first != 0, the first pre-smoothing pass (mode,
+// w) is folded in (first_pass_from_zero)
 __global__ __launch_bounds__(kBlock) void k_restrict(float* __restrict__ rc, float* __restrict__ zc,
                                                           const float* __restrict__ rf, const float* __restrict__ zf,
                                                           LvP Lf, int c1, int c2, const int* __restrict__ rows,
                                                           const int* __restrict__ nrows, int nrow_cap,
-                                                          double* __restrict__ zc64) {
+                                                          double* __restrict__ zc64, const float* __restrict__ adc,
+                                                          int first, float w, int mode) {
   static_assert(kBlock == 256, "4 fine rows x 64 fine z per block");
   __shared__ float s_d[8][32];
   const int nch = (c2 + 31) >> 5;
@@ -976,7 +994,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(float* __restrict__ rc, flo
       for (int j = 0; j < 8; ++j) acc = (float)((double)acc + 0.125 * (double)s_d[j][tid]);
       const size_t i = (size_t)row * c2 + ch * 32 + tid;
       rc[i] = acc;
-      zc[i] = 0.0f;
+      zc[i] = (first && ((xi + yi + ch * 32 + tid) & 1) == 0) ? first_pass_from_zero(acc, adc[i], w, mode) : 0.0f;
       if (zc64) zc64[i] = 0.0;  // the coarsest level's FP64 z
     }
     __syncthreads();
@@ -1161,11 +1179,15 @@ __global__ void k_cg_spmv(float* __restrict__ q, const float* __restrict__ p, Lv
 // x += alpha p, r -= alpha q (and r.r) on the updated cells; with zero_z also z = 0 on every cell
 // of the rows, the symmetric preconditioner's first step (z is read nowhere else).  rho is
 // sc[6 + slot]; block 0 records sc[5] = p.q, sc[2] = alpha and the NaN flag bit 1.
+// zero_z: z = 0 on the liquid rows for the next preconditioner application (symmetric V-cycle);
+// with first != 0 the level's first pre-smoothing colour pass is folded in (first_pass_from_zero,
+// mode 1, the relaxation w; adiag the level-0 diagonal) and the V-cycle skips that pass
 __global__ void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
                                const float* __restrict__ q, const uint8_t* __restrict__ cls, float* __restrict__ sc,
                                int slot, int* __restrict__ flag, const double* __restrict__ part_pq, int npq, int d0,
                                int d1, int d2, const int* __restrict__ rows, const int* __restrict__ nrows,
-                               double* __restrict__ partial, float* __restrict__ z, int zero_z) {
+                               double* __restrict__ partial, float* __restrict__ z, int zero_z,
+                               const float* __restrict__ adiag, int first, float w) {
   const float denom = (float)sum_partials_block(part_pq, npq);
   const float alpha = sc[6 + slot] / denom;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1191,16 +1213,24 @@ __global__ void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r, con
       const size_t base = (size_t)row * d2;
       for (int zi = it.z0; zi < d2; zi += it.zs) {
         const size_t i = base + zi;
-        if (zero_z) z[i] = 0.0f;
-        if (!inner || zi < 1 || zi > d2 - 2) continue;
+        const bool colour0 = first && ((xi + yi + zi) & 1) == 0;
+        float zv = 0.0f;
+        if (!inner || zi < 1 || zi > d2 - 2) {  // r stays: the first pass reads it as it is
+          if (colour0) zv = first_pass_from_zero(r[i], adiag[i], w, 1);
+          if (zero_z) z[i] = zv;
+          continue;
+        }
         int kc = cls[i];
         float xv = x[i], pv = p[i], rv = r[i], qv = q[i];
         asm volatile("" : "+v"(kc), "+v"(xv), "+v"(pv), "+v"(rv), "+v"(qv));
-        if (kc == 0) continue;
-        x[i] = xv + alpha * pv;
-        const float rn = rv - alpha * qv;
-        r[i] = rn;
-        v[0] += (double)rn * (double)rn;
+        if (kc != 0) {
+          x[i] = xv + alpha * pv;
+          rv = rv - alpha * qv;
+          r[i] = rv;
+          v[0] += (double)rv * (double)rv;
+        }
+        if (colour0) zv = first_pass_from_zero(rv, adiag[i], w, 1);
+        if (zero_z) z[i] = zv;
       }
     }
   block_partials<1>(v, partial);
@@ -1515,8 +1545,17 @@ unsigned row_sweep_blocks(const Level& L) {
 }
 
 // one red-black sweep on level l: Smooth(l, first) then Smooth(l, 1 - first).
-// Large levels use the fused tile kernel; small ones (launch-bound) two colour passes.
-void sweep(Ctx& c, int l, int first, int mode) {
+// whether level l is smoothed by two colour passes per sweep (the last branch of sweep() but one)
+bool colour_passes(const Ctx& c, int l) {
+  const Level& L = c.lv[l];
+  if (l > 0 && c.cfg.coarse_sweep == kCoarseRows && (long long)L.n <= c.cfg.row_sweep_max_cells) return false;
+  if (l > 0 && c.cfg.coarse_sweep == kCoarseTiles) return false;
+  return (long long)L.n < c.cfg.fused_sweep_min_cells;
+}
+
+// Large levels use the fused tile kernel; small ones (launch-bound) two colour passes.  skip_first:
+// the first colour pass has been folded into the kernel that zeroed z (colour passes only).
+void sweep(Ctx& c, int l, int first, int mode, bool skip_first = false) {
   Level& L = c.lv[l];
   if (l > 0 && c.cfg.coarse_sweep == kCoarseRows && (long long)L.n <= c.cfg.row_sweep_max_cells) {
     const unsigned nb = row_sweep_blocks(L);
@@ -1546,7 +1585,7 @@ void sweep(Ctx& c, int l, int first, int mode) {
   }
   if ((long long)L.n < c.cfg.fused_sweep_min_cells) {
     const unsigned nb = smooth_blocks(L);
-    for (int k = 0; k < 2; ++k) {
+    for (int k = skip_first ? 1 : 0; k < 2; ++k) {
       const int phase = k == 0 ? first : 1 - first;
       if (l == 0)
         hipLaunchKernelGGL(k_smooth<true>, dim3(nb), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), L.rows, L.nrows, phase,
@@ -1566,17 +1605,15 @@ void sweep(Ctx& c, int l, int first, int mode) {
 }
 
 // `fwd` sweeps starting with colour `first`, then `rev` starting with the other colour, on
-// level l: consecutive launches, timed as one run when profiling
-void sweeps(Ctx& c, int l, int first, int fwd, int rev, int mode) {
+// level l: consecutive launches, timed as one run when profiling.  skip_first: the first sweep's
+// first colour pass has been folded into the kernel that zeroed z (colour_passes(l) only).
+void sweeps(Ctx& c, int l, int first, int fwd, int rev, int mode, bool skip_first = false) {
   if (fwd + rev <= 0) return;
-  const bool fused = (long long)c.lv[l].n >= c.cfg.fused_sweep_min_cells ||
-                     (l > 0 && (c.cfg.coarse_sweep == kCoarseTiles ||
-                                (c.cfg.coarse_sweep == kCoarseRows &&
-                                 (long long)c.lv[l].n <= c.cfg.row_sweep_max_cells)));
+  const bool passes = colour_passes(c, l);
   ProfScope ps(c, l == 0 ? LH_PROF_SMOOTH_L0 : (l == 1 ? LH_PROF_SMOOTH_L1 : LH_PROF_SMOOTH_L2),
-               (fwd + rev) * (fused ? 1 : 2));
-  for (int it = 0; it < fwd; ++it) sweep(c, l, first, mode);
-  for (int it = 0; it < rev; ++it) sweep(c, l, 1 - first, mode);
+               (fwd + rev) * (passes ? 2 : 1) - (skip_first ? 1 : 0));
+  for (int it = 0; it < fwd; ++it) sweep(c, l, first, mode, skip_first && it == 0);
+  for (int it = 0; it < rev; ++it) sweep(c, l, 1 - first, mode, skip_first && fwd == 0 && it == 0);
 }
 
 // The coarsest level's sweeps in FP64 (coarsest_fp64): `fwd` row sweeps starting with colour 0,
@@ -1648,7 +1685,9 @@ void coarsest_chebyshev64(Ctx& c) {
 
 // ApplyPreconditioner, PressureSolver.cpp:719-772 (symmetric mode: z[0] = 0 first,
 // post-smoothing and the second half of the coarsest sweeps in reverse colour order)
-void apply_precond(Ctx& c, bool z_zeroed) {
+// first0: the caller's zeroing of z has also done level 0's first pre-smoothing colour pass
+// (k_cg_update_xr with first, symmetric V-cycle with colour passes on level 0)
+void apply_precond(Ctx& c, bool z_zeroed, bool first0 = false) {
   Level& L0 = c.lv[0];
   if (c.cfg.precond != LH_PRECOND_MULTIGRID) {
     hipLaunchKernelGGL(k_precond_simple, dim3(grid_for(L0.n)), dim3(kBlock), 0, c.s, L0.z, L0.r, L0.adiag, L0.cls,
@@ -1662,14 +1701,19 @@ void apply_precond(Ctx& c, bool z_zeroed) {
   // unless the caller has cleared it
   if (sym && !z_zeroed)
     hipLaunchKernelGGL(k_zero_rows, dim3(row_blocks(L0)), dim3(kBlock), 0, c.s, L0.z, L0.rows, L0.nrows, L0.d[2]);
+  bool folded = first0;  // level l - 1's first pre-smoothing pass done by the kernel that zeroed its z
   for (int l = 1; l < L; ++l) {
-    sweeps(c, l - 1, 0, c.cfg.num_pre, 0, mode);
+    sweeps(c, l - 1, 0, c.cfg.num_pre, 0, mode, folded);
     Level& Lc = c.lv[l];
     Level& Lf = c.lv[l - 1];
+    // the restriction zeroes z of level l; a level that pre-smooths with colour passes gets its
+    // first pass there
+    folded = l < L - 1 && c.cfg.num_pre > 0 && colour_passes(c, l);
     const size_t nch = div_up((size_t)Lc.d[2], (size_t)32);
     const unsigned nb = (unsigned)std::min<size_t>((size_t)Lc.d[0] * Lc.d[1] * nch, 16384);
     hipLaunchKernelGGL(k_restrict, dim3(nb), dim3(kBlock), 0, c.s, Lc.r, Lc.z, Lf.r, Lf.z, lvp(Lf), Lc.d[1], Lc.d[2],
-                       Lc.qrows, Lc.nqrows, Lc.d[0] * Lc.d[1], Lc.z64);
+                       Lc.qrows, Lc.nqrows, Lc.d[0] * Lc.d[1], Lc.z64, Lc.adiag, folded ? 1 : 0, c.cfg.omega,
+                       mode);
   }
   if (c.cfg.coarsest_cheb > 0)
     coarsest_chebyshev64(c);
@@ -1848,9 +1892,13 @@ void pcg_iteration_launches(Ctx& c, int slot, double* part_pq, double* part_rr) 
   Level& L0 = c.lv[0];
   const int nb = c.npartial;
   const int sym = c.cfg.precond == LH_PRECOND_MULTIGRID && c.cfg.smoother == LH_SMOOTHER_SYMMETRIC;
+  // the symmetric V-cycle zeroes level-0 z here; with colour passes on level 0 (and more than one
+  // level, so that level 0 pre-smooths) its first pass is folded in
+  const bool first0 = sym && c.lv.size() > 1 && c.cfg.num_pre > 0 && colour_passes(c, 0);
   hipLaunchKernelGGL(k_cg_update_xr, dim3(nb), dim3(kBlock), 0, c.s, c.x, L0.r, c.p, c.q, L0.cls, c.dsc, slot,
-                     c.dflag + 2, part_pq, nb, L0.d[0], L0.d[1], L0.d[2], L0.rows, L0.nrows, part_rr, L0.z, sym);
-  apply_precond(c, sym != 0);
+                     c.dflag + 2, part_pq, nb, L0.d[0], L0.d[1], L0.d[2], L0.rows, L0.nrows, part_rr, L0.z, sym,
+                     L0.adiag, first0 ? 1 : 0, c.cfg.omega);
+  apply_precond(c, sym != 0, first0);
   hipLaunchKernelGGL(k_cg_dot_rz, dim3(nb), dim3(kBlock), 0, c.s, L0.r, L0.z, L0.rows, L0.nrows, L0.d[2], c.partial,
                      part_rr, nb, c.dsc, c.dflag + 2, c.h_pin_dev, c.h_flag_dev);
 }
