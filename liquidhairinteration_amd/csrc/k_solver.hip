@@ -981,6 +981,9 @@ __global__ __launch_bounds__(kBlock) void k_restrict(float* __restrict__ rc, flo
     const int ch = k - (k / nch) * nch;
     const int xi = row / c1, yi = row - xi * c1;
     const int zi = ch * 32 + (wl >> 1);
+    const bool out = tid < 32 && ch * 32 + tid < c2;  // the threads that write the coarse cells
+    const size_t i = (size_t)row * c2 + ch * 32 + tid;
+    const float av = (first && out) ? adc[i] : 0.0f;  // issued before the fine-level work
     if (zi < c2) {
       const int x = 2 * xi + qx, y = 2 * yi + qy, w = 2 * zi + (wl & 1);
       const size_t f = ((size_t)x * Lf.d[1] + y) * Lf.d[2] + w;
@@ -988,13 +991,12 @@ __global__ __launch_bounds__(kBlock) void k_restrict(float* __restrict__ rc, flo
       s_d[q][wl >> 1] = g.v - g.ax;
     }
     __syncthreads();
-    if (tid < 32 && ch * 32 + tid < c2) {
+    if (out) {
       float acc = 0.0f;
 #pragma unroll
       for (int j = 0; j < 8; ++j) acc = (float)((double)acc + 0.125 * (double)s_d[j][tid]);
-      const size_t i = (size_t)row * c2 + ch * 32 + tid;
       rc[i] = acc;
-      zc[i] = (first && ((xi + yi + ch * 32 + tid) & 1) == 0) ? first_pass_from_zero(acc, adc[i], w, mode) : 0.0f;
+      zc[i] = (first && ((xi + yi + ch * 32 + tid) & 1) == 0) ? first_pass_from_zero(acc, av, w, mode) : 0.0f;
       if (zc64) zc64[i] = 0.0;  // the coarsest level's FP64 z
     }
     __syncthreads();
@@ -1222,14 +1224,15 @@ __global__ void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r, con
         }
         int kc = cls[i];
         float xv = x[i], pv = p[i], rv = r[i], qv = q[i];
-        asm volatile("" : "+v"(kc), "+v"(xv), "+v"(pv), "+v"(rv), "+v"(qv));
+        float av = first ? adiag[i] : 0.0f;  // with the other operands: one wait, not two
+        asm volatile("" : "+v"(kc), "+v"(xv), "+v"(pv), "+v"(rv), "+v"(qv), "+v"(av));
         if (kc != 0) {
           x[i] = xv + alpha * pv;
           rv = rv - alpha * qv;
           r[i] = rv;
           v[0] += (double)rv * (double)rv;
         }
-        if (colour0) zv = first_pass_from_zero(rv, adiag[i], w, 1);
+        if (colour0) zv = first_pass_from_zero(rv, av, w, 1);
         if (zero_z) z[i] = zv;
       }
     }
