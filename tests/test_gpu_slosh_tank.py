@@ -33,6 +33,8 @@ def slosh(N, refined_tol=None):
     cfg = json.load(open(lh.config_path("slosh_tank_256")))
     cfg["gridDimension"] = [N, N, N]
     cfg["Boundary"]["left"]["halfExtent"] = [0.5 - 3.0 / N] * 3
+    if N < 256:  # four levels on the reduced grids (five would leave a 2^3 coarsest level at 32^3)
+        cfg["SolverParameter"]["numLevel"] = 4
     if refined_tol is not None:
         cfg["refinedTolerance"] = refined_tol
     return cfg
