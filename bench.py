@@ -91,8 +91,9 @@ def algorithmic_bytes(kernel, n0, P, active, fused, per_launch=None, sweep_b=Non
 
     smooth_l0: one colour pass = half of a red-black sweep, whose minimum is 28 B/cell
                (Adiag + 3 face terms = 16 B, r 4 B, z read 4 B + write 4 B) -> 14 B per
-               cell of the level-0 rows holding liquid (the pass visits only those); the
-               fused one-launch sweep, when selected, is dense: 28 B/cell;
+               cell of the level-0 z-segments holding liquid (active["seg0"]: the pass
+               visits only those), or of the level-0 rows holding liquid when the level keeps
+               no segment list; the fused one-launch sweep, when selected, is dense: 28 B/cell;
     smooth_l1, smooth_l2: the same per cell of the rows holding liquid on level 1, and on
                level 2 (the coarsest of the three-level hierarchy): 28 B per sweep, times the
                sweeps of one launch (per_launch[l]: 1/2 for a colour pass, 1 for k_sweep_rows or
@@ -107,12 +108,13 @@ def algorithmic_bytes(kernel, n0, P, active, fused, per_launch=None, sweep_b=Non
     g2p:       72 B/particle (read position, write position, velocity, C) + 3
                velocity components read around the particles (~12 B/cell);
     liquid_phi: 12 B/particle + 4 B/cell.
-    active[l] = cells of the level-l rows that hold liquid.
+    active[l] = cells of the level-l rows that hold liquid; active["seg0"] = cells of the
+    level-0 z-segments that hold liquid (0 without a segment list).
     """
     per_launch = per_launch or {}
     sweep_b = sweep_b or {}
     return {
-        "smooth_l0": 28.0 * n0 if fused else 14.0 * active[0],
+        "smooth_l0": 28.0 * n0 if fused else 14.0 * (active.get("seg0") or active[0]),
         "smooth_l1": sweep_b.get(1, 28.0) * per_launch.get(1, 0.5) * active.get(1, 0),
         "smooth_l2": sweep_b.get(2, 28.0) * per_launch.get(2, 0.5) * active.get(2, 0),
         "spmv": 32.0 * active[0],
@@ -126,6 +128,16 @@ def algorithmic_bytes(kernel, n0, P, active, fused, per_launch=None, sweep_b=Non
 # symbols depend on the configuration (level_symbol)
 KERNEL_SYMBOL = {"smooth_l0": "k_smooth<true>", "spmv": "k_cg_p_spmv", "p2g": "k_p2g", "g2p": "k_g2p_advect",
                  "liquid_phi": "k_liquid_phi"}
+
+
+def kernel_symbols(cfg):
+    """KERNEL_SYMBOL for this configuration: the finest colour passes run over the level's z-segment
+    list (k_smooth_seg) when its z extent is a multiple of 64 cells (at most 31 segments)"""
+    sym = dict(KERNEL_SYMBOL)
+    d2 = cfg["gridDimension"][2]
+    if d2 % 64 == 0 and d2 // 64 <= 31:
+        sym["smooth_l0"] = "k_smooth_seg"
+    return sym
 
 
 def smooth_grid(d):
@@ -196,7 +208,7 @@ def pmc_reduce(fetch_csv, write_csv, workload, cfg):
     known bytes, and the scaled HBM bytes per dispatch"""
     import csv
 
-    symbols = dict(KERNEL_SYMBOL)
+    symbols = kernel_symbols(cfg)
     level_grid = {}
     for l in (1, 2):
         if l < cfg["SolverParameter"]["numLevel"]:
@@ -486,6 +498,8 @@ def main():
     for lvl in range(cfg["SolverParameter"]["numLevel"]):
         (_, _, d2), nr = model.GetSolver().LevelInfo(lvl)
         active[lvl], rows[lvl] = nr * d2, nr
+    seg_cells, nseg0 = model.GetSolver().LevelSegments(0)
+    active["seg0"] = seg_cells * nseg0
     fused = cfg["SolverParameter"].get("fusedSweepMinCells", 1 << 62) <= n0
     per_launch = {l: sweeps_per_launch(cfg, l) for l in range(1, cfg["SolverParameter"]["numLevel"])}
     sweep_b = {l: sweep_bytes(cfg, l) for l in range(1, cfg["SolverParameter"]["numLevel"])}
@@ -502,7 +516,7 @@ def main():
     launches, tot_ms = prof[dom]
     avg_s = tot_ms / 1e3 / max(launches, 1)
     ach = rate(dom)
-    symbol = level_symbol(cfg, int(dom[-1]))[0] if dom in ("smooth_l1", "smooth_l2") else KERNEL_SYMBOL[dom]
+    symbol = level_symbol(cfg, int(dom[-1]))[0] if dom in ("smooth_l1", "smooth_l2") else kernel_symbols(cfg)[dom]
     roof = {"kernel": dom, "symbol": symbol, "bound": "hbm", "achieved": round(ach, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(dom, workload, symbol), "launches": launches,
@@ -530,6 +544,7 @@ def main():
         "data": "synthetic (seeded jittered particle blocks)",
         "config": {"workload": workload,
                    "grid": list(scene.GetGridDimension()), "particles": P, "active_rows_per_level": rows,
+                   "active_segments_level0": nseg0,
                    "solver": "MGPCG L=%d pre/post=%d/%d coarsest=%s %s, tol %g %s" % (
                        cfg["SolverParameter"]["numLevel"], cfg["SolverParameter"]["numPreSmooth"],
                        cfg["SolverParameter"]["numPostSmooth"], coarsest_label(cfg),

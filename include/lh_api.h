@@ -159,6 +159,12 @@ int lh_get_solution64(lh_ctx* ctx, double* host, size_t count);
 /* multigrid level l of the built operator: its dims and the number of z-rows (x, y)
  * holding a liquid cell -- the rows the smoother and the PCG vector kernels visit */
 int lh_solver_level_info(lh_ctx* ctx, int level, int dims[3], int64_t* active_rows);
+/* multigrid level l of the built operator: the z-segments its colour passes visit when it keeps a
+ * segment list (the finest level, when its z extent is a multiple of 64 cells): the cells per
+ * segment (0 without a list) and the number of segments holding a liquid cell.  No counterpart in
+ * the reference: introspection of this library's work lists, for profiling (as
+ * lh_solver_level_info). */
+int lh_solver_level_segments(lh_ctx* ctx, int level, int* segment_cells, int64_t* segments);
 /* initial guess used by the following pressure solves (NULL removes it) */
 int lh_set_warm_start(lh_ctx* ctx, const float* x0, size_t count);
 

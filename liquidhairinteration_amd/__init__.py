@@ -127,6 +127,7 @@ def load_library():
         "lh_get_solution64": ([vp, dp, C.c_size_t], C.c_int),
         "lh_set_warm_start": ([vp, fp, C.c_size_t], C.c_int),
         "lh_solver_level_info": ([vp, C.c_int, ip, C.POINTER(C.c_int64)], C.c_int),
+        "lh_solver_level_segments": ([vp, C.c_int, ip, C.POINTER(C.c_int64)], C.c_int),
         "lh_write_positions": ([vp, C.c_char_p, C.c_int], C.c_int),
         "lh_save": ([vp, C.c_char_p], C.c_int),
         "lh_set_boundary": ([vp, C.c_char_p], C.c_int),
@@ -295,6 +296,14 @@ class CGSolver:
         rows = C.c_int64()
         self._h.check(self._h.L.lh_solver_level_info(self._h.h, int(level), dims, C.byref(rows)))
         return tuple(dims), int(rows.value)
+
+    def LevelSegments(self, level):
+        """(cells per z-segment, segments holding liquid) of level `level`'s colour-pass work list;
+        (0, 0) when the level keeps no segment list"""
+        cells = C.c_int()
+        n = C.c_int64()
+        self._h.check(self._h.L.lh_solver_level_segments(self._h.h, int(level), C.byref(cells), C.byref(n)))
+        return int(cells.value), int(n.value)
 
     def SetWarmStart(self, x0):
         if x0 is None:

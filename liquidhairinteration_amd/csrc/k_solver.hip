@@ -166,15 +166,70 @@ __device__ __forceinline__ XcdRun xcd_run(int nr, int R, int sub) {
   return q;
 }
 
-// flag[row] = 1 when the z-row holds a cell with cls != 0; one wave per row
-__global__ void k_row_flags(const uint8_t* __restrict__ cls, int* __restrict__ flag, int nrow, int d2) {
+// flag[row] = 1 when the z-row holds a cell with cls != 0; one wave per row.  With smask (d2 a
+// multiple of kSegZ == 64), also bit s of smask[row] when its z-segment s holds such a cell.
+static_assert(kSegZ == 64, "one wave-wide load per segment");
+__global__ void k_row_flags(const uint8_t* __restrict__ cls, int* __restrict__ flag, int nrow, int d2,
+                            int* __restrict__ smask) {
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   if (row >= nrow) return;
-  bool any = false;
-  for (int z = lane; z < d2; z += 64) any = any || cls[(size_t)row * d2 + z] != 0;
-  const bool hit = __ballot(any) != 0;
-  if (lane == 0) flag[row] = hit ? 1 : 0;
+  int mask = 0;
+  for (int s = 0; s * 64 < d2; ++s) {  // uniform trip count
+    const int z = s * 64 + lane;
+    const bool v = z < d2 && cls[(size_t)row * d2 + z] != 0;
+    if (__ballot(v) != 0) mask |= 1 << (s & 31);
+  }
+  if (lane == 0) {
+    flag[row] = mask != 0 ? 1 : 0;
+    if (smask) smask[row] = mask;
+  }
+}
+
+// The finest level's segment list: for the rows of the (ascending) row list, the ids row * nseg +
+// s of the segments that smask flags, ascending.  One block of 1024 threads; wave w owns a
+// contiguous run of 64-entry chunks of the row list: pass 1 counts its segments, the block forms
+// the waves' offsets, pass 2 writes each entry's segments at its offset plus the wave's inclusive
+// scan of the counts.
+__global__ __launch_bounds__(1024) void k_seg_compact(const int* __restrict__ rows, const int* __restrict__ nrows,
+                                                      const int* __restrict__ smask, int nseg,
+                                                      int* __restrict__ segs, int* __restrict__ nsegs) {
+  __shared__ int s_w[16];
+  const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+  const int nr = *nrows;
+  const int nch = (nr + 63) >> 6;
+  const int per = (nch + 15) >> 4;
+  const int c0 = min(nch, w * per), c1 = min(nch, c0 + per);
+  int cnt = 0;
+  for (int ch = c0; ch < c1; ++ch) {
+    const int i = ch * 64 + lane;
+    if (i < nr) cnt += __popc(smask[rows[i]]);
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+  if (lane == 0) s_w[w] = cnt;
+  __syncthreads();
+  int pos = 0;
+  for (int k = 0; k < w; ++k) pos += s_w[k];
+  for (int ch = c0; ch < c1; ++ch) {
+    const int i = ch * 64 + lane;
+    const int row = i < nr ? rows[i] : 0;
+    const int m = i < nr ? smask[row] : 0;
+    const int n = __popc(m);
+    int inc = n;  // inclusive scan of the entries' counts over the wave
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(inc, o);
+      if (lane >= o) inc += u;
+    }
+    int at = pos + inc - n;
+    for (int sg = 0; sg < nseg; ++sg)
+      if (m & (1 << sg)) segs[at++] = row * nseg + sg;
+    pos += __shfl(inc, 63);
+  }
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int k = 0; k < 16; ++k) tot += s_w[k];
+    *nsegs = tot;
+  }
 }
 // ascending list of the flagged rows (one block of 1024 threads; deterministic order).  Wave w
 // owns a contiguous run of 64-flag segments; each segment is one coalesced load.  Pass 1 counts
@@ -413,6 +468,42 @@ __global__ __launch_bounds__(kBlock) void k_smooth(float* __restrict__ z, const 
       } else {
         z[i] = zv + w * term;
       }
+    }
+  }
+}
+
+// The finest level's colour pass over its segment list (k_smooth<true> restricted to the z-segments
+// that hold a liquid cell: a segment without one has adiag 0 everywhere, and k_smooth skips such
+// cells): eight segments of kSegZ cells per 256-thread block, a thread one cell of the pass's colour
+// per segment; the segments in per-XCD runs of the list.
+__global__ __launch_bounds__(kBlock) void k_smooth_seg(float* __restrict__ z, const float* __restrict__ r, LvP L,
+                                                       const int* __restrict__ segs, const int* __restrict__ nsegs,
+                                                       int nseg, int phase, float w, int mode) {
+  static_assert(kBlock == 8 * (kSegZ / 2), "eight segments per block");
+  const int d1 = L.d[1], d2 = L.d[2];
+  const int sub = (int)threadIdx.x >> 5, h = (int)threadIdx.x & 31;
+  const int kcap = L.d[0] * d1 * nseg - 1;
+  const XcdRun run = xcd_run(*nsegs, 8, sub);
+  const int kstep = run.step;
+  int k = run.k;
+  int seg_next = segs[min(k, kcap)];
+  for (; k < run.end; k += kstep) {
+    const int id = seg_next;
+    seg_next = segs[min(k + kstep, kcap)];
+    const int row = id / nseg, sg = id - row * nseg;
+    const int xi = row / d1, yi = row - xi * d1;
+    const int zi = sg * kSegZ + 2 * h + ((xi + yi + phase) & 1);
+    const size_t i = (size_t)row * d2 + zi;
+    if (is_small(L.adiag[i])) continue;
+    const AxAt g = get_ax_full(L, z, r, xi, yi, zi, i);
+    const float ad = g.ad, rv = g.v, zv = g.zc;
+    if (is_small(ad)) continue;
+    float term = (rv - g.ax) / ad;
+    if (mode == 0) {
+      float wt = w * term;
+      z[i] = (float)((1.0 - (double)w) * (double)zv + (double)wt);
+    } else {
+      z[i] = zv + w * term;
     }
   }
 }
@@ -1478,8 +1569,11 @@ void launch_build_lhs(Ctx& c, float rho, float dt) {
                        L.d[2]);
     const int nrow = L.d[0] * L.d[1];
     hipLaunchKernelGGL(k_row_flags, dim3((unsigned)div_up(nrow, kBlock / 64)), dim3(kBlock), 0, c.s, L.cls, L.aflag,
-                       nrow, L.d[2]);
+                       nrow, L.d[2], L.smask);
     hipLaunchKernelGGL(k_row_compact, dim3(1), dim3(1024), 0, c.s, L.aflag, nrow, L.rows, L.nrows);
+    if (L.segs)
+      hipLaunchKernelGGL(k_seg_compact, dim3(1), dim3(1024), 0, c.s, L.rows, L.nrows, L.smask, L.d[2] / kSegZ, L.segs,
+                         L.nsegs);
   }
   // levels >= 1: rows where restriction makes r nonzero, finest first (each needs the finer
   // level's set) ...
@@ -1531,6 +1625,13 @@ unsigned smooth_blocks(const Level& L) {
   const int R = H >= kBlock ? 1 : kBlock / H;
   const size_t need = div_up((size_t)L.d[0] * L.d[1], (size_t)R);
   return (unsigned)(div_up(std::max<size_t>(1, std::min<size_t>(need, 8192)), 8) * 8);  // xcd_run: 8 | grid
+}
+
+// blocks for k_smooth_seg: eight segments per block over the list's capacity, capped (the kernel
+// strides); a multiple of 8 for xcd_run
+unsigned seg_blocks(const Level& L) {
+  const size_t need = div_up((size_t)L.d[0] * L.d[1] * (L.d[2] / kSegZ), (size_t)8);
+  return (unsigned)(div_up(std::max<size_t>(1, std::min<size_t>(need, 8192)), 8) * 8);
 }
 
 // blocks for a kernel over the cells of a level's row list (RowIter mapping; the kernel strides)
@@ -1590,7 +1691,10 @@ void sweep(Ctx& c, int l, int first, int mode, bool skip_first = false) {
     const unsigned nb = smooth_blocks(L);
     for (int k = skip_first ? 1 : 0; k < 2; ++k) {
       const int phase = k == 0 ? first : 1 - first;
-      if (l == 0)
+      if (l == 0 && L.segs)
+        hipLaunchKernelGGL(k_smooth_seg, dim3(seg_blocks(L)), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), L.segs, L.nsegs,
+                           L.d[2] / kSegZ, phase, c.cfg.omega, mode);
+      else if (l == 0)
         hipLaunchKernelGGL(k_smooth<true>, dim3(nb), dim3(kBlock), 0, c.s, L.z, L.r, lvp(L), L.rows, L.nrows, phase,
                            c.cfg.omega, mode);
       else

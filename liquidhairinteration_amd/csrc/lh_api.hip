@@ -246,6 +246,9 @@ int alloc_ctx(Ctx& c) {
          dalloc(c, q.zrows, (size_t)q.d[0] * q.d[1]) && dalloc(c, q.nzrows, 1) &&
          dalloc(c, q.zflag, (size_t)q.d[0] * q.d[1]) && dalloc(c, q.tflag, div_up(q.d[0], 4) * div_up(q.d[1], 4)) &&
          dalloc(c, q.tiles, div_up(q.d[0], 4) * div_up(q.d[1], 4)) && dalloc(c, q.ntiles, 1);
+    if (l == 0 && q.d[2] % kSegZ == 0 && q.d[2] / kSegZ <= 31)
+      ok = ok && dalloc(c, q.smask, (size_t)q.d[0] * q.d[1]) &&
+           dalloc(c, q.segs, (size_t)q.d[0] * q.d[1] * (q.d[2] / kSegZ)) && dalloc(c, q.nsegs, 1);
   }
   if (c.cfg.coarsest_fp64 && c.cfg.precond == LH_PRECOND_MULTIGRID) {
     // the FP64 coarsest sweeps are row sweeps: the coarsest level must be a coarse level swept by rows
@@ -326,6 +329,7 @@ void free_ctx(Ctx& c) {
   for (auto& q : c.lv) {
     F(q.adiag); F(q.t[0]); F(q.t[1]); F(q.t[2]); F(q.cls); F(q.cmask); F(q.r); F(q.z); F(q.z2); F(q.z64); F(q.z64b); F(q.d64); F(q.d64b);
     F(q.rows); F(q.nrows);
+    F(q.smask); F(q.segs); F(q.nsegs);
     F(q.aflag); F(q.qrows); F(q.nqrows); F(q.qflag); F(q.zrows); F(q.nzrows); F(q.zflag); F(q.tflag); F(q.tiles); F(q.ntiles);
   }
   F(c.p); F(c.p2); F(c.x); F(c.q); F(c.partial); F(c.dsc); F(c.dsc64); F(c.dmaxv); F(c.dflag);
@@ -1109,6 +1113,26 @@ int lh_solver_level_info(lh_ctx* ctx, int level, int dims[3], int64_t* active_ro
       hipMemcpyAsync(&n, L.nrows, sizeof(int), hipMemcpyDeviceToHost, c.s);
       if (!hip_ok(c, hipStreamSynchronize(c.s), "level info")) return LH_ERR_DEVICE;
       *active_rows = n;
+    }
+    return LH_OK;
+  });
+}
+
+int lh_solver_level_segments(lh_ctx* ctx, int level, int* segment_cells, int64_t* segments) {
+  return guarded(ctx, [=](Ctx& c) {
+    if (level < 0 || level >= (int)c.lv.size()) {
+      c.err = "lh_solver_level_segments: no such level";
+      return LH_ERR_ARG;
+    }
+    const Level& L = c.lv[level];
+    if (segment_cells) *segment_cells = L.segs ? kSegZ : 0;
+    if (segments) {
+      int n = 0;
+      if (L.segs) {
+        hipMemcpyAsync(&n, L.nsegs, sizeof(int), hipMemcpyDeviceToHost, c.s);
+        if (!hip_ok(c, hipStreamSynchronize(c.s), "level segments")) return LH_ERR_DEVICE;
+      }
+      *segments = n;
     }
     return LH_OK;
   });

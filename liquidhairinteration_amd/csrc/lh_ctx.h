@@ -77,6 +77,8 @@ struct GridP {
 enum { kCoarsePasses = 0, kCoarseRows = 1, kCoarseTiles = 2 };
 constexpr int kPinDiv = 24;  // first of the three pinned host doubles the divergence reduction writes
 
+constexpr int kSegZ = 64;  // cells per z-segment of the finest level's work list
+
 struct Level {
   int d[3] = {0, 0, 0};
   size_t n = 0;
@@ -98,6 +100,11 @@ struct Level {
   int* rows = nullptr;   // capacity d[0] * d[1]
   int* nrows = nullptr;  // device count
   int* aflag = nullptr;  // per-row flag of that set
+  // level 0 when d[2] is a multiple of kSegZ: the kSegZ-cell z-segments of those rows that hold a
+  // liquid cell, ascending as row * (d[2] / kSegZ) + segment (the finest colour passes' work list)
+  int* smask = nullptr;  // per row, bit s: segment s holds a cell with cls != 0
+  int* segs = nullptr;   // capacity d[0] * d[1] * d[2] / kSegZ
+  int* nsegs = nullptr;  // device count
   // levels >= 1, rows of the restriction: where it can produce a nonzero r (parents of the
   // finer level's liquid and restriction rows) or must clear z (the z-support below)
   int* qrows = nullptr;

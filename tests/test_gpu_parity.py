@@ -271,6 +271,11 @@ def test_dam_break_128_free_running_bit_exact():
     po, vo, _ = orc.particles()
     same(pg, po, "positions after 3 steps")
     same(vg, vo, "velocities after 3 steps")
+    # the finest colour passes ran over the z-segment list (128 = two 64-cell segments per row):
+    # every row holding liquid has one or two segments holding liquid
+    (_, _, d2), rows = m.GetSolver().LevelInfo(0)
+    cells, segs = m.GetSolver().LevelSegments(0)
+    assert d2 == 128 and cells == 64 and rows <= segs <= 2 * rows, (rows, cells, segs)
     scene.close()
 
 
