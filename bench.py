@@ -101,8 +101,9 @@ def algorithmic_bytes(kernel, n0, P, active, fused, per_launch=None, sweep_b=Non
                launch of the coarsest level's two Chebyshev steps (sweep_b[l]); with more than
                three levels smooth_l2 times every level from 2 down, its bytes count level 2's;
     spmv:      the p update fused with q = A p (k_cg_p_spmv, all but the first SpMV of a solve):
-               z and p read, new p and q written, 4 coefficient floats -> 32 B per cell of
-               those rows (the plain first SpMV moves 24 B);
+               z and p read, new p and q written, 4 coefficient floats -> 32 B per cell it
+               visits: of the level-0 z-segments holding liquid, or of the rows holding liquid
+               without a segment list (the plain first SpMV moves 24 B);
     p2g:       the three components in one launch: 60 B/particle (position, velocity,
                affine matrix) + 4 B per face written (~3 n0 faces);
     g2p:       72 B/particle (read position, write position, velocity, C) + 3
@@ -117,7 +118,7 @@ def algorithmic_bytes(kernel, n0, P, active, fused, per_launch=None, sweep_b=Non
         "smooth_l0": 28.0 * n0 if fused else 14.0 * (active.get("seg0") or active[0]),
         "smooth_l1": sweep_b.get(1, 28.0) * per_launch.get(1, 0.5) * active.get(1, 0),
         "smooth_l2": sweep_b.get(2, 28.0) * per_launch.get(2, 0.5) * active.get(2, 0),
-        "spmv": 32.0 * active[0],
+        "spmv": 32.0 * (active.get("seg0") or active[0]),
         "p2g": 60.0 * P + 12.0 * n0,
         "g2p": 72.0 * P + 12.0 * n0,
         "liquid_phi": 12.0 * P + 4.0 * n0,

@@ -150,6 +150,27 @@ __device__ __forceinline__ RowIter row_iter(int d2) {
   return it;
 }
 
+// An item of a level-0 work list: a z-row (nseg == 0: the list holds rows, every cell of the row)
+// or a z-segment (the list holds row * nseg + s: the kSegZ cells from kSegZ s).  Kernels map
+// threads to an item's cells as RowIter does to a row of the item's length (list_iter).
+struct WorkItem {
+  int row, zlo, zhi;
+};
+__device__ __forceinline__ WorkItem work_item(int id, int nseg, int d2) {
+  WorkItem w;
+  if (nseg == 0) {
+    w.row = id;
+    w.zlo = 0;
+    w.zhi = d2;
+  } else {
+    w.row = id / nseg;
+    w.zlo = (id - w.row * nseg) * kSegZ;
+    w.zhi = w.zlo + kSegZ;
+  }
+  return w;
+}
+__device__ __forceinline__ RowIter list_iter(int d2, int nseg) { return row_iter(nseg ? kSegZ : d2); }
+
 // A row list split into 8 contiguous runs, one per XCD: workgroups are dispatched to the 8 XCDs
 // round-robin, so workgroup b serves run b % 8, and a row's x/y neighbours (adjacent and nearby
 // list entries) are mostly read through the same XCD's L2.  The grid is a multiple of 8; R list
@@ -1280,7 +1301,7 @@ __global__ void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r, con
                                int slot, int* __restrict__ flag, const double* __restrict__ part_pq, int npq, int d0,
                                int d1, int d2, const int* __restrict__ rows, const int* __restrict__ nrows,
                                double* __restrict__ partial, float* __restrict__ z, int zero_z,
-                               const float* __restrict__ adiag, int first, float w) {
+                               const float* __restrict__ adiag, int first, float w, int nseg) {
   const float denom = (float)sum_partials_block(part_pq, npq);
   const float alpha = sc[6 + slot] / denom;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1289,22 +1310,23 @@ __global__ void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r, con
     if (isnan(alpha) || isinf(alpha)) atomicOr(flag, 1);
   }
   double v[1] = {0.0};
-  const RowIter it = row_iter(d2);
-  // the next row id is fetched with the current row's operands; a cell's class and its four
+  const RowIter it = list_iter(d2, nseg);
+  // the next item id is fetched with the current item's operands; a cell's class and its four
   // operands are loaded together
-  const int kcap = d0 * d1 - 1, kstep = gridDim.x * it.R;
+  const int kcap = d0 * d1 * max(nseg, 1) - 1, kstep = gridDim.x * it.R;
   int k = blockIdx.x * it.R + it.sub;
-  int row_next = rows[min(k, kcap)];
+  int id_next = rows[min(k, kcap)];
   const int nr = *nrows;
   if (it.on)
     for (; k < nr; k += kstep) {
-      const int row = row_next;
-      row_next = rows[min(k + kstep, kcap)];
+      const WorkItem wi = work_item(id_next, nseg, d2);
+      id_next = rows[min(k + kstep, kcap)];
+      const int row = wi.row;
       const int xi = row / d1, yi = row - xi * d1;
       const bool inner = xi >= 1 && xi <= d0 - 2 && yi >= 1 && yi <= d1 - 2;
       if (!inner && !zero_z) continue;
       const size_t base = (size_t)row * d2;
-      for (int zi = it.z0; zi < d2; zi += it.zs) {
+      for (int zi = wi.zlo + it.z0; zi < wi.zhi; zi += it.zs) {
         const size_t i = base + zi;
         const bool colour0 = first && ((xi + yi + zi) & 1) == 0;
         float zv = 0.0f;
@@ -1336,7 +1358,8 @@ __global__ void k_cg_update_xr(float* __restrict__ x, float* __restrict__ r, con
 __global__ void k_cg_dot_rz(const float* __restrict__ r, const float* __restrict__ z, const int* __restrict__ rows,
                             const int* __restrict__ nrows, int d2, double* __restrict__ partial,
                             const double* __restrict__ part_rr, int npart, float* __restrict__ sc,
-                            const int* __restrict__ flag, float* __restrict__ host_sc, int* __restrict__ host_flag) {
+                            const int* __restrict__ flag, float* __restrict__ host_sc, int* __restrict__ host_flag,
+                            int nseg) {
   if (blockIdx.x == 0) {  // uniform over the block
     const double rr = sum_partials_block(part_rr, npart);
     if (threadIdx.x == 0) {
@@ -1347,15 +1370,16 @@ __global__ void k_cg_dot_rz(const float* __restrict__ r, const float* __restrict
     }
   }
   double v[1] = {0.0};
-  const RowIter it = row_iter(d2);
+  const RowIter it = list_iter(d2, nseg);
   const int kstep = gridDim.x * it.R;
   int k = blockIdx.x * it.R + it.sub;
   const int nr = *nrows;
   if (it.on)
-    for (int row_next = k < nr ? rows[k] : 0; k < nr; k += kstep) {
-      const size_t base = (size_t)row_next * d2;
-      row_next = k + kstep < nr ? rows[k + kstep] : 0;
-      for (int zi = it.z0; zi < d2; zi += it.zs) v[0] += (double)r[base + zi] * (double)z[base + zi];
+    for (int id_next = k < nr ? rows[k] : 0; k < nr; k += kstep) {
+      const WorkItem wi = work_item(id_next, nseg, d2);
+      const size_t base = (size_t)wi.row * d2;
+      id_next = k + kstep < nr ? rows[k + kstep] : 0;
+      for (int zi = wi.zlo + it.z0; zi < wi.zhi; zi += it.zs) v[0] += (double)r[base + zi] * (double)z[base + zi];
     }
   block_partials<1>(v, partial);
 }
@@ -1371,7 +1395,7 @@ __global__ void k_cg_dot_rz(const float* __restrict__ r, const float* __restrict
 __global__ void k_cg_p_spmv(float* __restrict__ pout, float* __restrict__ q, const float* __restrict__ pin,
                             const float* __restrict__ z, LvP L, const int* __restrict__ rows,
                             const int* __restrict__ nrows, float* __restrict__ sc, int slot, int* __restrict__ flag,
-                            const double* __restrict__ part_rz, int npart, double* __restrict__ partial) {
+                            const double* __restrict__ part_rz, int npart, double* __restrict__ partial, int nseg) {
   const float newRZ = (float)sum_partials_block(part_rz, npart);
   const float beta = newRZ / sc[6 + slot];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1384,20 +1408,21 @@ __global__ void k_cg_p_spmv(float* __restrict__ pout, float* __restrict__ q, con
   const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
   const size_t sx = (size_t)d1 * d2, sy = (size_t)d2;
   double v[1] = {0.0};
-  const RowIter it = row_iter(d2);
-  const int kcap = d0 * d1 - 1;
+  const RowIter it = list_iter(d2, nseg);
+  const int kcap = d0 * d1 * max(nseg, 1) - 1;
   const XcdRun run = xcd_run(*nrows, it.R, it.sub);  // the grid (npartial blocks) is a multiple of 8
   const int kstep = run.step;
   int k = run.k;
-  int row_next = rows[min(k, kcap)];
+  int id_next = rows[min(k, kcap)];
   if (it.on)
     for (; k < run.end; k += kstep) {
-      const int row = row_next;
-      row_next = rows[min(k + kstep, kcap)];
+      const WorkItem wi = work_item(id_next, nseg, d2);
+      id_next = rows[min(k + kstep, kcap)];
+      const int row = wi.row;
       const int xi = row / d1, yi = row - xi * d1;
       const bool irow = xi >= 1 && xi <= d0 - 2 && yi >= 1 && yi <= d1 - 2;
       const size_t base = (size_t)row * d2;
-      for (int zi = it.z0; zi < d2; zi += it.zs) {
+      for (int zi = wi.zlo + it.z0; zi < wi.zhi; zi += it.zs) {
         const size_t i = base + zi;
         if (!(irow && zi >= 1 && zi <= d2 - 2)) {  // a cell on the box faces: the p update only
           if (L.cls[i] != 0) pout[i] = z[i] + beta * pin[i];
@@ -1993,6 +2018,19 @@ void build_system(Ctx& c, float rho, float dt) {
 // the configured one; in relative mode it is ||r|| <= tol * ||b||.  Result in c.x.
 namespace {
 
+// The level-0 work list of the PCG vector kernels (the x/r update, r.z, the p update + SpMV): the
+// z-segments holding liquid when the level keeps them, else the rows holding liquid.  The cells
+// they leave out have cls == 0 (r = 0, no x / r / p update, zero weight in every sum and stencil).
+struct WorkList {
+  const int* list;
+  const int* n;
+  int nseg;
+};
+WorkList level0_work(const Level& L0) {
+  if (L0.segs) return {L0.segs, L0.nsegs, L0.d[2] / kSegZ};
+  return {L0.rows, L0.nrows, 0};
+}
+
 // One PCG iteration up to the r.z the host waits for (PressureSolver.cpp:879-906): x/r update,
 // preconditioner, r.z.  part_pq / part_rr: the p.q and r.r partials in c.partial.
 void pcg_iteration_launches(Ctx& c, int slot, double* part_pq, double* part_rr) {
@@ -2002,12 +2040,13 @@ void pcg_iteration_launches(Ctx& c, int slot, double* part_pq, double* part_rr) 
   // the symmetric V-cycle zeroes level-0 z here; with colour passes on level 0 (and more than one
   // level, so that level 0 pre-smooths) its first pass is folded in
   const bool first0 = sym && c.lv.size() > 1 && c.cfg.num_pre > 0 && colour_passes(c, 0);
+  const WorkList wl = level0_work(L0);
   hipLaunchKernelGGL(k_cg_update_xr, dim3(nb), dim3(kBlock), 0, c.s, c.x, L0.r, c.p, c.q, L0.cls, c.dsc, slot,
-                     c.dflag + 2, part_pq, nb, L0.d[0], L0.d[1], L0.d[2], L0.rows, L0.nrows, part_rr, L0.z, sym,
-                     L0.adiag, first0 ? 1 : 0, c.cfg.omega);
+                     c.dflag + 2, part_pq, nb, L0.d[0], L0.d[1], L0.d[2], wl.list, wl.n, part_rr, L0.z, sym,
+                     L0.adiag, first0 ? 1 : 0, c.cfg.omega, wl.nseg);
   apply_precond(c, sym != 0, first0);
-  hipLaunchKernelGGL(k_cg_dot_rz, dim3(nb), dim3(kBlock), 0, c.s, L0.r, L0.z, L0.rows, L0.nrows, L0.d[2], c.partial,
-                     part_rr, nb, c.dsc, c.dflag + 2, c.h_pin_dev, c.h_flag_dev);
+  hipLaunchKernelGGL(k_cg_dot_rz, dim3(nb), dim3(kBlock), 0, c.s, L0.r, L0.z, wl.list, wl.n, L0.d[2], c.partial,
+                     part_rr, nb, c.dsc, c.dflag + 2, c.h_pin_dev, c.h_flag_dev, wl.nseg);
 }
 
 // the levels' ping-pong buffers, which the preconditioner's host code swaps as it launches
@@ -2141,8 +2180,9 @@ int pcg_solve(Ctx& c, const float* b, const float* x0, float tol, int tol_mode, 
     hipEventRecord(c.ev_iter, c.s);
     {  // p = z + beta p and the next iteration's q = A p, while the host waits for the event and decides
       ProfScope ps(c, LH_PROF_SPMV);
-      hipLaunchKernelGGL(k_cg_p_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.p2, c.q, c.p, L0.z, P0, L0.rows, L0.nrows,
-                         c.dsc, slot, c.dflag + 2, c.partial, nb, part_pq);
+      const WorkList wl = level0_work(L0);
+      hipLaunchKernelGGL(k_cg_p_spmv, dim3(nb), dim3(kBlock), 0, c.s, c.p2, c.q, c.p, L0.z, P0, wl.list, wl.n,
+                         c.dsc, slot, c.dflag + 2, c.partial, nb, part_pq, wl.nseg);
     }
     std::swap(c.p, c.p2);
     if (!hip_ok(c, hipEventSynchronize(c.ev_iter), "solve iteration")) return LH_ERR_DEVICE;
