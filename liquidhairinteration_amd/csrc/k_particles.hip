@@ -536,6 +536,304 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vx, float* __re
   }
 }
 
+// ------------------------------------------------------------------ P2G with shared records
+// The gather above has every lane evaluate every record of its union window, so the x and y
+// parts of one record's weights and momenta are evaluated by each of the ~3.5 lanes (z) of a
+// wave that reach it, although they depend only on the record and the wave's y row and x
+// positions.  Here a wave (one y row of 64 z columns at the tile's x positions) evaluates them
+// once per record: per x slab and y column it takes the records of the whole wave's z window
+// (half-cell bins [2 z0 - 3, 2 (z0 + 63) + 3]) up to kShareCap at a time, one or more per lane,
+// and leaves per record in LDS its z position, the z column of C, and per component the products
+// w_x w_y and the x and y parts of the affine term (the gather's operations, in its order).  Then
+// each lane runs the gather's loop over its own window's records in bin order, reading them from
+// LDS and finishing each with its z weight and z part: the same operations on the same values in
+// the same order, so the result is bit-identical to k_p2g.  A lane's window holds ~7 of the ~134
+// records of a column (one particle per half-cell), and a chunk holding the whole column keeps
+// every lane busy.
+constexpr int kShareCap = 128;  // records per chunk (2 per lane)
+struct P2GShared {
+  // per record: R0.z, C.col(0).z, C.col(1).z, C.col(2).z | Y0, Y1, Y2, v_x | v_y, v_z, A0[0..1] |
+  // A0[2], E0[0..2] | A1[0..2], E1[0] | E1[1..2], A2[0..1] | A2[2], E2[0..2]
+  float r[kShareCap][28];
+};
+
+// LDS hand-over between the lanes of one wave: the wave's LDS operations complete in order, so
+// keeping the compiler from moving them across this point is enough
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// a record's x and y terms for the wave's faces into LDS slot `slot` (the terms of p2g_acc that do
+// not depend on the face's z): A = w_x w_y, E = C.col(a).x (x - face x), Y = C.col(a).y (y - face y)
+template <bool kPow2, int NX, int NYZ>
+__device__ __forceinline__ void p2g_share_record(float4 R0, float4 R1, float4 R2, float4 R3, const P2GCol& k,
+                                                 const float ccx0[3], const float fx0[3], const float ccxh[3],
+                                                 const float fxh[3], P2GShared& sh, int slot) {
+  const float gx = kPow2 ? (R0.x - k.mn0) * k.dx0 : (R0.x - k.mn0) / k.dx0;
+  const float gy = kPow2 ? (R0.y - k.mn1) * k.dx1 : (R0.y - k.mn1) / k.dx1;
+  const float wyi = quad_w(gy - k.ccy_i), wyh = quad_w(gy - k.ccy_h);
+  float o[28];
+#pragma unroll
+  for (int f = 0; f < 28; ++f) o[f] = 0.f;
+  o[0] = R0.z;
+  o[1] = R1.z;
+  o[2] = R2.z;
+  o[3] = R3.z;
+  o[4] = R1.y * (R0.y - k.fy_h);
+  o[5] = R2.y * (R0.y - k.fy_i);
+  o[6] = R3.y * (R0.y - k.fy_h);
+  o[7] = R3.w;
+  o[8] = R1.w;
+  o[9] = R2.w;
+#pragma unroll
+  for (int f = 0; f < NX; ++f) {
+    o[10 + f] = quad_w(gx - ccx0[f]) * wyh;
+    o[13 + f] = R1.x * (R0.x - fx0[f]);
+  }
+#pragma unroll
+  for (int f = 0; f < NYZ; ++f) {
+    const float wx = quad_w(gx - ccxh[f]);
+    const float dxf = R0.x - fxh[f];
+    o[16 + f] = wx * wyi;
+    o[19 + f] = R2.x * dxf;
+    o[22 + f] = wx * wyh;
+    o[25 + f] = R3.x * dxf;
+  }
+  float4* d = reinterpret_cast<float4*>(sh.r[slot]);
+#pragma unroll
+  for (int q = 0; q < 7; ++q) d[q] = make_float4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+}
+
+__device__ __forceinline__ void p2g_share_load(const P2GShared& sh, int s, float4 v[7]) {
+  const float4* p = reinterpret_cast<const float4*>(sh.r[s]);
+#pragma unroll
+  for (int q = 0; q < 7; ++q) v[q] = p[q];
+}
+
+// one LDS slot's terms into this lane's faces (p2g_acc with the x and y terms read back)
+template <bool kPow2, int NX, int NYZ>
+__device__ __forceinline__ void p2g_share_add(const float4 v[7], const P2GCol& k, float ws[3][3], float vs[3][3]) {
+  float o[28];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    o[4 * q] = v[q].x;
+    o[4 * q + 1] = v[q].y;
+    o[4 * q + 2] = v[q].z;
+    o[4 * q + 3] = v[q].w;
+  }
+  const float z = o[0];
+  const float gz = kPow2 ? (z - k.mn2) * k.dx2 : (z - k.mn2) / k.dx2;
+  const float wzh = quad_w(gz - k.ccz_h);
+  const float th = z - k.fz_h;
+  const float yz0 = o[4] + o[1] * th, yz1 = o[5] + o[2] * th;
+#pragma unroll
+  for (int f = 0; f < NX; ++f) {
+    const float w = o[10 + f] * wzh;
+    ws[0][f] = ws[0][f] + w;
+    vs[0][f] = vs[0][f] + w * (o[7] + (o[13 + f] + yz0));
+  }
+  if (NYZ > 0) {
+    const float wzi = quad_w(gz - k.ccz_i);
+    const float yz2 = o[6] + o[3] * (z - k.fz_i);
+#pragma unroll
+    for (int f = 0; f < NYZ; ++f) {
+      const float w1 = o[16 + f] * wzh, w2 = o[22 + f] * wzi;
+      ws[1][f] = ws[1][f] + w1;
+      vs[1][f] = vs[1][f] + w1 * (o[8] + (o[19 + f] + yz1));
+      ws[2][f] = ws[2][f] + w2;
+      vs[2][f] = vs[2][f] + w2 * (o[9] + (o[25 + f] + yz2));
+    }
+  }
+}
+
+// one y column of a wave's slab: the records [s0, s0 + n) of the wave's window, of which this
+// lane's window is [s0 + a, s0 + b), kShareCap at a time
+template <bool kPow2, int NX, int NYZ>
+__device__ __forceinline__ void p2g_share_column(const float4* __restrict__ pos, const float4* __restrict__ rest,
+                                                 int s0, int n, int a, int b, const P2GCol& k, const float ccx0[3],
+                                                 const float fx0[3], const float ccxh[3], const float fxh[3],
+                                                 P2GShared& sh, int lane, float ws[3][3], float vs[3][3]) {
+  for (int cb = 0; cb < n; cb += kShareCap) {
+    const int cn = min(n - cb, kShareCap);
+#pragma unroll
+    for (int r = 0; r < kShareCap / 64; ++r) {
+      const int t = r * 64 + lane;
+      if (t < cn) {
+        const size_t i = (size_t)(s0 + cb + t);
+        p2g_share_record<kPow2, NX, NYZ>(pos[i], rest[3 * i], rest[3 * i + 1], rest[3 * i + 2], k, ccx0, fx0, ccxh,
+                                         fxh, sh, t);
+      }
+    }
+    wave_lds_sync();
+    const int lo = max(a, cb) - cb, hi = min(b, cb + cn) - cb;
+    for (int t = lo; t < hi; ++t) {
+      float4 v[7];
+      p2g_share_load(sh, t, v);
+      p2g_share_add<kPow2, NX, NYZ>(v, k, ws, vs);
+    }
+    wave_lds_sync();
+  }
+}
+
+// x positions per tile, as for the gather: a longer row evaluates each record for fewer tiles
+// but leaves fewer busy waves (same-box sweeps of 4 / 8 / 12 / 16: the 256^3 collision, 0.4
+// particles per cell, 1426 / 1543 / 1783 / 2122 us; the 256^3 sloshing tank, 2.9 per cell, 6460
+// / 5598 / 5474 / 5675 us)
+#ifndef LH_P2G_SHARED_SPARSE_MX
+#define LH_P2G_SHARED_SPARSE_MX 4
+#endif
+constexpr int kMXSharedSparse = LH_P2G_SHARED_SPARSE_MX, kMXSharedDense = 12;
+
+template <bool kPow2, int MX>
+__global__ void __launch_bounds__(256) k_p2g_shared(float* __restrict__ vx, float* __restrict__ vy,
+                                                     float* __restrict__ vz, const float4* __restrict__ pos,
+                                                     const float4* __restrict__ rest, const int* __restrict__ start,
+                                                     P2GP q) {
+  __shared__ P2GShared shm[kBlock / 64];
+  const int N0 = q.g.N[0], N1 = q.g.N[1], N2 = q.g.N[2];
+  int c[3], c0[3];
+  if (!tile_coords<MX, kPTY, kPTZ>(N0 + 1, N1 + 1, N2 + 1, c, c0)) return;
+  const int T[3] = {MX, kPTY, kPTZ};
+  int blo[3], bhi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    blo[k] = max(2 * c0[k] - 3, 0);
+    bhi[k] = min(2 * (c0[k] + T[k] - 1) + 3, q.H[k] - 1);
+  }
+  const bool any = tile_has_particles(start, blo, bhi, q.H[1], q.H[2]);
+  static_assert(kPTZ == 64, "one y per wave");
+  c[1] = __builtin_amdgcn_readfirstlane(c[1]);
+  // a wave past the last y row leaves; a lane past the last z stays (it evaluates records for
+  // the others) and owns no face
+  if (c[1] > N1) return;
+  const bool zok = c[2] <= N2;
+  const bool has[3] = {c[1] < N1 && c[2] < N2, c[1] <= N1 && c[2] < N2, c[1] < N1 && c[2] <= N2};
+  const int xend0 = min(c0[0] + MX, N0 + 1), xend12 = min(c0[0] + MX, N0);
+  float* const vout[3] = {vx, vy, vz};
+  auto finish = [&](int a, int x, float ws, float vs) {
+    if (!has[a]) return;
+    float v = !is_small(ws) ? vs / ws : 0.0f;
+    if (a == 1 || a == q.tank_axis) {
+      const int cc[3] = {x, c[1], c[2]};
+      bool in = true;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int h = (k == a) ? q.g.N[k] - 1 : q.g.N[k] - 2;
+        if (cc[k] < 1 || cc[k] > h) in = false;
+      }
+      if (in) {
+        if (a == 1) v += q.gdt;
+        if (a == q.tank_axis) v += q.adt;
+      }
+    }
+    const int d1 = N1 + (a == 1), d2 = N2 + (a == 2);
+    vout[a][((size_t)x * d1 + c[1]) * d2 + c[2]] = v;
+  };
+  if (!any) {
+    for (int x = c0[0]; x < xend0; ++x) finish(0, x, 0.f, 0.f);
+    for (int x = c0[0]; x < xend12; ++x) {
+      finish(1, x, 0.f, 0.f);
+      finish(2, x, 0.f, 0.f);
+    }
+    return;
+  }
+  const int lane = (int)threadIdx.x & 63;
+  P2GShared& sh = shm[threadIdx.x >> 6];
+  P2GCol k;
+  k.mn0 = q.g.mn[0];
+  k.mn1 = q.g.mn[1];
+  k.mn2 = q.g.mn[2];
+  k.dx0 = kPow2 ? q.inv[0] : q.g.dx[0];
+  k.dx1 = kPow2 ? q.inv[1] : q.g.dx[1];
+  k.dx2 = kPow2 ? q.inv[2] : q.g.dx[2];
+  k.ccy_i = (float)c[1];
+  k.ccy_h = (float)c[1] + 0.5f;
+  k.ccz_i = (float)c[2];
+  k.ccz_h = (float)c[2] + 0.5f;
+  k.fy_i = k.ccy_i * q.g.dx[1] + k.mn1;
+  k.fy_h = k.ccy_h * q.g.dx[1] + k.mn1;
+  k.fz_i = k.ccz_i * q.g.dx[2] + k.mn2;
+  k.fz_h = k.ccz_h * q.g.dx[2] + k.mn2;
+  const int H2 = q.H[2];
+  const int ylo = max(2 * c[1] - 3, 0), yhi = min(2 * c[1] + 3, q.H[1] - 1);
+  // the wave's union window in z, and this lane's (inside it: a lane past the last face has none)
+  const int wzlo = max(2 * c0[2] - 3, 0), wzhi = min(2 * (c0[2] + kPTZ - 1) + 3, H2 - 1);
+  const int zlo = max(2 * c[2] - 3, 0), zhi = min(2 * c[2] + 3, H2 - 1);
+  auto xlo = [&](int m, int x) { return max(m == 0 ? 2 * x - 3 : 2 * x - 2, 0); };
+  auto xhi = [&](int m, int x) { return min((m == 0 ? 2 * x - 3 : 2 * x - 2) + 5, q.H[0] - 1); };
+  float ws[3][3], vs[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int f = 0; f < 3; ++f) ws[a][f] = vs[a][f] = 0.f;
+  int cw0 = c0[0], cw12 = c0[0];
+  for (int hx = blo[0]; hx <= bhi[0]; ++hx) {
+    float ccx0[3], fx0[3], ccxh[3], fxh[3];
+    int nx = 0, nyz = 0;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const int x0 = cw0 + f, x1 = cw12 + f;
+      nx += (x0 < xend0 && xlo(0, x0) <= hx && hx <= xhi(0, x0)) ? 1 : 0;
+      nyz += (x1 < xend12 && xlo(1, x1) <= hx && hx <= xhi(1, x1)) ? 1 : 0;
+      ccx0[f] = (float)x0;
+      fx0[f] = ccx0[f] * q.g.dx[0] + k.mn0;
+      ccxh[f] = (float)x1 + 0.5f;
+      fxh[f] = ccxh[f] * q.g.dx[0] + k.mn0;
+    }
+    if (nx + nyz > 0) {
+      for (int hy = ylo; hy <= yhi; ++hy) {
+        const size_t k0 = ((size_t)hx * q.H[1] + hy) * H2;
+        const int s0 = start[k0 + wzlo];
+        const int n = start[k0 + wzhi + 1] - s0;
+        if (n == 0) continue;
+        int a = 0, b = 0;
+        if (zok) {
+          a = start[k0 + zlo] - s0;
+          b = start[k0 + zhi + 1] - s0;
+        }
+#define LH_P2G_CASE(A, B)                                                                                \
+  case A * 4 + B:                                                                                        \
+    p2g_share_column<kPow2, A, B>(pos, rest, s0, n, a, b, k, ccx0, fx0, ccxh, fxh, sh, lane, ws, vs); \
+    break;
+        switch (nx * 4 + nyz) {
+          LH_P2G_CASE(0, 1) LH_P2G_CASE(0, 2) LH_P2G_CASE(0, 3)
+          LH_P2G_CASE(1, 0) LH_P2G_CASE(1, 1) LH_P2G_CASE(1, 2) LH_P2G_CASE(1, 3)
+          LH_P2G_CASE(2, 0) LH_P2G_CASE(2, 1) LH_P2G_CASE(2, 2) LH_P2G_CASE(2, 3)
+          LH_P2G_CASE(3, 0) LH_P2G_CASE(3, 1) LH_P2G_CASE(3, 2) LH_P2G_CASE(3, 3)
+          default: break;
+        }
+      }
+#undef LH_P2G_CASE
+    }
+    while (cw0 < xend0 && xhi(0, cw0) <= hx) {
+      finish(0, cw0, ws[0][0], vs[0][0]);
+      ws[0][0] = ws[0][1];
+      vs[0][0] = vs[0][1];
+      ws[0][1] = ws[0][2];
+      vs[0][1] = vs[0][2];
+      ws[0][2] = 0.f;
+      vs[0][2] = 0.f;
+      ++cw0;
+    }
+    while (cw12 < xend12 && xhi(1, cw12) <= hx) {
+      finish(1, cw12, ws[1][0], vs[1][0]);
+      finish(2, cw12, ws[2][0], vs[2][0]);
+#pragma unroll
+      for (int a = 1; a < 3; ++a) {
+        ws[a][0] = ws[a][1];
+        vs[a][0] = vs[a][1];
+        ws[a][1] = ws[a][2];
+        vs[a][1] = vs[a][2];
+        ws[a][2] = 0.f;
+        vs[a][2] = 0.f;
+      }
+      ++cw12;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ liquid phi
 // min over particles in cells c +- dW of |centre - x_p| - r.  The min is
 // order-free, so bin rows are visited nearest first (the cell's own 2 x 2 rows)
@@ -1190,6 +1488,23 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
   const unsigned nt = tile_grid(div_up(c.g.N[0] + 1, mx) * div_up(c.g.N[1] + 1, kPTY) * div_up(c.g.N[2] + 1, kPTZ));
   const float4* pos = (const float4*)c.prec[c.cur];
   const float4* rest = pos + c.cap;
+  if (!c.cfg.p2g_gather) {
+    const int ms = dense ? kMXSharedDense : kMXSharedSparse;
+    const unsigned ns = tile_grid(div_up(c.g.N[0] + 1, ms) * div_up(c.g.N[1] + 1, kPTY) * div_up(c.g.N[2] + 1, kPTZ));
+    if (pow2 && dense)
+      hipLaunchKernelGGL((k_p2g_shared<true, kMXSharedDense>), dim3(ns), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2],
+                         pos, rest, c.bin_start, q);
+    else if (pow2)
+      hipLaunchKernelGGL((k_p2g_shared<true, kMXSharedSparse>), dim3(ns), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2],
+                         pos, rest, c.bin_start, q);
+    else if (dense)
+      hipLaunchKernelGGL((k_p2g_shared<false, kMXSharedDense>), dim3(ns), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2],
+                         pos, rest, c.bin_start, q);
+    else
+      hipLaunchKernelGGL((k_p2g_shared<false, kMXSharedSparse>), dim3(ns), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2],
+                         pos, rest, c.bin_start, q);
+    return;
+  }
   if (pow2 && dense)
     hipLaunchKernelGGL((k_p2g<true, kMXDense>), dim3(nt), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2], pos, rest,
                        c.bin_start, q);

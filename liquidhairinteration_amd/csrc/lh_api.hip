@@ -160,6 +160,9 @@ Config parse_config(const std::string& text) {
     throw std::runtime_error("Unknown coarsestSolver " + csol);
   }
   c.pcg_graph = (s.has("pcgGraph") && s.at("pcgGraph").as_bool()) ? 1 : 0;
+  std::string pf = j.get("p2gForm", "shared");
+  if (pf == "gather") c.p2g_gather = 1;
+  else if (pf != "shared") throw std::runtime_error("Unknown p2gForm " + pf);
   c.first_dt = (float)j.get("firstDt", 0.01);
   c.correct_volume = (j.has("correctVolume") && j.at("correctVolume").as_bool()) ? 1 : 0;
   c.steps = (int)j.get("steps", 0.0);

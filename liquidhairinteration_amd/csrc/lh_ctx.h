@@ -53,6 +53,9 @@ struct Config {
   // SolverParameter.pcgGraph: each PCG iteration's x/r update, preconditioner and r.z launched as
   // one instantiated hipGraph instead of one launch at a time
   int pcg_graph = 0;
+  // "p2gForm": "shared" (default: a wave's records evaluated once per wave and handed to the face
+  // lanes through LDS) or "gather" (every lane evaluates every record of its window)
+  int p2g_gather = 0;
   double refined_tol = 1e-9;
   int max_outer = 50;
   // boundary SDF program (postfix, 12 floats per node)

@@ -90,6 +90,24 @@ def precond(cfg, kind):
     return cfg
 
 
+def p2g_gather(cfg):
+    """P2G as the per-lane gather (p2gForm "gather"; default "shared": a wave's records evaluated
+    once and handed to the face lanes through LDS)"""
+    cfg["p2gForm"] = "gather"
+    return cfg
+
+
+def small_collision(N=40):
+    """two 0.3^3 blocks at opposite corners moving at +-2 m/s (0.4 particles per cell: P2G's tile
+    length for liquid that does not fill its tiles)"""
+    cfg = small_dam(N, "symmetric")
+    cfg["FluidBlocks"] = [{"min": [0.0, 0.0, 0.0], "max": [0.3, 0.3, 0.3], "velocity": [2.0, 0.0, 0.0],
+                           "velocityNoise": 0.05},
+                          {"min": [0.7, 0.7, 0.7], "max": [1.0, 1.0, 1.0], "velocity": [-2.0, 0.0, 0.0],
+                           "velocityNoise": 0.05}]
+    return cfg
+
+
 def cv_cycle(cfg, k):
     cfg["LiquidParameter"]["correctCycle"] = k
     return cfg
@@ -134,6 +152,10 @@ CASES = {
     "dam24_chebyshev_coarsest": lambda: chebyshev(small_dam(24, "symmetric"), 12, 0.02),
     "dam32_tank_sphere_chebyshev_odd": lambda: chebyshev(small_dam(32, "symmetric", tank=True), 7, 0.01),
     "tall_16x24x160_chebyshev_segments": lambda: chebyshev(tall_dam(), 9, 0.005, levels=2),
+    # both P2G forms at both tile lengths (liquid filling its tiles, and not)
+    "dam24_tank_sphere_p2g_gather": lambda: p2g_gather(small_dam(24, "symmetric", tank=True)),
+    "collision40": lambda: small_collision(40),
+    "collision40_p2g_gather": lambda: p2g_gather(small_collision(40)),
 }
 
 
@@ -219,7 +241,8 @@ def test_stages_bit_exact(case):
                                   "dam24_precond_none", "dam32_tank_sphere_jacobi",
                                   "dam24_symmetric_coarsest_fp64", "dam24_reference_coarsest_fp64",
                                   "dam24_chebyshev_coarsest", "dam32_tank_sphere_chebyshev_odd",
-                                  "tall_16x24x160_chebyshev_segments"])
+                                  "tall_16x24x160_chebyshev_segments", "collision40",
+                                  "collision40_p2g_gather"])
 def test_free_running_steps_bit_exact(case):
     print("case", case, flush=True)
     cfg = CASES[case]()
