@@ -133,8 +133,11 @@ KERNEL_SYMBOL = {"smooth_l0": "k_smooth<true>", "spmv": "k_cg_p_spmv", "p2g": "k
 
 def kernel_symbols(cfg):
     """KERNEL_SYMBOL for this configuration: the finest colour passes run over the level's z-segment
-    list (k_smooth_seg) when its z extent is a multiple of 64 cells (at most 31 segments)"""
+    list (k_smooth_seg) when its z extent is a multiple of 64 cells (at most 31 segments); P2G is
+    k_p2g_shared unless the config selects the plain gather ("p2gForm": "gather")"""
     sym = dict(KERNEL_SYMBOL)
+    if cfg.get("p2gForm", "shared") == "shared":
+        sym["p2g"] = "k_p2g_shared"
     d2 = cfg["gridDimension"][2]
     if d2 % 64 == 0 and d2 // 64 <= 31:
         sym["smooth_l0"] = "k_smooth_seg"
