@@ -260,7 +260,7 @@ inline unsigned tile_grid(size_t ntiles) {
 
 template <int TX = kTX, int TY = kTY, int TZ = kTZ>
 __device__ __forceinline__ bool tile_coords(int d0, int d1, int d2, int c[3], int c0[3]) {
-  static_assert(TY * TZ == kBlock, "one (y, z) per thread");
+  static_assert(TY * TZ <= kBlock, "one (y, z) per thread");
   const int ntx = (d0 + TX - 1) / TX, nty = (d1 + TY - 1) / TY, ntz = (d2 + TZ - 1) / TZ;
   const int x = (int)(blockIdx.x % kXcds), j = (int)(blockIdx.x / kXcds);
   const unsigned grp = (unsigned)(j / kXChunk);
@@ -684,18 +684,27 @@ __device__ __forceinline__ void p2g_share_column(const float4* __restrict__ pos,
 #ifndef LH_P2G_SHARED_SPARSE_MX
 #define LH_P2G_SHARED_SPARSE_MX 4
 #endif
-constexpr int kMXSharedSparse = LH_P2G_SHARED_SPARSE_MX, kMXSharedDense = 12;
+#ifndef LH_P2G_SHARED_DENSE_MX
+#define LH_P2G_SHARED_DENSE_MX 12
+#endif
+constexpr int kMXSharedSparse = LH_P2G_SHARED_SPARSE_MX, kMXSharedDense = LH_P2G_SHARED_DENSE_MX;
+
+// y rows (waves) per workgroup of the shared-record P2G: each wave holds 14 KB of LDS
+#ifndef LH_P2G_SHARED_TY
+#define LH_P2G_SHARED_TY 2
+#endif
+constexpr int kSTY = LH_P2G_SHARED_TY;
 
 template <bool kPow2, int MX>
-__global__ void __launch_bounds__(256) k_p2g_shared(float* __restrict__ vx, float* __restrict__ vy,
+__global__ void __launch_bounds__(64 * kSTY) k_p2g_shared(float* __restrict__ vx, float* __restrict__ vy,
                                                      float* __restrict__ vz, const float4* __restrict__ pos,
                                                      const float4* __restrict__ rest, const int* __restrict__ start,
                                                      P2GP q) {
-  __shared__ P2GShared shm[kBlock / 64];
+  __shared__ P2GShared shm[kSTY];
   const int N0 = q.g.N[0], N1 = q.g.N[1], N2 = q.g.N[2];
   int c[3], c0[3];
-  if (!tile_coords<MX, kPTY, kPTZ>(N0 + 1, N1 + 1, N2 + 1, c, c0)) return;
-  const int T[3] = {MX, kPTY, kPTZ};
+  if (!tile_coords<MX, kSTY, kPTZ>(N0 + 1, N1 + 1, N2 + 1, c, c0)) return;
+  const int T[3] = {MX, kSTY, kPTZ};
   int blo[3], bhi[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -1490,18 +1499,18 @@ void launch_p2g(Ctx& c, bool fuse_body_forces, float dt) {
   const float4* rest = pos + c.cap;
   if (!c.cfg.p2g_gather) {
     const int ms = dense ? kMXSharedDense : kMXSharedSparse;
-    const unsigned ns = tile_grid(div_up(c.g.N[0] + 1, ms) * div_up(c.g.N[1] + 1, kPTY) * div_up(c.g.N[2] + 1, kPTZ));
+    const unsigned ns = tile_grid(div_up(c.g.N[0] + 1, ms) * div_up(c.g.N[1] + 1, kSTY) * div_up(c.g.N[2] + 1, kPTZ));
     if (pow2 && dense)
-      hipLaunchKernelGGL((k_p2g_shared<true, kMXSharedDense>), dim3(ns), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2],
+      hipLaunchKernelGGL((k_p2g_shared<true, kMXSharedDense>), dim3(ns), dim3(64 * kSTY), 0, c.s, c.v[0], c.v[1], c.v[2],
                          pos, rest, c.bin_start, q);
     else if (pow2)
-      hipLaunchKernelGGL((k_p2g_shared<true, kMXSharedSparse>), dim3(ns), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2],
+      hipLaunchKernelGGL((k_p2g_shared<true, kMXSharedSparse>), dim3(ns), dim3(64 * kSTY), 0, c.s, c.v[0], c.v[1], c.v[2],
                          pos, rest, c.bin_start, q);
     else if (dense)
-      hipLaunchKernelGGL((k_p2g_shared<false, kMXSharedDense>), dim3(ns), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2],
+      hipLaunchKernelGGL((k_p2g_shared<false, kMXSharedDense>), dim3(ns), dim3(64 * kSTY), 0, c.s, c.v[0], c.v[1], c.v[2],
                          pos, rest, c.bin_start, q);
     else
-      hipLaunchKernelGGL((k_p2g_shared<false, kMXSharedSparse>), dim3(ns), dim3(kBlock), 0, c.s, c.v[0], c.v[1], c.v[2],
+      hipLaunchKernelGGL((k_p2g_shared<false, kMXSharedSparse>), dim3(ns), dim3(64 * kSTY), 0, c.s, c.v[0], c.v[1], c.v[2],
                          pos, rest, c.bin_start, q);
     return;
   }
