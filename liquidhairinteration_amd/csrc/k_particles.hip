@@ -543,18 +543,21 @@ __global__ void __launch_bounds__(256) k_p2g(float* __restrict__ vx, float* __re
 // positions.  Here a wave (one y row of 64 z columns at the tile's x positions) evaluates them
 // once per record: per x slab and y column it takes the records of the whole wave's z window
 // (half-cell bins [2 z0 - 3, 2 (z0 + 63) + 3]) up to kShareCap at a time, one or more per lane,
-// and leaves per record in LDS its z position, the z column of C, and per component the products
-// w_x w_y and the x and y parts of the affine term (the gather's operations, in its order).  Then
-// each lane runs the gather's loop over its own window's records in bin order, reading them from
-// LDS and finishing each with its z weight and z part: the same operations on the same values in
-// the same order, so the result is bit-identical to k_p2g.  A lane's window holds ~7 of the ~134
-// records of a column (one particle per half-cell), and a chunk holding the whole column keeps
-// every lane busy.
-constexpr int kShareCap = 128;  // records per chunk (2 per lane)
+// and leaves per record in LDS 24 floats: per component and active x face the product w_x w_y,
+// the y parts of the affine terms, the velocities, x, z and the x and z rows of C (the gather's
+// operations, in its order).  Then each lane runs the gather's loop over its own window's records
+// in bin order, reading them from LDS and finishing each with its x parts, its z weight and its z
+// part: the same operations on the same values in the same order, so the result is bit-identical
+// to k_p2g.  A lane's window holds ~7 of the ~134 records of a column (one particle per
+// half-cell), and a chunk holding most of a column keeps the lanes busy.  Storing the x parts
+// instead (28 floats) costs occupancy: 10 waves per CU instead of 12, 1 261 against 1 220 us on
+// the 256^3 collision.
+constexpr int kShareCap = 128;  // records per chunk (two rounds of the wave's lanes)
+constexpr int kShareF = 24;     // floats per record in LDS
 struct P2GShared {
-  // per record: R0.z, C.col(0).z, C.col(1).z, C.col(2).z | Y0, Y1, Y2, v_x | v_y, v_z, A0[0..1] |
-  // A0[2], E0[0..2] | A1[0..2], E1[0] | E1[1..2], A2[0..1] | A2[2], E2[0..2]
-  float r[kShareCap][28];
+  // per record: R0.z, C.col(0).z, C.col(1).z, C.col(2).z | Y0, Y1, Y2, v_x | v_y, v_z, R0.x, C.col(0).x |
+  // C.col(1).x, C.col(2).x, A0[0..1] | A0[2], A1[0..2] | A2[0..2], -
+  float r[kShareCap][kShareF];
 };
 
 // LDS hand-over between the lanes of one wave: the wave's LDS operations complete in order, so
@@ -565,8 +568,8 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// a record's x and y terms for the wave's faces into LDS slot `slot` (the terms of p2g_acc that do
-// not depend on the face's z): A = w_x w_y, E = C.col(a).x (x - face x), Y = C.col(a).y (y - face y)
+// a record's terms for the wave's faces into LDS slot `slot`: A = w_x w_y per component and active x
+// face, Y = C.col(a).y (y - face y), the velocities, x, z and the x and z rows of C
 template <bool kPow2, int NX, int NYZ>
 __device__ __forceinline__ void p2g_share_record(float4 R0, float4 R1, float4 R2, float4 R3, const P2GCol& k,
                                                  const float ccx0[3], const float fx0[3], const float ccxh[3],
@@ -574,9 +577,9 @@ __device__ __forceinline__ void p2g_share_record(float4 R0, float4 R1, float4 R2
   const float gx = kPow2 ? (R0.x - k.mn0) * k.dx0 : (R0.x - k.mn0) / k.dx0;
   const float gy = kPow2 ? (R0.y - k.mn1) * k.dx1 : (R0.y - k.mn1) / k.dx1;
   const float wyi = quad_w(gy - k.ccy_i), wyh = quad_w(gy - k.ccy_h);
-  float o[28];
+  float o[kShareF];
 #pragma unroll
-  for (int f = 0; f < 28; ++f) o[f] = 0.f;
+  for (int f = 0; f < kShareF; ++f) o[f] = 0.f;
   o[0] = R0.z;
   o[1] = R1.z;
   o[2] = R2.z;
@@ -587,41 +590,56 @@ __device__ __forceinline__ void p2g_share_record(float4 R0, float4 R1, float4 R2
   o[7] = R3.w;
   o[8] = R1.w;
   o[9] = R2.w;
+  o[10] = R0.x;
+  o[11] = R1.x;
+  o[12] = R2.x;
+  o[13] = R3.x;
 #pragma unroll
-  for (int f = 0; f < NX; ++f) {
-    o[10 + f] = quad_w(gx - ccx0[f]) * wyh;
-    o[13 + f] = R1.x * (R0.x - fx0[f]);
-  }
+  for (int f = 0; f < NX; ++f) o[14 + f] = quad_w(gx - ccx0[f]) * wyh;
 #pragma unroll
   for (int f = 0; f < NYZ; ++f) {
     const float wx = quad_w(gx - ccxh[f]);
-    const float dxf = R0.x - fxh[f];
-    o[16 + f] = wx * wyi;
-    o[19 + f] = R2.x * dxf;
-    o[22 + f] = wx * wyh;
-    o[25 + f] = R3.x * dxf;
+    o[17 + f] = wx * wyi;
+    o[20 + f] = wx * wyh;
   }
   float4* d = reinterpret_cast<float4*>(sh.r[slot]);
 #pragma unroll
-  for (int q = 0; q < 7; ++q) d[q] = make_float4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+  for (int q = 0; q < kShareF / 4; ++q) d[q] = make_float4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
 }
 
-__device__ __forceinline__ void p2g_share_load(const P2GShared& sh, int s, float4 v[7]) {
+__device__ __forceinline__ void p2g_share_load(const P2GShared& sh, int s, float4 v[kShareF / 4]) {
   const float4* p = reinterpret_cast<const float4*>(sh.r[s]);
 #pragma unroll
-  for (int q = 0; q < 7; ++q) v[q] = p[q];
+  for (int q = 0; q < kShareF / 4; ++q) v[q] = p[q];
 }
 
 // one LDS slot's terms into this lane's faces (p2g_acc with the x and y terms read back)
 template <bool kPow2, int NX, int NYZ>
-__device__ __forceinline__ void p2g_share_add(const float4 v[7], const P2GCol& k, float ws[3][3], float vs[3][3]) {
-  float o[28];
+__device__ __forceinline__ void p2g_share_add(const float4 v[kShareF / 4], const P2GCol& k, const float fx0[3],
+                                              const float fxh[3], float ws[3][3], float vs[3][3]) {
+  float o[kShareF];
 #pragma unroll
-  for (int q = 0; q < 7; ++q) {
+  for (int q = 0; q < kShareF / 4; ++q) {
     o[4 * q] = v[q].x;
     o[4 * q + 1] = v[q].y;
     o[4 * q + 2] = v[q].z;
     o[4 * q + 3] = v[q].w;
+  }
+  // per component and active x face: the product w_x w_y and the x part of the affine term, formed
+  // here from the record's x and C.col(a).x (the same operations as in p2g_acc)
+  float A0[3], E0[3], A1[3], E1[3], A2[3], E2[3];
+#pragma unroll
+  for (int f = 0; f < NX; ++f) {
+    A0[f] = o[14 + f];
+    E0[f] = o[11] * (o[10] - fx0[f]);
+  }
+#pragma unroll
+  for (int f = 0; f < NYZ; ++f) {
+    A1[f] = o[17 + f];
+    A2[f] = o[20 + f];
+    const float dxf = o[10] - fxh[f];
+    E1[f] = o[12] * dxf;
+    E2[f] = o[13] * dxf;
   }
   const float z = o[0];
   const float gz = kPow2 ? (z - k.mn2) * k.dx2 : (z - k.mn2) / k.dx2;
@@ -630,20 +648,20 @@ __device__ __forceinline__ void p2g_share_add(const float4 v[7], const P2GCol& k
   const float yz0 = o[4] + o[1] * th, yz1 = o[5] + o[2] * th;
 #pragma unroll
   for (int f = 0; f < NX; ++f) {
-    const float w = o[10 + f] * wzh;
+    const float w = A0[f] * wzh;
     ws[0][f] = ws[0][f] + w;
-    vs[0][f] = vs[0][f] + w * (o[7] + (o[13 + f] + yz0));
+    vs[0][f] = vs[0][f] + w * (o[7] + (E0[f] + yz0));
   }
   if (NYZ > 0) {
     const float wzi = quad_w(gz - k.ccz_i);
     const float yz2 = o[6] + o[3] * (z - k.fz_i);
 #pragma unroll
     for (int f = 0; f < NYZ; ++f) {
-      const float w1 = o[16 + f] * wzh, w2 = o[22 + f] * wzi;
+      const float w1 = A1[f] * wzh, w2 = A2[f] * wzi;
       ws[1][f] = ws[1][f] + w1;
-      vs[1][f] = vs[1][f] + w1 * (o[8] + (o[19 + f] + yz1));
+      vs[1][f] = vs[1][f] + w1 * (o[8] + (E1[f] + yz1));
       ws[2][f] = ws[2][f] + w2;
-      vs[2][f] = vs[2][f] + w2 * (o[9] + (o[25 + f] + yz2));
+      vs[2][f] = vs[2][f] + w2 * (o[9] + (E2[f] + yz2));
     }
   }
 }
@@ -658,7 +676,7 @@ __device__ __forceinline__ void p2g_share_column(const float4* __restrict__ pos,
   for (int cb = 0; cb < n; cb += kShareCap) {
     const int cn = min(n - cb, kShareCap);
 #pragma unroll
-    for (int r = 0; r < kShareCap / 64; ++r) {
+    for (int r = 0; r < (kShareCap + 63) / 64; ++r) {
       const int t = r * 64 + lane;
       if (t < cn) {
         const size_t i = (size_t)(s0 + cb + t);
@@ -669,9 +687,9 @@ __device__ __forceinline__ void p2g_share_column(const float4* __restrict__ pos,
     wave_lds_sync();
     const int lo = max(a, cb) - cb, hi = min(b, cb + cn) - cb;
     for (int t = lo; t < hi; ++t) {
-      float4 v[7];
+      float4 v[kShareF / 4];
       p2g_share_load(sh, t, v);
-      p2g_share_add<kPow2, NX, NYZ>(v, k, ws, vs);
+      p2g_share_add<kPow2, NX, NYZ>(v, k, fx0, fxh, ws, vs);
     }
     wave_lds_sync();
   }
