@@ -152,6 +152,8 @@ CASES = {
     "dam24_chebyshev_coarsest": lambda: chebyshev(small_dam(24, "symmetric"), 12, 0.02),
     "dam32_tank_sphere_chebyshev_odd": lambda: chebyshev(small_dam(32, "symmetric", tank=True), 7, 0.01),
     "tall_16x24x160_chebyshev_segments": lambda: chebyshev(tall_dam(), 9, 0.005, levels=2),
+    # four two-step launches on a 10^3 coarsest level
+    "dam40_chebyshev_8_steps": lambda: chebyshev(small_dam(40, "symmetric"), 8, 0.01),
     # both P2G forms at both tile lengths (liquid filling its tiles, and not)
     "dam24_tank_sphere_p2g_gather": lambda: p2g_gather(small_dam(24, "symmetric", tank=True)),
     "collision40": lambda: small_collision(40),
@@ -241,7 +243,8 @@ def test_stages_bit_exact(case):
                                   "dam24_precond_none", "dam32_tank_sphere_jacobi",
                                   "dam24_symmetric_coarsest_fp64", "dam24_reference_coarsest_fp64",
                                   "dam24_chebyshev_coarsest", "dam32_tank_sphere_chebyshev_odd",
-                                  "tall_16x24x160_chebyshev_segments", "collision40",
+                                  "tall_16x24x160_chebyshev_segments", "dam40_chebyshev_8_steps",
+                                  "collision40",
                                   "collision40_p2g_gather"])
 def test_free_running_steps_bit_exact(case):
     print("case", case, flush=True)
