@@ -696,9 +696,10 @@ __device__ __forceinline__ void p2g_share_column(const float4* __restrict__ pos,
 }
 
 // x positions per tile, as for the gather: a longer row evaluates each record for fewer tiles
-// but leaves fewer busy waves (same-box sweeps of 4 / 8 / 12 / 16: the 256^3 collision, 0.4
-// particles per cell, 1426 / 1543 / 1783 / 2122 us; the 256^3 sloshing tank, 2.9 per cell, 6460
-// / 5598 / 5474 / 5675 us)
+// but leaves fewer busy waves (same-box sweeps with 2-wave workgroups: the 256^3 collision, 0.4
+// particles per cell, 3 / 4 / 6 / 8 positions 1288 / 1262 / 1300 / 1517 us; the 256^3 sloshing
+// tank, 2.9 per cell, 8 / 10 / 12 / 16 positions 4846 / 4875 / 4780 / 4712 us, 12 the fastest
+// step).  The macros let a build sweep them again.
 #ifndef LH_P2G_SHARED_SPARSE_MX
 #define LH_P2G_SHARED_SPARSE_MX 4
 #endif
@@ -707,7 +708,10 @@ __device__ __forceinline__ void p2g_share_column(const float4* __restrict__ pos,
 #endif
 constexpr int kMXSharedSparse = LH_P2G_SHARED_SPARSE_MX, kMXSharedDense = LH_P2G_SHARED_DENSE_MX;
 
-// y rows (waves) per workgroup of the shared-record P2G: each wave holds 14 KB of LDS
+// y rows (waves) per workgroup of the shared-record P2G.  Each wave holds 12 KB of LDS, so the
+// workgroup size sets how many fit a CU: with 2 waves 6 workgroups, 12 waves (the VGPR limit at
+// ~130 registers); with 4 waves 8.  Same box: 4 / 2 / 1 waves 1435 / 1271 / 1283 us on the
+// collision, 5592 / 4870 / 4799 us on the sloshing tank (with 28-float records).
 #ifndef LH_P2G_SHARED_TY
 #define LH_P2G_SHARED_TY 2
 #endif
