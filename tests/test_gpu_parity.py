@@ -108,6 +108,38 @@ def small_collision(N=40):
     return cfg
 
 
+def odd_grid(dx):
+    """17 x 23 x 31 cells of dx (odd on every axis: partial P2G and solver tiles, multigrid levels
+    rounded up; dx = 1/32 takes the power-of-two paths, 1/24 the divisions)"""
+    cfg = small_dam(24, "symmetric")
+    N = [17, 23, 31]
+    mx = [n * dx for n in N]
+    cfg["gridDimension"] = N
+    cfg["maxCoord"] = mx
+    cfg["Boundary"]["child"]["center"] = [m / 2 for m in mx]
+    cfg["Boundary"]["child"]["halfExtent"] = [m / 2 - 3 * dx for m in mx]
+    cfg["FluidBlocks"] = [{"min": [0.0, 0.0, 0.0], "max": [0.5 * mx[0], 0.6 * mx[1], mx[2]],
+                           "velocity": [0.3, 0.0, -0.2], "velocityNoise": 0.05}]
+    return cfg
+
+
+def empty_scene():
+    """no fluid: zero particles (gravity still reaches the interior y-faces, Q8; nothing to solve)"""
+    cfg = small_dam(24, "symmetric")
+    cfg["FluidBlocks"] = []
+    return cfg
+
+
+def single_particle():
+    """one particle in one cell"""
+    cfg = small_dam(24, "symmetric")
+    h = 1.0 / 24
+    cfg["FluidBlocks"] = [{"min": [0.5, 0.5, 0.5], "max": [0.5 + h, 0.5 + h, 0.5 + h], "velocity": [0.5, 0.0, 0.0],
+                           "velocityNoise": 0.0}]
+    cfg["particlesPerCellAxis"] = 1
+    return cfg
+
+
 def cv_cycle(cfg, k):
     cfg["LiquidParameter"]["correctCycle"] = k
     return cfg
@@ -158,6 +190,12 @@ CASES = {
     "dam24_tank_sphere_p2g_gather": lambda: p2g_gather(small_dam(24, "symmetric", tank=True)),
     "collision40": lambda: small_collision(40),
     "collision40_p2g_gather": lambda: p2g_gather(small_collision(40)),
+    # ragged and degenerate inputs
+    "odd_17x23x31_pow2_dx": lambda: odd_grid(1.0 / 32),
+    "odd_17x23x31_dx_1_24": lambda: odd_grid(1.0 / 24),
+    "odd_17x23x31_p2g_gather": lambda: p2g_gather(odd_grid(1.0 / 32)),
+    "empty_scene": empty_scene,
+    "single_particle": single_particle,
 }
 
 
@@ -244,8 +282,8 @@ def test_stages_bit_exact(case):
                                   "dam24_symmetric_coarsest_fp64", "dam24_reference_coarsest_fp64",
                                   "dam24_chebyshev_coarsest", "dam32_tank_sphere_chebyshev_odd",
                                   "tall_16x24x160_chebyshev_segments", "dam40_chebyshev_8_steps",
-                                  "collision40",
-                                  "collision40_p2g_gather"])
+                                  "collision40", "collision40_p2g_gather", "odd_17x23x31_pow2_dx",
+                                  "odd_17x23x31_dx_1_24", "empty_scene", "single_particle"])
 def test_free_running_steps_bit_exact(case):
     print("case", case, flush=True)
     cfg = CASES[case]()
